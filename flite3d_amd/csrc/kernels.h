@@ -1,0 +1,1185 @@
+// Device kernels of the MI355X flow-solver hot path (gfx950, wave64).
+//
+// Layout (device numbering, RCM order, see solver.hip):
+//   node state      u[c*n+i] (SoA, 6 comps), uprev, p, dt, dtt, mu, vort,
+//                   divg, diss[5], src[6], tdiff
+//   rec0[8*i]       AoS per node: rho, u, v, w, rhoE, p, nu~, mu  (gathers)
+//   grec[GREC*i]    AoS per node (HLLC-2): grad 15 | 1/lo 5 | 1/hi 5 | xyz 3 |
+//                   lo 5 | hi 5   (lo/hi = nodeHelpArray 16-25 semantics)
+//   edges           ea[e], eb[e] (ends in reference orientation), w[3*e] AoS
+//   incidence       sliced ELL, one wavefront per slice of 64 nodes:
+//                   slot (k, lane) at ell_ptr[s] + 64*k + lane holds
+//                   int2(edge<<1 | role, neighbour); role 1 = node is eb
+//   boundary sides  ba, bb, bind, wb[3*s]; per node CSR bs_ptr/bs_list
+//                   (side<<1 | end)
+// Every edge loop of the reference (scatter to both ends) is evaluated here
+// as a gather at each node over its incidence list in a fixed order, so
+// results are deterministic and no atomics are used.  The HLLC flux is the
+// exception: it is evaluated once per edge into fbuf and gathered by the
+// stage-update kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/flite3d_amd.h"
+
+namespace fk {
+
+constexpr int SLICE = 64;
+constexpr int GREC = 38;
+constexpr int NTHR = 256;
+
+struct DevParams {
+  double gamma, mach, Re, Pr, Prt;
+  double inflow[2][6];
+  double inflow_temperature;
+  int isentropic, sa, order, autoclip, nsig, bterm;
+  double clip, max_inc, min_rho, max_turb, turb_k;
+  double prolong_relax, turb_prolong_relax, mg_bc_relax;
+};
+
+// node boundary class from nodeIndicatorArray ranges; bit 4 marks host
+// index < numberOfBoundaryNodes (boundary-first numbering)
+enum : int { NC_NONE = 0, NC_WALL = 1, NC_SYM = 2, NC_FAR1 = 3, NC_FAR2 = 4, NC_INVWALL = 9 };
+constexpr int NC_MASK = 15;
+constexpr int NC_BFIRST = 16;
+
+template <typename R>
+struct LevelDev {
+  int n, ne, nbs, gnum;
+  const int *ea, *eb;
+  const R *w;          // 3 per edge
+  const int *ba, *bb, *bind;
+  const R *wb;         // 3 per boundary side
+  const int *ell_ptr, *ell_k;
+  const int2 *ell;
+  const int *bs_ptr, *bs_list;
+  const R *vol, *xyz, *wd, *nrm;  // nrm: 3 per node AoS
+  const int *ncls;
+  R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
+  R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
+};
+
+__device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+
+// iterate the incidence of node i: f(edge, role, nbr)
+template <typename F>
+__device__ __forceinline__ void for_edges(const int *ell_ptr, const int *ell_k, const int2 *ell, int i, F f) {
+  int s = i / SLICE, lane = i % SLICE;
+  int base = ell_ptr[s], K = ell_k[s];
+  for (int k = 0; k < K; ++k) {
+    int2 v = ell[base + k * SLICE + lane];
+    if (v.x < 0) continue;
+    f(v.x >> 1, v.x & 1, v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// node preparation: makePressureField (DEF:13612-13630) + gather records
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_prep(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  int n = L.n;
+  R r = L.u[i], m1 = L.u[n + i], m2 = L.u[2 * n + i], m3 = L.u[3 * n + i], E = L.u[4 * n + i];
+  R pp = R(P.gamma - 1.0) * (E - R(0.5) * (m1 * m1 + m2 * m2 + m3 * m3) / r);
+  pp = pp > R(0.1) ? pp : R(0.1);
+  L.p[i] = pp;
+  R *q = L.rec0 + 8 * (size_t)i;
+  q[0] = r;
+  q[1] = m1 / r;
+  q[2] = m2 / r;
+  q[3] = m3 / r;
+  q[4] = E;
+  q[5] = pp;
+  q[6] = P.sa ? L.u[5 * n + i] : R(0);
+  q[7] = L.mu[i];
+}
+
+// gather records from the current state without recomputing p (parity ops)
+template <typename R>
+__global__ void k_pack(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  int n = L.n;
+  R r = L.u[i];
+  R *q = L.rec0 + 8 * (size_t)i;
+  q[0] = r;
+  q[1] = L.u[n + i] / r;
+  q[2] = L.u[2 * n + i] / r;
+  q[3] = L.u[3 * n + i] / r;
+  q[4] = L.u[4 * n + i];
+  q[5] = L.p[i];
+  q[6] = P.sa ? L.u[5 * n + i] : R(0);
+  q[7] = L.mu[i];
+}
+
+// ---------------------------------------------------------------------------
+// computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459), gathered
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
+  R g[15];
+  R mn[5], mx[5];
+  for (int c = 0; c < 15; ++c) g[c] = 0;
+  for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
+  bool any = false;
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    const R *qj = L.rec0 + 8 * (size_t)j;
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    R sg = role ? R(-1) : R(1);
+    any = true;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      R s = q0[c] + qj[c];
+      g[c] += sg * (s * wx);
+      g[5 + c] += sg * (s * wy);
+      g[10 + c] += sg * (s * wz);
+      R lo = rmin(q0[c], qj[c]);
+      R hi = rmin(-q0[c], -qj[c]);
+      mn[c] = rmin(mn[c], lo);
+      mx[c] = rmin(mx[c], hi);
+    }
+  });
+  (void)any;
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      R t = R(2) * q0[c] + (q0[c] + qo[c]);
+      g[c] += t * wx;
+      g[5 + c] += t * wy;
+      g[10 + c] += t * wz;
+    }
+  }
+  R vt = R(1) / L.vol[i];
+  R *out = L.grec + GREC * (size_t)i;
+#pragma unroll
+  for (int c = 0; c < 15; ++c) out[c] = g[c] * vt;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    R lo = mn[c] - q0[c];
+    R hi = -mx[c] - q0[c];
+    out[28 + c] = lo;
+    out[33 + c] = hi;
+    out[15 + c] = lo < R(0) ? R(1) / lo : R(0);
+    out[20 + c] = hi > R(0) ? R(1) / hi : R(0);
+  }
+  if (L.xyz) {
+    out[25] = L.xyz[3 * (size_t)i];
+    out[26] = L.xyz[3 * (size_t)i + 1];
+    out[27] = L.xyz[3 * (size_t)i + 2];
+  } else {
+    out[25] = out[26] = out[27] = 0;
+  }
+}
+
+// LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds
+template <typename R>
+__device__ __forceinline__ R led(R a, R ilo, R ihi, int nsig) {
+  R a1 = R(2) * a;
+  if (a1 > R(0)) {
+    if (ihi == R(0)) return R(0);
+    R t = a1 * ihi, t2 = t * t;
+    R den = nsig == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
+    return R(0.5) * a1 / den;
+  } else if (a1 < R(0)) {
+    if (ilo == R(0)) return R(0);
+    R t = a1 * ilo, t2 = t * t;
+    R den = nsig == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
+    return R(0.5) * a1 / den;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// HLLC interior flux (makeRHS, DEF:1794-2100), one evaluation per edge.
+// fbuf[NF*e + c]: flux leaving ea towards eb (+ to ea, - to eb);
+// component 5 is the SA convection (makeSARHS, DEF:6873-6880).
+// ---------------------------------------------------------------------------
+template <typename R, int ORDER, bool SA>
+__global__ void k_flux(LevelDev<R> L, DevParams P) {
+  int e = gtid();
+  if (e >= L.ne) return;
+  constexpr int NF = 6;
+  int ia = L.ea[e], ib = L.eb[e];
+  const R *A = L.rec0 + 8 * (size_t)ia;
+  const R *B = L.rec0 + 8 * (size_t)ib;
+  R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+  R gam = R(P.gamma), gm1 = gam - R(1);
+  R roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
+  R roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
+  R fsa = 0;
+  if (SA) {
+    R t1 = A[6], t2 = B[6];
+    fsa = wx * (u1 * t1 + u2 * t2) + wy * (v1 * t1 + v2 * t2) + wz * (w1 * t1 + w2 * t2);
+    R dtt = t1 - t2;
+    if (fabs(dtt) > R(1e-20)) {
+      R id = R(1) / dtt;
+      R ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
+      fsa = fsa + fabs(wx * ax + wy * ay + wz * az) * dtt;
+    }
+  }
+  R ru1 = roe1 * u1, rv1 = roe1 * v1, rw1 = roe1 * w1, rh1 = re1 + pr1;
+  R ru2 = roe2 * u2, rv2 = roe2 * v2, rw2 = roe2 * w2, rh2 = re2 + pr2;
+  // central part: only kept when no HLLC branch is taken (NaN wave speeds)
+  R fc[5];
+  fc[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
+  fc[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
+  fc[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
+  fc[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
+  fc[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
+  R nx = R(2) * wx, ny = R(2) * wy, nz = R(2) * wz;
+  R rnorm1 = sqrt(nx * nx + ny * ny + nz * nz);
+  R inv = rnorm1 > R(0) ? R(1) / rnorm1 : rnorm1;
+  R rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
+  if (ORDER > 1) {
+    const R *G1 = L.grec + GREC * (size_t)ia;
+    const R *G2 = L.grec + GREC * (size_t)ib;
+    R dx = G2[25] - G1[25], dy = G2[26] - G1[26], dz = G2[27] - G1[27];
+    R hl = R(0.5) * R(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
+    R d[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      R a = R(0.5) * (G1[j] * dx + G1[j + 5] * dy + G1[j + 10] * dz);
+      R b = R(0.5) * (G2[j] * dx + G2[j + 5] * dy + G2[j + 10] * dz);
+      if (P.autoclip == 1) {
+        a = led(a, G1[15 + j], G1[20 + j], P.nsig);
+        b = -led(-b, G2[15 + j], G2[20 + j], P.nsig);
+      }
+      R c;
+      if (a * b <= R(0)) {
+        c = 0;
+      } else {
+        c = fabs(a) <= fabs(b) ? a : b;
+        if (fabs(c) > hl && hl > R(0)) c = c > R(0) ? hl : -hl;
+      }
+      d[j] = c;
+    }
+    if (roe1 + d[0] > R(0.1)) roe1 = roe1 + d[0];
+    u1 += d[1]; v1 += d[2]; w1 += d[3];
+    if (re1 + d[4] > R(0.0001)) re1 = re1 + d[4];
+    pr1 = gm1 * (re1 - R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1) * roe1);
+    pr1 = pr1 > R(0.1) ? pr1 : R(0.1);
+    rh1 = re1 + pr1;
+    ru1 = roe1 * u1; rv1 = roe1 * v1; rw1 = roe1 * w1;
+    if (roe2 - d[0] > R(0.1)) roe2 = roe2 - d[0];
+    u2 -= d[1]; v2 -= d[2]; w2 -= d[3];
+    if (re2 - d[4] > R(0.0001)) re2 = re2 - d[4];
+    pr2 = gm1 * (re2 - R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2) * roe2);
+    pr2 = pr2 > R(0.1) ? pr2 : R(0.1);
+    rh2 = re2 + pr2;
+    ru2 = roe2 * u2; rv2 = roe2 * v2; rw2 = roe2 * w2;
+  }
+  R rinv1 = R(1) / roe1, rinv2 = R(1) / roe2;
+  R nv1 = u1 * rnx + v1 * rny + w1 * rnz;
+  R nv2 = u2 * rnx + v2 * rny + w2 * rnz;
+  R beta0 = sqrt(gm1 / (R(2) * gam));
+  R rroe = sqrt(roe2 * rinv1);
+  R iro = R(1) / (R(1) + rroe);
+  R ut = (u1 + u2 * rroe) * iro, vt = (v1 + v2 * rroe) * iro, wt = (w1 + w2 * rroe) * iro;
+  R qt = ut * rnx + vt * rny + wt * rnz;
+  R ht = (rh1 * rinv1 + rh2 * rinv2 * rroe) * iro;
+  R ct = sqrt(fabs(gm1 * (ht - R(0.5) * (ut * ut + vt * vt + wt * wt))));
+  R c1 = sqrt(gm1 * fabs(rh1 * rinv1 - R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1)));
+  R c2 = sqrt(gm1 * fabs(rh2 * rinv2 - R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2)));
+  R sl = fmin(fmin(nv1 - beta0 * c1, qt - ct), nv2 - beta0 * c2);
+  R sr = fmax(fmax(nv2 + beta0 * c2, qt + ct), nv1 + beta0 * c1);
+  R sm = roe2 * nv2 * (sr - nv2) - roe1 * nv1 * (sl - nv1) + pr1 - pr2;
+  R fact = roe2 * (sr - nv2) - roe1 * (sl - nv1);
+  if (fact != R(0)) fact = R(1) / fact;
+  sm = fact * sm;
+  R f[5];
+  if (sl > R(0) || (sl <= R(0) && sm > R(0))) {
+    // left flux (+ star correction)
+    f[0] = rnx * ru1 + rny * rv1 + rnz * rw1;
+    f[1] = rnx * (ru1 * u1 + pr1) + rny * (ru1 * v1) + rnz * (ru1 * w1);
+    f[2] = rnx * (rv1 * u1) + rny * (rv1 * v1 + pr1) + rnz * (rv1 * w1);
+    f[3] = rnx * (rw1 * u1) + rny * (rw1 * v1) + rnz * (rw1 * w1 + pr1);
+    f[4] = rnx * (rh1 * u1) + rny * (rh1 * v1) + rnz * (rh1 * w1);
+    if (!(sl > R(0))) {
+      R fl = sl - sm;
+      if (fl != R(0)) {
+        fl = R(1) / fl;
+        R a = sl - nv1;
+        R pls = roe1 * (nv1 - sl) * (nv1 - sm) + pr1;
+        R ps = pls - pr1;
+        R s0 = roe1 * a * fl, s1 = (a * roe1 * u1 + ps * rnx) * fl;
+        R s2 = (a * roe1 * v1 + ps * rny) * fl, s3 = (a * roe1 * w1 + ps * rnz) * fl;
+        R s4 = (a * re1 - pr1 * nv1 + pls * sm) * fl;
+        f[0] += sl * (s0 - roe1);
+        f[1] += sl * (s1 - ru1);
+        f[2] += sl * (s2 - rv1);
+        f[3] += sl * (s3 - rw1);
+        f[4] += sl * (s4 - re1);
+      }
+    }
+  } else if ((sm <= R(0) && sr >= R(0)) || sr < R(0)) {
+    f[0] = rnx * ru2 + rny * rv2 + rnz * rw2;
+    f[1] = rnx * (ru2 * u2 + pr2) + rny * (ru2 * v2) + rnz * (ru2 * w2);
+    f[2] = rnx * (rv2 * u2) + rny * (rv2 * v2 + pr2) + rnz * (rv2 * w2);
+    f[3] = rnx * (rw2 * u2) + rny * (rw2 * v2) + rnz * (rw2 * w2 + pr2);
+    f[4] = rnx * (rh2 * u2) + rny * (rh2 * v2) + rnz * (rh2 * w2);
+    if (!(sr < R(0))) {
+      R fr = sr - sm;
+      if (fr != R(0)) {
+        fr = R(1) / fr;
+        R a = sr - nv2;
+        R prs = roe2 * (nv2 - sr) * (nv2 - sm) + pr2;
+        R ps = prs - pr2;
+        R s0 = roe2 * a * fr, s1 = (a * roe2 * u2 + ps * rnx) * fr;
+        R s2 = (a * roe2 * v2 + ps * rny) * fr, s3 = (a * roe2 * w2 + ps * rnz) * fr;
+        R s4 = (a * re2 - pr2 * nv2 + prs * sm) * fr;
+        f[0] += sr * (s0 - roe2);
+        f[1] += sr * (s1 - ru2);
+        f[2] += sr * (s2 - rv2);
+        f[3] += sr * (s3 - rw2);
+        f[4] += sr * (s4 - re2);
+      }
+    }
+  } else {
+    R *o = L.fbuf + NF * (size_t)e;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = fc[c];
+    o[5] = fsa;
+    return;
+  }
+  R *o = L.fbuf + NF * (size_t)e;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) o[c] = f[c] * rnorm1;
+  o[5] = fsa;
+}
+
+// physical flux of one state through unit normal
+template <typename R>
+__device__ __forceinline__ void phys_flux(const R *q, R rnx, R rny, R rnz, R *f) {
+  R rho = q[0], u = q[1], v = q[2], w = q[3], pr = q[5], rh = q[4] + q[5];
+  R nv = u * rnx + v * rny + w * rnz;
+  f[0] = rho * nv;
+  f[1] = pr * rnx + rho * u * nv;
+  f[2] = pr * rny + rho * v * nv;
+  f[3] = pr * rnz + rho * w * nv;
+  f[4] = rh * nv;
+}
+
+// farfield Roe flux of one state against the inflow state (DEF:2258-2446)
+template <typename R>
+__device__ __forceinline__ void farfield_roe(const double *inf, const R *q, R gm1, R rnx, R rny, R rnz, R *f) {
+  R ro = R(inf[0]), uo = R(inf[1]), vo = R(inf[2]), wo = R(inf[3]), eo = R(inf[4]), po = R(inf[5]);
+  R rho = q[0], uu = q[1], vv = q[2], ww = q[3], eps = q[4], pr = q[5];
+  R rh = eps + pr, h = rh / rho;
+  R nv = uu * rnx + vv * rny + ww * rnz;
+  R fl1 = rho * nv, fl2 = pr * rnx + rho * uu * nv, fl3 = pr * rny + rho * vv * nv;
+  R fl4 = pr * rnz + rho * ww * nv, fl5 = rh * nv;
+  R epsr = eo * ro, hr = eo + po / ro;
+  R unr = uo * rnx + vo * rny + wo * rnz;
+  R fr1 = ro * unr, fr2 = rnx * po + ro * uo * unr, fr3 = rny * po + ro * vo * unr;
+  R fr4 = rnz * po + ro * wo * unr, fr5 = (epsr + po) * unr;
+  R di = sqrt(ro / rho), d1 = R(1) / (di + R(1));
+  R ui = (di * uo + uu) * d1, vi = (di * vo + vv) * d1, wi = (di * wo + ww) * d1, hi = (di * hr + h) * d1;
+  R af = R(0.5) * (ui * ui + vi * vi + wi * wi);
+  R ci2 = gm1 * (hi - af);
+  ci2 = ci2 > R(1e-5) ? ci2 : R(1e-5);
+  R ci = sqrt(ci2), ici = R(1) / ci;
+  R ucp = ui * rnx + vi * rny + wi * rnz;
+  R du1 = ro - rho, du2 = ro * uo - rho * uu, du3 = ro * vo - rho * vv;
+  R du4 = ro * wo - rho * ww, du5 = epsr - eps;
+  R rl1 = fabs(ucp + ci), rl2 = fabs(ucp - ci), rl3 = fabs(ucp);
+  R s1 = R(0.5) * (rl1 + rl2), s2 = R(0.5) * (rl1 - rl2);
+  R al1x = gm1 * (af * du1 - ui * du2 - vi * du3 - wi * du4 + du5);
+  R al2x = -ucp * du1 + du2 * rnx + du3 * rny + du4 * rnz;
+  R cc1 = ((s1 - rl3) * al1x / ci2) + (s2 * al2x * ici);
+  R cc2 = (s2 * al1x * ici) + (s1 - rl3) * al2x;
+  f[0] = R(0.5) * (fr1 + fl1) - R(0.5) * (rl3 * du1 + cc1);
+  f[1] = R(0.5) * (fr2 + fl2) - R(0.5) * (rl3 * du2 + cc1 * ui + cc2 * rnx);
+  f[2] = R(0.5) * (fr3 + fl3) - R(0.5) * (rl3 * du3 + cc1 * vi + cc2 * rny);
+  f[3] = R(0.5) * (fr4 + fl4) - R(0.5) * (rl3 * du4 + cc1 * wi + cc2 * rnz);
+  f[4] = R(0.5) * (fr5 + fl5) - R(0.5) * (rl3 * du5 + cc1 * hi + cc2 * ucp);
+}
+
+// boundary-side loop of makeRHS (DEF:2116-2446): contributions to both ends,
+// bbuf[10*s + 5*end + c]
+template <typename R>
+__global__ void k_bflux(LevelDev<R> L, DevParams P) {
+  int s = gtid();
+  if (s >= L.nbs) return;
+  int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
+  R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  R *o = L.bbuf + 10 * (size_t)s;
+  R rnorm = sqrt(wx * wx + wy * wy + wz * wz);
+  if (!(rnorm > R(1e-10)) || !(ind <= 4 || ind == 9)) {
+#pragma unroll
+    for (int c = 0; c < 10; ++c) o[c] = 0;
+    return;
+  }
+  R irn = R(1) / rnorm;
+  R rnx = wx * irn, rny = wy * irn, rnz = wz * irn;
+  const R *A = L.rec0 + 8 * (size_t)ia;
+  const R *B = L.rec0 + 8 * (size_t)ib;
+  R f1[5], f2[5];
+  if (ind <= 2 || ind == 9) {
+    phys_flux(A, rnx, rny, rnz, f1);
+    phys_flux(B, rnx, rny, rnz, f2);
+  } else {
+    R gm1 = R(P.gamma - 1.0);
+    farfield_roe(P.inflow[ind - 3], A, gm1, rnx, rny, rnz, f1);
+    farfield_roe(P.inflow[ind - 3], B, gm1, rnx, rny, rnz, f2);
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    if (P.bterm == 1) {
+      o[c] = rnorm * (R(3) * f1[c] + f2[c]);
+      o[5 + c] = rnorm * (f1[c] + R(3) * f2[c]);
+    } else {
+      o[c] = rnorm * R(4) * f1[c];
+      o[5 + c] = rnorm * R(4) * f2[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// makeTimeSteps (DEF:13335-13610), gathered
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_timesteps(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  R gam = R(P.gamma);
+  R cc = gam * qi[5] / qi[0];
+  cc = cc > R(0.001) ? cc : R(0.001);
+  R lam = 0, lamv = 0;
+  bool visc = P.Re > 0.0;
+  R am = visc ? R(1.0 / P.Re) : R(0);
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    R al = wx * wx + wy * wy + wz * wz;
+    R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
+    lam += fabs(vn) + sqrt(cc * al);
+    if (visc) {
+      const R *qj = L.rec0 + 8 * (size_t)j;
+      R vis = P.sa ? qj[7] + qj[6] * am : qj[7];
+      vis = am > vis ? am : vis;
+      lamv += R(2) * vis * al / qj[0];
+    }
+    (void)role;
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    R al = wx * wx + wy * wy + wz * wz;
+    R cco = gam * qo[5] / qo[0];
+    cco = cco > R(0.001) ? cco : R(0.001);
+    R vn1 = fabs(wx * qi[1] + wy * qi[2] + wz * qi[3]);
+    R vn2 = fabs(wx * qo[1] + wy * qo[2] + wz * qo[3]);
+    R c1 = sqrt(cc * al), c2 = sqrt(cco * al);
+    lam += vn1 + (vn1 + vn2) + c1 + (c1 + c2);
+    if (visc) {
+      R vis1 = P.sa ? qi[7] + qi[6] * am : qi[7];
+      R vis2 = P.sa ? qo[7] + qo[6] * am : qo[7];
+      vis1 = am > vis1 ? am : vis1;
+      vis2 = am > vis2 ? am : vis2;
+      R dv1 = R(2) * vis1 * al / qi[0], dv2 = R(2) * vis2 * al / qo[0];
+      lamv += dv1 + (dv1 + dv2);
+    }
+  }
+  R V = L.vol[i];
+  L.dt[i] = R(0.5) / (lam / V + lamv / (V * V));
+}
+
+// ---------------------------------------------------------------------------
+// makeViscosityTerm (DEF:5198-5893), pass 1: gradients of (u,v,w,T) and the
+// node loop (Sutherland, SA eddy viscosity, stress, heat flux, vorticity,
+// divergence, adiabatic wall, wall stress).  tau[9*i]: stress 6 + k grad T 3
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_visc_node(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  R gog1 = R(P.gamma / (P.gamma - 1.0));
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  R ai[4] = {qi[1], qi[2], qi[3], gog1 * qi[5] / qi[0]};
+  R g[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) g[c] = 0;
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    const R *qj = L.rec0 + 8 * (size_t)j;
+    R aj[4] = {qj[1], qj[2], qj[3], gog1 * qj[5] / qj[0]};
+    R ww[3] = {L.w[3 * e], L.w[3 * e + 1], L.w[3 * e + 2]};
+    R sg = role ? R(-1) : R(1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[4 * d + q] += sg * ((ai[q] + aj[q]) * ww[d]);
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    R ao[4] = {qo[1], qo[2], qo[3], gog1 * qo[5] / qo[0]};
+    R ww[3] = {L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[4 * d + q] += (R(2) * ai[q] + (ai[q] + ao[q])) * ww[d];
+  }
+  R vt = R(1) / L.vol[i];
+  R dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
+  R dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
+  R dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
+  R T = ai[3];
+  if (T <= R(0.001)) T = R(0.001);
+  R tc = R((P.gamma - 1.0) * P.mach * P.mach), Tin = R(P.inflow_temperature);
+  R tct = tc * T;
+  R mu = R(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + R(198.6)) / (tc * Tin * T + R(198.6))));
+  R k = R(1.0 / P.Pr) * mu;
+  L.mu[i] = mu;
+  L.rec0[8 * (size_t)i + 7] = mu;  // later SA kernels read the new laminar viscosity
+  L.vort[i] = sqrt((dudy - dvdx) * (dudy - dvdx) + (dvdz - dwdy) * (dvdz - dwdy) + (dwdx - dudz) * (dwdx - dudz));
+  L.divg[i] = dudx + dvdy + dwdz;
+  if (P.sa) {
+    double tvc = (1.0 / P.Re) * double(qi[0]) * double(qi[6]);
+    double ksi = tvc / double(mu);
+    double k3 = ksi * ksi * ksi;
+    tvc = tvc * k3 / (k3 + 357.9);
+    mu = mu + R(tvc);
+    k = k + R(tvc * (1.0 / P.Prt));
+  }
+  R t[9];
+  R two3 = R(2.0 / 3.0), four3 = R(4.0 / 3.0);
+  t[0] = mu * (four3 * dudx - two3 * (dvdy + dwdz));
+  t[1] = mu * (dudy + dvdx);
+  t[2] = mu * (dudz + dwdx);
+  t[3] = mu * (four3 * dvdy - two3 * (dudx + dwdz));
+  t[4] = mu * (dvdz + dwdy);
+  t[5] = mu * (four3 * dwdz - two3 * (dudx + dvdy));
+  t[6] = k * dTdx;
+  t[7] = k * dTdy;
+  t[8] = k * dTdz;
+  int cls = L.ncls[i] & NC_MASK;
+  if (L.gnum == 1 && cls != NC_NONE) {
+    R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+    if (cls == NC_WALL) {  // setTemperatureForAdiabaticWall (DEF:5170-5196)
+      R nt = t[6] * nx + t[7] * ny + t[8] * nz;
+      t[6] -= nt * nx;
+      t[7] -= nt * ny;
+      t[8] -= nt * nz;
+    }
+    L.ws[3 * (size_t)i] = t[0] * nx + t[1] * ny + t[2] * nz;
+    L.ws[3 * (size_t)i + 1] = t[1] * nx + t[3] * ny + t[4] * nz;
+    L.ws[3 * (size_t)i + 2] = t[2] * nx + t[4] * ny + t[5] * nz;
+  }
+  R *o = L.tau + 9 * (size_t)i;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) o[c] = t[c];
+}
+
+// makeViscosityTerm pass 2 (DEF:5640-5893): dissipation(:,2:5) = -sum
+template <typename R>
+__global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  const R *ti = L.tau + 9 * (size_t)i;
+  R t1[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) t1[c] = ti[c];
+  R u1 = qi[1], v1 = qi[2], w1 = qi[3];
+  R a[4] = {0, 0, 0, 0};
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    const R *qj = L.rec0 + 8 * (size_t)j;
+    const R *tj = L.tau + 9 * (size_t)j;
+    R u2 = qj[1], v2 = qj[2], w2 = qj[3];
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    R d[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) d[c] = t1[c] + tj[c];
+    R dut1 = u1 * t1[0] + u2 * tj[0] + v1 * t1[1] + v2 * tj[1] + w1 * t1[2] + w2 * tj[2];
+    R dut2 = u1 * t1[1] + u2 * tj[1] + v1 * t1[3] + v2 * tj[3] + w1 * t1[4] + w2 * tj[4];
+    R dut3 = u1 * t1[2] + u2 * tj[2] + v1 * t1[4] + v2 * tj[4] + w1 * t1[5] + w2 * tj[5];
+    R sg = role ? R(-1) : R(1);
+    a[0] += sg * (wx * d[0] + wy * d[1] + wz * d[2]);
+    a[1] += sg * (wx * d[1] + wy * d[3] + wz * d[4]);
+    a[2] += sg * (wx * d[2] + wy * d[4] + wz * d[5]);
+    a[3] += sg * (wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3));
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int ind = L.bind[s];
+    if (ind == 2 || ind == 9) continue;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    const R *to = L.tau + 9 * (size_t)o;
+    R u2 = qo[1], v2 = qo[2], w2 = qo[3];
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    R d[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) d[c] = t1[c] + to[c];
+    R du11 = u1 * t1[0] + v1 * t1[1] + w1 * t1[2], du12 = u2 * to[0] + v2 * to[1] + w2 * to[2];
+    R du21 = u1 * t1[1] + v1 * t1[3] + w1 * t1[4], du22 = u2 * to[1] + v2 * to[3] + w2 * to[4];
+    R du31 = u1 * t1[2] + v1 * t1[4] + w1 * t1[5], du32 = u2 * to[2] + v2 * to[4] + w2 * to[5];
+    R dut1 = du11 + du12, dut2 = du21 + du22, dut3 = du31 + du32;
+    a[0] += (wx * d[0] + wy * d[1] + wz * d[2]) + R(2) * (wx * t1[0] + wy * t1[1] + wz * t1[2]);
+    a[1] += (wx * d[1] + wy * d[3] + wz * d[4]) + R(2) * (wx * t1[1] + wy * t1[3] + wz * t1[4]);
+    a[2] += (wx * d[2] + wy * d[4] + wz * d[5]) + R(2) * (wx * t1[2] + wy * t1[4] + wz * t1[5]);
+    a[3] += (wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3)) +
+            R(2) * (wx * (t1[6] + du11) + wy * (t1[7] + du21) + wz * (t1[8] + du31));
+  }
+  int n = L.n;
+  L.diss[i] = 0;
+  L.diss[n + i] = -a[0];
+  L.diss[2 * n + i] = -a[1];
+  L.diss[3 * n + i] = -a[2];
+  L.diss[4 * n + i] = -a[3];
+  (void)P;
+}
+
+// ---------------------------------------------------------------------------
+// makeTurbulentTimeSteps (DEF:12878-13099)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  R ioRe = R(1.0 / P.Re), oos = R(1.5), cb2 = R(0.622), tk = R(P.turb_k);
+  R lam = 0, lamv = 0;
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    R al = wx * wx + wy * wy + wz * wz;
+    R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
+    lam += tk * sqrt(al) + fabs(vn);
+    const R *qj = L.rec0 + 8 * (size_t)j;
+    R vis = oos * (qj[7] + R(3) * cb2 * qj[6] * ioRe);
+    lamv += R(2) * vis * al;
+    (void)role;
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    R al = wx * wx + wy * wy + wz * wz;
+    R vn1 = tk * sqrt(al) + fabs(wx * qi[1] + wy * qi[2] + wz * qi[3]);
+    R vn2 = tk * sqrt(al) + fabs(wx * qo[1] + wy * qo[2] + wz * qo[3]);
+    lam += vn1 + (vn1 + vn2);
+    R dv1 = R(2) * (oos * (qi[7] + R(3) * cb2 * qi[6] * ioRe)) * al;
+    R dv2 = R(2) * (oos * (qo[7] + R(3) * cb2 * qo[6] * ioRe)) * al;
+    lamv += dv1 + (dv1 + dv2);
+  }
+  R V = L.vol[i];
+  R dt = R(0.5) / (lam / V + lamv / (V * V));
+  R wd = L.wd[i];
+  R di = wd > R(1e-10) ? R(1) / wd : R(1e10);
+  di = di * di;
+  R ik2 = R(1.0 / (0.41 * 0.41)), cb1 = R(0.1355), ct3 = R(1.1);
+  R cw1 = cb1 * ik2 + (R(1) + cb2) * oos;
+  R nu = qi[6];
+  R S = fabs(L.divg[i]) + cb1 * (L.vort[i] + (R(1) + ct3) * nu * ik2 * di * ioRe) + cw1 * nu * di * ioRe;
+  L.dtt[i] = dt / (R(1) + R(0.5) * S * dt);
+}
+
+// makeSADiffusion (DEF:7436-7808), pass 1: grad nu~ (dnu[4*i]: 3 + nodal
+// cb2 term)
+template <typename R>
+__global__ void k_sa_diff1(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  R ti = L.rec0[8 * (size_t)i + 6];
+  R g[3] = {0, 0, 0};
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    R s = ti + L.rec0[8 * (size_t)j + 6];
+    R sg = role ? R(-1) : R(1);
+    g[0] += sg * (L.w[3 * e] * s);
+    g[1] += sg * (L.w[3 * e + 1] * s);
+    g[2] += sg * (L.w[3 * e + 2] * s);
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    R to = L.rec0[8 * (size_t)o + 6];
+    R c = P.bterm == 1 ? (R(3) * ti + to) : R(4) * ti;
+    g[0] += c * L.wb[3 * s];
+    g[1] += c * L.wb[3 * s + 1];
+    g[2] += c * L.wb[3 * s + 2];
+  }
+  R vt = R(1) / L.vol[i];
+  R dx = g[0] * vt, dy = g[1] * vt, dz = g[2] * vt;
+  R fact = R(1.0 / P.Re * 0.622 / (2.0 / 3.0));
+  R *o = L.dnu + 4 * (size_t)i;
+  o[0] = dx;
+  o[1] = dy;
+  o[2] = dz;
+  o[3] = -(fact * (g[0] * dx + g[1] * dy + g[2] * dz));
+}
+
+// makeSADiffusion pass 2
+template <typename R>
+__global__ void k_sa_diff2(LevelDev<R> L, DevParams P) {
+  int i = gtid();
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  const R *di = L.dnu + 4 * (size_t)i;
+  R Re = R(P.Re);
+  R fact = R(1.0 / P.Re / (2.0 / 3.0));
+  R v1 = Re * qi[7] / qi[0] + qi[6];
+  R acc = 0;
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    const R *qj = L.rec0 + 8 * (size_t)j;
+    const R *dj = L.dnu + 4 * (size_t)j;
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    R v2 = Re * qj[7] / qj[0] + qj[6];
+    R fd = v1 * (wx * di[0] + wy * di[1] + wz * di[2]) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
+    acc += role ? fact * fd : -(fact * fd);
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const R *qo = L.rec0 + 8 * (size_t)o;
+    const R *dq = L.dnu + 4 * (size_t)o;
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    R vo = Re * qo[7] / qo[0] + qo[6];
+    R fd1 = v1 * (wx * di[0] + wy * di[1] + wz * di[2]);
+    R fd2 = vo * (wx * dq[0] + wy * dq[1] + wz * dq[2]);
+    acc -= P.bterm == 1 ? fact * (R(3) * fd1 + fd2) : fact * R(4) * fd1;
+  }
+  L.tdiff[i] = acc + di[3];
+}
+
+// ---------------------------------------------------------------------------
+// node-local pieces shared by the stage kernels
+// ---------------------------------------------------------------------------
+
+// makeSASourceTerm (DEF:9285-9370) for node i; returns the rhs(:,6) increment
+template <typename R>
+__device__ __forceinline__ double sa_source(LevelDev<R> &L, const DevParams &P, int i) {
+  double Re = P.Re, ioRe = 1.0 / Re;
+  double cb1 = 0.1355, cb2 = 0.622, k2 = 0.41 * 0.41, sigma = 2.0 / 3.0;
+  double cw1 = cb1 / k2 + (1.0 + cb2) / sigma, cw2 = 0.3, cw36 = 64.0, cv13 = 7.1 * 7.1 * 7.1;
+  double ct3 = 1.1, ct4 = 2.0;
+  int n = L.n;
+  double vt = L.vol[i];
+  double mc = L.u[5 * n + i];
+  mc = mc > 0.0 ? mc : 0.0;
+  double mu = L.mu[i];
+  if (mu < 0.1 / Re) {
+    mu = 0.1 * Re;
+    L.mu[i] = R(mu);
+  }
+  double ksi = double(L.u[i]) * ioRe * mc / mu;
+  ksi = ksi > 0.001 ? ksi : 0.001;
+  double ft2 = ct3 * exp(-ct4 * ksi * ksi);
+  double k3 = ksi * ksi * ksi;
+  double d = L.wd[i];
+  d = d > 1e-10 ? d : 1e-10;
+  double md = mc / d, d2 = d * d;
+  double fv1 = k3 / (k3 + cv13);
+  double fv2 = 1.0 - ksi / (1.0 + ksi * fv1);
+  double Sc = double(L.vort[i]) + ioRe * mc * fv2 / (k2 * d2);
+  double r;
+  if (fabs(Sc) > 1e-10) {
+    r = ioRe * mc / (Sc * k2 * d2);
+    r = r < 0.0 ? 0.0 : (r > 100.0 ? 100.0 : r);
+  } else {
+    r = 100.0;
+  }
+  double r2 = r * r, r6 = r2 * r2 * r2;
+  double g = r + cw2 * (r6 - r);
+  double g2 = g * g, g6 = g2 * g2 * g2;
+  double rad = (1.0 + cw36) / (g6 + cw36);
+  double fw = rad > 1e-10 ? g * pow(rad, 1.0 / 6.0) : pow(1.0 + cw36, 1.0 / 6.0);
+  return -vt * cb1 * (1.0 - ft2) * Sc * mc + ioRe * vt * (cw1 * fw - cb1 * ft2 / k2) * md * md -
+         vt * mc * double(L.divg[i]);
+}
+
+// setBCsOnIncrementField (DEF:4978-5140) for node i on r[0..5]
+template <typename R>
+__device__ __forceinline__ void bc_increment(const LevelDev<R> &L, const DevParams &P, int i, R *r) {
+  int cls = L.ncls[i] & NC_MASK;
+  if (cls == NC_NONE || cls == NC_FAR1 || cls == NC_FAR2) return;
+  R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+  if (cls == NC_WALL && P.Re > 0.0) {
+    r[1] = r[2] = r[3] = 0;
+    if (!P.isentropic) r[4] = 0;
+    if (P.sa) r[5] = 0;
+    return;
+  }
+  R ni = r[1] * nx + r[2] * ny + r[3] * nz;
+  r[1] -= ni * nx;
+  r[2] -= ni * ny;
+  r[3] -= ni * nz;
+  if (P.sa && cls != NC_SYM) r[5] = 0;
+}
+
+// setBCsOnSolutionField (DEF:4364-4976) for node i on q[0..5]
+template <typename R>
+__device__ __forceinline__ void bc_solution(const LevelDev<R> &L, const DevParams &P, int i, R *q) {
+  int cls = L.ncls[i] & NC_MASK;
+  if (cls == NC_NONE || cls == NC_FAR1 || cls == NC_FAR2) return;
+  R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+  if (cls == NC_WALL && P.Re > 0.0) {
+    q[1] = q[2] = q[3] = 0;
+    if (!P.isentropic) q[4] = q[0] * R(P.inflow_temperature / P.gamma);
+    if (P.sa) q[5] = 0;
+    return;
+  }
+  R nv = q[1] * nx + q[2] * ny + q[3] * nz;
+  q[1] -= nv * nx;
+  q[2] -= nv * ny;
+  q[3] -= nv * nz;
+  if (cls == NC_INVWALL && !P.isentropic)
+    q[4] = q[0] * R(P.inflow_temperature / P.gamma) + R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0];
+  if (P.sa && cls != NC_SYM) q[5] = 0;
+}
+
+// setOuterBoundaryConditions (DEF:4018-4251) for node i
+template <typename R>
+__device__ __forceinline__ void outer_bc(const LevelDev<R> &L, const DevParams &P, int i, R *q, R p) {
+  int cls = L.ncls[i] & NC_MASK;
+  if (cls != NC_FAR1 && cls != NC_FAR2) return;
+  const double *inf = P.inflow[cls - NC_FAR1];
+  R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+  R rho0 = R(inf[0]), U1 = R(inf[1]), U2 = R(inf[2]), U3 = R(inf[3]);
+  R gm1 = R(P.gamma - 1.0);
+  R nv = U1 * nx + U2 * ny + U3 * nz;
+  if (nv > R(0)) {
+    R rho = q[0];
+    (void)rho;
+    q[0] = rho0;
+    q[1] = U1 * rho0;
+    q[2] = U2 * rho0;
+    q[3] = U3 * rho0;
+    q[4] = P.mach > 1.0 ? R(inf[4]) * rho0 : p / gm1 + R(0.5) * (U1 * U1 + U2 * U2 + U3 * U3) * rho0;
+  } else if (P.mach < 1.0) {
+    q[4] = R(inf[5]) / gm1 + R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0];
+  }
+  if (P.sa) q[5] = R(0.1);
+}
+
+// gather of the convective residual of node i (interior fbuf + boundary)
+template <typename R, bool SA>
+__device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, R *r) {
+  for (int c = 0; c < 6; ++c) r[c] = 0;
+  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+    const R *f = L.fbuf + 6 * (size_t)e;
+    if (role) {
+#pragma unroll
+      for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] -= f[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] += f[c];
+    }
+    (void)j;
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k];
+    const R *b = L.bbuf + 10 * (size_t)(v >> 1) + 5 * (v & 1);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += b[c];
+  }
+}
+
+// Stage kernel modes
+//   MODE_RHS      rhs = makeRHS (+ makeSARHS/Source if SA)                 (parity)
+//   MODE_STAGE    singleIteration RK stage body (DEF:1074-1180)
+//   MODE_COARSE   restrictGrid coarse part (DEF:12185-12234)
+//   MODE_FINE     restrictGrid fine part (DEF:12236-12361), rhs := increment
+enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_COARSE = 2, MODE_FINE = 3 };
+
+struct StageArgs {
+  double dtj, dtj_t, alpha;  // CFL*alpha_k, turbCFL*alpha_k, alpha_k
+  int calc_turb;             // SA terms active in this stage
+  int fas;                   // subtract alpha*sourceTerm (coarse levels)
+  int has_src;               // level has a sourceTerm (gridNumber > 1)
+};
+
+template <typename R, int MODE, bool SA>
+__global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
+  int i = gtid();
+  if (i >= L.n) return;
+  int n = L.n;
+  R r[6];
+  gather_rhs<R, SA>(L, i, r);
+  bool sa_on = SA && S.calc_turb;
+  if (sa_on) r[5] += R(sa_source(L, P, i));
+  if (MODE == MODE_RHS) {
+    for (int c = 0; c < 6; ++c) L.rhs[(size_t)c * n + i] = r[c];
+    return;
+  }
+  for (int c = 0; c < 5; ++c) r[c] += L.diss[(size_t)c * n + i];
+  if (SA) r[5] += L.tdiff[i];
+  if (!SA) r[5] = 0;
+  // solveSystem (DEF:1407-1479)
+  R V = L.vol[i], dt = L.dt[i];
+  for (int c = 0; c < 5; ++c) r[c] = dt * r[c] / V;
+  if (SA) r[5] = L.dtt[i] * r[5] / V;
+  bc_increment(L, P, i, r);
+  // updateSolutionVector (DEF:1292-1405)
+  R up[6], q[6];
+  for (int c = 0; c < 6; ++c) up[c] = L.uprev[(size_t)c * n + i];
+  R dtj = R(S.dtj);
+  for (int c = 0; c < 5; ++c) r[c] = dtj * r[c];
+  if (SA) r[5] = R(S.dtj_t) * r[5];
+  R inc = r[0];
+  if (fabs(inc / up[0]) < R(P.max_inc)) {
+    q[0] = up[0] - inc;
+  } else {
+    R sg = inc / fabs(inc);
+    q[0] = up[0] - sg * R(P.max_inc) * up[0];
+  }
+  for (int c = 1; c < 5; ++c) q[c] = up[c] - r[c];
+  if (SA) {
+    R t = up[5] - r[5];
+    q[5] = t > R(0) ? t : R(0);
+  } else {
+    q[5] = L.u[5 * (size_t)n + i];
+  }
+  // density check on the grid's own u (DEF:1398-1403)
+  R ug0 = L.u[i];
+  if (MODE == MODE_FINE) {
+    if (ug0 < R(P.min_rho)) L.u[i] = R(P.min_rho);
+  } else if (q[0] < R(P.min_rho)) {
+    q[0] = R(P.min_rho);
+  }
+  bc_solution(L, P, i, q);
+  R pi = L.p[i];
+  if (MODE == MODE_STAGE) {
+    if (S.fas) {
+      R a = R(S.alpha);
+      for (int c = 0; c < 5; ++c) q[c] -= a * L.src[(size_t)c * n + i];
+      if (sa_on) q[5] -= a * L.src[5 * (size_t)n + i];
+    }
+    q[0] = q[0] > R(P.min_rho) ? q[0] : R(P.min_rho);
+    if (sa_on) {
+      q[5] = q[5] > R(0) ? q[5] : R(0);
+      if (q[5] > R(P.max_turb)) q[5] = R(P.max_turb);
+    }
+    outer_bc(L, P, i, q, pi);
+    for (int c = 0; c < 6; ++c) L.u[(size_t)c * n + i] = q[c];
+  } else if (MODE == MODE_COARSE) {
+    outer_bc(L, P, i, q, pi);
+    for (int c = 0; c < 6; ++c) {
+      L.src[(size_t)c * n + i] = q[c] - up[c];
+      L.u[(size_t)c * n + i] = up[c];
+    }
+  } else {  // MODE_FINE
+    if (S.has_src) {
+      for (int c = 0; c < 5; ++c) q[c] -= L.src[(size_t)c * n + i];
+      if (SA) q[5] -= L.src[5 * (size_t)n + i];
+    }
+    if (SA) {
+      q[5] = q[5] > R(0) ? q[5] : R(0);
+      R ug5 = L.u[5 * (size_t)n + i];
+      if (ug5 > R(P.max_turb)) L.u[5 * (size_t)n + i] = R(P.max_turb);
+    }
+    outer_bc(L, P, i, q, pi);
+    for (int c = 0; c < 5; ++c) L.rhs[(size_t)c * n + i] = q[c] - up[c];
+    L.rhs[5 * (size_t)n + i] = SA ? q[5] - up[5] : R(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// simple node kernels
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_copy_state(R *dst, const R *src, int n, int ncomp) {
+  int i = gtid();
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; ++c) dst[(size_t)c * n + i] = src[(size_t)c * n + i];
+}
+
+template <typename R>
+__global__ void k_fill(R *dst, R v, size_t cnt) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) dst[i] = v;
+}
+
+// node BC application on a state array (setBCsOnSolutionField + outer)
+template <typename R>
+__global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
+  int i = gtid();
+  if (i >= L.n) return;
+  int n = L.n;
+  R q[6];
+  for (int c = 0; c < 6; ++c) q[c] = a[(size_t)c * n + i];
+  bc_solution(L, P, i, q);
+  if (outer) outer_bc(L, P, i, q, L.p[i]);
+  for (int c = 0; c < 6; ++c) a[(size_t)c * n + i] = q[c];
+}
+
+// restriction (mapFineToCoarse, DEF:12572-12690) as a gather over the fine
+// members of each coarse node: out[c*nc+k] = sum_i rw_i in[c*nf+i]
+template <typename RF, typename RC>
+__global__ void k_restrict(const int *cptr, const int *clist, const double *rw, const RF *in, RC *out, int nc,
+                           int nf, int ncomp) {
+  int k = gtid();
+  if (k >= nc) return;
+  for (int c = 0; c < ncomp; ++c) {
+    double s = 0;
+    for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
+      int i = clist[q];
+      s += rw[i] * double(in[(size_t)c * nf + i]);
+    }
+    out[(size_t)c * nc + k] = RC(s);
+  }
+}
+
+// prolongateGrid (DEF:12432-12570) after the coarse correction rhs_H is
+// formed: inject, relax, add, BCs, clamps on the fine level
+template <typename RF, typename RC>
+__global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, const RC *crhs, int nc, int ncomp_add) {
+  int i = gtid();
+  if (i >= F.n) return;
+  int n = F.n;
+  int k = f2c[i];
+  RF d[6];
+  for (int c = 0; c < 6; ++c) d[c] = RF(crhs[(size_t)c * nc + k]);
+  for (int c = 0; c < 5; ++c) d[c] = RF(P.prolong_relax) * d[c];
+  d[5] = RF(P.turb_prolong_relax) * d[5];
+  if (F.ncls[i] & NC_BFIRST)
+    for (int c = 0; c < 5; ++c) d[c] = RF(P.mg_bc_relax) * d[c];
+  for (int c = 0; c < 6; ++c) F.rhs[(size_t)c * n + i] = d[c];
+  RF q[6];
+  for (int c = 0; c < 6; ++c) q[c] = F.u[(size_t)c * n + i];
+  for (int c = 0; c < ncomp_add; ++c) q[c] = q[c] + d[c];
+  bc_solution(F, P, i, q);
+  outer_bc(F, P, i, q, F.p[i]);
+  if (P.sa) q[5] = q[5] > RF(0) ? q[5] : RF(0);
+  q[0] = q[0] > RF(P.min_rho) ? q[0] : RF(P.min_rho);
+  for (int c = 0; c < 6; ++c) F.u[(size_t)c * n + i] = q[c];
+}
+
+// coarse correction v_H = u_H - BC(R u_h), BC'd as an increment
+template <typename R>
+__global__ void k_coarse_correction(LevelDev<R> K, DevParams P) {
+  int i = gtid();
+  if (i >= K.n) return;
+  int n = K.n;
+  R r[6];
+  for (int c = 0; c < 6; ++c) r[c] = K.u[(size_t)c * n + i] - K.uprev[(size_t)c * n + i];
+  if (!P.sa) r[5] = 0;
+  bc_increment(K, P, i, r);
+  for (int c = 0; c < 6; ++c) K.rhs[(size_t)c * n + i] = r[c];
+}
+
+// end of restrictGrid: rhs_H BC'd as increment, sourceTerm -= rhs_H,
+// u_H += rhs_H (DEF:12362-12428)
+template <typename R>
+__global__ void k_restrict_finish(LevelDev<R> K, DevParams P) {
+  int i = gtid();
+  if (i >= K.n) return;
+  int n = K.n;
+  R r[6];
+  for (int c = 0; c < 6; ++c) r[c] = K.rhs[(size_t)c * n + i];
+  bc_increment(K, P, i, r);
+  int nc = P.sa ? 6 : 5;
+  for (int c = 0; c < 6; ++c) K.rhs[(size_t)c * n + i] = r[c];
+  for (int c = 0; c < nc; ++c) {
+    K.src[(size_t)c * n + i] -= r[c];
+    K.u[(size_t)c * n + i] += r[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// monitor reductions (writeResiduals MG:1302-1445, calculateLiftAndDrag
+// DEF:14710-14806): per-block partials, reduced in a fixed order
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_monitor_partial(LevelDev<R> L, int nres, const int *host_index, double *part_sum, double *part_max,
+                                  int *part_idx) {
+  __shared__ double ssum[NTHR];
+  __shared__ double smax[NTHR];
+  __shared__ int sidx[NTHR];
+  int t = threadIdx.x;
+  int i = gtid();
+  int n = L.n;
+  for (int j = 0; j < nres; ++j) {
+    double d = 0;
+    int hi = 0x7fffffff;
+    if (i < n) {
+      d = double(L.u[(size_t)j * n + i]) - double(L.uprev[(size_t)j * n + i]);
+      hi = host_index[i];
+    }
+    ssum[t] = d * d;
+    smax[t] = fabs(d);
+    sidx[t] = i < n ? hi : 0x7fffffff;
+    __syncthreads();
+    for (int s = NTHR / 2; s > 0; s >>= 1) {
+      if (t < s) {
+        ssum[t] += ssum[t + s];
+        double a = smax[t], b = smax[t + s];
+        if (b > a || (b == a && sidx[t + s] < sidx[t])) {
+          smax[t] = b;
+          sidx[t] = sidx[t + s];
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      part_sum[(size_t)j * gridDim.x + blockIdx.x] = ssum[0];
+      part_max[(size_t)j * gridDim.x + blockIdx.x] = smax[0];
+      part_idx[(size_t)j * gridDim.x + blockIdx.x] = sidx[0];
+    }
+    __syncthreads();
+  }
+}
+
+// force integrand per wall node, reduced on the host in node order
+template <typename R>
+__global__ void k_force_terms(LevelDev<R> L, DevParams P, const int *wall_nodes, int nw, const R *warea, double *out,
+                              double f0, double f1, double f2, double n10, double n11, double n12, double n20,
+                              double n21, double n22, double groundplane, double d0, double d1, double d2) {
+  int k = gtid();
+  if (k >= nw) return;
+  int ip = wall_nodes[k];
+  double *o = out + 8 * (size_t)k;
+  double z = L.xyz ? double(L.xyz[3 * (size_t)ip + 2]) : 0.0;
+  for (int c = 0; c < 8; ++c) o[c] = 0;
+  if (!(z > groundplane + 0.01)) return;
+  double x = L.xyz ? double(L.xyz[3 * (size_t)ip]) : 0.0;
+  double y = L.xyz ? double(L.xyz[3 * (size_t)ip + 1]) : 0.0;
+  double xc = x - d0, yc = y - d1, zc = z - d2;
+  double nx = L.nrm[3 * (size_t)ip], ny = L.nrm[3 * (size_t)ip + 1], nz = L.nrm[3 * (size_t)ip + 2];
+  double a = warea[ip];
+  double pinf = 1.0 / (P.gamma * P.mach * P.mach);
+  double dp = double(L.p[ip]) - pinf;
+  double n1 = nx * n10 + ny * n11 + nz * n12, n2 = nx * n20 + ny * n21 + nz * n22, nt = nx * f0 + ny * f1 + nz * f2;
+  double lift = -a * dp * n1, latF = -a * dp * n2, drag = -a * dp * nt;
+  double latM = xc * a * dp * n2 - yc * a * dp * nt;
+  double longM = xc * a * dp * n1 - zc * a * dp * nt;
+  double fd = 0;
+  if (P.Re > 0.0) {
+    double w1 = L.ws[3 * (size_t)ip], w2 = L.ws[3 * (size_t)ip + 1], w3 = L.ws[3 * (size_t)ip + 2];
+    double s1 = w1 * n10 + w2 * n11 + w3 * n12, s2 = w1 * n20 + w2 * n21 + w3 * n22, st = w1 * f0 + w2 * f1 + w3 * f2;
+    lift += a * s1;
+    latF += a * s2;
+    drag += a * st;
+    fd = a * st;
+    latM += -xc * a * s2 + yc * a * st;
+    longM += -xc * a * s1 + zc * a * st;
+  }
+  o[0] = lift;
+  o[1] = drag;
+  o[2] = latF;
+  o[3] = fd;
+  o[4] = latM;
+  o[5] = longM;
+  o[6] = -double(L.p[ip]) * (a - pinf) * n1;
+  o[7] = -double(L.p[ip]) * (a - pinf) * nt;
+}
+
+}  // namespace fk

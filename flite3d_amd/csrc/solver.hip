@@ -1,0 +1,1137 @@
+// C-ABI and host orchestration of the MI355X flow-solver hot path.
+//
+// The entry points are declared in include/flite3d_amd.h; each names the
+// reference routine it replaces.  Kernels live in kernels.h.  One HIP stream
+// per context; all calls return after the stream has drained.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace fk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPTRY(x)                                                                  \
+  do {                                                                             \
+    hipError_t _e = (x);                                                           \
+    if (_e != hipSuccess)                                                          \
+      return fail(FLITE_E_HIP, std::string(#x) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+#define TRY(x)              \
+  do {                      \
+    int _rc = (x);          \
+    if (_rc != 0) return _rc; \
+  } while (0)
+
+inline int nblk(long long n, int t = NTHR) { return int((n + t - 1) / t); }
+
+// device allocation list owned by a level
+struct Pool {
+  std::vector<void *> ptrs;
+  ~Pool() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(T **out, size_t count) {
+    void *p = nullptr;
+    size_t bytes = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    ptrs.push_back(p);
+    *out = static_cast<T *>(p);
+    return 0;
+  }
+  template <typename T>
+  int upload(T **out, const std::vector<T> &v) {
+    TRY(alloc(out, v.size()));
+    if (!v.empty()) {
+      hipError_t e = hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    return 0;
+  }
+};
+
+// Reverse Cuthill-McKee ordering of the side graph; returns host -> device
+std::vector<int> rcm_order(int n, const std::vector<int> &e1, const std::vector<int> &e2) {
+  std::vector<int> deg(n, 0);
+  for (size_t e = 0; e < e1.size(); ++e) { deg[e1[e]]++; deg[e2[e]]++; }
+  std::vector<int> ptr(n + 1, 0);
+  for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + deg[i];
+  std::vector<int> adj(ptr[n]);
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (size_t e = 0; e < e1.size(); ++e) { adj[pos[e1[e]]++] = e2[e]; adj[pos[e2[e]]++] = e1[e]; }
+  for (int i = 0; i < n; ++i)
+    std::sort(adj.begin() + ptr[i], adj.begin() + ptr[i + 1],
+              [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  auto bfs = [&](int s, std::vector<int> *out, int *last) {
+    std::vector<int> q{s};
+    std::vector<char> vis(n, 0);
+    vis[s] = 1;
+    for (size_t h = 0; h < q.size(); ++h) {
+      int v = q[h];
+      for (int k = ptr[v]; k < ptr[v + 1]; ++k)
+        if (!vis[adj[k]]) { vis[adj[k]] = 1; q.push_back(adj[k]); }
+    }
+    if (last) *last = q.back();
+    if (out) *out = q;
+  };
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (seen[s0]) continue;
+    // pseudo-peripheral start: two BFS sweeps from the component's first node
+    int far = s0;
+    bfs(s0, nullptr, &far);
+    int start = far;
+    std::vector<int> q{start};
+    seen[start] = 1;
+    for (size_t h = 0; h < q.size(); ++h) {
+      int v = q[h];
+      for (int k = ptr[v]; k < ptr[v + 1]; ++k)
+        if (!seen[adj[k]]) { seen[adj[k]] = 1; q.push_back(adj[k]); }
+    }
+    order.insert(order.end(), q.begin(), q.end());
+  }
+  std::reverse(order.begin(), order.end());
+  std::vector<int> h2d(n);
+  for (int d = 0; d < n; ++d) h2d[order[d]] = d;
+  return h2d;
+}
+
+}  // namespace
+
+struct Level {
+  bool fp32 = false;
+  int n = 0, ne = 0, nbs = 0, nb = 0, gnum = 0, nfine = 0;
+  bool has_coords = false;
+  bool uploaded = false;
+  std::vector<int> h2d, d2h;
+  std::vector<int> wall_dev;  // wall nodes (indicator 1 range) in register order
+  std::vector<double> warea_host;
+  Pool pool;
+  // device arrays (element type per precision, stored untyped)
+  void *w, *wb, *vol, *xyz, *wd, *nrm, *warea;
+  void *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
+  void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
+  int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
+  int2 *ell;
+  // transfer from the next finer level (coarse levels)
+  int *f2c = nullptr, *cptr = nullptr, *clist = nullptr;
+  double *rw = nullptr;
+  // monitor scratch
+  double *m_sum = nullptr, *m_max = nullptr, *m_force = nullptr;
+  int *m_idx = nullptr;
+
+  template <typename R>
+  LevelDev<R> view() const {
+    LevelDev<R> v;
+    v.n = n; v.ne = ne; v.nbs = nbs; v.gnum = gnum;
+    v.ea = ea; v.eb = eb; v.w = (const R *)w;
+    v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
+    v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell = ell;
+    v.bs_ptr = bs_ptr; v.bs_list = bs_list;
+    v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
+    v.wd = (const R *)wd; v.nrm = (const R *)nrm; v.ncls = ncls;
+    v.u = (R *)u; v.uprev = (R *)uprev; v.rhs = (R *)rhs; v.diss = (R *)diss; v.src = (R *)src;
+    v.p = (R *)p; v.dt = (R *)dt; v.dtt = (R *)dtt; v.mu = (R *)mu; v.vort = (R *)vort;
+    v.divg = (R *)divg; v.tdiff = (R *)tdiff; v.rec0 = (R *)rec0; v.grec = (R *)grec;
+    v.tau = (R *)tau; v.dnu = (R *)dnu; v.fbuf = (R *)fbuf; v.bbuf = (R *)bbuf; v.ws = (R *)ws;
+    return v;
+  }
+};
+
+struct flite_ctx {
+  flite_params hp;
+  DevParams P;
+  int nl = 0, nbatch = 1;
+  std::vector<std::unique_ptr<Level>> lev;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+template <typename F>
+int with_level(const Level &L, F f) {
+  if (L.fp32) return f(L.view<float>());
+  return f(L.view<double>());
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int sync(flite_ctx *c) {
+  HIPTRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+DevParams make_dev_params(const flite_params &p) {
+  DevParams P;
+  P.gamma = p.gamma;
+  P.mach = p.mach_number;
+  P.Re = p.reynolds_number;
+  P.Pr = p.prandtl_number;
+  P.Prt = p.turbulent_prandtl_number;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 6; ++j) P.inflow[k][j] = p.inflow_field[k][j];
+  P.inflow_temperature = p.inflow_temperature;
+  P.isentropic = p.walls_are_isentropic;
+  P.sa = p.turbulence_model == 1;
+  P.order = p.hllc_flux_order;
+  P.autoclip = p.automatic_clipping;
+  P.nsig = p.reynolds_number < 1 ? 4 : 2;
+  P.bterm = p.boundary_term;
+  P.clip = p.hllc_clipping;
+  P.max_inc = p.max_increment_factor;
+  P.min_rho = p.minimum_density;
+  P.max_turb = p.max_turbulence_value;
+  P.turb_k = p.turb_k;
+  P.prolong_relax = p.prolongation_relaxation;
+  P.turb_prolong_relax = p.turb_prolongation_relaxation;
+  P.mg_bc_relax = p.multigrid_bc_relaxation;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// level upload
+// ---------------------------------------------------------------------------
+template <typename R>
+int upload_real(Pool &pool, void **out, const std::vector<double> &v) {
+  std::vector<R> t(v.begin(), v.end());
+  R *d;
+  TRY(pool.upload(&d, t));
+  *out = d;
+  return 0;
+}
+
+template <typename R>
+int alloc_real(Pool &pool, void **out, size_t count) {
+  R *d;
+  TRY(pool.alloc(&d, count));
+  *out = d;
+  return 0;
+}
+
+int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
+  if (!v) return fail(FLITE_E_ARG, "null level view");
+  auto Lp = std::make_unique<Level>();
+  Level &L = *Lp;
+  const int b = v->index_base;
+  if (b != 0 && b != 1) return fail(FLITE_E_ARG, "index_base must be 0 or 1");
+  L.gnum = level;
+  L.fp32 = c->hp.coarse_fp32 && level > 1;
+  L.n = v->n_nodes;
+  L.ne = v->n_sides;
+  L.nbs = v->n_bsides;
+  L.nb = v->n_bnodes;
+  const int n = L.n, ne = L.ne, nbs = L.nbs;
+  if (n <= 0 || ne < 0 || nbs < 0 || L.nb < 0 || L.nb > n) return fail(FLITE_E_ARG, "bad level sizes");
+  if (!v->side_index || !v->side_weights || !v->bside_index || !v->bside_weights || !v->node_volume ||
+      !v->wall_distance || !v->node_indicator_register || !v->node_indicator_array)
+    return fail(FLITE_E_ARG, "missing level array");
+  if (level > 1) {
+    if (!v->prolongation || !v->restriction) return fail(FLITE_E_ARG, "coarse level needs prolongation/restriction");
+    Level &F = *c->lev[level - 2];
+    if (!F.uploaded) return fail(FLITE_E_STATE, "upload the finer level first");
+    if (v->n_fine != F.n) return fail(FLITE_E_ARG, "n_fine does not match the finer level");
+    L.nfine = v->n_fine;
+  }
+  // host copies, 0-based, range-checked
+  std::vector<int> e1(ne), e2(ne);
+  for (int e = 0; e < ne; ++e) {
+    e1[e] = v->side_index[e] - b;
+    e2[e] = v->side_index[(size_t)ne + e] - b;
+    if (e1[e] < 0 || e1[e] >= n || e2[e] < 0 || e2[e] >= n) return fail(FLITE_E_ARG, "side index out of range");
+  }
+  std::vector<int> b1(nbs), b2(nbs), bi(nbs);
+  for (int s = 0; s < nbs; ++s) {
+    b1[s] = v->bside_index[s] - b;
+    b2[s] = v->bside_index[(size_t)nbs + s] - b;
+    bi[s] = v->bside_index[2 * (size_t)nbs + s];
+    if (b1[s] < 0 || b1[s] >= n || b2[s] < 0 || b2[s] >= n) return fail(FLITE_E_ARG, "boundary side index out of range");
+    if (!(bi[s] <= 4 || bi[s] == 9)) return fail(FLITE_E_UNSUPPORTED, "engine boundary sides are not supported");
+  }
+  for (int i = 0; i < n; ++i)
+    if (!(v->node_volume[i] > 0)) return fail(FLITE_E_ARG, "non-positive node volume");
+  // device numbering
+  L.h2d = rcm_order(n, e1, e2);
+  L.d2h.assign(n, 0);
+  for (int h = 0; h < n; ++h) L.d2h[L.h2d[h]] = h;
+  // node classes from nodeIndicatorArray ranges
+  std::vector<int> ncls(n, 0);
+  const int *reg = v->node_indicator_register;
+  for (int k = 1; k <= 10; ++k) {
+    int st = reg[2 * k - 2], en = reg[2 * k - 1];
+    if (en < st) continue;
+    if (st < 1 || en > L.nb) return fail(FLITE_E_ARG, "node indicator register out of range");
+    if (k >= 5 && k <= 8) return fail(FLITE_E_UNSUPPORTED, "engine boundary nodes are not supported");
+    if (k == 10) return fail(FLITE_E_UNSUPPORTED, "indicator 10 nodes are not supported");
+    for (int ib = st; ib <= en; ++ib) {
+      int h = v->node_indicator_array[ib - 1] - b;
+      if (h < 0 || h >= n) return fail(FLITE_E_ARG, "node indicator array out of range");
+      ncls[L.h2d[h]] = k;
+      if (k == 1) L.wall_dev.push_back(L.h2d[h]);
+    }
+  }
+  for (int h = 0; h < L.nb; ++h) ncls[L.h2d[h]] |= NC_BFIRST;
+  // normals (makeBoundaryNormals, DEF:15394-15460) and wall area
+  // (makeSurfaceArea, DEF:15462-15540) in host numbering
+  std::vector<double> nrm_h(3 * (size_t)n, 0.0), warea_h(n, 0.0);
+  for (int s = 0; s < nbs; ++s)
+    for (int k = 0; k < 3; ++k) {
+      double w = v->bside_weights[(size_t)k * nbs + s];
+      nrm_h[3 * (size_t)b1[s] + k] -= w;
+      nrm_h[3 * (size_t)b2[s] + k] -= w;
+    }
+  for (int i = 0; i < L.nb; ++i) {
+    double x = nrm_h[3 * i], y = nrm_h[3 * i + 1], z = nrm_h[3 * i + 2];
+    double nn = std::sqrt(x * x + y * y + z * z);
+    for (int k = 0; k < 3; ++k) nrm_h[3 * i + k] = nn > 1e-10 ? nrm_h[3 * i + k] / nn : 0.0;
+  }
+  if (level == 1)
+    for (int s = 0; s < nbs; ++s) {
+      double wx = v->bside_weights[s], wy = v->bside_weights[nbs + s], wz = v->bside_weights[2 * (size_t)nbs + s];
+      double a = std::sqrt(wx * wx + wy * wy + wz * wz);
+      warea_h[b1[s]] += 4.0 * a;
+      warea_h[b2[s]] += 4.0 * a;
+    }
+  // device edges sorted by their lower device end
+  std::vector<int> eorder(ne);
+  std::iota(eorder.begin(), eorder.end(), 0);
+  std::vector<int> da(ne), db(ne);
+  for (int e = 0; e < ne; ++e) { da[e] = L.h2d[e1[e]]; db[e] = L.h2d[e2[e]]; }
+  std::sort(eorder.begin(), eorder.end(), [&](int x, int y) {
+    int lx = std::min(da[x], db[x]), ly = std::min(da[y], db[y]);
+    if (lx != ly) return lx < ly;
+    return std::max(da[x], db[x]) < std::max(da[y], db[y]);
+  });
+  std::vector<int> ea(ne), eb(ne);
+  std::vector<double> w(3 * (size_t)ne);
+  for (int k = 0; k < ne; ++k) {
+    int e = eorder[k];
+    ea[k] = da[e];
+    eb[k] = db[e];
+    for (int q = 0; q < 3; ++q) w[3 * (size_t)k + q] = v->side_weights[(size_t)q * ne + e];
+  }
+  // sliced ELL incidence
+  std::vector<int> deg(n, 0);
+  for (int k = 0; k < ne; ++k) { deg[ea[k]]++; deg[eb[k]]++; }
+  int nslice = (n + SLICE - 1) / SLICE;
+  std::vector<int> ell_ptr(nslice + 1, 0), ell_k(nslice, 0);
+  for (int s = 0; s < nslice; ++s) {
+    int K = 0;
+    for (int i = s * SLICE; i < std::min(n, (s + 1) * SLICE); ++i) K = std::max(K, deg[i]);
+    ell_k[s] = K;
+    ell_ptr[s + 1] = ell_ptr[s] + K * SLICE;
+  }
+  std::vector<int2> ell(ell_ptr[nslice] ? ell_ptr[nslice] : 1, make_int2(-1, -1));
+  std::vector<int> fill(n, 0);
+  for (int k = 0; k < ne; ++k) {
+    for (int r = 0; r < 2; ++r) {
+      int i = r ? eb[k] : ea[k], j = r ? ea[k] : eb[k];
+      int s = i / SLICE, lane = i % SLICE;
+      ell[ell_ptr[s] + fill[i] * SLICE + lane] = make_int2((k << 1) | r, j);
+      fill[i]++;
+    }
+  }
+  // boundary sides in device numbering, per-node CSR
+  std::vector<int> ba(nbs), bb(nbs);
+  std::vector<double> wb(3 * (size_t)nbs);
+  for (int s = 0; s < nbs; ++s) {
+    ba[s] = L.h2d[b1[s]];
+    bb[s] = L.h2d[b2[s]];
+    for (int q = 0; q < 3; ++q) wb[3 * (size_t)s + q] = v->bside_weights[(size_t)q * nbs + s];
+  }
+  std::vector<int> bs_ptr(n + 1, 0);
+  for (int s = 0; s < nbs; ++s) { bs_ptr[ba[s] + 1]++; bs_ptr[bb[s] + 1]++; }
+  for (int i = 0; i < n; ++i) bs_ptr[i + 1] += bs_ptr[i];
+  std::vector<int> bs_list(bs_ptr[n] ? bs_ptr[n] : 1);
+  {
+    std::vector<int> pos(bs_ptr.begin(), bs_ptr.end() - 1);
+    for (int s = 0; s < nbs; ++s) {
+      bs_list[pos[ba[s]]++] = (s << 1) | 0;
+      bs_list[pos[bb[s]]++] = (s << 1) | 1;
+    }
+  }
+  // node arrays in device numbering
+  std::vector<double> vol(n), wd(n), nrm(3 * (size_t)n), warea(n), xyz;
+  for (int h = 0; h < n; ++h) {
+    int d = L.h2d[h];
+    vol[d] = v->node_volume[h];
+    wd[d] = v->wall_distance[h];
+    warea[d] = warea_h[h];
+    for (int k = 0; k < 3; ++k) nrm[3 * (size_t)d + k] = nrm_h[3 * (size_t)h + k];
+  }
+  L.warea_host = warea;
+  L.has_coords = level == 1 && v->coordinates != nullptr;
+  if (L.has_coords) {
+    xyz.resize(3 * (size_t)n);
+    for (int h = 0; h < n; ++h)
+      for (int k = 0; k < 3; ++k) xyz[3 * (size_t)L.h2d[h] + k] = v->coordinates[(size_t)k * n + h];
+  }
+  Pool &P = L.pool;
+  TRY(P.upload(&L.ea, ea));
+  TRY(P.upload(&L.eb, eb));
+  TRY(P.upload(&L.ba, ba));
+  TRY(P.upload(&L.bb, bb));
+  TRY(P.upload(&L.bind, bi));
+  TRY(P.upload(&L.ell_ptr, ell_ptr));
+  TRY(P.upload(&L.ell_k, ell_k));
+  TRY(P.upload(&L.ell, ell));
+  TRY(P.upload(&L.bs_ptr, bs_ptr));
+  TRY(P.upload(&L.bs_list, bs_list));
+  TRY(P.upload(&L.ncls, ncls));
+  TRY(P.upload(&L.host_index, L.d2h));
+  TRY(P.upload(&L.wall_nodes, L.wall_dev));
+  auto up_real = [&](void **out, const std::vector<double> &vv) {
+    return L.fp32 ? upload_real<float>(P, out, vv) : upload_real<double>(P, out, vv);
+  };
+  auto al_real = [&](void **out, size_t cnt) {
+    return L.fp32 ? alloc_real<float>(P, out, cnt) : alloc_real<double>(P, out, cnt);
+  };
+  TRY(up_real(&L.w, w));
+  TRY(up_real(&L.wb, wb));
+  TRY(up_real(&L.vol, vol));
+  TRY(up_real(&L.wd, wd));
+  TRY(up_real(&L.nrm, nrm));
+  TRY(up_real(&L.warea, warea));
+  if (L.has_coords) TRY(up_real(&L.xyz, xyz)); else L.xyz = nullptr;
+  TRY(al_real(&L.u, 6 * (size_t)n));
+  TRY(al_real(&L.uprev, 6 * (size_t)n));
+  TRY(al_real(&L.rhs, 6 * (size_t)n));
+  TRY(al_real(&L.diss, 5 * (size_t)n));
+  TRY(al_real(&L.src, 6 * (size_t)n));
+  TRY(al_real(&L.p, n));
+  TRY(al_real(&L.dt, n));
+  TRY(al_real(&L.dtt, n));
+  TRY(al_real(&L.mu, n));
+  TRY(al_real(&L.vort, n));
+  TRY(al_real(&L.divg, n));
+  TRY(al_real(&L.tdiff, n));
+  TRY(al_real(&L.rec0, 8 * (size_t)n));
+  bool need_grad = L.has_coords && c->hp.hllc_flux_order > 1;
+  TRY(al_real(&L.grec, need_grad ? GREC * (size_t)n : 1));
+  TRY(al_real(&L.tau, 9 * (size_t)n));
+  TRY(al_real(&L.dnu, 4 * (size_t)n));
+  TRY(al_real(&L.fbuf, 6 * (size_t)ne));
+  TRY(al_real(&L.bbuf, 10 * (size_t)nbs));
+  TRY(al_real(&L.ws, 3 * (size_t)n));
+  int nbm = nblk(n);
+  TRY(P.alloc(&L.m_sum, 7 * (size_t)nbm));
+  TRY(P.alloc(&L.m_max, 7 * (size_t)nbm));
+  TRY(P.alloc(&L.m_idx, 7 * (size_t)nbm));
+  TRY(P.alloc(&L.m_force, 8 * (size_t)std::max<size_t>(1, L.wall_dev.size())));
+  // transfer operators from the finer level
+  if (level > 1) {
+    Level &F = *c->lev[level - 2];
+    std::vector<int> f2c(F.n);
+    std::vector<double> rw(F.n);
+    std::vector<int> cnt(n + 1, 0);
+    for (int hf = 0; hf < F.n; ++hf) {
+      int hc = v->prolongation[hf] - b;
+      if (hc < 0 || hc >= n) return fail(FLITE_E_ARG, "prolongation index out of range");
+      f2c[F.h2d[hf]] = L.h2d[hc];
+      rw[F.h2d[hf]] = v->restriction[hf];
+      cnt[L.h2d[hc] + 1]++;
+    }
+    for (int k = 0; k < n; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int> clist(F.n);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int hf = 0; hf < F.n; ++hf) {  // fine members in host order
+      int dc = L.h2d[v->prolongation[hf] - b];
+      clist[pos[dc]++] = F.h2d[hf];
+    }
+    TRY(P.upload(&L.f2c, f2c));
+    TRY(P.upload(&L.rw, rw));
+    TRY(P.upload(&L.cptr, cnt));
+    TRY(P.upload(&L.clist, clist));
+  }
+  L.uploaded = true;
+  c->lev[level - 1] = std::move(Lp);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// operations
+// ---------------------------------------------------------------------------
+int op_prep(flite_ctx *c, Level &L, bool compute_p) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    if (compute_p) {
+      hipLaunchKernelGGL(k_prep<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    } else {
+      hipLaunchKernelGGL(k_pack<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    }
+    return check_launch();
+  });
+}
+
+int op_grad(flite_ctx *c, Level &L) {
+  if (!(L.has_coords && c->hp.hllc_flux_order > 1)) return 0;
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_grad_minmax<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_flux(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  bool o2 = L.has_coords && c->hp.hllc_flux_order > 1;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    if (L.ne > 0) {
+      if (o2 && P.sa) hipLaunchKernelGGL((k_flux<R, 2, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
+      else if (o2) hipLaunchKernelGGL((k_flux<R, 2, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
+      else if (P.sa) hipLaunchKernelGGL((k_flux<R, 1, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
+      else hipLaunchKernelGGL((k_flux<R, 1, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
+    }
+    if (L.nbs > 0) hipLaunchKernelGGL(k_bflux<R>, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    dim3 g(nblk(L.n)), t(NTHR);
+#define LAUNCH_STAGE(M)                                                                  \
+  if (P.sa) hipLaunchKernelGGL((k_stage<R, M, true>), g, t, 0, c->stream, V, P, S);      \
+  else hipLaunchKernelGGL((k_stage<R, M, false>), g, t, 0, c->stream, V, P, S);
+    switch (mode) {
+      case MODE_RHS: LAUNCH_STAGE(MODE_RHS); break;
+      case MODE_STAGE: LAUNCH_STAGE(MODE_STAGE); break;
+      case MODE_COARSE: LAUNCH_STAGE(MODE_COARSE); break;
+      default: LAUNCH_STAGE(MODE_FINE); break;
+    }
+#undef LAUNCH_STAGE
+    return check_launch();
+  });
+}
+
+int op_timesteps(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_timesteps<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_zero_diss(flite_ctx *c, Level &L) {
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    size_t cnt = 5 * (size_t)L.n;
+    hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.diss, R(0), cnt);
+    return check_launch();
+  });
+}
+
+int op_dissipation(flite_ctx *c, Level &L) {
+  if (!(c->hp.flux_type == 4 && L.gnum == 1) &&
+      !(c->hp.second_order_dissipation_factor == 0.0 && c->hp.fourth_order_dissipation_factor == 0.0))
+    return fail(FLITE_E_UNSUPPORTED, "JST artificial dissipation is not implemented");
+  return op_zero_diss(c, L);
+}
+
+int op_viscous(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_visc_node<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    hipLaunchKernelGGL(k_visc_flux<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_sa_timesteps(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_sa_timesteps<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_sa_diffusion(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_sa_diff1<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    hipLaunchKernelGGL(k_sa_diff2<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  });
+}
+
+int op_copy(flite_ctx *c, Level &L, void *dst, const void *src, int ncomp) {
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_copy_state<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, (R *)dst, (const R *)src, L.n,
+                       ncomp);
+    return check_launch();
+  });
+}
+
+int op_bc_state(flite_ctx *c, Level &L, void *a, int outer) {
+  DevParams P = c->P;
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_bc_state<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (R *)a, outer);
+    return check_launch();
+  });
+}
+
+// mapFineToCoarse of ncomp columns from array fa of F into array ka of K
+int op_restrict_array(flite_ctx *c, Level &F, Level &K, const void *fa, void *ka, int ncomp) {
+  auto go = [&](auto *fin, auto *kout) {
+    using RF = std::remove_const_t<std::remove_pointer_t<decltype(fin)>>;
+    using RC = std::remove_pointer_t<decltype(kout)>;
+    hipLaunchKernelGGL((k_restrict<RF, RC>), dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, K.cptr, K.clist, K.rw, fin,
+                       kout, K.n, F.n, ncomp);
+    return check_launch();
+  };
+  if (F.fp32 && K.fp32) return go((const float *)fa, (float *)ka);
+  if (!F.fp32 && K.fp32) return go((const double *)fa, (float *)ka);
+  if (F.fp32 && !K.fp32) return go((const float *)fa, (double *)ka);
+  return go((const double *)fa, (double *)ka);
+}
+
+int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
+  const flite_params &hp = c->hp;
+  if (calc_diss) {
+    TRY(op_prep(c, L, true));
+    TRY(op_timesteps(c, L));
+    TRY(op_dissipation(c, L));
+    if (hp.reynolds_number > 0.0) {
+      TRY(op_viscous(c, L));
+      if (c->P.sa) {
+        TRY(op_sa_timesteps(c, L));
+        TRY(op_sa_diffusion(c, L));
+      }
+    }
+  }
+  for (int k = 1; k <= 3; ++k) {
+    TRY(op_prep(c, L, true));
+    TRY(op_grad(c, L));
+    TRY(op_flux(c, L));
+    StageArgs S;
+    S.alpha = hp.rk_coefficients[k - 1];
+    S.dtj = hp.cfl_number * S.alpha;
+    S.dtj_t = hp.turbulent_cfl_number * S.alpha;
+    S.calc_turb = 1;
+    S.fas = L.gnum > 1;
+    S.has_src = L.gnum > 1;
+    TRY(op_stage(c, L, MODE_STAGE, S));
+  }
+  return 0;
+}
+
+int do_time_iterations(flite_ctx *c, int level, int steps, bool calc_diss) {
+  Level &L = *c->lev[level - 1];
+  for (int s = 0; s < steps; ++s) {
+    TRY(op_copy(c, L, L.uprev, L.u, c->P.sa ? 6 : 5));
+    TRY(do_single_iteration(c, L, calc_diss));
+  }
+  return 0;
+}
+
+int do_restrict(flite_ctx *c, int fine_level) {
+  Level &F = *c->lev[fine_level - 1];
+  Level &K = *c->lev[fine_level];
+  const flite_params &hp = c->hp;
+  bool visc = hp.reynolds_number > 0.0;
+  // coarse part (DEF:12158-12234)
+  TRY(op_restrict_array(c, F, K, F.u, K.u, 6));
+  TRY(op_restrict_array(c, F, K, F.mu, K.mu, 1));
+  TRY(op_prep(c, K, true));
+  TRY(op_bc_state(c, K, K.u, 1));
+  TRY(op_copy(c, K, K.uprev, K.u, 6));
+  TRY(op_prep(c, K, true));
+  TRY(op_timesteps(c, K));
+  TRY(op_dissipation(c, K));
+  if (visc) TRY(op_viscous(c, K));
+  TRY(op_flux(c, K));
+  if (c->P.sa) {
+    TRY(op_sa_timesteps(c, K));
+    TRY(op_sa_diffusion(c, K));
+  }
+  StageArgs S;
+  S.alpha = 1.0;
+  S.dtj = hp.cfl_number;
+  S.dtj_t = hp.turbulent_cfl_number;
+  S.calc_turb = 1;
+  S.fas = 0;
+  S.has_src = 1;
+  TRY(op_stage(c, K, MODE_COARSE, S));
+  // fine part (DEF:12236-12361)
+  TRY(op_prep(c, F, true));
+  TRY(op_dissipation(c, F));
+  if (visc) TRY(op_viscous(c, F));
+  TRY(op_grad(c, F));
+  TRY(op_flux(c, F));
+  if (c->P.sa) TRY(op_sa_diffusion(c, F));
+  S.has_src = F.gnum > 1;
+  TRY(op_stage(c, F, MODE_FINE, S));
+  // coarse forcing (DEF:12362-12428)
+  TRY(op_restrict_array(c, F, K, F.rhs, K.rhs, 6));
+  DevParams P = c->P;
+  TRY(with_level(K, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_restrict_finish<R>, dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  }));
+  return 0;
+}
+
+int do_prolongate(flite_ctx *c, int coarse_level) {
+  Level &K = *c->lev[coarse_level - 1];
+  Level &F = *c->lev[coarse_level - 2];
+  DevParams P = c->P;
+  TRY(op_restrict_array(c, F, K, F.u, K.uprev, 6));
+  TRY(op_bc_state(c, K, K.uprev, 1));
+  TRY(with_level(K, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_coarse_correction<R>, dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, V, P);
+    return check_launch();
+  }));
+  int ncomp = (F.gnum < c->hp.number_of_turbulence_grids && P.sa) ? 6 : 5;
+  auto go = [&](auto FV, auto *crhs) {
+    using RF = std::remove_pointer_t<decltype(FV.u)>;
+    using RC = std::remove_pointer_t<decltype(crhs)>;
+    hipLaunchKernelGGL((k_prolong_fine<RF, RC>), dim3(nblk(F.n)), dim3(NTHR), 0, c->stream, FV, P, K.f2c,
+                       (const RC *)crhs, K.n, ncomp);
+    return check_launch();
+  };
+  if (F.fp32) {
+    auto FV = F.view<float>();
+    return K.fp32 ? go(FV, (float *)K.rhs) : go(FV, (double *)K.rhs);
+  }
+  auto FV = F.view<double>();
+  return K.fp32 ? go(FV, (float *)K.rhs) : go(FV, (double *)K.rhs);
+}
+
+int do_cycle(flite_ctx *c, int cycle) {
+  const flite_params &hp = c->hp;
+  int ng = c->nl, nrs = hp.number_of_relaxation_steps;
+  if (cycle > hp.number_of_turbulence_steps && c->P.sa)
+    return fail(FLITE_E_UNSUPPORTED, "frozen turbulence (cycle > numberOfTurbulenceSteps) is not implemented");
+  if (hp.multigrid_scheme <= 0) {
+    if (ng > 1) return fail(FLITE_E_UNSUPPORTED, "multigridScheme <= 0 with more than one grid is not implemented");
+    for (int i = 1; i < ng; ++i) {
+      Level &F = *c->lev[i - 1];
+      TRY(op_copy(c, F, F.uprev, F.u, 6));
+      TRY(op_restrict_array(c, F, *c->lev[i], F.u, c->lev[i]->u, 6));
+    }
+    Level &Kc = *c->lev[ng - 1];
+    TRY(with_level(Kc, [&](auto V) {
+      using R = std::remove_pointer_t<decltype(V.u)>;
+      hipLaunchKernelGGL(k_fill<R>, dim3(nblk(6LL * Kc.n)), dim3(NTHR), 0, c->stream, V.src, R(0), 6 * (size_t)Kc.n);
+      return check_launch();
+    }));
+    TRY(do_time_iterations(c, ng, nrs, true));
+  } else if (hp.multigrid_scheme == 1) {
+    for (int i = 1; i < ng; ++i) {
+      TRY(do_time_iterations(c, i, nrs, i == 1));
+      TRY(do_restrict(c, i));
+      Level &F = *c->lev[i - 1];
+      TRY(op_restrict_array(c, F, *c->lev[i], F.u, c->lev[i]->u, 6));
+    }
+    TRY(do_time_iterations(c, ng, nrs, ng == 1));
+    for (int i = ng; i >= 2; --i) {
+      TRY(do_prolongate(c, i));
+      TRY(do_time_iterations(c, i - 1, nrs, (i - 1) == 1));
+    }
+  } else if (hp.multigrid_scheme == 3) {
+    int sweeps = ng == 1 ? 1 : ng - 1, finest = 1;
+    for (int s = 1; s <= sweeps; ++s) {
+      for (int j = finest; j <= ng - 1; ++j) {
+        TRY(do_time_iterations(c, j, nrs, j == finest));
+        TRY(do_restrict(c, j));
+      }
+      TRY(do_time_iterations(c, ng, nrs, ng == 1));
+      finest = finest == 1 ? (ng >= 2 ? ng - 1 : ng) : finest - 1;
+      for (int j = ng; j >= finest + 1; --j) TRY(do_prolongate(c, j));
+    }
+  } else {
+    return fail(FLITE_E_UNSUPPORTED, "multigrid scheme not implemented");
+  }
+  return 0;
+}
+
+int check_ctx(flite_ctx *c) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  for (int l = 0; l < c->nl; ++l)
+    if (!c->lev[l] || !c->lev[l]->uploaded) return fail(FLITE_E_STATE, "not all levels uploaded");
+  return 0;
+}
+
+int check_level(flite_ctx *c, int level) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
+  if (!c->lev[level - 1] || !c->lev[level - 1]->uploaded) return fail(FLITE_E_STATE, "level not uploaded");
+  return 0;
+}
+
+void *field_ptr(Level &L, int field, int *ncomp, size_t *stride_nodes) {
+  *stride_nodes = (size_t)L.n;
+  switch (field) {
+    case FLITE_FIELD_U: *ncomp = 6; return L.u;
+    case FLITE_FIELD_UPREV: *ncomp = 6; return L.uprev;
+    case FLITE_FIELD_RHS: *ncomp = 6; return L.rhs;
+    case FLITE_FIELD_DISSIPATION: *ncomp = 5; return L.diss;
+    case FLITE_FIELD_PRESSURE: *ncomp = 1; return L.p;
+    case FLITE_FIELD_DT: *ncomp = 1; return L.dt;
+    case FLITE_FIELD_DT_TURB: *ncomp = 1; return L.dtt;
+    case FLITE_FIELD_LAMINAR_VISCOSITY: *ncomp = 1; return L.mu;
+    case FLITE_FIELD_VORTICITY: *ncomp = 1; return L.vort;
+    case FLITE_FIELD_DIVERGENCE: *ncomp = 1; return L.divg;
+    case FLITE_FIELD_SOURCE: *ncomp = 6; return L.src;
+    case FLITE_FIELD_TURB_DIFFUSION: *ncomp = 1; return L.tdiff;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *flite_last_error(void) { return g_err.c_str(); }
+
+int flite_field_ncomp(int field) {
+  static const int nc[FLITE_FIELD_COUNT] = {6, 6, 6, 5, 1, 1, 1, 1, 1, 1, 6, 1, 25, 3};
+  return field >= 0 && field < FLITE_FIELD_COUNT ? nc[field] : -1;
+}
+
+int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx **out) {
+  if (!p || !out) return fail(FLITE_E_ARG, "null argument");
+  *out = nullptr;
+  if (n_levels < 1 || n_levels > 16) return fail(FLITE_E_ARG, "n_levels must be 1..16");
+  if (n_batch != 1) return fail(FLITE_E_UNSUPPORTED, "batched conditions are not implemented yet");
+  if (p->flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "only fluxType=4 (HLLC) is implemented");
+  if (p->turbulence_model != 0 && p->turbulence_model != 1)
+    return fail(FLITE_E_UNSUPPORTED, "only turbulenceModel 0/1 are implemented");
+  if (p->hllc_flux_order > 1 && p->gradient_method != 1)
+    return fail(FLITE_E_UNSUPPORTED, "only Green-Gauss gradients are implemented");
+  if (p->turbulence_model == 1 && !(p->reynolds_number > 0.0))
+    return fail(FLITE_E_ARG, "Spalart-Allmaras needs ReynoldsNumber > 0");
+  int ndev = 0;
+  HIPTRY(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(FLITE_E_HIP, "no HIP device");
+  auto c = std::make_unique<flite_ctx>();
+  c->hp = *p;
+  c->P = make_dev_params(*p);
+  c->nl = n_levels;
+  c->nbatch = n_batch;
+  c->lev.resize(n_levels);
+  HIPTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return 0;
+}
+
+void flite_ctx_destroy(flite_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->lev.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int flite_level_upload(flite_ctx *c, int level, const flite_level_view *v) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
+  TRY(sync(c));
+  return upload_level(c, level, v);
+}
+
+int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  if (!out || ld < L.n) return fail(FLITE_E_ARG, "bad output buffer");
+  TRY(sync(c));
+  const size_t es = L.fp32 ? 4 : 8;
+  auto to_host = [&](const void *dptr, size_t count, std::vector<double> &h) -> int {
+    h.resize(count);
+    if (L.fp32) {
+      std::vector<float> t(count);
+      HIPTRY(hipMemcpy(t.data(), dptr, count * es, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < count; ++k) h[k] = t[k];
+    } else {
+      HIPTRY(hipMemcpy(h.data(), dptr, count * es, hipMemcpyDeviceToHost));
+    }
+    return 0;
+  };
+  std::vector<double> h;
+  if (field == FLITE_FIELD_GRADIENT) {
+    if (!(L.has_coords && c->hp.hllc_flux_order > 1)) return fail(FLITE_E_STATE, "no gradient on this level");
+    TRY(to_host(L.grec, GREC * (size_t)L.n, h));
+    for (int hh = 0; hh < L.n; ++hh) {
+      const double *g = &h[GREC * (size_t)L.h2d[hh]];
+      for (int k = 0; k < 15; ++k) out[(size_t)k * ld + hh] = g[k];
+      for (int k = 0; k < 5; ++k) {
+        out[(size_t)(15 + k) * ld + hh] = g[28 + k];
+        out[(size_t)(20 + k) * ld + hh] = g[33 + k];
+      }
+    }
+    return 0;
+  }
+  if (field == FLITE_FIELD_WALL_STRESS) {
+    TRY(to_host(L.ws, 3 * (size_t)L.n, h));
+    for (int hh = 0; hh < L.n; ++hh)
+      for (int k = 0; k < 3; ++k) out[(size_t)k * ld + hh] = h[3 * (size_t)L.h2d[hh] + k];
+    return 0;
+  }
+  int nc;
+  size_t stride;
+  void *p = field_ptr(L, field, &nc, &stride);
+  if (!p) return fail(FLITE_E_ARG, "bad field");
+  TRY(to_host(p, nc * stride, h));
+  for (int k = 0; k < nc; ++k)
+    for (int hh = 0; hh < L.n; ++hh) out[(size_t)k * ld + hh] = h[(size_t)k * stride + L.h2d[hh]];
+  return 0;
+}
+
+int flite_field_set(flite_ctx *c, int level, int field, const double *in, int ld) {
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  if (!in || ld < L.n) return fail(FLITE_E_ARG, "bad input buffer");
+  int nc;
+  size_t stride;
+  void *p = field_ptr(L, field, &nc, &stride);
+  if (!p) return fail(FLITE_E_ARG, "field cannot be set");
+  TRY(sync(c));
+  std::vector<double> h(nc * stride);
+  for (int k = 0; k < nc; ++k)
+    for (int hh = 0; hh < L.n; ++hh) h[(size_t)k * stride + L.h2d[hh]] = in[(size_t)k * ld + hh];
+  if (L.fp32) {
+    std::vector<float> t(h.begin(), h.end());
+    HIPTRY(hipMemcpy(p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPTRY(hipMemcpy(p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int flite_state_set(flite_ctx *c, int level, const double *u, int ld, int ncomp, int batch) {
+  if (batch != 0) return fail(FLITE_E_UNSUPPORTED, "batch index must be 0");
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  if (ncomp < 5 || ncomp > 6) return fail(FLITE_E_ARG, "ncomp must be 5 or 6");
+  std::vector<double> full(6 * (size_t)L.n, 0.0);
+  for (int k = 0; k < ncomp; ++k)
+    for (int i = 0; i < L.n; ++i) full[(size_t)k * L.n + i] = u[(size_t)k * ld + i];
+  return flite_field_set(c, level, FLITE_FIELD_U, full.data(), L.n);
+}
+
+int flite_state_get(flite_ctx *c, int level, double *u, int ld, int ncomp, int batch) {
+  if (batch != 0) return fail(FLITE_E_UNSUPPORTED, "batch index must be 0");
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  if (ncomp < 1 || ncomp > 6) return fail(FLITE_E_ARG, "ncomp must be 1..6");
+  std::vector<double> full(6 * (size_t)L.n);
+  TRY(flite_field_get(c, level, FLITE_FIELD_U, full.data(), L.n));
+  for (int k = 0; k < ncomp; ++k)
+    for (int i = 0; i < L.n; ++i) u[(size_t)k * ld + i] = full[(size_t)k * L.n + i];
+  return 0;
+}
+
+int flite_init_from_state(flite_ctx *c) {
+  TRY(check_ctx(c));
+  double mu0 = c->hp.reynolds_number > 0.0 ? 1.0 / c->hp.reynolds_number : 0.0;
+  for (int l = 0; l < c->nl; ++l) {
+    Level &L = *c->lev[l];
+    TRY(with_level(L, [&](auto V) {
+      using R = std::remove_pointer_t<decltype(V.u)>;
+      hipLaunchKernelGGL(k_fill<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), (size_t)L.n);
+      return check_launch();
+    }));
+    if (l > 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
+  }
+  return sync(c);
+}
+
+int flite_init_freestream(flite_ctx *c) {
+  TRY(check_ctx(c));
+  Level &L = *c->lev[0];
+  std::vector<double> u(6 * (size_t)L.n, 0.0);
+  for (int k = 0; k < 5; ++k)
+    for (int i = 0; i < L.n; ++i) u[(size_t)k * L.n + i] = c->hp.inflow_field[0][k];
+  if (c->P.sa)
+    for (int i = 0; i < L.n; ++i) u[5 * (size_t)L.n + i] = 0.1;
+  TRY(flite_field_set(c, 1, FLITE_FIELD_U, u.data(), L.n));
+  TRY(op_prep(c, L, true));
+  TRY(op_bc_state(c, L, L.u, 0));
+  return flite_init_from_state(c);
+}
+
+int flite_do_time_iterations(flite_ctx *c, int level, int steps, int calc_diss, int calc_turb, int iteration) {
+  TRY(check_level(c, level));
+  (void)iteration;
+  if (c->P.sa && !calc_turb) return fail(FLITE_E_UNSUPPORTED, "calculateTurbulence=.false. is not implemented");
+  TRY(do_time_iterations(c, level, steps, calc_diss != 0));
+  return sync(c);
+}
+
+int flite_restrict(flite_ctx *c, int fine_level, int calc_turb) {
+  TRY(check_ctx(c));
+  if (fine_level < 1 || fine_level >= c->nl) return fail(FLITE_E_ARG, "fine_level out of range");
+  if (c->P.sa && !calc_turb) return fail(FLITE_E_UNSUPPORTED, "calculateTurbulence=.false. is not implemented");
+  TRY(do_restrict(c, fine_level));
+  return sync(c);
+}
+
+int flite_prolongate(flite_ctx *c, int coarse_level) {
+  TRY(check_ctx(c));
+  if (coarse_level < 2 || coarse_level > c->nl) return fail(FLITE_E_ARG, "coarse_level out of range");
+  TRY(do_prolongate(c, coarse_level));
+  return sync(c);
+}
+
+int flite_map_u_fine_to_coarse(flite_ctx *c, int fine_level) {
+  TRY(check_ctx(c));
+  if (fine_level < 1 || fine_level >= c->nl) return fail(FLITE_E_ARG, "fine_level out of range");
+  Level &F = *c->lev[fine_level - 1];
+  TRY(op_restrict_array(c, F, *c->lev[fine_level], F.u, c->lev[fine_level]->u, 6));
+  return sync(c);
+}
+
+int flite_cycle(flite_ctx *c, int cycle_number) {
+  TRY(check_ctx(c));
+  TRY(do_cycle(c, cycle_number));
+  return sync(c);
+}
+
+int flite_apply(flite_ctx *c, int level, int op) {
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  switch (op) {
+    case FLITE_OP_PRESSURE:
+    case FLITE_OP_PRESSURE_ABS: TRY(op_prep(c, L, true)); break;
+    case FLITE_OP_TIMESTEPS: TRY(op_prep(c, L, false)); TRY(op_timesteps(c, L)); break;
+    case FLITE_OP_DISSIPATION: TRY(op_dissipation(c, L)); break;
+    case FLITE_OP_VISCOUS:
+      if (!(c->hp.reynolds_number > 0.0)) return fail(FLITE_E_ARG, "viscous term needs ReynoldsNumber > 0");
+      TRY(op_prep(c, L, false));
+      TRY(op_viscous(c, L));
+      break;
+    case FLITE_OP_RHS: {
+      TRY(op_prep(c, L, false));
+      TRY(op_grad(c, L));
+      TRY(op_flux(c, L));
+      StageArgs S{};
+      S.calc_turb = 1;
+      TRY(op_stage(c, L, MODE_RHS, S));
+      break;
+    }
+    case FLITE_OP_SA_TIMESTEPS:
+      if (!c->P.sa) return fail(FLITE_E_ARG, "needs turbulenceModel=1");
+      TRY(op_prep(c, L, false));
+      TRY(op_sa_timesteps(c, L));
+      break;
+    case FLITE_OP_SA_DIFFUSION:
+      if (!c->P.sa) return fail(FLITE_E_ARG, "needs turbulenceModel=1");
+      TRY(op_prep(c, L, false));
+      TRY(op_sa_diffusion(c, L));
+      break;
+    default: return fail(FLITE_E_ARG, "bad op");
+  }
+  return sync(c);
+}
+
+int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
+  TRY(check_ctx(c));
+  Level &L = *c->lev[0];
+  int nres = c->P.sa ? 6 : 5;
+  int nb = nblk(L.n);
+  DevParams P = c->P;
+  const flite_params &hp = c->hp;
+  double PI = 4.0 * std::atan(1.0);
+  double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
+  double ft[3] = {std::cos(ar) * std::cos(br), std::sin(br), std::sin(ar)};
+  double fn1[3] = {-ft[2], 0.0, ft[0]};
+  double fn2[3] = {-ft[1], ft[0], 0.0};
+  int nw = (int)L.wall_dev.size();
+  TRY(with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    hipLaunchKernelGGL(k_monitor_partial<R>, dim3(nb), dim3(NTHR), 0, c->stream, V, nres, L.host_index, L.m_sum,
+                       L.m_max, L.m_idx);
+    if (nw > 0)
+      hipLaunchKernelGGL(k_force_terms<R>, dim3(nblk(nw)), dim3(NTHR), 0, c->stream, V, P, L.wall_nodes, nw,
+                         (const R *)L.warea, L.m_force, ft[0], ft[1], ft[2], fn1[0], fn1[1], fn1[2], fn2[0], fn2[1],
+                         fn2[2], hp.groundplane, hp.datum[0], hp.datum[1], hp.datum[2]);
+    return check_launch();
+  }));
+  std::vector<double> s(7 * (size_t)nb), m(7 * (size_t)nb), f(8 * (size_t)std::max(1, nw));
+  std::vector<int> ix(7 * (size_t)nb);
+  HIPTRY(hipMemcpyAsync(s.data(), L.m_sum, sizeof(double) * nres * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipMemcpyAsync(m.data(), L.m_max, sizeof(double) * nres * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipMemcpyAsync(ix.data(), L.m_idx, sizeof(int) * nres * nb, hipMemcpyDeviceToHost, c->stream));
+  if (nw > 0) HIPTRY(hipMemcpyAsync(f.data(), L.m_force, sizeof(double) * 8 * nw, hipMemcpyDeviceToHost, c->stream));
+  TRY(sync(c));
+  for (int j = 0; j < 7; ++j) { res[j] = 0; maxres[j] = 0; maxidx[j] = 0; }
+  for (int j = 0; j < nres; ++j) {
+    double acc = 0, mx = 0;
+    int mi = 0x7fffffff;
+    for (int b = 0; b < nb; ++b) {
+      acc += s[(size_t)j * nb + b];
+      double v = m[(size_t)j * nb + b];
+      int vi = ix[(size_t)j * nb + b];
+      if (v > mx || (v == mx && v > 0 && vi < mi)) { mx = v; mi = vi; }
+    }
+    res[j] = std::sqrt(acc);
+    maxres[j] = mx;
+    maxidx[j] = mx > 0 ? mi : 0;
+  }
+  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < nw; ++k)
+    for (int q = 0; q < 8; ++q) tot[q] += f[8 * (size_t)k + q];
+  double fct = 2.0 / hp.reference_area;
+  forces[0] = fct * tot[0];
+  forces[1] = fct * tot[1];
+  forces[2] = fct * tot[2];
+  forces[3] = fct * tot[3];
+  forces[4] = tot[4];
+  forces[5] = tot[5];
+  forces[6] = fct * tot[6];
+  forces[7] = fct * tot[7];
+  forces[8] = 0.0;
+  for (int j = 0; j < nres; ++j)
+    if (std::isnan(res[j])) return fail(FLITE_E_NAN, "NaN in residual");
+  return 0;
+}
+
+int flite_sync(flite_ctx *c) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  return sync(c);
+}
+
+}  // extern "C"

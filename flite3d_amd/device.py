@@ -1,0 +1,146 @@
+"""Python host binding of the HIP library (libflite3d_amd.so) via its C-ABI.
+
+Mirrors the calls the Fortran host makes through flite3d_amd/fortran/
+flite_gpu.f90; each method names the reference routine it stands for.  The
+library must be built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .params import FIELD_NCOMP, FIELDS, OPS, FliteLevelView, FliteParams, level_view
+
+_bound = None
+
+
+class DeviceError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"flite3d_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _bound
+    if _bound is None:
+        L = _native.device()
+        P = ctypes.POINTER
+        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.flite_ctx_create.argtypes = [P(FliteParams), i, i, P(vp)]
+        L.flite_level_upload.argtypes = [vp, i, P(FliteLevelView)]
+        L.flite_ctx_destroy.argtypes = [vp]
+        L.flite_ctx_destroy.restype = None
+        L.flite_last_error.restype = ctypes.c_char_p
+        L.flite_field_get.argtypes = [vp, i, i, P(d), i]
+        L.flite_field_set.argtypes = [vp, i, i, P(d), i]
+        L.flite_state_set.argtypes = [vp, i, P(d), i, i, i]
+        L.flite_state_get.argtypes = [vp, i, P(d), i, i, i]
+        L.flite_apply.argtypes = [vp, i, i]
+        L.flite_init_freestream.argtypes = [vp]
+        L.flite_init_from_state.argtypes = [vp]
+        L.flite_do_time_iterations.argtypes = [vp, i, i, i, i, i]
+        L.flite_restrict.argtypes = [vp, i, i]
+        L.flite_prolongate.argtypes = [vp, i]
+        L.flite_map_u_fine_to_coarse.argtypes = [vp, i]
+        L.flite_cycle.argtypes = [vp, i]
+        L.flite_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
+        L.flite_sync.argtypes = [vp]
+        _bound = L
+    return _bound
+
+
+class Solver:
+    """One flite_ctx: the multigrid levels of one case on one GPU."""
+
+    def __init__(self, params: FliteParams, levels, n_batch=1):
+        self.L = lib()
+        self.params = params
+        self.levels = levels
+        ctx = ctypes.c_void_p()
+        self._check(self.L.flite_ctx_create(ctypes.byref(params), len(levels),
+                                            n_batch, ctypes.byref(ctx)))
+        self.ctx = ctx
+        for k, lv in enumerate(levels):
+            v = level_view(lv, k == 0)
+            self._check(self.L.flite_level_upload(self.ctx, k + 1, ctypes.byref(v)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DeviceError(rc, self.L.flite_last_error().decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.flite_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n(self, level):
+        return self.levels[level - 1].n_nodes
+
+    def get(self, level, field):
+        f = FIELDS[field] if isinstance(field, str) else field
+        n = self.n(level)
+        out = np.zeros((FIELD_NCOMP[f], n))
+        self._check(self.L.flite_field_get(
+            self.ctx, level, f, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+    def set(self, level, field, arr):
+        f = FIELDS[field] if isinstance(field, str) else field
+        n = self.n(level)
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(FIELD_NCOMP[f], n)
+        self._check(self.L.flite_field_set(
+            self.ctx, level, f, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+
+    def apply(self, level, op):
+        self._check(self.L.flite_apply(self.ctx, level, OPS[op] if isinstance(op, str) else op))
+
+    def init_freestream(self):
+        """setUpInitialField2 (DEF:3713-3996) + multiGridCycles preamble (MG:586-602)."""
+        self._check(self.L.flite_init_freestream(self.ctx))
+
+    def init_from_state(self):
+        self._check(self.L.flite_init_from_state(self.ctx))
+
+    def do_time_iterations(self, level, steps, calc_diss, calc_turb=True, iteration=1):
+        """doTimeIterations (DEF:966-1000)."""
+        self._check(self.L.flite_do_time_iterations(self.ctx, level, steps, int(calc_diss),
+                                                    int(calc_turb), iteration))
+
+    def restrict(self, fine_level, calc_turb=True):
+        """restrictGrid (DEF:12141-12430)."""
+        self._check(self.L.flite_restrict(self.ctx, fine_level, int(calc_turb)))
+
+    def prolongate(self, coarse_level):
+        """prolongateGrid (DEF:12432-12570)."""
+        self._check(self.L.flite_prolongate(self.ctx, coarse_level))
+
+    def map_u_fine_to_coarse(self, fine_level):
+        """mapFineToCoarse of u (DEF:12572-12690)."""
+        self._check(self.L.flite_map_u_fine_to_coarse(self.ctx, fine_level))
+
+    def cycle(self, number):
+        """oneCycle (MG:907-1196)."""
+        self._check(self.L.flite_cycle(self.ctx, number))
+
+    def sync(self):
+        self._check(self.L.flite_sync(self.ctx))
+
+    def monitor(self):
+        """writeResiduals reductions (MG:1271-1445) + calculateLiftAndDrag (DEF:14710-14870)."""
+        res = (ctypes.c_double * 7)()
+        mx = (ctypes.c_double * 7)()
+        idx = (ctypes.c_int * 7)()
+        forces = (ctypes.c_double * 9)()
+        rc = self.L.flite_monitor(self.ctx, res, mx, idx, forces)
+        if rc not in (0, -3):
+            self._check(rc)
+        return dict(res=np.array(res[:]), maxres=np.array(mx[:]), maxidx=np.array(idx[:]),
+                    forces=np.array(forces[:]), rc=rc)

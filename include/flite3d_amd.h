@@ -1,0 +1,186 @@
+/*
+ * flite3d_amd.h -- C-ABI of the MI355X flow-solver hot path.
+ *
+ * Drop-in for the Fortran module-procedure seam between FLITE3D's
+ * MultigridSolver driver (SolverSource/mgnsg3d_mg_par.F90) and its
+ * GridSolver module (SolverSource/mgnsg3d_def_par.F90).  A Fortran host binds
+ * these entry points through ISO_C_BINDING (flite3d_amd/fortran/flite_gpu.f90)
+ * and keeps run.inp, <case>.inp, <case>.sol_1 and the .res/.rsd outputs.
+ *
+ * Conventions
+ *   - plain pointers and sizes, no library types in any signature;
+ *   - node-based arrays use the reference layout: column-major SoA with
+ *     leading dimension N (u(N,6) is u[c*N + i]);
+ *   - index arrays are 0- or 1-based as flite_level_view.index_base says;
+ *   - every call is synchronous with respect to the calling host thread and
+ *     returns 0 on success or a negative FLITE_E_* code; flite_last_error()
+ *     returns the message of the last failure on the calling thread;
+ *   - one HIP stream per context; contexts are not shared between threads.
+ */
+#ifndef FLITE3D_AMD_H
+#define FLITE3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FLITE_OK = 0,
+  FLITE_E_ARG = -1,      /* bad argument / shape */
+  FLITE_E_HIP = -2,      /* HIP runtime error */
+  FLITE_E_NAN = -3,      /* NaN in the monitor residual (reference STOPs, mgnsg3d_mg_par.F90:1464-1470) */
+  FLITE_E_UNSUPPORTED = -4, /* option outside the implemented path */
+  FLITE_E_STATE = -5     /* call order (e.g. level not uploaded) */
+};
+
+/* Scalars of InputVariablesData (mgnsg3d_def_par.F90:64-196) used by the
+ * hot path, after readInputVariables' post-processing
+ * (mgnsg3d_def_par.F90:386-415).  Field names follow the ivd fields. */
+typedef struct flite_params {
+  double gamma;
+  double mach_number;
+  double reynolds_number;          /* 0 => Euler */
+  double prandtl_number;
+  double turbulent_prandtl_number;
+  double alpha, beta;              /* degrees */
+  double inflow_field[2][6];       /* ivd%inflowField(j,k) at [k-1][j-1] */
+  double inflow_temperature;
+  int walls_are_isentropic;
+  int turbulence_model;            /* 0 none, 1 Spalart-Allmaras */
+  int flux_type;                   /* 4 = HLLC */
+  int hllc_flux_order;             /* 1 or 2 */
+  int automatic_clipping;          /* 1 = LED local clipping */
+  int gradient_method;             /* 1 = Green-Gauss (Galerkin) */
+  double hllc_clipping;
+  int boundary_term;               /* 1 = FE (3:1), 2 = FV */
+  double cfl_number;
+  double turbulent_cfl_number;
+  double rk_coefficients[3];       /* 0.6, 0.6, 1.0 (mgnsg3d_def_par.F90:14240-14244) */
+  double max_increment_factor;
+  double minimum_density;
+  double max_turbulence_value;
+  double turb_k;
+  double second_order_dissipation_factor;
+  double fourth_order_dissipation_factor;
+  int multigrid_scheme;            /* 1 V-cycle, 3 growing V */
+  int number_of_relaxation_steps;
+  int number_of_turbulence_grids;
+  int number_of_turbulence_steps;
+  double prolongation_relaxation;
+  double turb_prolongation_relaxation;
+  double multigrid_bc_relaxation;
+  double reference_area;
+  double groundplane;
+  double datum[3];
+  int coarse_fp32;                 /* 1: levels >= 2 smooth and transfer in FP32 */
+} flite_params;
+
+/* Non-owning view of one level as read from <case>.sol_1
+ * (readGridComputationData, mgnsg3d_def_par.F90:13812-14708).  The library
+ * copies, renumbers and keeps its own device copies. */
+typedef struct flite_level_view {
+  int index_base;                  /* 0 (C) or 1 (Fortran) */
+  int n_nodes;
+  int n_sides;
+  int n_bsides;
+  int n_bnodes;
+  const int *side_index;           /* sideIndexArray(E,2), column-major */
+  const double *side_weights;      /* sideWeightsArray(E,3) */
+  const int *bside_index;          /* brp%sideIndexArray(Eb,3): i1,i2,indicator */
+  const double *bside_weights;     /* brp%sideWeightsArray(Eb,3) */
+  const double *node_volume;       /* nodeVolume(N) */
+  const double *coordinates;       /* coordinates(N,3); NULL on coarse levels (treated as 0) */
+  const double *wall_distance;     /* wallDistance(N) */
+  const int *node_indicator_register; /* nodeIndicatorRegister(-20:0), 21 ints, 1-based ranges */
+  const int *node_indicator_array; /* nodeIndicatorArray(Nb) */
+  /* coarse levels only (NULL on level 1) */
+  int n_fine;                      /* nodes of the next finer level */
+  const int *prolongation;         /* pod%prolongationArray(n_fine): fine -> coarse */
+  const double *restriction;       /* rod%restrictionArray(n_fine): V_f / V_c */
+} flite_level_view;
+
+typedef struct flite_ctx flite_ctx;
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* Replaces the allocation half of readComputationData (mgnsg3d_mg_par.F90:1198-1268). */
+int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx **out);
+/* Replaces readGridComputationData's device side (mgnsg3d_def_par.F90:13812-14708). */
+int flite_level_upload(flite_ctx *ctx, int level, const flite_level_view *v);
+void flite_ctx_destroy(flite_ctx *ctx);
+const char *flite_last_error(void);
+
+/* ---- state ------------------------------------------------------------ */
+/* Solution u(N,ncomp) in host numbering, leading dimension ld >= N. */
+int flite_state_set(flite_ctx *ctx, int level, const double *u, int ld, int ncomp, int batch);
+int flite_state_get(flite_ctx *ctx, int level, double *u, int ld, int ncomp, int batch);
+/* Freestream start (setUpInitialField2, mgnsg3d_def_par.F90:3713-3996) on
+ * level 1 followed by the cycle preamble of multiGridCycles
+ * (mgnsg3d_mg_par.F90:586-602): laminar viscosity and restricted u on
+ * every level. */
+int flite_init_freestream(flite_ctx *ctx);
+/* Same preamble when u on level 1 came from flite_state_set (restart). */
+int flite_init_from_state(flite_ctx *ctx);
+
+/* ---- stepping (mgnsg3d_mg_par.F90:939-1186 callers) --------------------- */
+/* doTimeIterations (mgnsg3d_def_par.F90:966-1000) */
+int flite_do_time_iterations(flite_ctx *ctx, int level, int relaxation_steps,
+                             int calculate_dissipation, int calculate_turbulence,
+                             int iteration_number);
+/* restrictGrid (mgnsg3d_def_par.F90:12141-12430), fine_level -> fine_level+1 */
+int flite_restrict(flite_ctx *ctx, int fine_level, int calculate_turbulence);
+/* prolongateGrid (mgnsg3d_def_par.F90:12432-12570), coarse_level -> coarse_level-1 */
+int flite_prolongate(flite_ctx *ctx, int coarse_level);
+/* mapFineToCoarse of u (mgnsg3d_def_par.F90:12572-12690), as oneCycle calls it */
+int flite_map_u_fine_to_coarse(flite_ctx *ctx, int fine_level);
+/* oneCycle (mgnsg3d_mg_par.F90:907-1196), schemes 0, 1 and 3 */
+int flite_cycle(flite_ctx *ctx, int cycle_number);
+
+/* ---- monitor ------------------------------------------------------------ */
+/* writeResiduals' reductions (mgnsg3d_mg_par.F90:1271-1445) and
+ * calculateLiftAndDrag (mgnsg3d_def_par.F90:14710-14870) on level 1:
+ *   res[7]     sqrt(sum (u_j - uprev_j)^2), j < nres (5, or 6 with SA)
+ *   maxres[7]  max |u_j - uprev_j|, maxidx[7] its node (host numbering, index_base)
+ *   forces[9]  lift, drag, latF, frictionDrag, latM, longM, pressureLift, pressureDrag, 0 */
+int flite_monitor(flite_ctx *ctx, double res[7], double maxres[7], int maxidx[7], double forces[9]);
+
+/* ---- parity / inspection ------------------------------------------------ */
+/* Named node fields of a level, host numbering, column-major (N, ncomp):
+ *   FLITE_FIELD_U (6), UPREV (6), RHS (6), DISSIPATION (5), PRESSURE, DT,
+ *   DT_TURB, LAMINAR_VISCOSITY, VORTICITY, DIVERGENCE, SOURCE (6),
+ *   TURB_DIFFUSION, GRADIENT (25: nodeHelpArray 1-15 gradients, 16-25 limits),
+ *   WALL_STRESS (3, boundary nodes only, zeros elsewhere) */
+enum {
+  FLITE_FIELD_U = 0, FLITE_FIELD_UPREV, FLITE_FIELD_RHS, FLITE_FIELD_DISSIPATION,
+  FLITE_FIELD_PRESSURE, FLITE_FIELD_DT, FLITE_FIELD_DT_TURB,
+  FLITE_FIELD_LAMINAR_VISCOSITY, FLITE_FIELD_VORTICITY, FLITE_FIELD_DIVERGENCE,
+  FLITE_FIELD_SOURCE, FLITE_FIELD_TURB_DIFFUSION, FLITE_FIELD_GRADIENT,
+  FLITE_FIELD_WALL_STRESS, FLITE_FIELD_COUNT
+};
+int flite_field_get(flite_ctx *ctx, int level, int field, double *out, int ld);
+int flite_field_set(flite_ctx *ctx, int level, int field, const double *in, int ld);
+int flite_field_ncomp(int field);
+
+/* Single operations of the smoother, for parity tests against the CPU
+ * restatement; each reads and writes the level state like its reference
+ * routine. */
+enum {
+  FLITE_OP_PRESSURE = 0,      /* makePressureField (13612) */
+  FLITE_OP_PRESSURE_ABS,      /* makePressureField; p = |p| (1024-1025) */
+  FLITE_OP_TIMESTEPS,         /* makeTimeSteps (13335) */
+  FLITE_OP_DISSIPATION,       /* makeArtificialDissipation2 early-return semantics (11484) */
+  FLITE_OP_VISCOUS,           /* makeViscosityTerm into dissipation (5198) */
+  FLITE_OP_RHS,               /* rhs = 0; makeRHS (1525); with SA also
+                                 makeSARHS + makeSASourceTerm (6809, 9285) */
+  FLITE_OP_SA_TIMESTEPS,      /* makeTurbulentTimeSteps (12878) */
+  FLITE_OP_SA_DIFFUSION,      /* makeSADiffusion (7436) */
+  FLITE_OP_COUNT
+};
+int flite_apply(flite_ctx *ctx, int level, int op);
+
+/* Synchronise and report per-context timing counters (ms). */
+int flite_sync(flite_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

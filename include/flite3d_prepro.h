@@ -1,0 +1,84 @@
+/*
+ * flite3d_prepro.h -- C-ABI of the grid preprocessor (host side, C++).
+ *
+ * Builds the edge-based ("side-based") median-dual data structure and the
+ * agglomerated multigrid levels that FLITE3D's Gen3d writes into
+ * <case>.sol_1, from a tetrahedral mesh plus oriented boundary triangles.
+ * The arrays it returns are exactly the records the solver reads per level
+ * (reference: SolverSource/mgnsg3d_def_par.F90:791-865 boundary block,
+ * :13843-13862 sides, :13867-13906 fine nodes, :13991-14007 coarse nodes,
+ * :444-545 prolongation/restriction operators).
+ *
+ * Reference producers restated here:
+ *   median-dual side coefficients and volumes  PreProSource/gen3d_grid.f90:786-855, 2092-2332
+ *   boundary side coefficients                 PreProSource/gen3d_boundary.f90:289-372, 511-620
+ *   boundary node indicator priority           PreProSource/gen3d_boundary.f90:1590-1662
+ *   agglomeration (mergeNodes, createSides)    PreProSource/gen3d_grid.f90:2358-2901, 3558-3611
+ *   boundary agglomeration                     PreProSource/gen3d_boundary.f90:922-1259
+ *   coarse wall distance                       PreProSource/gen3d_grid.f90:5823-5850
+ *
+ * Conventions: all indices are 0-based on this interface; the .sol writer in
+ * flite3d_amd/solfile.py converts to the 1-based Fortran records.  Boundary
+ * nodes are numbered first on every level (reference relies on it,
+ * mgnsg3d_def_par.F90:12504-12513).  Every function returns 0 on success or
+ * a negative code; fp_last_error() gives the message.
+ */
+#ifndef FLITE3D_PREPRO_H
+#define FLITE3D_PREPRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fp_mesh fp_mesh;
+
+/* Build the fine level and n_levels-1 agglomerated coarse levels.
+ *   xyz        [n_nodes*3] row-major coordinates
+ *   tets       [n_tets*4]  node indices (any orientation; made positive)
+ *   btri       [n_btri*3]  boundary triangles, right-hand normal pointing
+ *                          INTO the domain (the .plt convention the
+ *                          coefficient formula assumes)
+ *   btri_ind   [n_btri]    boundary indicator per triangle (1 wall,
+ *                          2 symmetry, 3/4 farfield, ... as in .bco)
+ * Node numbering of the result: boundary nodes first (in input order),
+ * then interior nodes (in input order).  fp_fine_old_index() maps back. */
+fp_mesh *fp_mesh_build(int n_nodes, const double *xyz, int n_tets,
+                       const int *tets, int n_btri, const int *btri,
+                       const int *btri_ind, int n_levels);
+void fp_mesh_free(fp_mesh *m);
+const char *fp_last_error(void);
+
+int fp_mesh_nlevels(const fp_mesh *m);
+
+/* sizes of level l (0 = fine) */
+int fp_level_sizes(const fp_mesh *m, int level, int *n_nodes, int *n_sides,
+                   int *n_bsides, int *n_bnodes);
+
+/* Copy level arrays out.  Any pointer may be NULL to skip that array.
+ *   side_index  [2*n_sides]   column-major (i1 column, then i2 column)
+ *   side_w      [3*n_sides]   column-major (wx, wy, wz columns)
+ *   bside_index [3*n_bsides]  column-major (i1, i2, indicator)
+ *   bside_w     [3*n_bsides]  column-major
+ *   volume      [n_nodes]
+ *   coords      [3*n_nodes]   column-major; zeros on coarse levels
+ *   wall_dist   [n_nodes]
+ *   wall_node   [n_nodes]     nearest wall node (fine numbering of that level)
+ *   reg         [21]          nodeIndicatorRegister(-20..0), 1-based ranges
+ *   node_ind_array [n_bnodes] boundary nodes grouped by indicator 1..9
+ *   node_ind    [n_bnodes]    indicator of each boundary node
+ *   seed        [n_nodes]     fine-level seed node of each coarse node
+ *   f2c         [n_fine]      prolongationArray: fine node -> coarse node
+ *   rweight     [n_fine]      restrictionArray: V_fine / V_coarse           */
+int fp_level_copy(const fp_mesh *m, int level, int *side_index,
+                  double *side_w, int *bside_index, double *bside_w,
+                  double *volume, double *coords, double *wall_dist,
+                  int *wall_node, int *reg, int *node_ind_array,
+                  int *node_ind, int *seed, int *f2c, double *rweight);
+
+/* fine-level node i (new numbering) -> index in the xyz passed in */
+int fp_fine_old_index(const fp_mesh *m, int *old_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

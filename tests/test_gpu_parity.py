@@ -1,0 +1,159 @@
+"""GPU path (libflite3d_amd.so through its C-ABI) against the CPU oracle.
+
+Bar: per component, max_i |gpu - oracle| <= tol * max(max_i |oracle|, floor)
+with tol = 1e-11 for single operations on the FP64 path (the reference's
+per-node residual tolerance), and tol = 1e-9 for states after whole V-cycles,
+where rounding differences of the summation order are amplified by the
+nonlinear limiters over several hundred kernel launches.
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from flite3d_amd.device import Solver
+
+from cases import euler, laminar, levels, rans
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b, floor=1e-30):
+    a = np.atleast_2d(a)
+    b = np.atleast_2d(b)
+    out = []
+    for c in range(b.shape[0]):
+        sc = max(np.abs(b[c]).max(), floor)
+        out.append(np.abs(a[c] - b[c]).max() / sc)
+    return np.array(out)
+
+
+def pair(iv, lv, fp32=False):
+    P = iv.to_struct(coarse_fp32=fp32)
+    return Solver(P, list(lv)), Oracle(iv.to_struct(), list(lv))
+
+
+def perturb(s, o, level, rng, amp=0.02):
+    """Same smooth-plus-noise state on both sides."""
+    u = o.get(level, "U")
+    n = u.shape[1]
+    u = u * (1.0 + amp * rng.uniform(-1, 1, size=u.shape))
+    s.set(level, "U", u)
+    o.set(level, "U", u)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("level", [1, 2])
+def test_rhs_euler(order, level):
+    print(f"[case] rhs euler order={order} level={level}", flush=True)
+    lv = levels(8, 3)
+    s, o = pair(euler(order), lv)
+    s.init_freestream(); o.init_freestream()
+    perturb(s, o, level, np.random.default_rng(10 + level))
+    for x in (s, o):
+        x.apply(level, "PRESSURE")
+        x.apply(level, "RHS")
+    assert np.abs(s.get(level, "PRESSURE") - o.get(level, "PRESSURE")).max() < 1e-13
+    e = rel_err(s.get(level, "RHS")[:5], o.get(level, "RHS")[:5])
+    assert e.max() < 1e-11, e
+    if order == 2 and level == 1:
+        eg = rel_err(s.get(1, "GRADIENT"), o.get(1, "GRADIENT"))
+        assert eg.max() < 1e-11, eg
+
+
+def test_timesteps_and_viscous():
+    print("[case] timesteps/viscous laminar", flush=True)
+    lv = levels(8, 3, first_cell=0.01)
+    s, o = pair(laminar(), lv)
+    s.init_freestream(); o.init_freestream()
+    rng = np.random.default_rng(3)
+    perturb(s, o, 1, rng)
+    for x in (s, o):
+        x.apply(1, "PRESSURE")
+        x.apply(1, "TIMESTEPS")
+        x.apply(1, "DISSIPATION")
+        x.apply(1, "VISCOUS")
+    assert rel_err(s.get(1, "DT"), o.get(1, "DT")).max() < 1e-11
+    for f in ("DISSIPATION", "LAMINAR_VISCOSITY", "VORTICITY", "DIVERGENCE"):
+        e = rel_err(s.get(1, f), o.get(1, f))
+        assert e.max() < 1e-11, (f, e)
+    e = rel_err(s.get(1, "WALL_STRESS"), o.get(1, "WALL_STRESS"))
+    assert e.max() < 1e-11, e
+
+
+def test_sa_terms():
+    print("[case] SA terms", flush=True)
+    lv = levels(8, 3, first_cell=0.005)
+    s, o = pair(rans(), lv)
+    s.init_freestream(); o.init_freestream()
+    rng = np.random.default_rng(4)
+    perturb(s, o, 1, rng)
+    u = o.get(1, "U")
+    u[5] = 0.1 + 5.0 * rng.uniform(0, 1, size=u.shape[1])
+    s.set(1, "U", u); o.set(1, "U", u)
+    for x in (s, o):
+        x.apply(1, "PRESSURE")
+        x.apply(1, "TIMESTEPS")
+        x.apply(1, "VISCOUS")
+        x.apply(1, "SA_TIMESTEPS")
+        x.apply(1, "SA_DIFFUSION")
+        x.apply(1, "RHS")
+    for f in ("DT", "DISSIPATION", "DT_TURB", "TURB_DIFFUSION", "RHS"):
+        e = rel_err(s.get(1, f), o.get(1, f))
+        assert e.max() < 1e-11, (f, e)
+
+
+@pytest.mark.parametrize("case", ["euler1", "euler2", "rans"])
+def test_time_iterations(case):
+    print(f"[case] time iterations {case}", flush=True)
+    if case == "rans":
+        lv, iv = levels(8, 3, first_cell=0.005), rans()
+    else:
+        lv, iv = levels(8, 3), euler(int(case[-1]))
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    for x in (s, o):
+        x.do_time_iterations(1, 2, True)
+    e = rel_err(s.get(1, "U")[: (6 if case == "rans" else 5)], o.get(1, "U")[: (6 if case == "rans" else 5)])
+    assert e.max() < 1e-11, e
+
+
+@pytest.mark.parametrize("case", ["euler1", "rans"])
+def test_restrict_prolong(case):
+    print(f"[case] restrict/prolong {case}", flush=True)
+    if case == "rans":
+        lv, iv = levels(8, 3, first_cell=0.005), rans()
+    else:
+        lv, iv = levels(8, 3), euler(1)
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    nc = 6 if case == "rans" else 5
+    for x in (s, o):
+        x.do_time_iterations(1, 1, True)
+        x.restrict(1)
+    for f in ("U", "SOURCE", "RHS"):
+        e = rel_err(s.get(2, f)[:nc], o.get(2, f)[:nc], floor=1e-12)
+        assert e.max() < 1e-10, (f, e)
+    for x in (s, o):
+        x.do_time_iterations(2, 1, False)
+        x.prolongate(2)
+    e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
+    assert e.max() < 1e-10, e
+
+
+@pytest.mark.parametrize("case,scheme", [("euler1", 1), ("euler2", 3), ("rans", 1), ("rans", 3)])
+def test_cycles_and_monitor(case, scheme):
+    print(f"[case] cycles {case} scheme {scheme}", flush=True)
+    if case == "rans":
+        lv, iv = levels(8, 3, first_cell=0.005), rans(multigridscheme=scheme)
+    else:
+        lv, iv = levels(8, 3), euler(int(case[-1]), multigridscheme=scheme)
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    for c in range(1, 6):
+        s.cycle(c); o.cycle(c)
+        ms, mo = s.monitor(), o.monitor()
+        assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (c, ms["res"], mo["res"])
+        assert np.allclose(ms["forces"], mo["forces"], rtol=1e-8, atol=1e-14), (c, ms["forces"], mo["forces"])
+    nc = 6 if case == "rans" else 5
+    e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
+    assert e.max() < 1e-9, e
