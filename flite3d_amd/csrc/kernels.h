@@ -52,6 +52,8 @@ struct LevelDev {
   const R *wb;         // 3 per boundary side
   const int *ell_ptr, *ell_k;
   const int2 *ell;
+  const R *ellw;       // side weights per ELL slot, SoA: [c * nslot + slot]
+  long long nslot;
   const int *bs_ptr, *bs_list;
   const R *vol, *xyz, *wd, *nrm;  // nrm: 3 per node AoS
   const int *ncls;
@@ -75,6 +77,21 @@ __device__ __forceinline__ void for_edges(const int *ell_ptr, const int *ell_k, 
     int2 v = ell[base + k * SLICE + lane];
     if (v.x < 0) continue;
     f(v.x >> 1, v.x & 1, v.y);
+  }
+}
+
+// same, with the side weights read from the slot-ordered copy (coalesced):
+// f(role, nbr, wx, wy, wz)
+template <typename R, typename F>
+__device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, F f) {
+  int s = i / SLICE, lane = i % SLICE;
+  int base = L.ell_ptr[s], K = L.ell_k[s];
+  const long long ns = L.nslot;
+  for (int k = 0; k < K; ++k) {
+    int slot = base + k * SLICE + lane;
+    int2 v = L.ell[slot];
+    if (v.x < 0) continue;
+    f(v.x & 1, v.y, L.ellw[slot], L.ellw[ns + slot], L.ellw[2 * ns + slot]);
   }
 }
 
@@ -133,9 +150,8 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 15; ++c) g[c] = 0;
   for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
   bool any = false;
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
     R sg = role ? R(-1) : R(1);
     any = true;
 #pragma unroll
@@ -463,8 +479,7 @@ __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
   R lam = 0, lamv = 0;
   bool visc = P.Re > 0.0;
   R am = visc ? R(1.0 / P.Re) : R(0);
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += fabs(vn) + sqrt(cc * al);
@@ -516,10 +531,10 @@ __global__ void k_visc_node(LevelDev<R> L, DevParams P) {
   R g[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) g[c] = 0;
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     R aj[4] = {qj[1], qj[2], qj[3], gog1 * qj[5] / qj[0]};
-    R ww[3] = {L.w[3 * e], L.w[3 * e + 1], L.w[3 * e + 2]};
+    R ww[3] = {wx, wy, wz};
     R sg = role ? R(-1) : R(1);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -600,11 +615,10 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 9; ++c) t1[c] = ti[c];
   R u1 = qi[1], v1 = qi[2], w1 = qi[3];
   R a[4] = {0, 0, 0, 0};
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     const R *tj = L.tau + 9 * (size_t)j;
     R u2 = qj[1], v2 = qj[2], w2 = qj[3];
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
     R d[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) d[c] = t1[c] + tj[c];
@@ -658,8 +672,7 @@ __global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
   const R *qi = L.rec0 + 8 * (size_t)i;
   R ioRe = R(1.0 / P.Re), oos = R(1.5), cb2 = R(0.622), tk = R(P.turb_k);
   R lam = 0, lamv = 0;
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += tk * sqrt(al) + fabs(vn);
@@ -701,12 +714,12 @@ __global__ void k_sa_diff1(LevelDev<R> L, DevParams P) {
   if (i >= L.n) return;
   R ti = L.rec0[8 * (size_t)i + 6];
   R g[3] = {0, 0, 0};
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     R s = ti + L.rec0[8 * (size_t)j + 6];
     R sg = role ? R(-1) : R(1);
-    g[0] += sg * (L.w[3 * e] * s);
-    g[1] += sg * (L.w[3 * e + 1] * s);
-    g[2] += sg * (L.w[3 * e + 2] * s);
+    g[0] += sg * (wx * s);
+    g[1] += sg * (wy * s);
+    g[2] += sg * (wz * s);
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -738,10 +751,9 @@ __global__ void k_sa_diff2(LevelDev<R> L, DevParams P) {
   R fact = R(1.0 / P.Re / (2.0 / 3.0));
   R v1 = Re * qi[7] / qi[0] + qi[6];
   R acc = 0;
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     const R *dj = L.dnu + 4 * (size_t)j;
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
     R v2 = Re * qj[7] / qj[0] + qj[6];
     R fd = v1 * (wx * di[0] + wy * di[1] + wz * di[2]) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
     acc += role ? fact * fd : -(fact * fd);

@@ -137,6 +137,8 @@ struct Level {
   void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
   int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
   int2 *ell;
+  void *ellw;
+  long long nslot = 0;
   // transfer from the next finer level (coarse levels)
   int *f2c = nullptr, *cptr = nullptr, *clist = nullptr;
   double *rw = nullptr;
@@ -150,7 +152,7 @@ struct Level {
     v.n = n; v.ne = ne; v.nbs = nbs; v.gnum = gnum;
     v.ea = ea; v.eb = eb; v.w = (const R *)w;
     v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
-    v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell = ell;
+    v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell = ell; v.ellw = (const R *)ellw; v.nslot = nslot;
     v.bs_ptr = bs_ptr; v.bs_list = bs_list;
     v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
     v.wd = (const R *)wd; v.nrm = (const R *)nrm; v.ncls = ncls;
@@ -168,6 +170,14 @@ struct flite_ctx {
   int nl = 0, nbatch = 1;
   std::vector<std::unique_ptr<Level>> lev;
   hipStream_t stream = nullptr;
+  // kernel timing (flite_timing_enable)
+  std::string timed;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used;
+  double timed_ms = 0;
+  int timed_count = 0;
+  // residual evaluations per level in the last counted cycle
+  std::vector<int> evals;
 };
 
 namespace {
@@ -186,6 +196,52 @@ int check_launch() {
 
 int sync(flite_ctx *c) {
   HIPTRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+hipEvent_t take_event(flite_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// scoped timing of one launch of kernel class `name` on level `gnum`
+struct Timed {
+  flite_ctx *c;
+  bool on;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(flite_ctx *c_, const char *name, int gnum) : c(c_) {
+    on = !c->timed.empty() && c->timed == std::string(name) + "_l" + std::to_string(gnum);
+    if (on) {
+      a = take_event(c);
+      b = take_event(c);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (on) {
+      (void)hipEventRecord(b, c->stream);
+      c->ev_used.push_back({a, b});
+    }
+  }
+};
+
+int harvest_timing(flite_ctx *c) {
+  HIPTRY(hipStreamSynchronize(c->stream));
+  for (auto &pr : c->ev_used) {
+    float ms = 0;
+    HIPTRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->timed_ms += ms;
+    c->timed_count += 1;
+    c->ev_pool.push_back(pr.first);
+    c->ev_pool.push_back(pr.second);
+  }
+  c->ev_used.clear();
   return 0;
 }
 
@@ -348,16 +404,21 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
     ell_k[s] = K;
     ell_ptr[s + 1] = ell_ptr[s] + K * SLICE;
   }
-  std::vector<int2> ell(ell_ptr[nslice] ? ell_ptr[nslice] : 1, make_int2(-1, -1));
+  const size_t nslot = ell_ptr[nslice] ? ell_ptr[nslice] : 1;
+  std::vector<int2> ell(nslot, make_int2(-1, -1));
+  std::vector<double> ellw(3 * nslot, 0.0);
   std::vector<int> fill(n, 0);
   for (int k = 0; k < ne; ++k) {
     for (int r = 0; r < 2; ++r) {
       int i = r ? eb[k] : ea[k], j = r ? ea[k] : eb[k];
       int s = i / SLICE, lane = i % SLICE;
-      ell[ell_ptr[s] + fill[i] * SLICE + lane] = make_int2((k << 1) | r, j);
+      size_t slot = ell_ptr[s] + fill[i] * SLICE + lane;
+      ell[slot] = make_int2((k << 1) | r, j);
+      for (int q = 0; q < 3; ++q) ellw[q * nslot + slot] = w[3 * (size_t)k + q];
       fill[i]++;
     }
   }
+  L.nslot = (long long)nslot;
   // boundary sides in device numbering, per-node CSR
   std::vector<int> ba(nbs), bb(nbs);
   std::vector<double> wb(3 * (size_t)nbs);
@@ -414,6 +475,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
     return L.fp32 ? alloc_real<float>(P, out, cnt) : alloc_real<double>(P, out, cnt);
   };
   TRY(up_real(&L.w, w));
+  TRY(up_real(&L.ellw, ellw));
   TRY(up_real(&L.wb, wb));
   TRY(up_real(&L.vol, vol));
   TRY(up_real(&L.wd, wd));
@@ -480,6 +542,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
 // ---------------------------------------------------------------------------
 int op_prep(flite_ctx *c, Level &L, bool compute_p) {
   DevParams P = c->P;
+  Timed tm(c, "prep", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (compute_p) {
@@ -494,6 +557,7 @@ int op_prep(flite_ctx *c, Level &L, bool compute_p) {
 int op_grad(flite_ctx *c, Level &L) {
   if (!(L.has_coords && c->hp.hllc_flux_order > 1)) return 0;
   DevParams P = c->P;
+  Timed tm(c, "grad", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     hipLaunchKernelGGL(k_grad_minmax<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
@@ -504,21 +568,27 @@ int op_grad(flite_ctx *c, Level &L) {
 int op_flux(flite_ctx *c, Level &L) {
   DevParams P = c->P;
   bool o2 = L.has_coords && c->hp.hllc_flux_order > 1;
+  if (L.gnum <= (int)c->evals.size()) c->evals[L.gnum - 1] += 1;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (L.ne > 0) {
+      Timed tm(c, "flux", L.gnum);
       if (o2 && P.sa) hipLaunchKernelGGL((k_flux<R, 2, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
       else if (o2) hipLaunchKernelGGL((k_flux<R, 2, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
       else if (P.sa) hipLaunchKernelGGL((k_flux<R, 1, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
       else hipLaunchKernelGGL((k_flux<R, 1, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
     }
-    if (L.nbs > 0) hipLaunchKernelGGL(k_bflux<R>, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P);
+    if (L.nbs > 0) {
+      Timed tm(c, "bflux", L.gnum);
+      hipLaunchKernelGGL(k_bflux<R>, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P);
+    }
     return check_launch();
   });
 }
 
 int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
   DevParams P = c->P;
+  Timed tm(c, "stage", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     dim3 g(nblk(L.n)), t(NTHR);
@@ -538,6 +608,7 @@ int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
 
 int op_timesteps(flite_ctx *c, Level &L) {
   DevParams P = c->P;
+  Timed tm(c, "timesteps", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     hipLaunchKernelGGL(k_timesteps<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
@@ -563,6 +634,7 @@ int op_dissipation(flite_ctx *c, Level &L) {
 
 int op_viscous(flite_ctx *c, Level &L) {
   DevParams P = c->P;
+  Timed tm(c, "viscous", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     hipLaunchKernelGGL(k_visc_node<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
@@ -1026,6 +1098,7 @@ int flite_map_u_fine_to_coarse(flite_ctx *c, int fine_level) {
 
 int flite_cycle(flite_ctx *c, int cycle_number) {
   TRY(check_ctx(c));
+  c->evals.assign(c->nl, 0);
   TRY(do_cycle(c, cycle_number));
   return sync(c);
 }
@@ -1132,6 +1205,39 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
 int flite_sync(flite_ctx *c) {
   if (!c) return fail(FLITE_E_ARG, "null context");
   return sync(c);
+}
+
+int flite_timing_enable(flite_ctx *c, const char *name) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  TRY(harvest_timing(c));
+  c->timed = name ? name : "";
+  c->timed_ms = 0;
+  c->timed_count = 0;
+  return 0;
+}
+
+int flite_timing_get(flite_ctx *c, double *total_ms, int *launches) {
+  if (!c || !total_ms || !launches) return fail(FLITE_E_ARG, "null argument");
+  TRY(harvest_timing(c));
+  *total_ms = c->timed_ms;
+  *launches = c->timed_count;
+  return 0;
+}
+
+int flite_cycle_evaluations(flite_ctx *c, int *evals_per_level) {
+  TRY(check_ctx(c));
+  if (!evals_per_level) return fail(FLITE_E_ARG, "null argument");
+  for (int l = 0; l < c->nl; ++l) evals_per_level[l] = l < (int)c->evals.size() ? c->evals[l] : 0;
+  return 0;
+}
+
+int flite_level_sizes(flite_ctx *c, int level, int *n_nodes, int *n_sides, int *n_bsides) {
+  TRY(check_level(c, level));
+  Level &L = *c->lev[level - 1];
+  if (n_nodes) *n_nodes = L.n;
+  if (n_sides) *n_sides = L.ne;
+  if (n_bsides) *n_bsides = L.nbs;
+  return 0;
 }
 
 }  // extern "C"

@@ -144,3 +144,45 @@ class Solver:
             self._check(rc)
         return dict(res=np.array(res[:]), maxres=np.array(mx[:]), maxidx=np.array(idx[:]),
                     forces=np.array(forces[:]), rc=rc)
+
+
+def _bind_extra():
+    L = lib()
+    if getattr(L, "_extra_bound", False):
+        return L
+    P = ctypes.POINTER
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.flite_timing_enable.argtypes = [vp, ctypes.c_char_p]
+    L.flite_timing_get.argtypes = [vp, P(d), P(i)]
+    L.flite_cycle_evaluations.argtypes = [vp, P(i)]
+    L.flite_level_sizes.argtypes = [vp, i, P(i), P(i), P(i)]
+    L._extra_bound = True
+    return L
+
+
+def timing_enable(solver, name):
+    """Record HIP events around every launch of kernel class `name` (e.g. 'flux_l1')."""
+    L = _bind_extra()
+    solver._check(L.flite_timing_enable(solver.ctx, None if name is None else name.encode()))
+
+
+def timing_get(solver):
+    L = _bind_extra()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int()
+    solver._check(L.flite_timing_get(solver.ctx, ctypes.byref(ms), ctypes.byref(cnt)))
+    return ms.value, cnt.value
+
+
+def cycle_evaluations(solver):
+    L = _bind_extra()
+    out = (ctypes.c_int * len(solver.levels))()
+    solver._check(L.flite_cycle_evaluations(solver.ctx, out))
+    return list(out)
+
+
+def level_sizes(solver, level):
+    L = _bind_extra()
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    solver._check(L.flite_level_sizes(solver.ctx, level, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    return a.value, b.value, c.value

@@ -177,8 +177,21 @@ enum {
 };
 int flite_apply(flite_ctx *ctx, int level, int op);
 
-/* Synchronise and report per-context timing counters (ms). */
+/* Wait for all work queued by the context. */
 int flite_sync(flite_ctx *ctx);
+
+/* Kernel timing with HIP events recorded on the context's stream around
+ * every launch of the named kernel class (e.g. "flux_l1", "stage_l1",
+ * "grad_l1"; "_lN" = level N).  flite_timing_enable(ctx, name) starts
+ * recording (name NULL stops); flite_timing_get returns the summed
+ * duration (ms) and the number of launches since enabling. */
+int flite_timing_enable(flite_ctx *ctx, const char *name);
+int flite_timing_get(flite_ctx *ctx, double *total_ms, int *launches);
+/* Per-cycle counts of side-loop evaluations: writes, per level, the number
+ * of residual (flux) evaluations one flite_cycle performs. */
+int flite_cycle_evaluations(flite_ctx *ctx, int *evals_per_level);
+/* Sizes of a level as uploaded: nodes, sides, boundary sides. */
+int flite_level_sizes(flite_ctx *ctx, int level, int *n_nodes, int *n_sides, int *n_bsides);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,7 @@ def lib():
         L.or_cycle.argtypes = [vp, i]
         L.or_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
         L.or_hllc_flux.argtypes = [P(FliteParams), P(d), d, P(d), d, P(d), P(d)]
+        L.or_farfield_flux.argtypes = [P(FliteParams), i, P(d), d, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -145,4 +146,14 @@ def hllc_flux(params, ul, pl, ur, pr, w):
     b = (ctypes.c_double * 5)(*ur)
     ww = (ctypes.c_double * 3)(*w)
     lib().or_hllc_flux(ctypes.byref(params), a, pl, b, pr, ww, f)
+    return np.array(f[:])
+
+
+def farfield_flux(params, indicator, state, p, n):
+    f = (ctypes.c_double * 5)()
+    a = (ctypes.c_double * 5)(*state)
+    nn = (ctypes.c_double * 3)(*n)
+    rc = lib().or_farfield_flux(ctypes.byref(params), indicator, a, p, nn, f)
+    if rc != 0:
+        raise OracleError(lib().or_last_error().decode())
     return np.array(f[:])

@@ -906,6 +906,13 @@ static void farfield_roe(const flite_params *P, const double *inf, const double 
   f[4] = 0.5 * (fr5 + fl5) - 0.5 * (rl3 * du5 + cc1 * hi + cc2 * ucp);
 }
 
+int or_farfield_flux(const flite_params *p, int indicator, const double s[5], double pr,
+                     const double n[3], double f[5]) {
+  if (indicator != 3 && indicator != 4) return fail(FLITE_E_ARG, "farfield indicator must be 3 or 4");
+  farfield_roe(p, p->inflow_field[indicator - 3], s, pr, n[0], n[1], n[2], f);
+  return 0;
+}
+
 /* makeRHS (DEF:1525-2590), HLLC interior flux + boundary loop; rhs must be
  * zeroed by the caller (the reference does grp%rhs = 0 before the call). */
 static int make_rhs(or_ctx *C, orl *L) {

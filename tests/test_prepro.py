@@ -1,0 +1,102 @@
+"""Grid preprocessor (Gen3d restatement) invariants (CPU).
+
+- dual volumes: each node gets exactly a quarter of every incident tet
+  (median dual), so V_i = sum V_tet/4, and sum V = box minus bump volume of
+  the piecewise-linear mesh;
+- side-coefficient closure per node, sum_j +-2 w_ij + 4 sum w_b = 0 (the
+  solver's "max side error" check, DEF:14430-14568) on every level;
+- sizes of the Kuhn bump box (N, E, boundary nodes/sides) in closed form;
+- boundary-first numbering, indicator priority (farfield over wall) and the
+  nodeIndicatorRegister layout (gen3d_grid.f90:4541-4558);
+- agglomeration: every fine node mapped, coarse volume = sum of fine volumes,
+  restriction weights V_f/V_c, fine boundary nodes map to coarse boundary
+  nodes.
+"""
+import numpy as np
+import pytest
+
+from flite3d_amd import mesh
+
+
+@pytest.fixture(scope="module")
+def box():
+    xyz, tets, btri, bind = mesh.bump_box(6, jitter=0.15, seed=0)
+    lv, old = mesh.build_levels(xyz, tets, btri, bind, 3)
+    return xyz, tets, btri, bind, lv, old
+
+
+def tet_volumes(xyz, tets):
+    a, b, c, d = (xyz[tets[:, k]] for k in range(4))
+    return np.abs(np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)) / 6.0
+
+
+def test_sizes_closed_form():
+    for n in (3, 5, 8):
+        xyz, tets, btri, bind = mesh.bump_box(n)
+        lv, _ = mesh.build_levels(xyz, tets, btri, bind, 1)
+        L = lv[0]
+        assert L.n_nodes == (n + 1) ** 3
+        assert L.n_sides == 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+        assert L.n_bnodes == 6 * n * n + 2
+        assert L.n_bsides == 18 * n * n
+        assert tets.shape[0] == 6 * n ** 3
+
+
+def test_dual_volumes(box):
+    xyz, tets, btri, bind, lv, old = box
+    L = lv[0]
+    vt = tet_volumes(xyz, tets)
+    ref = np.zeros(xyz.shape[0])
+    for k in range(4):
+        np.add.at(ref, tets[:, k], vt / 4.0)
+    assert np.allclose(L.volume, ref[old], rtol=1e-12)
+    assert abs(L.volume.sum() - vt.sum()) < 1e-13
+
+
+def test_closure_all_levels(box):
+    *_, lv, old = box
+    for L in lv:
+        chk = np.zeros((L.n_nodes, 3))
+        i1, i2 = L.side_index
+        b1, b2 = L.bside_index[0], L.bside_index[1]
+        for c in range(3):
+            np.add.at(chk[:, c], i1, 2 * L.side_w[c])
+            np.add.at(chk[:, c], i2, -2 * L.side_w[c])
+            np.add.at(chk[:, c], b1, 4 * L.bside_w[c])
+            np.add.at(chk[:, c], b2, 4 * L.bside_w[c])
+        assert np.abs(chk).max() < 1e-14, np.abs(chk).max()
+
+
+def test_boundary_numbering_and_indicators(box):
+    xyz, tets, btri, bind, lv, old = box
+    L = lv[0]
+    bnodes = set(np.unique(btri))
+    assert set(old[: L.n_bnodes]) == bnodes
+    assert not (set(old[L.n_bnodes:]) & bnodes)
+    wall_faces_nodes = set(np.unique(btri[bind == 1]))
+    far_faces_nodes = set(np.unique(btri[bind == 3]))
+    for i in range(L.n_bnodes):
+        o = old[i]
+        expect = 3 if o in far_faces_nodes else 1
+        assert L.node_ind[i] == expect
+    reg = L.reg
+    nw = int((L.node_ind == 1).sum())
+    nf = int((L.node_ind == 3).sum())
+    assert reg[0] == 1 and reg[1] == nw
+    assert reg[4] == nw + 1 and reg[5] == nw + nf
+    assert np.all(L.node_ind[L.node_ind_array[:nw]] == 1)
+
+
+def test_agglomeration(box):
+    *_, lv, old = box
+    for F, C in zip(lv[:-1], lv[1:]):
+        assert C.f2c.shape[0] == F.n_nodes
+        assert C.f2c.min() == 0 and C.f2c.max() == C.n_nodes - 1
+        vol = np.zeros(C.n_nodes)
+        np.add.at(vol, C.f2c, F.volume)
+        assert np.allclose(vol, C.volume, rtol=1e-14)
+        assert np.allclose(C.rweight, F.volume / C.volume[C.f2c], rtol=1e-14)
+        assert C.f2c[: F.n_bnodes].max() < C.n_bnodes
+        assert C.n_nodes < F.n_nodes / 3
+        assert np.all(C.side_index[0] < C.side_index[1])
+        assert np.all(C.bside_index[0] <= C.bside_index[1])
