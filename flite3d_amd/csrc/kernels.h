@@ -4,12 +4,14 @@
 //   node state      u[c*n+i] (SoA, 6 comps), uprev, p, dt, dtt, mu, vort,
 //                   divg, diss[5], src[6], tdiff
 //   rec0[8*i]       AoS per node: rho, u, v, w, rhoE, p, nu~, mu  (gathers)
-//   grec[GREC*i]    AoS per node (HLLC-2): grad 15 | 1/lo 5 | 1/hi 5 | xyz 3 |
-//                   lo 5 | hi 5   (lo/hi = nodeHelpArray 16-25 semantics)
+//   grec[GREC*i]    AoS per node (HLLC-2): grad 15 | 1/lo 5 | 1/hi 5 | pad
+//                   (lo/hi = nodeHelpArray 16-25: min/max over sides - q)
 //   edges           ea[e], eb[e] (ends in reference orientation), w[3*e] AoS
 //   incidence       sliced ELL, one wavefront per slice of 64 nodes:
 //                   slot (k, lane) at ell_ptr[s] + 64*k + lane holds
-//                   int2(edge<<1 | role, neighbour); role 1 = node is eb
+//                   ell_e = edge<<1 | role (role 1: node is eb), ell_nbr,
+//                   and ellw[3][slot] = the side weights signed for the
+//                   node (+w as ea, -w as eb)
 //   boundary sides  ba, bb, bind, wb[3*s]; per node CSR bs_ptr/bs_list
 //                   (side<<1 | end)
 // Every edge loop of the reference (scatter to both ends) is evaluated here
@@ -25,7 +27,7 @@
 namespace fk {
 
 constexpr int SLICE = 64;
-constexpr int GREC = 38;
+constexpr int GREC = 26;
 constexpr int NTHR = 256;
 
 struct DevParams {
@@ -51,8 +53,8 @@ struct LevelDev {
   const int *ba, *bb, *bind;
   const R *wb;         // 3 per boundary side
   const int *ell_ptr, *ell_k;
-  const int2 *ell;
-  const R *ellw;       // side weights per ELL slot, SoA: [c * nslot + slot]
+  const int *ell_e, *ell_nbr;
+  const R *ellw;       // signed side weights per ELL slot, SoA: [c * nslot + slot]
   long long nslot;
   const int *bs_ptr, *bs_list;
   const R *vol, *xyz, *wd, *nrm;  // nrm: 3 per node AoS
@@ -63,25 +65,41 @@ struct LevelDev {
 
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// XCD-aware block order: the hardware hands workgroup b to XCD b % 8; this
+// remaps so that each XCD walks one contiguous range of logical blocks and
+// its L2 sees one compact window of nodes/sides instead of every 8th block.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+  int b = blockIdx.x, nb = gridDim.x;
+  if (nb < 2 * NXCD) return b;
+  int x = b % NXCD, k = b / NXCD;
+  int q = nb / NXCD, r = nb % NXCD;
+  int start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return start + k;
+}
+__device__ __forceinline__ int gtid_x() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 template <typename R>
 __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
-// iterate the incidence of node i: f(edge, role, nbr)
-template <typename F>
-__device__ __forceinline__ void for_edges(const int *ell_ptr, const int *ell_k, const int2 *ell, int i, F f) {
+// iterate the incidence of node i: f(edge, role)
+template <typename R, typename F>
+__device__ __forceinline__ void for_edges(const LevelDev<R> &L, int i, F f) {
   int s = i / SLICE, lane = i % SLICE;
-  int base = ell_ptr[s], K = ell_k[s];
+  int base = L.ell_ptr[s], K = L.ell_k[s];
   for (int k = 0; k < K; ++k) {
-    int2 v = ell[base + k * SLICE + lane];
-    if (v.x < 0) continue;
-    f(v.x >> 1, v.x & 1, v.y);
+    int v = L.ell_e[base + k * SLICE + lane];
+    if (v < 0) continue;
+    f(v >> 1, v & 1);
   }
 }
 
-// same, with the side weights read from the slot-ordered copy (coalesced):
-// f(role, nbr, wx, wy, wz)
+// neighbours with the node-signed side weights (+w when the node is the
+// side's first end, -w when it is the second): f(nbr, wx, wy, wz).  A
+// reference scatter "+f to i1, -f to i2" with f linear in w becomes "+f(w_s)"
+// here, with the same rounding since the sign flip is exact.
 template <typename R, typename F>
 __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, F f) {
   int s = i / SLICE, lane = i % SLICE;
@@ -89,9 +107,9 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, F f) {
   const long long ns = L.nslot;
   for (int k = 0; k < K; ++k) {
     int slot = base + k * SLICE + lane;
-    int2 v = L.ell[slot];
-    if (v.x < 0) continue;
-    f(v.x & 1, v.y, L.ellw[slot], L.ellw[ns + slot], L.ellw[2 * ns + slot]);
+    int j = L.ell_nbr[slot];
+    if (j < 0) continue;
+    f(j, L.ellw[slot], L.ellw[ns + slot], L.ellw[2 * ns + slot]);
   }
 }
 
@@ -100,7 +118,7 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, F f) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_prep(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   int n = L.n;
   R r = L.u[i], m1 = L.u[n + i], m2 = L.u[2 * n + i], m3 = L.u[3 * n + i], E = L.u[4 * n + i];
@@ -121,7 +139,7 @@ __global__ void k_prep(LevelDev<R> L, DevParams P) {
 // gather records from the current state without recomputing p (parity ops)
 template <typename R>
 __global__ void k_pack(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   int n = L.n;
   R r = L.u[i];
@@ -141,7 +159,7 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
@@ -150,16 +168,15 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 15; ++c) g[c] = 0;
   for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
   bool any = false;
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
-    R sg = role ? R(-1) : R(1);
     any = true;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       R s = q0[c] + qj[c];
-      g[c] += sg * (s * wx);
-      g[5 + c] += sg * (s * wy);
-      g[10 + c] += sg * (s * wz);
+      g[c] += s * wx;
+      g[5 + c] += s * wy;
+      g[10 + c] += s * wz;
       R lo = rmin(q0[c], qj[c]);
       R hi = rmin(-q0[c], -qj[c]);
       mn[c] = rmin(mn[c], lo);
@@ -188,18 +205,10 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 5; ++c) {
     R lo = mn[c] - q0[c];
     R hi = -mx[c] - q0[c];
-    out[28 + c] = lo;
-    out[33 + c] = hi;
     out[15 + c] = lo < R(0) ? R(1) / lo : R(0);
     out[20 + c] = hi > R(0) ? R(1) / hi : R(0);
   }
-  if (L.xyz) {
-    out[25] = L.xyz[3 * (size_t)i];
-    out[26] = L.xyz[3 * (size_t)i + 1];
-    out[27] = L.xyz[3 * (size_t)i + 2];
-  } else {
-    out[25] = out[26] = out[27] = 0;
-  }
+  out[25] = 0;
 }
 
 // LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds
@@ -227,7 +236,7 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi, int nsig) {
 // ---------------------------------------------------------------------------
 template <typename R, int ORDER, bool SA>
 __global__ void k_flux(LevelDev<R> L, DevParams P) {
-  int e = gtid();
+  int e = gtid_x();
   if (e >= L.ne) return;
   constexpr int NF = 6;
   int ia = L.ea[e], ib = L.eb[e];
@@ -264,7 +273,8 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
   if (ORDER > 1) {
     const R *G1 = L.grec + GREC * (size_t)ia;
     const R *G2 = L.grec + GREC * (size_t)ib;
-    R dx = G2[25] - G1[25], dy = G2[26] - G1[26], dz = G2[27] - G1[27];
+    const R *X1 = L.xyz + 3 * (size_t)ia, *X2 = L.xyz + 3 * (size_t)ib;
+    R dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
     R hl = R(0.5) * R(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
     R d[5];
 #pragma unroll
@@ -429,7 +439,7 @@ __device__ __forceinline__ void farfield_roe(const double *inf, const R *q, R gm
 // bbuf[10*s + 5*end + c]
 template <typename R>
 __global__ void k_bflux(LevelDev<R> L, DevParams P) {
-  int s = gtid();
+  int s = gtid_x();
   if (s >= L.nbs) return;
   int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
   R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
@@ -470,7 +480,7 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   R gam = R(P.gamma);
@@ -479,7 +489,7 @@ __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
   R lam = 0, lamv = 0;
   bool visc = P.Re > 0.0;
   R am = visc ? R(1.0 / P.Re) : R(0);
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += fabs(vn) + sqrt(cc * al);
@@ -489,7 +499,6 @@ __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
       vis = am > vis ? am : vis;
       lamv += R(2) * vis * al / qj[0];
     }
-    (void)role;
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -523,7 +532,7 @@ __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_visc_node(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   R gog1 = R(P.gamma / (P.gamma - 1.0));
   const R *qi = L.rec0 + 8 * (size_t)i;
@@ -531,15 +540,14 @@ __global__ void k_visc_node(LevelDev<R> L, DevParams P) {
   R g[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) g[c] = 0;
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     R aj[4] = {qj[1], qj[2], qj[3], gog1 * qj[5] / qj[0]};
     R ww[3] = {wx, wy, wz};
-    R sg = role ? R(-1) : R(1);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) g[4 * d + q] += sg * ((ai[q] + aj[q]) * ww[d]);
+      for (int q = 0; q < 4; ++q) g[4 * d + q] += (ai[q] + aj[q]) * ww[d];
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -606,7 +614,7 @@ __global__ void k_visc_node(LevelDev<R> L, DevParams P) {
 // makeViscosityTerm pass 2 (DEF:5640-5893): dissipation(:,2:5) = -sum
 template <typename R>
 __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   const R *ti = L.tau + 9 * (size_t)i;
@@ -615,7 +623,7 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 9; ++c) t1[c] = ti[c];
   R u1 = qi[1], v1 = qi[2], w1 = qi[3];
   R a[4] = {0, 0, 0, 0};
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     const R *tj = L.tau + 9 * (size_t)j;
     R u2 = qj[1], v2 = qj[2], w2 = qj[3];
@@ -625,11 +633,10 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
     R dut1 = u1 * t1[0] + u2 * tj[0] + v1 * t1[1] + v2 * tj[1] + w1 * t1[2] + w2 * tj[2];
     R dut2 = u1 * t1[1] + u2 * tj[1] + v1 * t1[3] + v2 * tj[3] + w1 * t1[4] + w2 * tj[4];
     R dut3 = u1 * t1[2] + u2 * tj[2] + v1 * t1[4] + v2 * tj[4] + w1 * t1[5] + w2 * tj[5];
-    R sg = role ? R(-1) : R(1);
-    a[0] += sg * (wx * d[0] + wy * d[1] + wz * d[2]);
-    a[1] += sg * (wx * d[1] + wy * d[3] + wz * d[4]);
-    a[2] += sg * (wx * d[2] + wy * d[4] + wz * d[5]);
-    a[3] += sg * (wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3));
+    a[0] += wx * d[0] + wy * d[1] + wz * d[2];
+    a[1] += wx * d[1] + wy * d[3] + wz * d[4];
+    a[2] += wx * d[2] + wy * d[4] + wz * d[5];
+    a[3] += wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3);
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -667,19 +674,18 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   R ioRe = R(1.0 / P.Re), oos = R(1.5), cb2 = R(0.622), tk = R(P.turb_k);
   R lam = 0, lamv = 0;
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += tk * sqrt(al) + fabs(vn);
     const R *qj = L.rec0 + 8 * (size_t)j;
     R vis = oos * (qj[7] + R(3) * cb2 * qj[6] * ioRe);
     lamv += R(2) * vis * al;
-    (void)role;
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -710,21 +716,21 @@ __global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
 // cb2 term)
 template <typename R>
 __global__ void k_sa_diff1(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
-  R ti = L.rec0[8 * (size_t)i + 6];
+  const R *nu = L.u + 5 * (size_t)L.n;
+  R ti = nu[i];
   R g[3] = {0, 0, 0};
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
-    R s = ti + L.rec0[8 * (size_t)j + 6];
-    R sg = role ? R(-1) : R(1);
-    g[0] += sg * (wx * s);
-    g[1] += sg * (wy * s);
-    g[2] += sg * (wz * s);
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
+    R s = ti + nu[j];
+    g[0] += wx * s;
+    g[1] += wy * s;
+    g[2] += wz * s;
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
     int o = (v & 1) ? L.ba[s] : L.bb[s];
-    R to = L.rec0[8 * (size_t)o + 6];
+    R to = nu[o];
     R c = P.bterm == 1 ? (R(3) * ti + to) : R(4) * ti;
     g[0] += c * L.wb[3 * s];
     g[1] += c * L.wb[3 * s + 1];
@@ -743,7 +749,7 @@ __global__ void k_sa_diff1(LevelDev<R> L, DevParams P) {
 // makeSADiffusion pass 2
 template <typename R>
 __global__ void k_sa_diff2(LevelDev<R> L, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   const R *di = L.dnu + 4 * (size_t)i;
@@ -751,12 +757,12 @@ __global__ void k_sa_diff2(LevelDev<R> L, DevParams P) {
   R fact = R(1.0 / P.Re / (2.0 / 3.0));
   R v1 = Re * qi[7] / qi[0] + qi[6];
   R acc = 0;
-  for_edges_w(L, i, [&](int role, int j, R wx, R wy, R wz) {
+  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
     const R *qj = L.rec0 + 8 * (size_t)j;
     const R *dj = L.dnu + 4 * (size_t)j;
     R v2 = Re * qj[7] / qj[0] + qj[6];
     R fd = v1 * (wx * di[0] + wy * di[1] + wz * di[2]) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
-    acc += role ? fact * fd : -(fact * fd);
+    acc -= fact * fd;
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k], s = v >> 1;
@@ -886,7 +892,7 @@ __device__ __forceinline__ void outer_bc(const LevelDev<R> &L, const DevParams &
 template <typename R, bool SA>
 __device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, R *r) {
   for (int c = 0; c < 6; ++c) r[c] = 0;
-  for_edges(L.ell_ptr, L.ell_k, L.ell, i, [&](int e, int role, int j) {
+  for_edges(L, i, [&](int e, int role) {
     const R *f = L.fbuf + 6 * (size_t)e;
     if (role) {
 #pragma unroll
@@ -895,7 +901,6 @@ __device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, R *r) {
 #pragma unroll
       for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] += f[c];
     }
-    (void)j;
   });
   for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
     int v = L.bs_list[k];
@@ -921,7 +926,7 @@ struct StageArgs {
 
 template <typename R, int MODE, bool SA>
 __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   int n = L.n;
   R r[6];
@@ -1009,7 +1014,7 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_copy_state(R *dst, const R *src, int n, int ncomp) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= n) return;
   for (int c = 0; c < ncomp; ++c) dst[(size_t)c * n + i] = src[(size_t)c * n + i];
 }
@@ -1023,7 +1028,7 @@ __global__ void k_fill(R *dst, R v, size_t cnt) {
 // node BC application on a state array (setBCsOnSolutionField + outer)
 template <typename R>
 __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= L.n) return;
   int n = L.n;
   R q[6];
@@ -1038,23 +1043,26 @@ __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
 template <typename RF, typename RC>
 __global__ void k_restrict(const int *cptr, const int *clist, const double *rw, const RF *in, RC *out, int nc,
                            int nf, int ncomp) {
-  int k = gtid();
+  int k = gtid_x();
   if (k >= nc) return;
-  for (int c = 0; c < ncomp; ++c) {
-    double s = 0;
-    for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
-      int i = clist[q];
-      s += rw[i] * double(in[(size_t)c * nf + i]);
-    }
-    out[(size_t)c * nc + k] = RC(s);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
+    int i = clist[q];
+    double r = rw[i];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < ncomp) acc[c] += r * double(in[(size_t)c * nf + i]);
   }
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    if (c < ncomp) out[(size_t)c * nc + k] = RC(acc[c]);
 }
 
 // prolongateGrid (DEF:12432-12570) after the coarse correction rhs_H is
 // formed: inject, relax, add, BCs, clamps on the fine level
 template <typename RF, typename RC>
 __global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, const RC *crhs, int nc, int ncomp_add) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= F.n) return;
   int n = F.n;
   int k = f2c[i];
@@ -1078,7 +1086,7 @@ __global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, cons
 // coarse correction v_H = u_H - BC(R u_h), BC'd as an increment
 template <typename R>
 __global__ void k_coarse_correction(LevelDev<R> K, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= K.n) return;
   int n = K.n;
   R r[6];
@@ -1092,7 +1100,7 @@ __global__ void k_coarse_correction(LevelDev<R> K, DevParams P) {
 // u_H += rhs_H (DEF:12362-12428)
 template <typename R>
 __global__ void k_restrict_finish(LevelDev<R> K, DevParams P) {
-  int i = gtid();
+  int i = gtid_x();
   if (i >= K.n) return;
   int n = K.n;
   R r[6];

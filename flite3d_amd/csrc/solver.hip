@@ -122,6 +122,14 @@ std::vector<int> rcm_order(int n, const std::vector<int> &e1, const std::vector<
 
 }  // namespace
 
+// host copy of one uploaded level view, 0-based, validated
+struct Staged {
+  int n = 0, ne = 0, nbs = 0, nb = 0, nfine = 0;
+  std::vector<int> e1, e2, b1, b2, bi, nia, f2c;
+  std::vector<double> w, wb, vol, xyz, wd, rw;  // w/wb/xyz column-major as in the view
+  int reg[21];
+};
+
 struct Level {
   bool fp32 = false;
   int n = 0, ne = 0, nbs = 0, nb = 0, gnum = 0, nfine = 0;
@@ -136,7 +144,7 @@ struct Level {
   void *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
   int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
-  int2 *ell;
+  int *ell_e, *ell_nbr;
   void *ellw;
   long long nslot = 0;
   // transfer from the next finer level (coarse levels)
@@ -152,7 +160,7 @@ struct Level {
     v.n = n; v.ne = ne; v.nbs = nbs; v.gnum = gnum;
     v.ea = ea; v.eb = eb; v.w = (const R *)w;
     v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
-    v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell = ell; v.ellw = (const R *)ellw; v.nslot = nslot;
+    v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell_e = ell_e; v.ell_nbr = ell_nbr; v.ellw = (const R *)ellw; v.nslot = nslot;
     v.bs_ptr = bs_ptr; v.bs_list = bs_list;
     v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
     v.wd = (const R *)wd; v.nrm = (const R *)nrm; v.ncls = ncls;
@@ -178,6 +186,10 @@ struct flite_ctx {
   int timed_count = 0;
   // residual evaluations per level in the last counted cycle
   std::vector<int> evals;
+  // uploaded level views, built on the device at first use
+  std::vector<Staged> staged;
+  std::vector<char> staged_ok;
+  bool built = false;
 };
 
 namespace {
@@ -292,63 +304,120 @@ int alloc_real(Pool &pool, void **out, size_t count) {
   return 0;
 }
 
-int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
+int stage_level(flite_ctx *c, int level, const flite_level_view *v, Staged &S) {
   if (!v) return fail(FLITE_E_ARG, "null level view");
-  auto Lp = std::make_unique<Level>();
-  Level &L = *Lp;
   const int b = v->index_base;
   if (b != 0 && b != 1) return fail(FLITE_E_ARG, "index_base must be 0 or 1");
-  L.gnum = level;
-  L.fp32 = c->hp.coarse_fp32 && level > 1;
-  L.n = v->n_nodes;
-  L.ne = v->n_sides;
-  L.nbs = v->n_bsides;
-  L.nb = v->n_bnodes;
-  const int n = L.n, ne = L.ne, nbs = L.nbs;
-  if (n <= 0 || ne < 0 || nbs < 0 || L.nb < 0 || L.nb > n) return fail(FLITE_E_ARG, "bad level sizes");
+  S.n = v->n_nodes;
+  S.ne = v->n_sides;
+  S.nbs = v->n_bsides;
+  S.nb = v->n_bnodes;
+  const int n = S.n, ne = S.ne, nbs = S.nbs;
+  if (n <= 0 || ne < 0 || nbs < 0 || S.nb < 0 || S.nb > n) return fail(FLITE_E_ARG, "bad level sizes");
   if (!v->side_index || !v->side_weights || !v->bside_index || !v->bside_weights || !v->node_volume ||
       !v->wall_distance || !v->node_indicator_register || !v->node_indicator_array)
     return fail(FLITE_E_ARG, "missing level array");
+  S.e1.resize(ne);
+  S.e2.resize(ne);
+  for (int e = 0; e < ne; ++e) {
+    S.e1[e] = v->side_index[e] - b;
+    S.e2[e] = v->side_index[(size_t)ne + e] - b;
+    if (S.e1[e] < 0 || S.e1[e] >= n || S.e2[e] < 0 || S.e2[e] >= n) return fail(FLITE_E_ARG, "side index out of range");
+  }
+  S.w.assign(v->side_weights, v->side_weights + 3 * (size_t)ne);
+  S.b1.resize(nbs);
+  S.b2.resize(nbs);
+  S.bi.resize(nbs);
+  for (int s = 0; s < nbs; ++s) {
+    S.b1[s] = v->bside_index[s] - b;
+    S.b2[s] = v->bside_index[(size_t)nbs + s] - b;
+    S.bi[s] = v->bside_index[2 * (size_t)nbs + s];
+    if (S.b1[s] < 0 || S.b1[s] >= n || S.b2[s] < 0 || S.b2[s] >= n)
+      return fail(FLITE_E_ARG, "boundary side index out of range");
+    if (!(S.bi[s] <= 4 || S.bi[s] == 9)) return fail(FLITE_E_UNSUPPORTED, "engine boundary sides are not supported");
+  }
+  S.wb.assign(v->bside_weights, v->bside_weights + 3 * (size_t)nbs);
+  S.vol.assign(v->node_volume, v->node_volume + n);
+  for (int i = 0; i < n; ++i)
+    if (!(S.vol[i] > 0)) return fail(FLITE_E_ARG, "non-positive node volume");
+  S.wd.assign(v->wall_distance, v->wall_distance + n);
+  if (level == 1 && v->coordinates) S.xyz.assign(v->coordinates, v->coordinates + 3 * (size_t)n);
+  for (int k = 0; k < 21; ++k) S.reg[k] = v->node_indicator_register[k];
+  S.nia.resize(S.nb);
+  for (int i = 0; i < S.nb; ++i) {
+    S.nia[i] = v->node_indicator_array[i] - b;
+    if (S.nia[i] < 0 || S.nia[i] >= n) return fail(FLITE_E_ARG, "node indicator array out of range");
+  }
+  for (int k = 1; k <= 10; ++k) {
+    int st = S.reg[2 * k - 2], en = S.reg[2 * k - 1];
+    if (en < st) continue;
+    if (st < 1 || en > S.nb) return fail(FLITE_E_ARG, "node indicator register out of range");
+    if (k >= 5 && k <= 8) return fail(FLITE_E_UNSUPPORTED, "engine boundary nodes are not supported");
+    if (k == 10) return fail(FLITE_E_UNSUPPORTED, "indicator 10 nodes are not supported");
+  }
   if (level > 1) {
     if (!v->prolongation || !v->restriction) return fail(FLITE_E_ARG, "coarse level needs prolongation/restriction");
-    Level &F = *c->lev[level - 2];
-    if (!F.uploaded) return fail(FLITE_E_STATE, "upload the finer level first");
-    if (v->n_fine != F.n) return fail(FLITE_E_ARG, "n_fine does not match the finer level");
-    L.nfine = v->n_fine;
+    S.nfine = v->n_fine;
+    if (S.nfine <= 0) return fail(FLITE_E_ARG, "bad n_fine");
+    S.f2c.resize(S.nfine);
+    for (int i = 0; i < S.nfine; ++i) {
+      S.f2c[i] = v->prolongation[i] - b;
+      if (S.f2c[i] < 0 || S.f2c[i] >= n) return fail(FLITE_E_ARG, "prolongation index out of range");
+    }
+    S.rw.assign(v->restriction, v->restriction + S.nfine);
   }
-  // host copies, 0-based, range-checked
-  std::vector<int> e1(ne), e2(ne);
-  for (int e = 0; e < ne; ++e) {
-    e1[e] = v->side_index[e] - b;
-    e2[e] = v->side_index[(size_t)ne + e] - b;
-    if (e1[e] < 0 || e1[e] >= n || e2[e] < 0 || e2[e] >= n) return fail(FLITE_E_ARG, "side index out of range");
+  return 0;
+}
+
+// Device numbering: RCM on the coarsest level; every finer level lists the
+// members of each coarse node contiguously, in the coarse order, so that
+// restriction and prolongation are contiguous-range operations and each
+// agglomerate's nodes share cache lines.
+std::vector<std::vector<int>> nested_orders(const std::vector<Staged> &st) {
+  int nl = (int)st.size();
+  std::vector<std::vector<int>> h2d(nl);
+  h2d[nl - 1] = rcm_order(st[nl - 1].n, st[nl - 1].e1, st[nl - 1].e2);
+  for (int l = nl - 2; l >= 0; --l) {
+    const Staged &K = st[l + 1];
+    int nc = K.n, nf = st[l].n;
+    std::vector<int> cd2h(nc);
+    for (int h = 0; h < nc; ++h) cd2h[h2d[l + 1][h]] = h;
+    std::vector<int> cnt(nc + 1, 0);
+    for (int i = 0; i < nf; ++i) cnt[K.f2c[i] + 1]++;
+    for (int k = 0; k < nc; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int> members(nf);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < nf; ++i) members[pos[K.f2c[i]]++] = i;
+    h2d[l].assign(nf, 0);
+    int d = 0;
+    for (int dc = 0; dc < nc; ++dc) {
+      int hc = cd2h[dc];
+      for (int q = cnt[hc]; q < cnt[hc + 1]; ++q) h2d[l][members[q]] = d++;
+    }
   }
-  std::vector<int> b1(nbs), b2(nbs), bi(nbs);
-  for (int s = 0; s < nbs; ++s) {
-    b1[s] = v->bside_index[s] - b;
-    b2[s] = v->bside_index[(size_t)nbs + s] - b;
-    bi[s] = v->bside_index[2 * (size_t)nbs + s];
-    if (b1[s] < 0 || b1[s] >= n || b2[s] < 0 || b2[s] >= n) return fail(FLITE_E_ARG, "boundary side index out of range");
-    if (!(bi[s] <= 4 || bi[s] == 9)) return fail(FLITE_E_UNSUPPORTED, "engine boundary sides are not supported");
-  }
-  for (int i = 0; i < n; ++i)
-    if (!(v->node_volume[i] > 0)) return fail(FLITE_E_ARG, "non-positive node volume");
-  // device numbering
-  L.h2d = rcm_order(n, e1, e2);
+  return h2d;
+}
+
+int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int> &h2d) {
+  auto Lp = std::make_unique<Level>();
+  Level &L = *Lp;
+  L.gnum = level;
+  L.fp32 = c->hp.coarse_fp32 && level > 1;
+  L.n = S.n;
+  L.ne = S.ne;
+  L.nbs = S.nbs;
+  L.nb = S.nb;
+  L.nfine = S.nfine;
+  const int n = L.n, ne = L.ne, nbs = L.nbs;
+  L.h2d = h2d;
   L.d2h.assign(n, 0);
   for (int h = 0; h < n; ++h) L.d2h[L.h2d[h]] = h;
   // node classes from nodeIndicatorArray ranges
   std::vector<int> ncls(n, 0);
-  const int *reg = v->node_indicator_register;
   for (int k = 1; k <= 10; ++k) {
-    int st = reg[2 * k - 2], en = reg[2 * k - 1];
-    if (en < st) continue;
-    if (st < 1 || en > L.nb) return fail(FLITE_E_ARG, "node indicator register out of range");
-    if (k >= 5 && k <= 8) return fail(FLITE_E_UNSUPPORTED, "engine boundary nodes are not supported");
-    if (k == 10) return fail(FLITE_E_UNSUPPORTED, "indicator 10 nodes are not supported");
+    int st = S.reg[2 * k - 2], en = S.reg[2 * k - 1];
     for (int ib = st; ib <= en; ++ib) {
-      int h = v->node_indicator_array[ib - 1] - b;
-      if (h < 0 || h >= n) return fail(FLITE_E_ARG, "node indicator array out of range");
+      int h = S.nia[ib - 1];
       ncls[L.h2d[h]] = k;
       if (k == 1) L.wall_dev.push_back(L.h2d[h]);
     }
@@ -359,9 +428,9 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   std::vector<double> nrm_h(3 * (size_t)n, 0.0), warea_h(n, 0.0);
   for (int s = 0; s < nbs; ++s)
     for (int k = 0; k < 3; ++k) {
-      double w = v->bside_weights[(size_t)k * nbs + s];
-      nrm_h[3 * (size_t)b1[s] + k] -= w;
-      nrm_h[3 * (size_t)b2[s] + k] -= w;
+      double w = S.wb[(size_t)k * nbs + s];
+      nrm_h[3 * (size_t)S.b1[s] + k] -= w;
+      nrm_h[3 * (size_t)S.b2[s] + k] -= w;
     }
   for (int i = 0; i < L.nb; ++i) {
     double x = nrm_h[3 * i], y = nrm_h[3 * i + 1], z = nrm_h[3 * i + 2];
@@ -370,16 +439,16 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   }
   if (level == 1)
     for (int s = 0; s < nbs; ++s) {
-      double wx = v->bside_weights[s], wy = v->bside_weights[nbs + s], wz = v->bside_weights[2 * (size_t)nbs + s];
+      double wx = S.wb[s], wy = S.wb[nbs + s], wz = S.wb[2 * (size_t)nbs + s];
       double a = std::sqrt(wx * wx + wy * wy + wz * wz);
-      warea_h[b1[s]] += 4.0 * a;
-      warea_h[b2[s]] += 4.0 * a;
+      warea_h[S.b1[s]] += 4.0 * a;
+      warea_h[S.b2[s]] += 4.0 * a;
     }
   // device edges sorted by their lower device end
   std::vector<int> eorder(ne);
   std::iota(eorder.begin(), eorder.end(), 0);
   std::vector<int> da(ne), db(ne);
-  for (int e = 0; e < ne; ++e) { da[e] = L.h2d[e1[e]]; db[e] = L.h2d[e2[e]]; }
+  for (int e = 0; e < ne; ++e) { da[e] = L.h2d[S.e1[e]]; db[e] = L.h2d[S.e2[e]]; }
   std::sort(eorder.begin(), eorder.end(), [&](int x, int y) {
     int lx = std::min(da[x], db[x]), ly = std::min(da[y], db[y]);
     if (lx != ly) return lx < ly;
@@ -391,7 +460,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
     int e = eorder[k];
     ea[k] = da[e];
     eb[k] = db[e];
-    for (int q = 0; q < 3; ++q) w[3 * (size_t)k + q] = v->side_weights[(size_t)q * ne + e];
+    for (int q = 0; q < 3; ++q) w[3 * (size_t)k + q] = S.w[(size_t)q * ne + e];
   }
   // sliced ELL incidence
   std::vector<int> deg(n, 0);
@@ -405,7 +474,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
     ell_ptr[s + 1] = ell_ptr[s] + K * SLICE;
   }
   const size_t nslot = ell_ptr[nslice] ? ell_ptr[nslice] : 1;
-  std::vector<int2> ell(nslot, make_int2(-1, -1));
+  std::vector<int> ell_e(nslot, -1), ell_nbr(nslot, -1);
   std::vector<double> ellw(3 * nslot, 0.0);
   std::vector<int> fill(n, 0);
   for (int k = 0; k < ne; ++k) {
@@ -413,8 +482,9 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
       int i = r ? eb[k] : ea[k], j = r ? ea[k] : eb[k];
       int s = i / SLICE, lane = i % SLICE;
       size_t slot = ell_ptr[s] + fill[i] * SLICE + lane;
-      ell[slot] = make_int2((k << 1) | r, j);
-      for (int q = 0; q < 3; ++q) ellw[q * nslot + slot] = w[3 * (size_t)k + q];
+      ell_e[slot] = (k << 1) | r;
+      ell_nbr[slot] = j;
+      for (int q = 0; q < 3; ++q) ellw[q * nslot + slot] = r ? -w[3 * (size_t)k + q] : w[3 * (size_t)k + q];
       fill[i]++;
     }
   }
@@ -423,9 +493,9 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   std::vector<int> ba(nbs), bb(nbs);
   std::vector<double> wb(3 * (size_t)nbs);
   for (int s = 0; s < nbs; ++s) {
-    ba[s] = L.h2d[b1[s]];
-    bb[s] = L.h2d[b2[s]];
-    for (int q = 0; q < 3; ++q) wb[3 * (size_t)s + q] = v->bside_weights[(size_t)q * nbs + s];
+    ba[s] = L.h2d[S.b1[s]];
+    bb[s] = L.h2d[S.b2[s]];
+    for (int q = 0; q < 3; ++q) wb[3 * (size_t)s + q] = S.wb[(size_t)q * nbs + s];
   }
   std::vector<int> bs_ptr(n + 1, 0);
   for (int s = 0; s < nbs; ++s) { bs_ptr[ba[s] + 1]++; bs_ptr[bb[s] + 1]++; }
@@ -442,27 +512,28 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   std::vector<double> vol(n), wd(n), nrm(3 * (size_t)n), warea(n), xyz;
   for (int h = 0; h < n; ++h) {
     int d = L.h2d[h];
-    vol[d] = v->node_volume[h];
-    wd[d] = v->wall_distance[h];
+    vol[d] = S.vol[h];
+    wd[d] = S.wd[h];
     warea[d] = warea_h[h];
     for (int k = 0; k < 3; ++k) nrm[3 * (size_t)d + k] = nrm_h[3 * (size_t)h + k];
   }
   L.warea_host = warea;
-  L.has_coords = level == 1 && v->coordinates != nullptr;
+  L.has_coords = level == 1 && !S.xyz.empty();
   if (L.has_coords) {
     xyz.resize(3 * (size_t)n);
     for (int h = 0; h < n; ++h)
-      for (int k = 0; k < 3; ++k) xyz[3 * (size_t)L.h2d[h] + k] = v->coordinates[(size_t)k * n + h];
+      for (int k = 0; k < 3; ++k) xyz[3 * (size_t)L.h2d[h] + k] = S.xyz[(size_t)k * n + h];
   }
   Pool &P = L.pool;
   TRY(P.upload(&L.ea, ea));
   TRY(P.upload(&L.eb, eb));
   TRY(P.upload(&L.ba, ba));
   TRY(P.upload(&L.bb, bb));
-  TRY(P.upload(&L.bind, bi));
+  TRY(P.upload(&L.bind, S.bi));
   TRY(P.upload(&L.ell_ptr, ell_ptr));
   TRY(P.upload(&L.ell_k, ell_k));
-  TRY(P.upload(&L.ell, ell));
+  TRY(P.upload(&L.ell_e, ell_e));
+  TRY(P.upload(&L.ell_nbr, ell_nbr));
   TRY(P.upload(&L.bs_ptr, bs_ptr));
   TRY(P.upload(&L.bs_list, bs_list));
   TRY(P.upload(&L.ncls, ncls));
@@ -514,19 +585,15 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
     std::vector<double> rw(F.n);
     std::vector<int> cnt(n + 1, 0);
     for (int hf = 0; hf < F.n; ++hf) {
-      int hc = v->prolongation[hf] - b;
-      if (hc < 0 || hc >= n) return fail(FLITE_E_ARG, "prolongation index out of range");
+      int hc = S.f2c[hf];
       f2c[F.h2d[hf]] = L.h2d[hc];
-      rw[F.h2d[hf]] = v->restriction[hf];
+      rw[F.h2d[hf]] = S.rw[hf];
       cnt[L.h2d[hc] + 1]++;
     }
     for (int k = 0; k < n; ++k) cnt[k + 1] += cnt[k];
     std::vector<int> clist(F.n);
     std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-    for (int hf = 0; hf < F.n; ++hf) {  // fine members in host order
-      int dc = L.h2d[v->prolongation[hf] - b];
-      clist[pos[dc]++] = F.h2d[hf];
-    }
+    for (int df = 0; df < F.n; ++df) clist[pos[f2c[df]]++] = df;  // fine members in device order
     TRY(P.upload(&L.f2c, f2c));
     TRY(P.upload(&L.rw, rw));
     TRY(P.upload(&L.cptr, cnt));
@@ -534,6 +601,32 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   }
   L.uploaded = true;
   c->lev[level - 1] = std::move(Lp);
+  return 0;
+}
+
+int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
+  if (level > 1) {
+    if (c->staged.size() < (size_t)level - 1 || !c->staged_ok[level - 2])
+      return fail(FLITE_E_STATE, "upload the finer level first");
+    if (v && v->n_fine != c->staged[level - 2].n) return fail(FLITE_E_ARG, "n_fine does not match the finer level");
+  }
+  Staged S;
+  TRY(stage_level(c, level, v, S));
+  c->staged[level - 1] = std::move(S);
+  c->staged_ok[level - 1] = 1;
+  c->built = false;
+  for (auto &l : c->lev) l.reset();
+  return 0;
+}
+
+// builds the device levels once every level is staged
+int ensure_built(flite_ctx *c) {
+  if (c->built) return 0;
+  for (int l = 0; l < c->nl; ++l)
+    if (!c->staged_ok[l]) return fail(FLITE_E_STATE, "not all levels uploaded");
+  auto orders = nested_orders(c->staged);
+  for (int l = 0; l < c->nl; ++l) TRY(build_level(c, l + 1, c->staged[l], orders[l]));
+  c->built = true;
   return 0;
 }
 
@@ -859,15 +952,14 @@ int do_cycle(flite_ctx *c, int cycle) {
 
 int check_ctx(flite_ctx *c) {
   if (!c) return fail(FLITE_E_ARG, "null context");
-  for (int l = 0; l < c->nl; ++l)
-    if (!c->lev[l] || !c->lev[l]->uploaded) return fail(FLITE_E_STATE, "not all levels uploaded");
+  TRY(ensure_built(c));
   return 0;
 }
 
 int check_level(flite_ctx *c, int level) {
   if (!c) return fail(FLITE_E_ARG, "null context");
   if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
-  if (!c->lev[level - 1] || !c->lev[level - 1]->uploaded) return fail(FLITE_E_STATE, "level not uploaded");
+  TRY(ensure_built(c));
   return 0;
 }
 
@@ -925,6 +1017,8 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   c->nl = n_levels;
   c->nbatch = n_batch;
   c->lev.resize(n_levels);
+  c->staged.resize(n_levels);
+  c->staged_ok.assign(n_levels, 0);
   HIPTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   *out = c.release();
   return 0;
@@ -969,9 +1063,9 @@ int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
     for (int hh = 0; hh < L.n; ++hh) {
       const double *g = &h[GREC * (size_t)L.h2d[hh]];
       for (int k = 0; k < 15; ++k) out[(size_t)k * ld + hh] = g[k];
-      for (int k = 0; k < 5; ++k) {
-        out[(size_t)(15 + k) * ld + hh] = g[28 + k];
-        out[(size_t)(20 + k) * ld + hh] = g[33 + k];
+      for (int k = 0; k < 5; ++k) {  // limits from their stored reciprocals
+        out[(size_t)(15 + k) * ld + hh] = g[15 + k] != 0.0 ? 1.0 / g[15 + k] : 0.0;
+        out[(size_t)(20 + k) * ld + hh] = g[20 + k] != 0.0 ? 1.0 / g[20 + k] : 0.0;
       }
     }
     return 0;
