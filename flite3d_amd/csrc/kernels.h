@@ -36,6 +36,7 @@ struct DevParams {
   double inflow_temperature;
   int isentropic, sa, order, autoclip, nsig, bterm;
   double clip, max_inc, min_rho, max_turb, turb_k;
+  double beta0;  // Davis wave-speed factor sqrt((gamma - 1) / (2 gamma))
   double prolong_relax, turb_prolong_relax, mg_bc_relax;
 };
 
@@ -211,22 +212,18 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   out[25] = 0;
 }
 
-// LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds
-template <typename R>
-__device__ __forceinline__ R led(R a, R ilo, R ihi, int nsig) {
+// LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds.
+// Branch-free: the bound is selected by the sign of a, so a wavefront runs
+// the root and the division once whatever the signs of its lanes.
+template <int NSIG, typename R>
+__device__ __forceinline__ R led(R a, R ilo, R ihi) {
   R a1 = R(2) * a;
-  if (a1 > R(0)) {
-    if (ihi == R(0)) return R(0);
-    R t = a1 * ihi, t2 = t * t;
-    R den = nsig == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
-    return R(0.5) * a1 / den;
-  } else if (a1 < R(0)) {
-    if (ilo == R(0)) return R(0);
-    R t = a1 * ilo, t2 = t * t;
-    R den = nsig == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
-    return R(0.5) * a1 / den;
-  }
-  return a;
+  R iv = a1 > R(0) ? ihi : ilo;
+  R t = a1 * iv, t2 = t * t;
+  R den = NSIG == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
+  R r = R(0.5) * a1 / den;
+  r = iv == R(0) ? R(0) : r;
+  return (a1 > R(0) || a1 < R(0)) ? r : a;  // a == 0 (or NaN) passes through
 }
 
 // ---------------------------------------------------------------------------
@@ -234,7 +231,7 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi, int nsig) {
 // fbuf[NF*e + c]: flux leaving ea towards eb (+ to ea, - to eb);
 // component 5 is the SA convection (makeSARHS, DEF:6873-6880).
 // ---------------------------------------------------------------------------
-template <typename R, int ORDER, bool SA>
+template <typename R, int ORDER, bool SA, int NSIG>
 __global__ void k_flux(LevelDev<R> L, DevParams P) {
   int e = gtid_x();
   if (e >= L.ne) return;
@@ -282,8 +279,8 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
       R a = R(0.5) * (G1[j] * dx + G1[j + 5] * dy + G1[j + 10] * dz);
       R b = R(0.5) * (G2[j] * dx + G2[j + 5] * dy + G2[j + 10] * dz);
       if (P.autoclip == 1) {
-        a = led(a, G1[15 + j], G1[20 + j], P.nsig);
-        b = -led(-b, G2[15 + j], G2[20 + j], P.nsig);
+        a = led<NSIG>(a, G1[15 + j], G1[20 + j]);
+        b = -led<NSIG>(-b, G2[15 + j], G2[20 + j]);
       }
       R c;
       if (a * b <= R(0)) {
@@ -312,7 +309,7 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
   R rinv1 = R(1) / roe1, rinv2 = R(1) / roe2;
   R nv1 = u1 * rnx + v1 * rny + w1 * rnz;
   R nv2 = u2 * rnx + v2 * rny + w2 * rnz;
-  R beta0 = sqrt(gm1 / (R(2) * gam));
+  R beta0 = R(P.beta0);
   R rroe = sqrt(roe2 * rinv1);
   R iro = R(1) / (R(1) + rroe);
   R ut = (u1 + u2 * rroe) * iro, vt = (v1 + v2 * rroe) * iro, wt = (w1 + w2 * rroe) * iro;
@@ -327,62 +324,45 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
   R fact = roe2 * (sr - nv2) - roe1 * (sl - nv1);
   if (fact != R(0)) fact = R(1) / fact;
   sm = fact * sm;
-  R f[5];
-  if (sl > R(0) || (sl <= R(0) && sm > R(0))) {
-    // left flux (+ star correction)
-    f[0] = rnx * ru1 + rny * rv1 + rnz * rw1;
-    f[1] = rnx * (ru1 * u1 + pr1) + rny * (ru1 * v1) + rnz * (ru1 * w1);
-    f[2] = rnx * (rv1 * u1) + rny * (rv1 * v1 + pr1) + rnz * (rv1 * w1);
-    f[3] = rnx * (rw1 * u1) + rny * (rw1 * v1) + rnz * (rw1 * w1 + pr1);
-    f[4] = rnx * (rh1 * u1) + rny * (rh1 * v1) + rnz * (rh1 * w1);
-    if (!(sl > R(0))) {
-      R fl = sl - sm;
-      if (fl != R(0)) {
-        fl = R(1) / fl;
-        R a = sl - nv1;
-        R pls = roe1 * (nv1 - sl) * (nv1 - sm) + pr1;
-        R ps = pls - pr1;
-        R s0 = roe1 * a * fl, s1 = (a * roe1 * u1 + ps * rnx) * fl;
-        R s2 = (a * roe1 * v1 + ps * rny) * fl, s3 = (a * roe1 * w1 + ps * rnz) * fl;
-        R s4 = (a * re1 - pr1 * nv1 + pls * sm) * fl;
-        f[0] += sl * (s0 - roe1);
-        f[1] += sl * (s1 - ru1);
-        f[2] += sl * (s2 - rv1);
-        f[3] += sl * (s3 - rw1);
-        f[4] += sl * (s4 - re1);
-      }
-    }
-  } else if ((sm <= R(0) && sr >= R(0)) || sr < R(0)) {
-    f[0] = rnx * ru2 + rny * rv2 + rnz * rw2;
-    f[1] = rnx * (ru2 * u2 + pr2) + rny * (ru2 * v2) + rnz * (ru2 * w2);
-    f[2] = rnx * (rv2 * u2) + rny * (rv2 * v2 + pr2) + rnz * (rv2 * w2);
-    f[3] = rnx * (rw2 * u2) + rny * (rw2 * v2) + rnz * (rw2 * w2 + pr2);
-    f[4] = rnx * (rh2 * u2) + rny * (rh2 * v2) + rnz * (rh2 * w2);
-    if (!(sr < R(0))) {
-      R fr = sr - sm;
-      if (fr != R(0)) {
-        fr = R(1) / fr;
-        R a = sr - nv2;
-        R prs = roe2 * (nv2 - sr) * (nv2 - sm) + pr2;
-        R ps = prs - pr2;
-        R s0 = roe2 * a * fr, s1 = (a * roe2 * u2 + ps * rnx) * fr;
-        R s2 = (a * roe2 * v2 + ps * rny) * fr, s3 = (a * roe2 * w2 + ps * rnz) * fr;
-        R s4 = (a * re2 - pr2 * nv2 + prs * sm) * fr;
-        f[0] += sr * (s0 - roe2);
-        f[1] += sr * (s1 - ru2);
-        f[2] += sr * (s2 - rv2);
-        f[3] += sr * (s3 - rw2);
-        f[4] += sr * (s4 - re2);
-      }
-    }
-  } else {
-    R *o = L.fbuf + NF * (size_t)e;
+  // upwind side K: left when sl > 0 or (sl <= 0 < sm), right when
+  // sm <= 0 <= sr or sr < 0; neither only for NaN wave speeds, where the
+  // central flux is kept.  Both sides share one code path (selected
+  // operands), so a wavefront evaluates the star state once.
+  bool lft = sl > R(0) || (sl <= R(0) && sm > R(0));
+  bool rgt = (sm <= R(0) && sr >= R(0)) || sr < R(0);
+  R *o = L.fbuf + NF * (size_t)e;
+  if (!lft && !rgt) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = fc[c];
     o[5] = fsa;
     return;
   }
-  R *o = L.fbuf + NF * (size_t)e;
+  R rk = lft ? roe1 : roe2, uk = lft ? u1 : u2, vk = lft ? v1 : v2, wk = lft ? w1 : w2;
+  R ek = lft ? re1 : re2, pk = lft ? pr1 : pr2, nvk = lft ? nv1 : nv2;
+  R ruk = lft ? ru1 : ru2, rvk = lft ? rv1 : rv2, rwk = lft ? rw1 : rw2, rhk = lft ? rh1 : rh2;
+  R sk = lft ? sl : sr;
+  bool star = lft ? !(sl > R(0)) : !(sr < R(0));
+  R f[5];
+  f[0] = rnx * ruk + rny * rvk + rnz * rwk;
+  f[1] = rnx * (ruk * uk + pk) + rny * (ruk * vk) + rnz * (ruk * wk);
+  f[2] = rnx * (rvk * uk) + rny * (rvk * vk + pk) + rnz * (rvk * wk);
+  f[3] = rnx * (rwk * uk) + rny * (rwk * vk) + rnz * (rwk * wk + pk);
+  f[4] = rnx * (rhk * uk) + rny * (rhk * vk) + rnz * (rhk * wk);
+  R fk = sk - sm;
+  if (star && fk != R(0)) {
+    fk = R(1) / fk;
+    R a = sk - nvk;
+    R pks = rk * (nvk - sk) * (nvk - sm) + pk;
+    R ps = pks - pk;
+    R s0 = rk * a * fk, s1 = (a * rk * uk + ps * rnx) * fk;
+    R s2 = (a * rk * vk + ps * rny) * fk, s3 = (a * rk * wk + ps * rnz) * fk;
+    R s4 = (a * ek - pk * nvk + pks * sm) * fk;
+    f[0] += sk * (s0 - rk);
+    f[1] += sk * (s1 - ruk);
+    f[2] += sk * (s2 - rvk);
+    f[3] += sk * (s3 - rwk);
+    f[4] += sk * (s4 - ek);
+  }
 #pragma unroll
   for (int c = 0; c < 5; ++c) o[c] = f[c] * rnorm1;
   o[5] = fsa;

@@ -278,6 +278,7 @@ DevParams make_dev_params(const flite_params &p) {
   P.min_rho = p.minimum_density;
   P.max_turb = p.max_turbulence_value;
   P.turb_k = p.turb_k;
+  P.beta0 = std::sqrt((p.gamma - 1.0) / (2.0 * p.gamma));
   P.prolong_relax = p.prolongation_relaxation;
   P.turb_prolong_relax = p.turb_prolongation_relaxation;
   P.mg_bc_relax = p.multigrid_bc_relaxation;
@@ -666,10 +667,19 @@ int op_flux(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (L.ne > 0) {
       Timed tm(c, "flux", L.gnum);
-      if (o2 && P.sa) hipLaunchKernelGGL((k_flux<R, 2, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
-      else if (o2) hipLaunchKernelGGL((k_flux<R, 2, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
-      else if (P.sa) hipLaunchKernelGGL((k_flux<R, 1, true>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
-      else hipLaunchKernelGGL((k_flux<R, 1, false>), dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P);
+      dim3 g(nblk(L.ne)), b(NTHR);
+      bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
+      if (o2 && P.sa) {
+        if (n4) hipLaunchKernelGGL((k_flux<R, 2, true, 4>), g, b, 0, c->stream, V, P);
+        else hipLaunchKernelGGL((k_flux<R, 2, true, 2>), g, b, 0, c->stream, V, P);
+      } else if (o2) {
+        if (n4) hipLaunchKernelGGL((k_flux<R, 2, false, 4>), g, b, 0, c->stream, V, P);
+        else hipLaunchKernelGGL((k_flux<R, 2, false, 2>), g, b, 0, c->stream, V, P);
+      } else if (P.sa) {
+        hipLaunchKernelGGL((k_flux<R, 1, true, 2>), g, b, 0, c->stream, V, P);
+      } else {
+        hipLaunchKernelGGL((k_flux<R, 1, false, 2>), g, b, 0, c->stream, V, P);
+      }
     }
     if (L.nbs > 0) {
       Timed tm(c, "bflux", L.gnum);
