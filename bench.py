@@ -85,6 +85,36 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1"):
                 res0=r0, res=res[0], forces=m["forces"][:4].tolist())
 
 
+def run_converge(levels, fp32, drop, cap_s, every=250):
+    """Cycles the solver from freestream until the monitor residual (L2 of
+    the density change over the last fine smoothing step, relative to cycle
+    1) falls by `drop` decades or `cap_s` seconds pass; returns the history."""
+    from flite3d_amd import device
+    s = device.Solver(params(fp32), levels)
+    s.init_freestream()
+    hist = []
+    r0 = None
+    c = 1
+    s.sync()
+    t0 = time.perf_counter()
+    while True:
+        s.cycle(c)
+        m = s.monitor()
+        r0 = r0 or m["res"][0]
+        d = float(np.log10(m["res"][0] / r0)) if m["res"][0] > 0 else -99.0
+        el = time.perf_counter() - t0
+        if c % every == 0 or d <= -drop or el > cap_s or not np.isfinite(d):
+            hist.append((c, round(el, 3), round(d, 4), float(m["forces"][1]), float(m["forces"][0])))
+            log(f"[converge] {'fp32-coarse' if fp32 else 'fp64'} cycle {c} t={el:.1f}s "
+                f"drop={d:.3f} drag={m['forces'][1]:.12e}")
+        if d <= -drop or el > cap_s or not np.isfinite(d):
+            break
+        c += 1
+    s.close()
+    return dict(cycles=c, time=el, drop=d, reached=d <= -drop, drag=hist[-1][3],
+                lift=hist[-1][4], history=hist)
+
+
 def cpu_baseline(levels, budget_s=30.0):
     """The FP64 CPU restatement (oracle, one core) on one cycle of the same
     workload, or fewer fine-level smoothing steps if a cycle exceeds the
@@ -111,6 +141,9 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the timed steps: run each precision mode to a 1e-8 "
+                         "monitor drop or this many seconds, and report the histories")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -118,6 +151,14 @@ def main():
         sys.exit(2)
     levels = build_case(a.n, a.levels)
     sides = [l.n_sides for l in levels]
+    if a.converge > 0:
+        out = {}
+        for tag, fp32 in (("mixed", True), ("fp64", False)):
+            out[tag] = run_converge(levels, fp32, 8.0, a.converge)
+        out["drag_mixed_minus_fp64"] = out["mixed"]["drag"] - out["fp64"]["drag"]
+        out["workload"] = f"RANS-SA bump box {a.n}^3 hexes, {a.levels} MG levels, growing-V"
+        print(json.dumps(out), flush=True)
+        return
     results = {}
     for tag, fp32 in ((("fp64", False),) if a.fp64_only else (("mixed", True), ("fp64", False))):
         log(f"[bench] GPU run {tag}: warmup {a.warmup}, steps {a.steps}")

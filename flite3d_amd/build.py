@@ -2,8 +2,10 @@
 
 * ``libflite3d_prepro.so`` -- host C++ grid preprocessor (g++).
 * ``libflite3d_amd.so``    -- HIP kernels for gfx950 plus the C-ABI (hipcc).
+* ``fortran/_build/mgnsg3d_gpu`` -- the Fortran host driver over the C-ABI
+  (flang), with its ISO_C_BINDING modules.
 
-Both land next to this file so that they travel with the source tree.
+All land under this directory so that they travel with the source tree.
 """
 import os
 import subprocess
@@ -53,9 +55,33 @@ def build_device():
     return DEVICE_SO
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN, "_build")
+FORTRAN_EXE = os.path.join(FORTRAN_BUILD, "mgnsg3d_gpu")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+# module order: each file uses only the ones before it
+FORTRAN_SOURCES = ["flite_gpu.f90", "flite_input.f90", "flite_gridio.f90", "mgnsg3d_gpu.f90"]
+
+
+def build_fortran():
+    """Compile the Fortran host (modules + driver) and link it against
+    libflite3d_amd.so, found at run time next to the package ($ORIGIN)."""
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    objs = []
+    for f in FORTRAN_SOURCES:
+        o = os.path.join(FORTRAN_BUILD, f.replace(".f90", ".o"))
+        _run([FLANG, "-O2", "-c", "-module-dir", FORTRAN_BUILD, "-I", FORTRAN_BUILD,
+              os.path.join(FORTRAN, f), "-o", o])
+        objs.append(o)
+    _run([FLANG, "-O2"] + objs + ["-o", FORTRAN_EXE, "-L", HERE, "-lflite3d_amd",
+                                  "-Wl,-rpath,$ORIGIN/../.."])
+    return FORTRAN_EXE
+
+
 def build_all():
     build_prepro()
     build_device()
+    build_fortran()
 
 
 if __name__ == "__main__":

@@ -124,6 +124,7 @@ std::vector<int> rcm_order(int n, const std::vector<int> &e1, const std::vector<
 
 // host copy of one uploaded level view, 0-based, validated
 struct Staged {
+  int base = 0;  // index_base of the uploaded view
   int n = 0, ne = 0, nbs = 0, nb = 0, nfine = 0;
   std::vector<int> e1, e2, b1, b2, bi, nia, f2c;
   std::vector<double> w, wb, vol, xyz, wd, rw;  // w/wb/xyz column-major as in the view
@@ -309,6 +310,7 @@ int stage_level(flite_ctx *c, int level, const flite_level_view *v, Staged &S) {
   if (!v) return fail(FLITE_E_ARG, "null level view");
   const int b = v->index_base;
   if (b != 0 && b != 1) return fail(FLITE_E_ARG, "index_base must be 0 or 1");
+  S.base = b;
   S.n = v->n_nodes;
   S.ne = v->n_sides;
   S.nbs = v->n_bsides;
@@ -1286,7 +1288,7 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
     }
     res[j] = std::sqrt(acc);
     maxres[j] = mx;
-    maxidx[j] = mx > 0 ? mi : 0;
+    maxidx[j] = (mx > 0 ? mi : 0) + c->staged[0].base;  // reference starts at node 1
   }
   double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int k = 0; k < nw; ++k)

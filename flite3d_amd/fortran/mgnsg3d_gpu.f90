@@ -1,0 +1,172 @@
+! mgnsg3d_gpu -- steady multigrid driver whose grid work runs on the GPU.
+!
+! The control flow, inputs and outputs are those of the reference driver
+! (SolverSource/mgnsg3d_mg_par.F90): run.inp names the case (MG:107-110),
+! <case>.inp holds the &inputVariables namelist (DEF:201-422), <case>.sol_1
+! the preprocessed grids (MG:1198-1268), <case>.rst_1 an optional start-up
+! field (restartNumber > 0, MG:247-260); multiGridCycles' loop (MG:606-800)
+! runs oneCycle either numberOfMGIterations times or, when that is negative,
+! until the monitor residual has fallen by that many decades; every cycle
+! appends a line to <case>.rsd (writeResiduals, MG:1478-1514), every
+! writeToFileInterval cycles and at the end the solution goes to <case>.res_1
+! (writeResultsToFile, MG:1518-1531), and liftdrag.res gets the final forces
+! (MG:847-880).  Only single-domain runs exist here: the MPI partitioning
+! collapses to one domain on one GPU.
+!
+! Usage: mgnsg3d_gpu [--coarse-fp32]
+!   --coarse-fp32  smooth and transfer on the coarse levels in FP32
+!                  (the fine level, the solution update and the forces stay FP64)
+program mgnsg3d_gpu
+  use, intrinsic :: iso_c_binding
+  use flite_gpu
+  use flite_input
+  use flite_gridio
+  implicit none
+
+  type(InputVariablesData) :: ivd
+  type(GridLevelData), allocatable, target :: grp(:)
+  type(flite_params) :: p
+  type(flite_level_view) :: v
+  character(len=256) :: fileName, arg
+  integer :: numberOfGrids, i, l, nvar, nres, k
+  logical :: coarseFp32
+  real(c_double), allocatable :: u(:, :)
+  real(c_double) :: res(7), maxres(7), forces(9)
+  integer(c_int) :: maxind(7)
+  real(c_double) :: initialResidual, currentResidual, stopAtResidualReduction, currentRes
+  real(c_double) :: averageTimePerCycle
+  integer(c_int64_t) :: t0, t1, rate
+
+  coarseFp32 = .false.
+  do k = 1, command_argument_count()
+     call get_command_argument(k, arg)
+     if (trim(arg) == "--coarse-fp32") then
+        coarseFp32 = .true.
+     else
+        write (*, '(A)') "usage: mgnsg3d_gpu [--coarse-fp32]"
+        error stop "ERROR: unknown argument"
+     end if
+  end do
+
+  open (1, file='run.inp', status='old')
+  read (1, "(a)") fileName
+  close (1)
+  fileName = adjustl(fileName)
+
+  open (21, file=trim(fileName)//".inp", form='formatted', status='old')
+  call readInputVariables(ivd, 21)
+  close (21)
+  call check_supported(ivd)
+  nvar = 5
+  if (ivd%turbulenceModel == 1) nvar = 6
+  nres = nvar
+
+  write (*, '(A)') "reading "//trim(fileName)//".sol_1 ..."
+  open (20, file=trim(fileName)//".sol_1", form='unformatted', status='old', access='sequential')
+  call readComputationData(20, ivd%numberOfGridsToUse, grp, numberOfGrids)
+  close (20)
+  write (*, '(A,I3,A,I10,A,I11,A)') "Number of grids:", numberOfGrids, "  fine grid:", &
+       grp(1)%numberOfNodes, " nodes,", grp(1)%numberOfSides, " sides"
+
+  p = to_flite_params(ivd, coarseFp32)
+  call flite_check(flite_ctx_create(p, numberOfGrids, 1, gpu_ctx), "flite_ctx_create")
+  do l = 1, numberOfGrids
+     v = level_view(grp(l), l == 1)
+     call flite_check(flite_level_upload(gpu_ctx, l, v), "readGridComputationData")
+  end do
+
+  allocate (u(grp(1)%numberOfNodes, nvar))
+  if (ivd%restartNumber > 0) then
+     open (22, file=trim(fileName)//".rst_1", form='unformatted', status='old')
+     call readStartupField(22, u, grp(1)%numberOfNodes, nvar)
+     close (22)
+     call flite_check(flite_state_set(gpu_ctx, 1, u, grp(1)%numberOfNodes, nvar, 0), "setUpInitialField")
+     call flite_check(flite_init_from_state(gpu_ctx), "setUpInitialField")
+  else
+     call flite_check(flite_init_freestream(gpu_ctx), "setUpInitialField")
+  end if
+
+  initialResidual = ivd%initialResidual
+  currentResidual = 10000.0_c_double
+  stopAtResidualReduction = -1.0_c_double
+  if (ivd%numberOfMGIterations < 0) stopAtResidualReduction = abs(dble(ivd%numberOfMGIterations))
+  averageTimePerCycle = 0.0_c_double
+  call system_clock(count_rate=rate)
+
+  write (*, '(A)') "Starting timestepping..."
+  i = 1
+  do
+     if (stopAtResidualReduction < 0.0_c_double) then
+        if (i > ivd%numberOfMGIterations) exit
+     else
+        if (.not. (log(initialResidual) - log(currentResidual) < log(10.0_c_double)*stopAtResidualReduction)) exit
+     end if
+     call system_clock(t0)
+     call gpu_oneCycle(i)
+     call flite_check(flite_sync(gpu_ctx), "oneCycle")
+     call system_clock(t1)
+     averageTimePerCycle = averageTimePerCycle + dble(t1 - t0)/dble(rate)
+
+     call writeResiduals(i)
+     if (mod(i, ivd%writeToFileInterval) == 0) call writeResultsToFile()
+     i = i + 1
+     if (stopAtResidualReduction >= 0.0_c_double .and. i > ivd%maximumnumberofcycles) exit
+  end do
+  call writeResultsToFile()
+
+  call gpu_monitor(res, maxres, maxind, forces)
+  open (28, file='liftdrag.res', form='formatted', status='unknown')
+  write (28, '(7E14.5)') ivd%physicalTimeStep, forces(1:6)
+  close (28)
+  if (i > 1) write (*, *) "Average time per cycle: ", averageTimePerCycle/dble(i - 1)
+
+  call flite_ctx_destroy(gpu_ctx)
+
+contains
+
+  ! writeResiduals (MG:1271-1516): screen line, NaN check, .rsd line
+  subroutine writeResiduals(iterationNumber)
+    integer, intent(in) :: iterationNumber
+    integer :: j
+    call gpu_monitor(res, maxres, maxind, forces)
+    currentResidual = res(1)
+    if (initialResidual == 10000.0_c_double) then
+       initialResidual = res(1)
+       write (*, *) ' Initial Residual is : ', initialResidual
+    end if
+    write (*, 501) iterationNumber, res(1), " (", maxres(1), ") ", res(2), " (", maxres(2), ") ", &
+         res(3), " (", maxres(3), ") ", res(4), " (", maxres(4), ") ", res(5), " (", maxres(5), ") ", &
+         forces(1), forces(2)
+    do j = 1, nres
+       if (isnan(res(j))) then
+          write (*, *) "---> NAN in residual, exiting now ...."
+          write (*, *) "---> stop in sub writeResiduals"
+          stop
+       end if
+    end do
+    if (res(1)/initialResidual > 1.0e-15_c_double) then
+       currentRes = log(res(1)/initialResidual)
+    else
+       currentRes = -100.0_c_double
+    end if
+    if (iterationNumber == 1) then
+       open (25, file=trim(fileName)//".rsd", form='formatted', status='unknown')
+    else
+       open (25, file=trim(fileName)//".rsd", form='formatted', status='unknown', position='append')
+    end if
+    write (25, 504) iterationNumber, currentRes/log(10.0_c_double), forces(1:6), &
+         (log(res(j))/log(10.0_c_double), j=2, nres)
+    close (25)
+501 format(I7, E14.5, A, E8.2, A, E14.5, A, E8.2, A, E14.5, A, E8.2, A, E14.5, A, E8.2, A, E14.5, A, E8.2, A, &
+         E12.4, E12.4)
+504 format(I7, E14.5, E14.5, E14.5, 12(f13.6, 1x))
+  end subroutine writeResiduals
+
+  subroutine writeResultsToFile()
+    call flite_check(flite_state_get(gpu_ctx, 1, u, grp(1)%numberOfNodes, nvar, 0), "writeResultsToFile")
+    open (26, file=trim(fileName)//".res_1", form='unformatted', status='replace')
+    call writeGridResults(26, u, grp(1)%numberOfNodes, nvar)
+    close (26)
+  end subroutine writeResultsToFile
+
+end program mgnsg3d_gpu
