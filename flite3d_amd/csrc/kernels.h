@@ -85,12 +85,119 @@ __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
-// iterate the incidence of node i: f(edge, role)
-template <typename R, typename F>
-__device__ __forceinline__ void for_edges(const LevelDev<R> &L, int i, F f) {
+// Forward-mode dual number: a value and one directional derivative.  The
+// residual's side and boundary fluxes are templated on their arithmetic
+// type; evaluated on Dual they give the exact derivative along the seeded
+// direction (the tangent), and seeded along each input in turn the rows of
+// the transposed Jacobian products (the adjoint).  Comparisons and
+// selections act on the value, so the derivative is that of the branch the
+// value takes.  Kept in its own namespace so that its sqrt/fabs/fmin/fmax
+// are found by argument lookup only and never hide the double versions.
+namespace ad {
+struct Dual {
+  double v, d;
+  __host__ __device__ Dual() : v(0), d(0) {}
+  __host__ __device__ Dual(double x) : v(x), d(0) {}
+  __host__ __device__ Dual(double x, double y) : v(x), d(y) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ Dual &operator+=(Dual &a, Dual b) { return a = a + b; }
+__device__ __forceinline__ Dual &operator-=(Dual &a, Dual b) { return a = a - b; }
+__device__ __forceinline__ Dual &operator*=(Dual &a, Dual b) { return a = a * b; }
+__device__ __forceinline__ bool operator<(Dual a, Dual b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(Dual a, Dual b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+__device__ __forceinline__ bool operator==(Dual a, Dual b) { return a.v == b.v; }
+__device__ __forceinline__ bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+__device__ __forceinline__ Dual sqrt(Dual a) {
+  double r = ::sqrt(a.v);
+  return Dual(r, r > 0.0 ? a.d / (2.0 * r) : 0.0);
+}
+__device__ __forceinline__ Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+__device__ __forceinline__ Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
+__device__ __forceinline__ Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
+}  // namespace ad
+
+// N consecutive values of a gather record in 16-byte loads (8-byte loads for
+// FP32 records whose stride is not a multiple of 16 bytes): fewer memory
+// instructions per lane for the scattered record reads.  p must be aligned
+// to 16 bytes (FP64, or FP32 with N % 4 == 0) or 8 bytes (other FP32).
+template <int N, typename R>
+__device__ __forceinline__ void ldrec(const R *p, R *o) {
+  if constexpr (sizeof(R) == 8) {
+    static_assert(N % 2 == 0, "FP64 records load in pairs");
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      double2 v = q[k];
+      o[2 * k] = v.x;
+      o[2 * k + 1] = v.y;
+    }
+  } else if constexpr (N % 4 == 0) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int k = 0; k < N / 4; ++k) {
+      float4 v = q[k];
+      o[4 * k] = v.x;
+      o[4 * k + 1] = v.y;
+      o[4 * k + 2] = v.z;
+      o[4 * k + 3] = v.w;
+    }
+  } else {
+    static_assert(N % 2 == 0, "FP32 records load in pairs");
+    const float2 *q = reinterpret_cast<const float2 *>(p);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      float2 v = q[k];
+      o[2 * k] = v.x;
+      o[2 * k + 1] = v.y;
+    }
+  }
+}
+
+// Lanes per node in the gather kernels.  SUB consecutive lanes of a
+// wavefront share node i = t / SUB and split its incidence lists
+// round-robin (lane `sub` takes slots sub, sub+SUB, ...); their partial
+// results are combined by a fixed xor butterfly, so every lane of the group
+// ends with the same bits whatever the scheduling.  SUB = 1 on large levels;
+// on small coarse levels SUB > 1 puts more lanes on the chip and shortens
+// each lane's chain of dependent gathers.
+template <int SUB>
+__device__ __forceinline__ void node_lanes(int &i, int &sub) {
+  int t = gtid_x();
+  i = t / SUB;
+  sub = t % SUB;
+}
+template <int SUB, typename T>
+__device__ __forceinline__ T lsum(T v) {
+#pragma unroll
+  for (int m = 1; m < SUB; m <<= 1) v += __shfl_xor(v, m, SUB);
+  return v;
+}
+template <int SUB, typename T>
+__device__ __forceinline__ T lmin(T v) {
+#pragma unroll
+  for (int m = 1; m < SUB; m <<= 1) {
+    T o = __shfl_xor(v, m, SUB);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// iterate this lane's share of the incidence of node i: f(edge, role)
+template <int SUB, typename R, typename F>
+__device__ __forceinline__ void for_edges(const LevelDev<R> &L, int i, int sub, F f) {
   int s = i / SLICE, lane = i % SLICE;
   int base = L.ell_ptr[s], K = L.ell_k[s];
-  for (int k = 0; k < K; ++k) {
+  for (int k = sub; k < K; k += SUB) {
     int v = L.ell_e[base + k * SLICE + lane];
     if (v < 0) continue;
     f(v >> 1, v & 1);
@@ -101,12 +208,12 @@ __device__ __forceinline__ void for_edges(const LevelDev<R> &L, int i, F f) {
 // side's first end, -w when it is the second): f(nbr, wx, wy, wz).  A
 // reference scatter "+f to i1, -f to i2" with f linear in w becomes "+f(w_s)"
 // here, with the same rounding since the sign flip is exact.
-template <typename R, typename F>
-__device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, F f) {
+template <int SUB, typename R, typename F>
+__device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, int sub, F f) {
   int s = i / SLICE, lane = i % SLICE;
   int base = L.ell_ptr[s], K = L.ell_k[s];
   const long long ns = L.nslot;
-  for (int k = 0; k < K; ++k) {
+  for (int k = sub; k < K; k += SUB) {
     int slot = base + k * SLICE + lane;
     int j = L.ell_nbr[slot];
     if (j < 0) continue;
@@ -158,9 +265,10 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 // computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459), gathered
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, int SUB>
 __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
+  int i, sub;
+  node_lanes<SUB>(i, sub);
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
@@ -169,8 +277,9 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 15; ++c) g[c] = 0;
   for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
   bool any = false;
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
-    const R *qj = L.rec0 + 8 * (size_t)j;
+  for_edges_w<SUB>(L, i, sub, [&](int j, R wx, R wy, R wz) {
+    R qj[8];
+    ldrec<8>(L.rec0 + 8 * (size_t)j, qj);
     any = true;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
@@ -185,7 +294,7 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
     }
   });
   (void)any;
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+  for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
     int v = L.bs_list[k], s = v >> 1;
     int o = (v & 1) ? L.ba[s] : L.bb[s];
     const R *qo = L.rec0 + 8 * (size_t)o;
@@ -197,6 +306,16 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
       g[5 + c] += t * wy;
       g[10 + c] += t * wz;
     }
+  }
+  if (SUB > 1) {
+#pragma unroll
+    for (int c = 0; c < 15; ++c) g[c] = lsum<SUB>(g[c]);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      mn[c] = lmin<SUB>(mn[c]);
+      mx[c] = lmin<SUB>(mx[c]);
+    }
+    if (sub) return;
   }
   R vt = R(1) / L.vol[i];
   R *out = L.grec + GREC * (size_t)i;
@@ -226,137 +345,132 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
   return (a1 > R(0) || a1 < R(0)) ? r : a;  // a == 0 (or NaN) passes through
 }
 
-// ---------------------------------------------------------------------------
-// HLLC interior flux (makeRHS, DEF:1794-2100), one evaluation per edge.
-// fbuf[NF*e + c]: flux leaving ea towards eb (+ to ea, - to eb);
-// component 5 is the SA convection (makeSARHS, DEF:6873-6880).
-// ---------------------------------------------------------------------------
-template <typename R, int ORDER, bool SA, int NSIG>
-__global__ void k_flux(LevelDev<R> L, DevParams P) {
-  int e = gtid_x();
-  if (e >= L.ne) return;
-  constexpr int NF = 6;
-  int ia = L.ea[e], ib = L.eb[e];
-  const R *A = L.rec0 + 8 * (size_t)ia;
-  const R *B = L.rec0 + 8 * (size_t)ib;
-  R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
-  R gam = R(P.gamma), gm1 = gam - R(1);
-  R roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
-  R roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
-  R fsa = 0;
+// HLLC interior flux of one side (makeRHS, DEF:1794-2100) from the two
+// ends' gather records A, B (rho, u, v, w, rhoE, p, nu~, mu) and the side
+// weights; writes o[0..5] (component 5: SA convection, DEF:6873-6880).
+// T is the arithmetic type: the level's R for the residual, a dual number
+// for its tangent and adjoint (first order only: the MUSCL branch reads the
+// level's gradient records and is built for T = R).
+template <typename T, typename R, int ORDER, bool SA, int NSIG>
+__device__ __forceinline__ void side_flux(const LevelDev<R> &L, const DevParams &P, int ia, int ib, const T *A,
+                                          const T *B, T wx, T wy, T wz, T *o) {
+  T gam = T(P.gamma), gm1 = gam - T(1);
+  T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
+  T roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
+  T fsa = 0;
   if (SA) {
-    R t1 = A[6], t2 = B[6];
+    T t1 = A[6], t2 = B[6];
     fsa = wx * (u1 * t1 + u2 * t2) + wy * (v1 * t1 + v2 * t2) + wz * (w1 * t1 + w2 * t2);
-    R dtt = t1 - t2;
-    if (fabs(dtt) > R(1e-20)) {
-      R id = R(1) / dtt;
-      R ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
+    T dtt = t1 - t2;
+    if (fabs(dtt) > T(1e-20)) {
+      T id = T(1) / dtt;
+      T ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
       fsa = fsa + fabs(wx * ax + wy * ay + wz * az) * dtt;
     }
   }
-  R ru1 = roe1 * u1, rv1 = roe1 * v1, rw1 = roe1 * w1, rh1 = re1 + pr1;
-  R ru2 = roe2 * u2, rv2 = roe2 * v2, rw2 = roe2 * w2, rh2 = re2 + pr2;
+  T ru1 = roe1 * u1, rv1 = roe1 * v1, rw1 = roe1 * w1, rh1 = re1 + pr1;
+  T ru2 = roe2 * u2, rv2 = roe2 * v2, rw2 = roe2 * w2, rh2 = re2 + pr2;
   // central part: only kept when no HLLC branch is taken (NaN wave speeds)
-  R fc[5];
+  T fc[5];
   fc[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
   fc[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
   fc[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
   fc[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
   fc[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
-  R nx = R(2) * wx, ny = R(2) * wy, nz = R(2) * wz;
-  R rnorm1 = sqrt(nx * nx + ny * ny + nz * nz);
-  R inv = rnorm1 > R(0) ? R(1) / rnorm1 : rnorm1;
-  R rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
-  if (ORDER > 1) {
-    const R *G1 = L.grec + GREC * (size_t)ia;
-    const R *G2 = L.grec + GREC * (size_t)ib;
-    const R *X1 = L.xyz + 3 * (size_t)ia, *X2 = L.xyz + 3 * (size_t)ib;
-    R dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
-    R hl = R(0.5) * R(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
-    R d[5];
+  T nx = T(2) * wx, ny = T(2) * wy, nz = T(2) * wz;
+  T rnorm1 = sqrt(nx * nx + ny * ny + nz * nz);
+  T inv = rnorm1 > T(0) ? T(1) / rnorm1 : rnorm1;
+  T rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
+  if constexpr (ORDER > 1) {
+    T G1[GREC], G2[GREC];
+    ldrec<GREC>(L.grec + GREC * (size_t)ia, G1);
+    ldrec<GREC>(L.grec + GREC * (size_t)ib, G2);
+    const T *X1 = L.xyz + 3 * (size_t)ia, *X2 = L.xyz + 3 * (size_t)ib;
+    T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
+    T hl = T(0.5) * T(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
+    T d[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      R a = R(0.5) * (G1[j] * dx + G1[j + 5] * dy + G1[j + 10] * dz);
-      R b = R(0.5) * (G2[j] * dx + G2[j + 5] * dy + G2[j + 10] * dz);
+      T a = T(0.5) * (G1[j] * dx + G1[j + 5] * dy + G1[j + 10] * dz);
+      T b = T(0.5) * (G2[j] * dx + G2[j + 5] * dy + G2[j + 10] * dz);
       if (P.autoclip == 1) {
         a = led<NSIG>(a, G1[15 + j], G1[20 + j]);
         b = -led<NSIG>(-b, G2[15 + j], G2[20 + j]);
       }
-      R c;
-      if (a * b <= R(0)) {
+      T c;
+      if (a * b <= T(0)) {
         c = 0;
       } else {
         c = fabs(a) <= fabs(b) ? a : b;
-        if (fabs(c) > hl && hl > R(0)) c = c > R(0) ? hl : -hl;
+        if (fabs(c) > hl && hl > T(0)) c = c > T(0) ? hl : -hl;
       }
       d[j] = c;
     }
-    if (roe1 + d[0] > R(0.1)) roe1 = roe1 + d[0];
+    if (roe1 + d[0] > T(0.1)) roe1 = roe1 + d[0];
     u1 += d[1]; v1 += d[2]; w1 += d[3];
-    if (re1 + d[4] > R(0.0001)) re1 = re1 + d[4];
-    pr1 = gm1 * (re1 - R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1) * roe1);
-    pr1 = pr1 > R(0.1) ? pr1 : R(0.1);
+    if (re1 + d[4] > T(0.0001)) re1 = re1 + d[4];
+    pr1 = gm1 * (re1 - T(0.5) * (u1 * u1 + v1 * v1 + w1 * w1) * roe1);
+    pr1 = pr1 > T(0.1) ? pr1 : T(0.1);
     rh1 = re1 + pr1;
     ru1 = roe1 * u1; rv1 = roe1 * v1; rw1 = roe1 * w1;
-    if (roe2 - d[0] > R(0.1)) roe2 = roe2 - d[0];
+    if (roe2 - d[0] > T(0.1)) roe2 = roe2 - d[0];
     u2 -= d[1]; v2 -= d[2]; w2 -= d[3];
-    if (re2 - d[4] > R(0.0001)) re2 = re2 - d[4];
-    pr2 = gm1 * (re2 - R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2) * roe2);
-    pr2 = pr2 > R(0.1) ? pr2 : R(0.1);
+    if (re2 - d[4] > T(0.0001)) re2 = re2 - d[4];
+    pr2 = gm1 * (re2 - T(0.5) * (u2 * u2 + v2 * v2 + w2 * w2) * roe2);
+    pr2 = pr2 > T(0.1) ? pr2 : T(0.1);
     rh2 = re2 + pr2;
     ru2 = roe2 * u2; rv2 = roe2 * v2; rw2 = roe2 * w2;
   }
-  R rinv1 = R(1) / roe1, rinv2 = R(1) / roe2;
-  R nv1 = u1 * rnx + v1 * rny + w1 * rnz;
-  R nv2 = u2 * rnx + v2 * rny + w2 * rnz;
-  R beta0 = R(P.beta0);
-  R rroe = sqrt(roe2 * rinv1);
-  R iro = R(1) / (R(1) + rroe);
-  R ut = (u1 + u2 * rroe) * iro, vt = (v1 + v2 * rroe) * iro, wt = (w1 + w2 * rroe) * iro;
-  R qt = ut * rnx + vt * rny + wt * rnz;
-  R ht = (rh1 * rinv1 + rh2 * rinv2 * rroe) * iro;
-  R ct = sqrt(fabs(gm1 * (ht - R(0.5) * (ut * ut + vt * vt + wt * wt))));
-  R c1 = sqrt(gm1 * fabs(rh1 * rinv1 - R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1)));
-  R c2 = sqrt(gm1 * fabs(rh2 * rinv2 - R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2)));
-  R sl = fmin(fmin(nv1 - beta0 * c1, qt - ct), nv2 - beta0 * c2);
-  R sr = fmax(fmax(nv2 + beta0 * c2, qt + ct), nv1 + beta0 * c1);
-  R sm = roe2 * nv2 * (sr - nv2) - roe1 * nv1 * (sl - nv1) + pr1 - pr2;
-  R fact = roe2 * (sr - nv2) - roe1 * (sl - nv1);
-  if (fact != R(0)) fact = R(1) / fact;
+  T rinv1 = T(1) / roe1, rinv2 = T(1) / roe2;
+  T nv1 = u1 * rnx + v1 * rny + w1 * rnz;
+  T nv2 = u2 * rnx + v2 * rny + w2 * rnz;
+  T beta0 = T(P.beta0);
+  T rroe = sqrt(roe2 * rinv1);
+  T iro = T(1) / (T(1) + rroe);
+  T ut = (u1 + u2 * rroe) * iro, vt = (v1 + v2 * rroe) * iro, wt = (w1 + w2 * rroe) * iro;
+  T qt = ut * rnx + vt * rny + wt * rnz;
+  T ht = (rh1 * rinv1 + rh2 * rinv2 * rroe) * iro;
+  T ct = sqrt(fabs(gm1 * (ht - T(0.5) * (ut * ut + vt * vt + wt * wt))));
+  T c1 = sqrt(gm1 * fabs(rh1 * rinv1 - T(0.5) * (u1 * u1 + v1 * v1 + w1 * w1)));
+  T c2 = sqrt(gm1 * fabs(rh2 * rinv2 - T(0.5) * (u2 * u2 + v2 * v2 + w2 * w2)));
+  T sl = fmin(fmin(nv1 - beta0 * c1, qt - ct), nv2 - beta0 * c2);
+  T sr = fmax(fmax(nv2 + beta0 * c2, qt + ct), nv1 + beta0 * c1);
+  T sm = roe2 * nv2 * (sr - nv2) - roe1 * nv1 * (sl - nv1) + pr1 - pr2;
+  T fact = roe2 * (sr - nv2) - roe1 * (sl - nv1);
+  if (fact != T(0)) fact = T(1) / fact;
   sm = fact * sm;
   // upwind side K: left when sl > 0 or (sl <= 0 < sm), right when
   // sm <= 0 <= sr or sr < 0; neither only for NaN wave speeds, where the
   // central flux is kept.  Both sides share one code path (selected
   // operands), so a wavefront evaluates the star state once.
-  bool lft = sl > R(0) || (sl <= R(0) && sm > R(0));
-  bool rgt = (sm <= R(0) && sr >= R(0)) || sr < R(0);
-  R *o = L.fbuf + NF * (size_t)e;
+  bool lft = sl > T(0) || (sl <= T(0) && sm > T(0));
+  bool rgt = (sm <= T(0) && sr >= T(0)) || sr < T(0);
   if (!lft && !rgt) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = fc[c];
     o[5] = fsa;
     return;
   }
-  R rk = lft ? roe1 : roe2, uk = lft ? u1 : u2, vk = lft ? v1 : v2, wk = lft ? w1 : w2;
-  R ek = lft ? re1 : re2, pk = lft ? pr1 : pr2, nvk = lft ? nv1 : nv2;
-  R ruk = lft ? ru1 : ru2, rvk = lft ? rv1 : rv2, rwk = lft ? rw1 : rw2, rhk = lft ? rh1 : rh2;
-  R sk = lft ? sl : sr;
-  bool star = lft ? !(sl > R(0)) : !(sr < R(0));
-  R f[5];
+  T rk = lft ? roe1 : roe2, uk = lft ? u1 : u2, vk = lft ? v1 : v2, wk = lft ? w1 : w2;
+  T ek = lft ? re1 : re2, pk = lft ? pr1 : pr2, nvk = lft ? nv1 : nv2;
+  T ruk = lft ? ru1 : ru2, rvk = lft ? rv1 : rv2, rwk = lft ? rw1 : rw2, rhk = lft ? rh1 : rh2;
+  T sk = lft ? sl : sr;
+  bool star = lft ? !(sl > T(0)) : !(sr < T(0));
+  T f[5];
   f[0] = rnx * ruk + rny * rvk + rnz * rwk;
   f[1] = rnx * (ruk * uk + pk) + rny * (ruk * vk) + rnz * (ruk * wk);
   f[2] = rnx * (rvk * uk) + rny * (rvk * vk + pk) + rnz * (rvk * wk);
   f[3] = rnx * (rwk * uk) + rny * (rwk * vk) + rnz * (rwk * wk + pk);
   f[4] = rnx * (rhk * uk) + rny * (rhk * vk) + rnz * (rhk * wk);
-  R fk = sk - sm;
-  if (star && fk != R(0)) {
-    fk = R(1) / fk;
-    R a = sk - nvk;
-    R pks = rk * (nvk - sk) * (nvk - sm) + pk;
-    R ps = pks - pk;
-    R s0 = rk * a * fk, s1 = (a * rk * uk + ps * rnx) * fk;
-    R s2 = (a * rk * vk + ps * rny) * fk, s3 = (a * rk * wk + ps * rnz) * fk;
-    R s4 = (a * ek - pk * nvk + pks * sm) * fk;
+  T fk = sk - sm;
+  if (star && fk != T(0)) {
+    fk = T(1) / fk;
+    T a = sk - nvk;
+    T pks = rk * (nvk - sk) * (nvk - sm) + pk;
+    T ps = pks - pk;
+    T s0 = rk * a * fk, s1 = (a * rk * uk + ps * rnx) * fk;
+    T s2 = (a * rk * vk + ps * rny) * fk, s3 = (a * rk * wk + ps * rnz) * fk;
+    T s4 = (a * ek - pk * nvk + pks * sm) * fk;
     f[0] += sk * (s0 - rk);
     f[1] += sk * (s1 - ruk);
     f[2] += sk * (s2 - rvk);
@@ -366,6 +480,20 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
 #pragma unroll
   for (int c = 0; c < 5; ++c) o[c] = f[c] * rnorm1;
   o[5] = fsa;
+}
+
+// HLLC interior flux (makeRHS, DEF:1794-2100), one evaluation per edge.
+// fbuf[6*e + c]: flux leaving ea towards eb (+ to ea, - to eb).
+template <typename R, int ORDER, bool SA, int NSIG>
+__global__ void k_flux(LevelDev<R> L, DevParams P) {
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  int ia = L.ea[e], ib = L.eb[e];
+  R A[8], B[8];
+  ldrec<8>(L.rec0 + 8 * (size_t)ia, A);
+  ldrec<8>(L.rec0 + 8 * (size_t)ib, B);
+  R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+  side_flux<R, R, ORDER, SA, NSIG>(L, P, ia, ib, A, B, wx, wy, wz, L.fbuf + 6 * (size_t)e);
 }
 
 // physical flux of one state through unit normal
@@ -456,35 +584,274 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// makeTimeSteps (DEF:13335-13610), gathered
+// Tangent and adjoint of the level-1 convective residual R(U) (makeRHS,
+// DEF:1525-2512: interior HLLC sides + boundary sides) for first-order
+// HLLC without viscous or turbulence terms.  The side and boundary fluxes
+// are re-evaluated on dual numbers from the conserved state, so nothing per
+// side is stored between the forward and the reverse sweeps.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void k_timesteps(LevelDev<R> L, DevParams P) {
+// gather record of a conserved state exactly as k_prep forms it
+template <typename T>
+__device__ __forceinline__ void state_record(const T *U, double gamma, T *A) {
+  T r = U[0], m1 = U[1], m2 = U[2], m3 = U[3], E = U[4];
+  T pp = T(gamma - 1.0) * (E - T(0.5) * (m1 * m1 + m2 * m2 + m3 * m3) / r);
+  pp = pp > T(0.1) ? pp : T(0.1);
+  A[0] = r;
+  A[1] = m1 / r;
+  A[2] = m2 / r;
+  A[3] = m3 / r;
+  A[4] = E;
+  A[5] = pp;
+  A[6] = T(0);
+  A[7] = T(0);
+}
+
+// boundary flux of one end (the per-end flux of k_bflux): F(U)·n̂
+template <typename T>
+__device__ __forceinline__ void bside_end_flux(const DevParams &P, int ind, const T *U, double rnx, double rny,
+                                               double rnz, T *f) {
+  T A[8];
+  state_record(U, P.gamma, A);
+  if (ind <= 2 || ind == 9) {
+    phys_flux(A, T(rnx), T(rny), T(rnz), f);
+  } else {
+    farfield_roe(P.inflow[ind - 3], A, T(P.gamma - 1.0), T(rnx), T(rny), T(rnz), f);
+  }
+}
+
+// tangent: fbuf/bbuf <- d(fluxes)/dU along du; the stage gather (MODE_RHS)
+// then sums them into rhs = (dR/dU) du
+__global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  Dual Ua[5], Ub[5], A[8], B[8], o[6];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia], du[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib], du[c * n + ib]);
+  }
+  state_record(Ua, P.gamma, A);
+  state_record(Ub, P.gamma, B);
+  side_flux<Dual, double, 1, false, 2>(L, P, ia, ib, A, B, Dual(L.w[3 * e]), Dual(L.w[3 * e + 1]),
+                                       Dual(L.w[3 * e + 2]), o);
+  double *f = L.fbuf + 6 * (size_t)e;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) f[c] = o[c].d;
+  f[5] = 0;
+}
+
+__global__ void k_bflux_tangent(LevelDev<double> L, DevParams P, const double *du) {
+  using ad::Dual;
+  int s = gtid_x();
+  if (s >= L.nbs) return;
+  size_t n = L.n;
+  int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
+  double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  double *o = L.bbuf + 10 * (size_t)s;
+  double rnorm = ::sqrt(wx * wx + wy * wy + wz * wz);
+  if (!(rnorm > 1e-10) || !(ind <= 4 || ind == 9)) {
+#pragma unroll
+    for (int c = 0; c < 10; ++c) o[c] = 0;
+    return;
+  }
+  double irn = 1.0 / rnorm;
+  Dual Ua[5], Ub[5], f1[5], f2[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia], du[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib], du[c * n + ib]);
+  }
+  bside_end_flux(P, ind, Ua, wx * irn, wy * irn, wz * irn, f1);
+  bside_end_flux(P, ind, Ub, wx * irn, wy * irn, wz * irn, f2);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    if (P.bterm == 1) {
+      o[c] = rnorm * (3.0 * f1[c].d + f2[c].d);
+      o[5 + c] = rnorm * (f1[c].d + 3.0 * f2[c].d);
+    } else {
+      o[c] = rnorm * 4.0 * f1[c].d;
+      o[5 + c] = rnorm * 4.0 * f2[c].d;
+    }
+  }
+}
+
+// adjoint, interior sides: side e adds +f to R(ea) and -f to R(eb), so its
+// share of (dR/dU)^T lam is (df/dU_ea)^T d and (df/dU_eb)^T d with
+// d = lam(ea) - lam(eb).  abuf[10*e + k]: k < 5 for U_ea, k >= 5 for U_eb.
+__global__ void k_flux_adjoint(LevelDev<double> L, DevParams P, const double *lam, double *abuf) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  double ua[5], ub[5], d[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    ua[c] = L.u[c * n + ia];
+    ub[c] = L.u[c * n + ib];
+    d[c] = lam[c * n + ia] - lam[c * n + ib];
+  }
+  Dual wx(L.w[3 * e]), wy(L.w[3 * e + 1]), wz(L.w[3 * e + 2]);
+#pragma unroll 1
+  for (int k = 0; k < 10; ++k) {
+    Dual Ua[5], Ub[5], A[8], B[8], o[6];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Ua[c] = Dual(ua[c], k == c ? 1.0 : 0.0);
+      Ub[c] = Dual(ub[c], k == c + 5 ? 1.0 : 0.0);
+    }
+    state_record(Ua, P.gamma, A);
+    state_record(Ub, P.gamma, B);
+    side_flux<Dual, double, 1, false, 2>(L, P, ia, ib, A, B, wx, wy, wz, o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
+    abuf[10 * (size_t)e + k] = acc;
+  }
+}
+
+// adjoint, boundary sides: R(a) += |w|(3F_a + F_b), R(b) += |w|(F_a + 3F_b)
+// (boundaryTerm 1; 4F_a and 4F_b for 2), so U_a receives
+// |w| (dF_a/dU_a)^T (3 lam_a + lam_b), and alike for U_b.  bbuf layout as
+// the residual's: [10*s + 5*end + k].
+__global__ void k_bflux_adjoint(LevelDev<double> L, DevParams P, const double *lam) {
+  using ad::Dual;
+  int s = gtid_x();
+  if (s >= L.nbs) return;
+  size_t n = L.n;
+  int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
+  double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  double *o = L.bbuf + 10 * (size_t)s;
+  double rnorm = ::sqrt(wx * wx + wy * wy + wz * wz);
+  if (!(rnorm > 1e-10) || !(ind <= 4 || ind == 9)) {
+#pragma unroll
+    for (int c = 0; c < 10; ++c) o[c] = 0;
+    return;
+  }
+  double irn = 1.0 / rnorm;
+#pragma unroll 1
+  for (int end = 0; end < 2; ++end) {
+    int me = end ? ib : ia;
+    double wt[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double la = lam[c * n + ia], lb = lam[c * n + ib];
+      if (P.bterm == 1) wt[c] = rnorm * (end ? la + 3.0 * lb : 3.0 * la + lb);
+      else wt[c] = rnorm * 4.0 * (end ? lb : la);
+    }
+#pragma unroll 1
+    for (int k = 0; k < 5; ++k) {
+      Dual U[5], f[5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) U[c] = Dual(L.u[c * n + me], k == c ? 1.0 : 0.0);
+      bside_end_flux(P, ind, U, wx * irn, wy * irn, wz * irn, f);
+      double acc = 0;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc += wt[c] * f[c].d;
+      o[5 * end + k] = acc;
+    }
+  }
+}
+
+// adjoint gather: out(i) = sum of node i's shares over its sides
+__global__ void k_adjoint_gather(LevelDev<double> L, const double *abuf, double *out) {
   int i = gtid_x();
   if (i >= L.n) return;
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  R gam = R(P.gamma);
-  R cc = gam * qi[5] / qi[0];
-  cc = cc > R(0.001) ? cc : R(0.001);
-  R lam = 0, lamv = 0;
-  bool visc = P.Re > 0.0;
-  R am = visc ? R(1.0 / P.Re) : R(0);
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
+  double r[5] = {0, 0, 0, 0, 0};
+  for_edges<1>(L, i, 0, [&](int e, int role) {
+    const double *a = abuf + 10 * (size_t)e + 5 * role;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += a[c];
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k];
+    const double *b = L.bbuf + 10 * (size_t)(v >> 1) + 5 * (v & 1);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += b[c];
+  }
+  size_t n = L.n;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) out[c * n + i] = r[c];
+  out[5 * n + i] = 0;
+}
+
+// partial derivative of the pressure drag (calculateLiftAndDrag,
+// DEF:14757-14803: drag = (2/S) sum_wall -A_i (p_i - p_inf) n_i.t) with
+// respect to the conserved state of each node; zero off the wall
+__global__ void k_drag_state_gradient(LevelDev<double> L, DevParams P, const int *wall_nodes, int nw,
+                                      const double *warea, double fct, double t0, double t1, double t2,
+                                      double groundplane, double *out) {
+  int k = gtid();
+  if (k >= nw) return;
+  int ip = wall_nodes[k];
+  double z = L.xyz ? L.xyz[3 * (size_t)ip + 2] : 0.0;
+  if (!(z > groundplane + 0.01)) return;
+  size_t n = L.n;
+  double r = L.u[ip], m1 = L.u[n + ip], m2 = L.u[2 * n + ip], m3 = L.u[3 * n + ip], E = L.u[4 * n + ip];
+  double g1 = P.gamma - 1.0;
+  double pp = g1 * (E - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r);
+  double nt = L.nrm[3 * (size_t)ip] * t0 + L.nrm[3 * (size_t)ip + 1] * t1 + L.nrm[3 * (size_t)ip + 2] * t2;
+  double s = -fct * warea[ip] * nt;
+  if (!(pp > 0.1)) s = 0.0;  // clamped pressure: no dependence
+  double ir = 1.0 / r;
+  out[ip] += s * g1 * 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * ir * ir;
+  out[n + ip] += -s * g1 * m1 * ir;
+  out[2 * n + ip] += -s * g1 * m2 * ir;
+  out[3 * n + ip] += -s * g1 * m3 * ir;
+  out[4 * n + ip] += s * g1;
+}
+
+// ---------------------------------------------------------------------------
+// Node-gather "parts".  Each reference edge loop of the time-step, viscous
+// and Spalart-Allmaras routines is a part with the same five hooks:
+//   init(i)                    node-local set-up
+//   edge(j, w)                 one interior side of node i (signed weights)
+//   bside(s, o, ind, w)        one boundary side of node i (o: other end)
+//   reduce<SUB>()              combine the lanes' partial sums
+//   finish(i)                  node-local epilogue and stores
+// k_gather runs up to three parts over ONE walk of the incidence lists, so
+// loops that read the same neighbours share their gathers.  Every part
+// accumulates in the same order whether it runs alone or fused.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct NoPart {
+  __device__ void init(const LevelDev<R> &, const DevParams &, int, const R *) {}
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *, R, R, R) {}
+  __device__ void bside(const LevelDev<R> &, const DevParams &, const R *, int, int, int, const R *, R, R, R) {}
+  template <int SUB>
+  __device__ void reduce() {}
+  __device__ void finish(const LevelDev<R> &, const DevParams &, int, const R *) {}
+};
+
+// makeTimeSteps (DEF:13335-13610)
+template <typename R>
+struct TimeStepPart {
+  R lam, lamv, cc, am, gam;
+  bool visc;
+  __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *qi) {
+    gam = R(P.gamma);
+    cc = gam * qi[5] / qi[0];
+    cc = cc > R(0.001) ? cc : R(0.001);
+    lam = 0;
+    lamv = 0;
+    visc = P.Re > 0.0;
+    am = visc ? R(1.0 / P.Re) : R(0);
+  }
+  __device__ void edge(const LevelDev<R> &, const DevParams &P, const R *qi, int, const R *qj, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += fabs(vn) + sqrt(cc * al);
-    if (visc) {
-      const R *qj = L.rec0 + 8 * (size_t)j;
+    if (visc) {  // neighbour's viscosity (DEF:13432-13433)
       R vis = P.sa ? qj[7] + qj[6] * am : qj[7];
       vis = am > vis ? am : vis;
       lamv += R(2) * vis * al / qj[0];
     }
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
-    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  }
+  __device__ void bside(const LevelDev<R> &, const DevParams &P, const R *qi, int, int, int, const R *qo, R wx, R wy,
+                        R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R cco = gam * qo[5] / qo[0];
     cco = cco > R(0.001) ? cco : R(0.001);
@@ -501,110 +868,125 @@ __global__ void k_timesteps(LevelDev<R> L, DevParams P) {
       lamv += dv1 + (dv1 + dv2);
     }
   }
-  R V = L.vol[i];
-  L.dt[i] = R(0.5) / (lam / V + lamv / (V * V));
-}
+  template <int SUB>
+  __device__ void reduce() {
+    lam = lsum<SUB>(lam);
+    lamv = lsum<SUB>(lamv);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &, int i, const R *) {
+    R V = L.vol[i];
+    L.dt[i] = R(0.5) / (lam / V + lamv / (V * V));
+  }
+};
 
-// ---------------------------------------------------------------------------
 // makeViscosityTerm (DEF:5198-5893), pass 1: gradients of (u,v,w,T) and the
 // node loop (Sutherland, SA eddy viscosity, stress, heat flux, vorticity,
-// divergence, adiabatic wall, wall stress).  tau[9*i]: stress 6 + k grad T 3
-// ---------------------------------------------------------------------------
-template <typename R>
-__global__ void k_visc_node(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
-  if (i >= L.n) return;
-  R gog1 = R(P.gamma / (P.gamma - 1.0));
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  R ai[4] = {qi[1], qi[2], qi[3], gog1 * qi[5] / qi[0]};
-  R g[12];
+// divergence, adiabatic wall, wall stress).  tau[9*i]: stress 6 + k grad T 3.
+// WRITE_REC also stores the new laminar viscosity in the gather record;
+// when fused with makeTimeSteps (which reads the neighbours' previous one)
+// that store is left to k_mu_record after the pass.
+template <typename R, bool WRITE_REC>
+struct ViscNodePart {
+  R gog1, ai[4], g[12];
+  __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *qi) {
+    gog1 = R(P.gamma / (P.gamma - 1.0));
+    ai[0] = qi[1];
+    ai[1] = qi[2];
+    ai[2] = qi[3];
+    ai[3] = gog1 * qi[5] / qi[0];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) g[c] = 0;
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
-    const R *qj = L.rec0 + 8 * (size_t)j;
+    for (int c = 0; c < 12; ++c) g[c] = 0;
+  }
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *qj, R wx, R wy, R wz) {
     R aj[4] = {qj[1], qj[2], qj[3], gog1 * qj[5] / qj[0]};
     R ww[3] = {wx, wy, wz};
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int q = 0; q < 4; ++q) g[4 * d + q] += (ai[q] + aj[q]) * ww[d];
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
+  }
+  __device__ void bside(const LevelDev<R> &, const DevParams &, const R *, int, int, int, const R *qo, R wx, R wy,
+                        R wz) {
     R ao[4] = {qo[1], qo[2], qo[3], gog1 * qo[5] / qo[0]};
-    R ww[3] = {L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2]};
+    R ww[3] = {wx, wy, wz};
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int q = 0; q < 4; ++q) g[4 * d + q] += (R(2) * ai[q] + (ai[q] + ao[q])) * ww[d];
   }
-  R vt = R(1) / L.vol[i];
-  R dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
-  R dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
-  R dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
-  R T = ai[3];
-  if (T <= R(0.001)) T = R(0.001);
-  R tc = R((P.gamma - 1.0) * P.mach * P.mach), Tin = R(P.inflow_temperature);
-  R tct = tc * T;
-  R mu = R(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + R(198.6)) / (tc * Tin * T + R(198.6))));
-  R k = R(1.0 / P.Pr) * mu;
-  L.mu[i] = mu;
-  L.rec0[8 * (size_t)i + 7] = mu;  // later SA kernels read the new laminar viscosity
-  L.vort[i] = sqrt((dudy - dvdx) * (dudy - dvdx) + (dvdz - dwdy) * (dvdz - dwdy) + (dwdx - dudz) * (dwdx - dudz));
-  L.divg[i] = dudx + dvdy + dwdz;
-  if (P.sa) {
-    double tvc = (1.0 / P.Re) * double(qi[0]) * double(qi[6]);
-    double ksi = tvc / double(mu);
-    double k3 = ksi * ksi * ksi;
-    tvc = tvc * k3 / (k3 + 357.9);
-    mu = mu + R(tvc);
-    k = k + R(tvc * (1.0 / P.Prt));
-  }
-  R t[9];
-  R two3 = R(2.0 / 3.0), four3 = R(4.0 / 3.0);
-  t[0] = mu * (four3 * dudx - two3 * (dvdy + dwdz));
-  t[1] = mu * (dudy + dvdx);
-  t[2] = mu * (dudz + dwdx);
-  t[3] = mu * (four3 * dvdy - two3 * (dudx + dwdz));
-  t[4] = mu * (dvdz + dwdy);
-  t[5] = mu * (four3 * dwdz - two3 * (dudx + dvdy));
-  t[6] = k * dTdx;
-  t[7] = k * dTdy;
-  t[8] = k * dTdz;
-  int cls = L.ncls[i] & NC_MASK;
-  if (L.gnum == 1 && cls != NC_NONE) {
-    R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
-    if (cls == NC_WALL) {  // setTemperatureForAdiabaticWall (DEF:5170-5196)
-      R nt = t[6] * nx + t[7] * ny + t[8] * nz;
-      t[6] -= nt * nx;
-      t[7] -= nt * ny;
-      t[8] -= nt * nz;
-    }
-    L.ws[3 * (size_t)i] = t[0] * nx + t[1] * ny + t[2] * nz;
-    L.ws[3 * (size_t)i + 1] = t[1] * nx + t[3] * ny + t[4] * nz;
-    L.ws[3 * (size_t)i + 2] = t[2] * nx + t[4] * ny + t[5] * nz;
-  }
-  R *o = L.tau + 9 * (size_t)i;
+  template <int SUB>
+  __device__ void reduce() {
 #pragma unroll
-  for (int c = 0; c < 9; ++c) o[c] = t[c];
-}
+    for (int c = 0; c < 12; ++c) g[c] = lsum<SUB>(g[c]);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &P, int i, const R *qi) {
+    R vt = R(1) / L.vol[i];
+    R dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
+    R dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
+    R dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
+    R T = ai[3];
+    if (T <= R(0.001)) T = R(0.001);
+    R tc = R((P.gamma - 1.0) * P.mach * P.mach), Tin = R(P.inflow_temperature);
+    R tct = tc * T;
+    R mu = R(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + R(198.6)) / (tc * Tin * T + R(198.6))));
+    R k = R(1.0 / P.Pr) * mu;
+    L.mu[i] = mu;
+    if (WRITE_REC) L.rec0[8 * (size_t)i + 7] = mu;
+    L.vort[i] = sqrt((dudy - dvdx) * (dudy - dvdx) + (dvdz - dwdy) * (dvdz - dwdy) + (dwdx - dudz) * (dwdx - dudz));
+    L.divg[i] = dudx + dvdy + dwdz;
+    if (P.sa) {
+      double tvc = (1.0 / P.Re) * double(qi[0]) * double(qi[6]);
+      double ksi = tvc / double(mu);
+      double k3 = ksi * ksi * ksi;
+      tvc = tvc * k3 / (k3 + 357.9);
+      mu = mu + R(tvc);
+      k = k + R(tvc * (1.0 / P.Prt));
+    }
+    R t[9];
+    R two3 = R(2.0 / 3.0), four3 = R(4.0 / 3.0);
+    t[0] = mu * (four3 * dudx - two3 * (dvdy + dwdz));
+    t[1] = mu * (dudy + dvdx);
+    t[2] = mu * (dudz + dwdx);
+    t[3] = mu * (four3 * dvdy - two3 * (dudx + dwdz));
+    t[4] = mu * (dvdz + dwdy);
+    t[5] = mu * (four3 * dwdz - two3 * (dudx + dvdy));
+    t[6] = k * dTdx;
+    t[7] = k * dTdy;
+    t[8] = k * dTdz;
+    int cls = L.ncls[i] & NC_MASK;
+    if (L.gnum == 1 && cls != NC_NONE) {
+      R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+      if (cls == NC_WALL) {  // setTemperatureForAdiabaticWall (DEF:5170-5196)
+        R nt = t[6] * nx + t[7] * ny + t[8] * nz;
+        t[6] -= nt * nx;
+        t[7] -= nt * ny;
+        t[8] -= nt * nz;
+      }
+      L.ws[3 * (size_t)i] = t[0] * nx + t[1] * ny + t[2] * nz;
+      L.ws[3 * (size_t)i + 1] = t[1] * nx + t[3] * ny + t[4] * nz;
+      L.ws[3 * (size_t)i + 2] = t[2] * nx + t[4] * ny + t[5] * nz;
+    }
+    R *o = L.tau + 9 * (size_t)i;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o[c] = t[c];
+  }
+};
 
 // makeViscosityTerm pass 2 (DEF:5640-5893): dissipation(:,2:5) = -sum
 template <typename R>
-__global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
-  if (i >= L.n) return;
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  const R *ti = L.tau + 9 * (size_t)i;
-  R t1[9];
+struct ViscFluxPart {
+  R t1[9], u1, v1, w1, a[4];
+  __device__ void init(const LevelDev<R> &L, const DevParams &, int i, const R *qi) {
+    const R *ti = L.tau + 9 * (size_t)i;
 #pragma unroll
-  for (int c = 0; c < 9; ++c) t1[c] = ti[c];
-  R u1 = qi[1], v1 = qi[2], w1 = qi[3];
-  R a[4] = {0, 0, 0, 0};
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
-    const R *qj = L.rec0 + 8 * (size_t)j;
+    for (int c = 0; c < 9; ++c) t1[c] = ti[c];
+    u1 = qi[1];
+    v1 = qi[2];
+    w1 = qi[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = 0;
+  }
+  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
     const R *tj = L.tau + 9 * (size_t)j;
     R u2 = qj[1], v2 = qj[2], w2 = qj[3];
     R d[9];
@@ -617,16 +999,12 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
     a[1] += wx * d[1] + wy * d[3] + wz * d[4];
     a[2] += wx * d[2] + wy * d[4] + wz * d[5];
     a[3] += wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3);
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int ind = L.bind[s];
-    if (ind == 2 || ind == 9) continue;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
+  }
+  __device__ void bside(const LevelDev<R> &L, const DevParams &, const R *, int, int o, int ind, const R *qo, R wx,
+                        R wy, R wz) {
+    if (ind == 2 || ind == 9) return;  // symmetry / inviscid wall carry no viscous flux
     const R *to = L.tau + 9 * (size_t)o;
     R u2 = qo[1], v2 = qo[2], w2 = qo[3];
-    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
     R d[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) d[c] = t1[c] + to[c];
@@ -640,38 +1018,42 @@ __global__ void k_visc_flux(LevelDev<R> L, DevParams P) {
     a[3] += (wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3)) +
             R(2) * (wx * (t1[6] + du11) + wy * (t1[7] + du21) + wz * (t1[8] + du31));
   }
-  int n = L.n;
-  L.diss[i] = 0;
-  L.diss[n + i] = -a[0];
-  L.diss[2 * n + i] = -a[1];
-  L.diss[3 * n + i] = -a[2];
-  L.diss[4 * n + i] = -a[3];
-  (void)P;
-}
+  template <int SUB>
+  __device__ void reduce() {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = lsum<SUB>(a[c]);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &, int i, const R *) {
+    int n = L.n;
+    L.diss[i] = 0;
+    L.diss[n + i] = -a[0];
+    L.diss[2 * n + i] = -a[1];
+    L.diss[3 * n + i] = -a[2];
+    L.diss[4 * n + i] = -a[3];
+  }
+};
 
-// ---------------------------------------------------------------------------
 // makeTurbulentTimeSteps (DEF:12878-13099)
-// ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
-  if (i >= L.n) return;
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  R ioRe = R(1.0 / P.Re), oos = R(1.5), cb2 = R(0.622), tk = R(P.turb_k);
-  R lam = 0, lamv = 0;
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
+struct SATimeStepPart {
+  R ioRe, oos, cb2, tk, lam, lamv;
+  __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *) {
+    ioRe = R(1.0 / P.Re);
+    oos = R(1.5);
+    cb2 = R(0.622);
+    tk = R(P.turb_k);
+    lam = 0;
+    lamv = 0;
+  }
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *qi, int, const R *qj, R wx, R wy, R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += tk * sqrt(al) + fabs(vn);
-    const R *qj = L.rec0 + 8 * (size_t)j;
     R vis = oos * (qj[7] + R(3) * cb2 * qj[6] * ioRe);
     lamv += R(2) * vis * al;
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
-    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  }
+  __device__ void bside(const LevelDev<R> &, const DevParams &, const R *qi, int, int, int, const R *qo, R wx, R wy,
+                        R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn1 = tk * sqrt(al) + fabs(wx * qi[1] + wy * qi[2] + wz * qi[3]);
     R vn2 = tk * sqrt(al) + fabs(wx * qo[1] + wy * qo[2] + wz * qo[3]);
@@ -680,82 +1062,149 @@ __global__ void k_sa_timesteps(LevelDev<R> L, DevParams P) {
     R dv2 = R(2) * (oos * (qo[7] + R(3) * cb2 * qo[6] * ioRe)) * al;
     lamv += dv1 + (dv1 + dv2);
   }
-  R V = L.vol[i];
-  R dt = R(0.5) / (lam / V + lamv / (V * V));
-  R wd = L.wd[i];
-  R di = wd > R(1e-10) ? R(1) / wd : R(1e10);
-  di = di * di;
-  R ik2 = R(1.0 / (0.41 * 0.41)), cb1 = R(0.1355), ct3 = R(1.1);
-  R cw1 = cb1 * ik2 + (R(1) + cb2) * oos;
-  R nu = qi[6];
-  R S = fabs(L.divg[i]) + cb1 * (L.vort[i] + (R(1) + ct3) * nu * ik2 * di * ioRe) + cw1 * nu * di * ioRe;
-  L.dtt[i] = dt / (R(1) + R(0.5) * S * dt);
-}
+  template <int SUB>
+  __device__ void reduce() {
+    lam = lsum<SUB>(lam);
+    lamv = lsum<SUB>(lamv);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &, int i, const R *qi) {
+    R V = L.vol[i];
+    R dt = R(0.5) / (lam / V + lamv / (V * V));
+    R wd = L.wd[i];
+    R di = wd > R(1e-10) ? R(1) / wd : R(1e10);
+    di = di * di;
+    R ik2 = R(1.0 / (0.41 * 0.41)), cb1 = R(0.1355), ct3 = R(1.1);
+    R cw1 = cb1 * ik2 + (R(1) + cb2) * oos;
+    R nu = qi[6];
+    R S = fabs(L.divg[i]) + cb1 * (L.vort[i] + (R(1) + ct3) * nu * ik2 * di * ioRe) + cw1 * nu * di * ioRe;
+    L.dtt[i] = dt / (R(1) + R(0.5) * S * dt);
+  }
+};
 
 // makeSADiffusion (DEF:7436-7808), pass 1: grad nu~ (dnu[4*i]: 3 + nodal
-// cb2 term)
+// cb2 term).  nu~ is read from the state u(:,6).
 template <typename R>
-__global__ void k_sa_diff1(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
-  if (i >= L.n) return;
-  const R *nu = L.u + 5 * (size_t)L.n;
-  R ti = nu[i];
-  R g[3] = {0, 0, 0};
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
-    R s = ti + nu[j];
+struct SADiff1Part {
+  R ti, g[3];
+  __device__ void init(const LevelDev<R> &L, const DevParams &, int i, const R *) {
+    ti = L.u[5 * (size_t)L.n + i];
+    g[0] = g[1] = g[2] = 0;
+  }
+  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *, R wx, R wy, R wz) {
+    R s = ti + L.u[5 * (size_t)L.n + j];
     g[0] += wx * s;
     g[1] += wy * s;
     g[2] += wz * s;
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    R to = nu[o];
-    R c = P.bterm == 1 ? (R(3) * ti + to) : R(4) * ti;
-    g[0] += c * L.wb[3 * s];
-    g[1] += c * L.wb[3 * s + 1];
-    g[2] += c * L.wb[3 * s + 2];
   }
-  R vt = R(1) / L.vol[i];
-  R dx = g[0] * vt, dy = g[1] * vt, dz = g[2] * vt;
-  R fact = R(1.0 / P.Re * 0.622 / (2.0 / 3.0));
-  R *o = L.dnu + 4 * (size_t)i;
-  o[0] = dx;
-  o[1] = dy;
-  o[2] = dz;
-  o[3] = -(fact * (g[0] * dx + g[1] * dy + g[2] * dz));
-}
+  __device__ void bside(const LevelDev<R> &L, const DevParams &P, const R *, int, int o, int, const R *, R wx, R wy,
+                        R wz) {
+    R to = L.u[5 * (size_t)L.n + o];
+    R c = P.bterm == 1 ? (R(3) * ti + to) : R(4) * ti;
+    g[0] += c * wx;
+    g[1] += c * wy;
+    g[2] += c * wz;
+  }
+  template <int SUB>
+  __device__ void reduce() {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c] = lsum<SUB>(g[c]);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &P, int i, const R *) {
+    R vt = R(1) / L.vol[i];
+    R dx = g[0] * vt, dy = g[1] * vt, dz = g[2] * vt;
+    R fact = R(1.0 / P.Re * 0.622 / (2.0 / 3.0));
+    R *o = L.dnu + 4 * (size_t)i;
+    o[0] = dx;
+    o[1] = dy;
+    o[2] = dz;
+    o[3] = -(fact * (g[0] * dx + g[1] * dy + g[2] * dz));
+  }
+};
 
 // makeSADiffusion pass 2
 template <typename R>
-__global__ void k_sa_diff2(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
-  if (i >= L.n) return;
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  const R *di = L.dnu + 4 * (size_t)i;
-  R Re = R(P.Re);
-  R fact = R(1.0 / P.Re / (2.0 / 3.0));
-  R v1 = Re * qi[7] / qi[0] + qi[6];
-  R acc = 0;
-  for_edges_w(L, i, [&](int j, R wx, R wy, R wz) {
-    const R *qj = L.rec0 + 8 * (size_t)j;
+struct SADiff2Part {
+  R Re, fact, v1, acc, d0, d1, d2, d3;
+  __device__ void init(const LevelDev<R> &L, const DevParams &P, int i, const R *qi) {
+    const R *di = L.dnu + 4 * (size_t)i;
+    d0 = di[0];
+    d1 = di[1];
+    d2 = di[2];
+    d3 = di[3];
+    Re = R(P.Re);
+    fact = R(1.0 / P.Re / (2.0 / 3.0));
+    v1 = Re * qi[7] / qi[0] + qi[6];
+    acc = 0;
+  }
+  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
     const R *dj = L.dnu + 4 * (size_t)j;
     R v2 = Re * qj[7] / qj[0] + qj[6];
-    R fd = v1 * (wx * di[0] + wy * di[1] + wz * di[2]) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
+    R fd = v1 * (wx * d0 + wy * d1 + wz * d2) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
     acc -= fact * fd;
-  });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
+  }
+  __device__ void bside(const LevelDev<R> &L, const DevParams &P, const R *, int, int o, int, const R *qo, R wx, R wy,
+                        R wz) {
     const R *dq = L.dnu + 4 * (size_t)o;
-    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
     R vo = Re * qo[7] / qo[0] + qo[6];
-    R fd1 = v1 * (wx * di[0] + wy * di[1] + wz * di[2]);
+    R fd1 = v1 * (wx * d0 + wy * d1 + wz * d2);
     R fd2 = vo * (wx * dq[0] + wy * dq[1] + wz * dq[2]);
     acc -= P.bterm == 1 ? fact * (R(3) * fd1 + fd2) : fact * R(4) * fd1;
   }
-  L.tdiff[i] = acc + di[3];
+  template <int SUB>
+  __device__ void reduce() {
+    acc = lsum<SUB>(acc);
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &, int i, const R *) { L.tdiff[i] = acc + d3; }
+};
+
+// one walk of node i's interior and boundary sides feeding up to three parts
+template <typename R, int SUB, typename A, typename B, typename C>
+__global__ void k_gather(LevelDev<R> L, DevParams P) {
+  int i, sub;
+  node_lanes<SUB>(i, sub);
+  if (i >= L.n) return;
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  A a;
+  B b;
+  C c;
+  a.init(L, P, i, qi);
+  b.init(L, P, i, qi);
+  c.init(L, P, i, qi);
+  for_edges_w<SUB>(L, i, sub, [&](int j, R wx, R wy, R wz) {
+    R qj[8];
+    ldrec<8>(L.rec0 + 8 * (size_t)j, qj);
+    a.edge(L, P, qi, j, qj, wx, wy, wz);
+    b.edge(L, P, qi, j, qj, wx, wy, wz);
+    c.edge(L, P, qi, j, qj, wx, wy, wz);
+  });
+  for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    int ind = L.bind[s];
+    R qo[8];
+    ldrec<8>(L.rec0 + 8 * (size_t)o, qo);
+    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    a.bside(L, P, qi, s, o, ind, qo, wx, wy, wz);
+    b.bside(L, P, qi, s, o, ind, qo, wx, wy, wz);
+    c.bside(L, P, qi, s, o, ind, qo, wx, wy, wz);
+  }
+  if (SUB > 1) {
+    a.template reduce<SUB>();
+    b.template reduce<SUB>();
+    c.template reduce<SUB>();
+    if (sub) return;
+  }
+  a.finish(L, P, i, qi);
+  b.finish(L, P, i, qi);
+  c.finish(L, P, i, qi);
+}
+
+// new laminar viscosity into the gather records after a fused pass
+template <typename R>
+__global__ void k_mu_record(LevelDev<R> L) {
+  int i = gtid_x();
+  if (i >= L.n) return;
+  L.rec0[8 * (size_t)i + 7] = L.mu[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -869,11 +1318,12 @@ __device__ __forceinline__ void outer_bc(const LevelDev<R> &L, const DevParams &
 }
 
 // gather of the convective residual of node i (interior fbuf + boundary)
-template <typename R, bool SA>
-__device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, R *r) {
+template <typename R, bool SA, int SUB>
+__device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, int sub, R *r) {
   for (int c = 0; c < 6; ++c) r[c] = 0;
-  for_edges(L, i, [&](int e, int role) {
-    const R *f = L.fbuf + 6 * (size_t)e;
+  for_edges<SUB>(L, i, sub, [&](int e, int role) {
+    R f[6];
+    ldrec<6>(L.fbuf + 6 * (size_t)e, f);
     if (role) {
 #pragma unroll
       for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] -= f[c];
@@ -882,11 +1332,15 @@ __device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, R *r) {
       for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] += f[c];
     }
   });
-  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+  for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
     int v = L.bs_list[k];
     const R *b = L.bbuf + 10 * (size_t)(v >> 1) + 5 * (v & 1);
 #pragma unroll
     for (int c = 0; c < 5; ++c) r[c] += b[c];
+  }
+  if (SUB > 1) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[c] = lsum<SUB>(r[c]);
   }
 }
 
@@ -904,13 +1358,15 @@ struct StageArgs {
   int has_src;               // level has a sourceTerm (gridNumber > 1)
 };
 
-template <typename R, int MODE, bool SA>
+template <typename R, int MODE, bool SA, int SUB>
 __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
-  int i = gtid_x();
+  int i, sub;
+  node_lanes<SUB>(i, sub);
   if (i >= L.n) return;
   int n = L.n;
   R r[6];
-  gather_rhs<R, SA>(L, i, r);
+  gather_rhs<R, SA, SUB>(L, i, sub, r);
+  if (sub) return;
   bool sa_on = SA && S.calc_turb;
   if (sa_on) r[5] += R(sa_source(L, P, i));
   if (MODE == MODE_RHS) {
@@ -1019,20 +1475,24 @@ __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
 }
 
 // restriction (mapFineToCoarse, DEF:12572-12690) as a gather over the fine
-// members of each coarse node: out[c*nc+k] = sum_i rw_i in[c*nf+i]
-template <typename RF, typename RC>
-__global__ void k_restrict(const int *cptr, const int *clist, const double *rw, const RF *in, RC *out, int nc,
-                           int nf, int ncomp) {
-  int k = gtid_x();
+// members of each coarse node: out[c*nc+k] = sum_i rw_i in[c*nf+i].  In the
+// nested device order the members of coarse node k are the contiguous fine
+// range [cptr[k], cptr[k+1]); SUB lanes per coarse node read it coalesced.
+template <typename RF, typename RC, int SUB>
+__global__ void k_restrict(const int *cptr, const double *rw, const RF *in, RC *out, int nc, int nf, int ncomp) {
+  int k, sub;
+  node_lanes<SUB>(k, sub);
   if (k >= nc) return;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
-    int i = clist[q];
+  for (int i = cptr[k] + sub; i < cptr[k + 1]; i += SUB) {
     double r = rw[i];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
       if (c < ncomp) acc[c] += r * double(in[(size_t)c * nf + i]);
   }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) acc[c] = lsum<SUB>(acc[c]);
+  if (sub) return;
 #pragma unroll
   for (int c = 0; c < 6; ++c)
     if (c < ncomp) out[(size_t)c * nc + k] = RC(acc[c]);

@@ -14,6 +14,8 @@
 #include <queue>
 #include <string>
 #include <vector>
+#include <cstdlib>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -134,6 +136,7 @@ struct Staged {
 struct Level {
   bool fp32 = false;
   int n = 0, ne = 0, nbs = 0, nb = 0, gnum = 0, nfine = 0;
+  int lanes = 1;  // lanes per node in the gather kernels (1, 4 or 16)
   bool has_coords = false;
   bool uploaded = false;
   std::vector<int> h2d, d2h;
@@ -153,6 +156,7 @@ struct Level {
   double *rw = nullptr;
   // monitor scratch
   double *m_sum = nullptr, *m_max = nullptr, *m_force = nullptr;
+  double *abuf = nullptr;  // adjoint side shares, 10 per side (allocated on first use)
   int *m_idx = nullptr;
 
   template <typename R>
@@ -177,6 +181,7 @@ struct flite_ctx {
   flite_params hp;
   DevParams P;
   int nl = 0, nbatch = 1;
+  int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
   std::vector<std::unique_ptr<Level>> lev;
   hipStream_t stream = nullptr;
   // kernel timing (flite_timing_enable)
@@ -199,6 +204,16 @@ template <typename F>
 int with_level(const Level &L, F f) {
   if (L.fp32) return f(L.view<float>());
   return f(L.view<double>());
+}
+
+// f(std::integral_constant<int, SUB>) for the lanes-per-node variants built
+template <typename F>
+int with_lanes(int lanes, F f) {
+  switch (lanes) {
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    default: return f(std::integral_constant<int, 1>{});
+  }
 }
 
 int check_launch() {
@@ -407,6 +422,9 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   L.gnum = level;
   L.fp32 = c->hp.coarse_fp32 && level > 1;
   L.n = S.n;
+  // enough lanes to give the chip >= ~1M threads on mid-size levels, and
+  // short per-lane gather chains on the small coarse ones
+  L.lanes = c->lanes_override ? c->lanes_override : (S.n >= (1 << 20) ? 1 : S.n >= (1 << 16) ? 4 : 16);
   L.ne = S.ne;
   L.nbs = S.nbs;
   L.nb = S.nb;
@@ -597,6 +615,8 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     std::vector<int> clist(F.n);
     std::vector<int> pos(cnt.begin(), cnt.end() - 1);
     for (int df = 0; df < F.n; ++df) clist[pos[f2c[df]]++] = df;  // fine members in device order
+    for (int df = 0; df < F.n; ++df)  // nested order: members of each coarse node are contiguous
+      if (clist[df] != df) return fail(FLITE_E_STATE, "fine level is not in nested order of its coarse level");
     TRY(P.upload(&L.f2c, f2c));
     TRY(P.upload(&L.rw, rw));
     TRY(P.upload(&L.cptr, cnt));
@@ -656,8 +676,11 @@ int op_grad(flite_ctx *c, Level &L) {
   Timed tm(c, "grad", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_grad_minmax<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    return check_launch();
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      hipLaunchKernelGGL((k_grad_minmax<R, SUB>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0, c->stream, V, P);
+      return check_launch();
+    });
   });
 }
 
@@ -696,18 +719,21 @@ int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
   Timed tm(c, "stage", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    dim3 g(nblk(L.n)), t(NTHR);
-#define LAUNCH_STAGE(M)                                                                  \
-  if (P.sa) hipLaunchKernelGGL((k_stage<R, M, true>), g, t, 0, c->stream, V, P, S);      \
-  else hipLaunchKernelGGL((k_stage<R, M, false>), g, t, 0, c->stream, V, P, S);
-    switch (mode) {
-      case MODE_RHS: LAUNCH_STAGE(MODE_RHS); break;
-      case MODE_STAGE: LAUNCH_STAGE(MODE_STAGE); break;
-      case MODE_COARSE: LAUNCH_STAGE(MODE_COARSE); break;
-      default: LAUNCH_STAGE(MODE_FINE); break;
-    }
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      dim3 g(nblk((long long)L.n * SUB)), t(NTHR);
+#define LAUNCH_STAGE(M)                                                                       \
+  if (P.sa) hipLaunchKernelGGL((k_stage<R, M, true, SUB>), g, t, 0, c->stream, V, P, S);      \
+  else hipLaunchKernelGGL((k_stage<R, M, false, SUB>), g, t, 0, c->stream, V, P, S);
+      switch (mode) {
+        case MODE_RHS: LAUNCH_STAGE(MODE_RHS); break;
+        case MODE_STAGE: LAUNCH_STAGE(MODE_STAGE); break;
+        case MODE_COARSE: LAUNCH_STAGE(MODE_COARSE); break;
+        default: LAUNCH_STAGE(MODE_FINE); break;
+      }
 #undef LAUNCH_STAGE
-    return check_launch();
+      return check_launch();
+    });
   });
 }
 
@@ -716,8 +742,12 @@ int op_timesteps(flite_ctx *c, Level &L) {
   Timed tm(c, "timesteps", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_timesteps<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    return check_launch();
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      return check_launch();
+    });
   });
 }
 
@@ -742,9 +772,14 @@ int op_viscous(flite_ctx *c, Level &L) {
   Timed tm(c, "viscous", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_visc_node<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    hipLaunchKernelGGL(k_visc_flux<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    return check_launch();
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      return check_launch();
+    });
   });
 }
 
@@ -752,8 +787,12 @@ int op_sa_timesteps(flite_ctx *c, Level &L) {
   DevParams P = c->P;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_sa_timesteps<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    return check_launch();
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      hipLaunchKernelGGL((k_gather<R, SUB, SATimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      return check_launch();
+    });
   });
 }
 
@@ -761,9 +800,60 @@ int op_sa_diffusion(flite_ctx *c, Level &L) {
   DevParams P = c->P;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_sa_diff1<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    hipLaunchKernelGGL(k_sa_diff2<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
-    return check_launch();
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      hipLaunchKernelGGL((k_gather<R, SUB, SADiff1Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      hipLaunchKernelGGL((k_gather<R, SUB, SADiff2Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      return check_launch();
+    });
+  });
+}
+
+// The viscous-flow terms of one smoothing step in two walks of the
+// incidence lists instead of up to six:
+//   pass A: [makeTimeSteps] + makeViscosityTerm pass 1 + [makeSADiffusion pass 1]
+//   pass B: makeViscosityTerm pass 2 + [makeTurbulentTimeSteps] + [makeSADiffusion pass 2]
+// Pass B only needs pass A's node results (mu, tau, vorticity, divergence,
+// grad nu~), which is the dependency order of the separate routines.  With
+// time steps in pass A they read the previous laminar viscosity, so the new
+// one reaches the gather records after the pass (k_mu_record).
+int op_step_terms(flite_ctx *c, Level &L, bool timesteps, bool sa_timesteps) {
+  DevParams P = c->P;
+  bool sa = P.sa;
+  Timed tm(c, "viscous", L.gnum);
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    return with_lanes(L.lanes, [&](auto sc) {
+      constexpr int SUB = decltype(sc)::value;
+      if (timesteps && sa) {
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      } else if (timesteps) {
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      } else if (sa) {
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      } else {
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      }
+      if (timesteps)
+        hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V);
+      if (sa && sa_timesteps) {
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SATimeStepPart<R>, SADiff2Part<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      } else if (sa) {
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SADiff2Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      } else {
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+                         c->stream, V, P);
+      }
+      return check_launch();
+    });
   });
 }
 
@@ -790,8 +880,9 @@ int op_restrict_array(flite_ctx *c, Level &F, Level &K, const void *fa, void *ka
   auto go = [&](auto *fin, auto *kout) {
     using RF = std::remove_const_t<std::remove_pointer_t<decltype(fin)>>;
     using RC = std::remove_pointer_t<decltype(kout)>;
-    hipLaunchKernelGGL((k_restrict<RF, RC>), dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, K.cptr, K.clist, K.rw, fin,
-                       kout, K.n, F.n, ncomp);
+    constexpr int SUB = 4;  // about 8 fine members per coarse node
+    hipLaunchKernelGGL((k_restrict<RF, RC, SUB>), dim3(nblk((long long)K.n * SUB)), dim3(NTHR), 0, c->stream, K.cptr,
+                       K.rw, fin, kout, K.n, F.n, ncomp);
     return check_launch();
   };
   if (F.fp32 && K.fp32) return go((const float *)fa, (float *)ka);
@@ -804,14 +895,12 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
   const flite_params &hp = c->hp;
   if (calc_diss) {
     TRY(op_prep(c, L, true));
-    TRY(op_timesteps(c, L));
     TRY(op_dissipation(c, L));
     if (hp.reynolds_number > 0.0) {
-      TRY(op_viscous(c, L));
-      if (c->P.sa) {
-        TRY(op_sa_timesteps(c, L));
-        TRY(op_sa_diffusion(c, L));
-      }
+      // makeTimeSteps, makeViscosityTerm, makeTurbulentTimeSteps, makeSADiffusion
+      TRY(op_step_terms(c, L, true, true));
+    } else {
+      TRY(op_timesteps(c, L));
     }
   }
   for (int k = 1; k <= 3; ++k) {
@@ -851,14 +940,13 @@ int do_restrict(flite_ctx *c, int fine_level) {
   TRY(op_bc_state(c, K, K.u, 1));
   TRY(op_copy(c, K, K.uprev, K.u, 6));
   TRY(op_prep(c, K, true));
-  TRY(op_timesteps(c, K));
   TRY(op_dissipation(c, K));
-  if (visc) TRY(op_viscous(c, K));
-  TRY(op_flux(c, K));
-  if (c->P.sa) {
-    TRY(op_sa_timesteps(c, K));
-    TRY(op_sa_diffusion(c, K));
+  if (visc) {
+    TRY(op_step_terms(c, K, true, true));
+  } else {
+    TRY(op_timesteps(c, K));
   }
+  TRY(op_flux(c, K));
   StageArgs S;
   S.alpha = 1.0;
   S.dtj = hp.cfl_number;
@@ -870,10 +958,9 @@ int do_restrict(flite_ctx *c, int fine_level) {
   // fine part (DEF:12236-12361)
   TRY(op_prep(c, F, true));
   TRY(op_dissipation(c, F));
-  if (visc) TRY(op_viscous(c, F));
+  if (visc) TRY(op_step_terms(c, F, false, false));
   TRY(op_grad(c, F));
   TRY(op_flux(c, F));
-  if (c->P.sa) TRY(op_sa_diffusion(c, F));
   S.has_src = F.gnum > 1;
   TRY(op_stage(c, F, MODE_FINE, S));
   // coarse forcing (DEF:12362-12428)
@@ -1026,6 +1113,13 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   auto c = std::make_unique<flite_ctx>();
   c->hp = *p;
   c->P = make_dev_params(*p);
+  if (const char *e = std::getenv("FLITE_GATHER_LANES")) {
+    int v = std::atoi(e);
+    if (v != 1 && v != 4 && v != 16) {
+      return fail(FLITE_E_ARG, "FLITE_GATHER_LANES must be 1, 4 or 16");
+    }
+    c->lanes_override = v;
+  }
   c->nl = n_levels;
   c->nbatch = n_batch;
   c->lev.resize(n_levels);
@@ -1306,6 +1400,94 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
   for (int j = 0; j < nres; ++j)
     if (std::isnan(res[j])) return fail(FLITE_E_NAN, "NaN in residual");
   return 0;
+}
+
+
+// ---- tangent / adjoint of the level-1 residual ---------------------------
+namespace {
+int check_linearisable(flite_ctx *c) {
+  TRY(check_ctx(c));
+  const Level &L = *c->lev[0];
+  const flite_params &hp = c->hp;
+  if (L.fp32) return fail(FLITE_E_UNSUPPORTED, "level 1 must be FP64");
+  if (L.has_coords && hp.hllc_flux_order > 1)
+    return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: first-order HLLC (HLLCFluxOrder=1) only");
+  if (hp.reynolds_number > 0.0 || hp.turbulence_model != 0)
+    return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: inviscid flow (ReynoldsNumber=0) only");
+  return 0;
+}
+
+// host (N, 5) column-major -> level-1 device array src (6 components)
+int put_vector5(flite_ctx *c, const double *x, int ld) {
+  Level &L = *c->lev[0];
+  if (!x || ld < L.n) return fail(FLITE_E_ARG, "bad input vector");
+  std::vector<double> h(6 * (size_t)L.n, 0.0);
+  for (int k = 0; k < 5; ++k)
+    for (int i = 0; i < L.n; ++i) h[(size_t)k * L.n + i] = x[(size_t)k * ld + i];
+  return flite_field_set(c, 1, FLITE_FIELD_SOURCE, h.data(), L.n);
+}
+
+int get_vector5(flite_ctx *c, double *y, int ld) {
+  Level &L = *c->lev[0];
+  if (!y || ld < L.n) return fail(FLITE_E_ARG, "bad output vector");
+  std::vector<double> h(6 * (size_t)L.n);
+  TRY(flite_field_get(c, 1, FLITE_FIELD_RHS, h.data(), L.n));
+  for (int k = 0; k < 5; ++k)
+    for (int i = 0; i < L.n; ++i) y[(size_t)k * ld + i] = h[(size_t)k * L.n + i];
+  return 0;
+}
+}  // namespace
+
+int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  TRY(put_vector5(c, du, ld_du));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  if (L.ne > 0) hipLaunchKernelGGL(k_flux_tangent, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_tangent, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  TRY(check_launch());
+  StageArgs S{};
+  TRY(op_stage(c, L, MODE_RHS, S));
+  return get_vector5(c, out, ld_out);
+}
+
+int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
+  TRY(put_vector5(c, lam, ld_lam));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  if (L.ne > 0) hipLaunchKernelGGL(k_flux_adjoint, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf);
+  if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_adjoint, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  hipLaunchKernelGGL(k_adjoint_gather, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, L.abuf, V.rhs);
+  TRY(check_launch());
+  return get_vector5(c, out, ld_out);
+}
+
+int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  if (!dJdU || ld < L.n) return fail(FLITE_E_ARG, "bad output vector");
+  const flite_params &hp = c->hp;
+  TRY(op_prep(c, L, true));
+  double res[7], mx[7], forces[9];
+  int idx[7];
+  TRY(flite_monitor(c, res, mx, idx, forces));
+  if (drag) *drag = forces[1];
+  double PI = 4.0 * std::atan(1.0);
+  double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
+  double t0 = std::cos(ar) * std::cos(br), t1 = std::sin(br), t2 = std::sin(ar);
+  LevelDev<double> V = L.view<double>();
+  size_t cnt = 6 * (size_t)L.n;
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.rhs, 0.0, cnt);
+  int nw = (int)L.wall_dev.size();
+  if (nw > 0)
+    hipLaunchKernelGGL(k_drag_state_gradient, dim3(nblk(nw)), dim3(NTHR), 0, c->stream, V, c->P, L.wall_nodes, nw,
+                       (const double *)L.warea, 2.0 / hp.reference_area, t0, t1, t2, hp.groundplane, V.rhs);
+  TRY(check_launch());
+  return get_vector5(c, dJdU, ld);
 }
 
 int flite_sync(flite_ctx *c) {

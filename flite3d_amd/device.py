@@ -47,6 +47,9 @@ def lib():
         L.flite_cycle.argtypes = [vp, i]
         L.flite_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
         L.flite_sync.argtypes = [vp]
+        L.flite_tangent_apply.argtypes = [vp, P(d), i, P(d), i]
+        L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
+        L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         _bound = L
     return _bound
 
@@ -144,6 +147,39 @@ class Solver:
             self._check(rc)
         return dict(res=np.array(res[:]), maxres=np.array(mx[:]), maxidx=np.array(idx[:]),
                     forces=np.array(forces[:]), rc=rc)
+
+
+    # ---- linearisation of the level-1 residual (first-order HLLC, inviscid)
+    def _vec5(self, x):
+        n = self.n(1)
+        return np.ascontiguousarray(np.asarray(x, dtype=np.float64)[:5].reshape(5, n))
+
+    def tangent(self, du):
+        """(dR/dU) du of the level-1 residual at the current state."""
+        n = self.n(1)
+        x = self._vec5(du)
+        out = np.zeros((5, n))
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_tangent_apply(self.ctx, x.ctypes.data_as(D), n, out.ctypes.data_as(D), n))
+        return out
+
+    def adjoint(self, lam):
+        """(dR/dU)^T lam of the level-1 residual at the current state."""
+        n = self.n(1)
+        x = self._vec5(lam)
+        out = np.zeros((5, n))
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_adjoint_apply(self.ctx, x.ctypes.data_as(D), n, out.ctypes.data_as(D), n))
+        return out
+
+    def drag_gradient(self):
+        """(pressure drag, dDrag/dU) at the current state."""
+        n = self.n(1)
+        out = np.zeros((5, n))
+        drag = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_drag_state_gradient(self.ctx, ctypes.byref(drag), out.ctypes.data_as(D), n))
+        return drag.value, out
 
 
 def _bind_extra():

@@ -177,6 +177,24 @@ enum {
 };
 int flite_apply(flite_ctx *ctx, int level, int op);
 
+/* ---- linearisation (discrete tangent and adjoint) ---------------------- */
+/* The level-1 convective residual R(U) of makeRHS (mgnsg3d_def_par.F90:
+ * 1525-2512: interior HLLC sides and boundary sides, as FLITE_OP_RHS
+ * computes it) differentiated exactly at the current state, for first-order
+ * HLLC and inviscid flow (other settings return FLITE_E_UNSUPPORTED).
+ * Vectors are (N, 5) column-major in host numbering with leading dimension
+ * ld.  The reference has no differentiated code; these entry points serve
+ * the sensitivity calculations a maintainer adds on top of it.
+ *   flite_tangent_apply:  out = (dR/dU) du
+ *   flite_adjoint_apply:  out = (dR/dU)^T lam  (a reversed side loop that
+ *                         re-evaluates each side's states; nothing stored)
+ *   flite_drag_state_gradient: the pressure drag of calculateLiftAndDrag
+ *                         (mgnsg3d_def_par.F90:14757-14803) at the current
+ *                         state and its partial derivative dJ/dU. */
+int flite_tangent_apply(flite_ctx *ctx, const double *du, int ld_du, double *out, int ld_out);
+int flite_adjoint_apply(flite_ctx *ctx, const double *lam, int ld_lam, double *out, int ld_out);
+int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld);
+
 /* Wait for all work queued by the context. */
 int flite_sync(flite_ctx *ctx);
 

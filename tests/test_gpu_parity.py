@@ -157,3 +157,37 @@ def test_cycles_and_monitor(case, scheme):
     nc = 6 if case == "rans" else 5
     e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
     assert e.max() < 1e-9, e
+
+
+@pytest.mark.parametrize("lanes", ["1", "4", "16"])
+def test_gather_lanes(lanes, monkeypatch):
+    """Every lanes-per-node variant of the gather kernels (forced on all
+    levels) against the oracle: the single operations to the per-operation
+    bar, then three growing-V cycles with restriction on every level."""
+    print(f"[case] gather lanes {lanes}", flush=True)
+    monkeypatch.setenv("FLITE_GATHER_LANES", lanes)
+    lv = levels(8, 3, first_cell=0.005)
+    s, o = pair(rans(multigridscheme=3), lv)
+    s.init_freestream(); o.init_freestream()
+    rng = np.random.default_rng(7)
+    perturb(s, o, 1, rng)
+    u = o.get(1, "U")
+    u[5] = 0.1 + 5.0 * rng.uniform(0, 1, size=u.shape[1])
+    s.set(1, "U", u); o.set(1, "U", u)
+    for x in (s, o):
+        x.apply(1, "PRESSURE")
+        x.apply(1, "TIMESTEPS")
+        x.apply(1, "VISCOUS")
+        x.apply(1, "SA_TIMESTEPS")
+        x.apply(1, "SA_DIFFUSION")
+        x.apply(1, "RHS")
+    for f in ("DT", "DISSIPATION", "DT_TURB", "TURB_DIFFUSION", "RHS", "GRADIENT"):
+        e = rel_err(s.get(1, f), o.get(1, f))
+        assert e.max() < 1e-11, (f, e)
+    s.init_freestream(); o.init_freestream()
+    for c in (1, 2, 3):
+        s.cycle(c); o.cycle(c)
+    ms, mo = s.monitor(), o.monitor()
+    assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
+    e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
+    assert e.max() < 1e-9, e
