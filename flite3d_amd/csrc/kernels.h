@@ -382,9 +382,8 @@ __device__ __forceinline__ void side_flux(const LevelDev<R> &L, const DevParams 
   T inv = rnorm1 > T(0) ? T(1) / rnorm1 : rnorm1;
   T rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
   if constexpr (ORDER > 1) {
-    T G1[GREC], G2[GREC];
-    ldrec<GREC>(L.grec + GREC * (size_t)ia, G1);
-    ldrec<GREC>(L.grec + GREC * (size_t)ib, G2);
+    const T *G1 = L.grec + GREC * (size_t)ia;
+    const T *G2 = L.grec + GREC * (size_t)ib;
     const T *X1 = L.xyz + 3 * (size_t)ia, *X2 = L.xyz + 3 * (size_t)ib;
     T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
     T hl = T(0.5) * T(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 #include <cstdlib>
+#include <cstdint>
 #include <type_traits>
 
 #include "kernels.h"
@@ -391,10 +392,73 @@ int stage_level(flite_ctx *c, int level, const flite_level_view *v, Staged &S) {
 // members of each coarse node contiguously, in the coarse order, so that
 // restriction and prolongation are contiguous-range operations and each
 // agglomerate's nodes share cache lines.
+// Z-order (Morton) of points: each axis is first replaced by its rank among
+// the points, so that graded spacings (a boundary layer) still spread over
+// all the curve's cells; ties keep the input order.  Returns host -> device.
+std::vector<int> morton_order(const std::vector<double> &x, const std::vector<double> &y,
+                              const std::vector<double> &z) {
+  int n = (int)x.size();
+  auto ranks = [n](const std::vector<double> &v) {
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return v[a] < v[b]; });
+    std::vector<uint32_t> r(n);
+    for (int k = 0; k < n; ++k) r[idx[k]] = (uint32_t)((uint64_t)k * ((1u << 21) - 1) / std::max(1, n - 1));
+    return r;
+  };
+  auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  std::vector<uint32_t> rx = ranks(x), ry = ranks(y), rz = ranks(z);
+  std::vector<uint64_t> key(n);
+  for (int i = 0; i < n; ++i) key[i] = spread(rx[i]) | spread(ry[i]) << 1 | spread(rz[i]) << 2;
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<int> h2d(n);
+  for (int d = 0; d < n; ++d) h2d[idx[d]] = d;
+  return h2d;
+}
+
+// order of the coarsest level: Z-order of the centroids of its level-1
+// members when level 1 has coordinates (FLITE_NODE_ORDER=rcm selects reverse
+// Cuthill-McKee of the coarsest side graph instead)
+std::vector<int> coarsest_order(const std::vector<Staged> &st) {
+  int nl = (int)st.size();
+  const Staged &F = st[0], &C = st[nl - 1];
+  const char *e = std::getenv("FLITE_NODE_ORDER");
+  bool rcm = e && std::string(e) == "rcm";
+  if (rcm || F.xyz.empty()) return rcm_order(C.n, C.e1, C.e2);
+  std::vector<int> m(F.n);
+  std::iota(m.begin(), m.end(), 0);
+  for (int l = 1; l < nl; ++l)
+    for (int i = 0; i < F.n; ++i) m[i] = st[l].f2c[m[i]];
+  std::vector<double> x(C.n, 0.0), y(C.n, 0.0), z(C.n, 0.0), cnt(C.n, 0.0);
+  for (int i = 0; i < F.n; ++i) {
+    x[m[i]] += F.xyz[i];
+    y[m[i]] += F.xyz[(size_t)F.n + i];
+    z[m[i]] += F.xyz[2 * (size_t)F.n + i];
+    cnt[m[i]] += 1.0;
+  }
+  for (int k = 0; k < C.n; ++k) {
+    double w = cnt[k] > 0 ? 1.0 / cnt[k] : 0.0;
+    x[k] *= w;
+    y[k] *= w;
+    z[k] *= w;
+  }
+  return morton_order(x, y, z);
+}
+
 std::vector<std::vector<int>> nested_orders(const std::vector<Staged> &st) {
   int nl = (int)st.size();
   std::vector<std::vector<int>> h2d(nl);
-  h2d[nl - 1] = rcm_order(st[nl - 1].n, st[nl - 1].e1, st[nl - 1].e2);
+  h2d[nl - 1] = coarsest_order(st);
   for (int l = nl - 2; l >= 0; --l) {
     const Staged &K = st[l + 1];
     int nc = K.n, nf = st[l].n;

@@ -206,6 +206,30 @@ module flite_gpu
        integer(c_int), value :: level, op
      end function flite_apply
 
+     integer(c_int) function flite_tangent_apply(ctx, du, ld_du, out, ld_out) bind(C, name="flite_tangent_apply")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: du(*)
+       integer(c_int), value :: ld_du, ld_out
+       real(c_double), intent(out) :: out(*)
+     end function flite_tangent_apply
+
+     integer(c_int) function flite_adjoint_apply(ctx, lam, ld_lam, out, ld_out) bind(C, name="flite_adjoint_apply")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: lam(*)
+       integer(c_int), value :: ld_lam, ld_out
+       real(c_double), intent(out) :: out(*)
+     end function flite_adjoint_apply
+
+     integer(c_int) function flite_drag_state_gradient(ctx, drag, dJdU, ld) bind(C, name="flite_drag_state_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: drag
+       real(c_double), intent(out) :: dJdU(*)
+       integer(c_int), value :: ld
+     end function flite_drag_state_gradient
+
      integer(c_int) function flite_sync(ctx) bind(C, name="flite_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
