@@ -32,6 +32,22 @@ PEAK_HBM_GBS = 8000.0
 ALGO_BYTES_PER_EDGE = {"flux": 76.0, "grad": 67.9, "stage": 19.7}
 
 
+def flux_traffic():
+    """HBM bytes per launch of the level-1 flux kernel, from the committed
+    summary of the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over
+    this benchmark, scaled by the copy-kernel calibration it records; None
+    when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "k_flux_hllc2_sa_rans128_pmc.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -45,9 +61,9 @@ def build_case(n, n_levels):
     return lv
 
 
-def params(fp32):
+def params(fp32, re=6.0e6, mach=0.7, alpha=1.0):
     from flite3d_amd.params import InputVariables
-    iv = InputVariables(reynoldsnumber=6.0e6, machnumber=0.7, alpha=1.0, fluxtype=4,
+    iv = InputVariables(reynoldsnumber=re, machnumber=mach, alpha=alpha, fluxtype=4,
                         hllcfluxorder=2, automaticlipping=1, hllcclipping=1.0,
                         gradientmethod=1, turbulencemodel=1, wallsareisentropic=True,
                         cflnumber=1.0, turbulentcflnumber=1.0, multigridscheme=3,
@@ -55,9 +71,9 @@ def params(fp32):
     return iv.to_struct(coarse_fp32=fp32)
 
 
-def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1"):
+def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None):
     from flite3d_amd import device
-    s = device.Solver(params(fp32), levels)
+    s = device.Solver(prm(fp32) if prm else params(fp32), levels)
     s.init_freestream()
     c = 1
     r0 = None
@@ -115,6 +131,122 @@ def run_converge(levels, fp32, drop, cap_s, every=250):
                 lift=hist[-1][4], history=hist)
 
 
+def euler_params(mach, alpha, order=1, scheme=1, cfl=2.0):
+    from flite3d_amd.params import InputVariables
+    return InputVariables(reynoldsnumber=0.0, machnumber=mach, alpha=alpha, fluxtype=4,
+                          hllcfluxorder=order, automaticlipping=1, cflnumber=cfl,
+                          multigridscheme=scheme, restartnumber=0).to_struct()
+
+
+def bench_config1(cycles):
+    """Euler on the jittered 20^3 bump box, 3 levels, 200 cycles: the GPU and
+    the FP64 CPU restatement (one core) on the same cycles.  The reference's
+    Roe flux cannot serve (its interior Roe branch is defective), so this is
+    the parity mode: HLLC-1, 3-stage RK, CFL 2, V-cycle."""
+    from flite3d_amd import device, mesh
+    import oracle
+    lv = mesh.bump_levels(20, 3, jitter=0.15, seed=0)
+    P = euler_params(0.5, 2.0)
+    E = [l.n_sides for l in lv]
+    s = device.Solver(P, lv)
+    s.init_freestream()
+    evals = None
+    s.sync()
+    t0 = time.perf_counter()
+    for c in range(1, cycles + 1):
+        s.cycle(c)
+        ms = s.monitor()
+        if evals is None:
+            evals = device.cycle_evaluations(s)
+    s.sync()
+    tg = time.perf_counter() - t0
+    ug = s.get(1, "U")[:5]
+    s.close()
+    o = oracle.Oracle(P, lv)
+    o.init_freestream()
+    t0 = time.perf_counter()
+    for c in range(1, cycles + 1):
+        o.cycle(c)
+        mo = o.monitor()
+    tc = time.perf_counter() - t0
+    uo = o.get(1, "U")[:5]
+    o.close()
+    err = max(float(np.abs(ug[k] - uo[k]).max() / np.abs(uo[k]).max()) for k in range(5))
+    ge = sum(e * n for e, n in zip(E, evals)) * cycles
+    return {"metric": "MG cycles/s, Euler 20^3 jittered bump box, 3 levels", "config": 1,
+            "cycles": cycles, "gpu_ms_per_cycle": round(1e3 * tg / cycles, 4),
+            "gpu_gedges": round(ge / tg / 1e9, 4), "cpu_ms_per_cycle": round(1e3 * tc / cycles, 3),
+            "cpu_cores": 1, "cpu_kind": "port", "gpu_over_cpu": round(tc / tg, 2),
+            "state_rel_err_gpu_vs_cpu": err, "drag_gpu": float(ms["forces"][1]),
+            "drag_cpu": float(mo["forces"][1]), "nodes": lv[0].n_nodes, "sides": E}
+
+
+def bench_config3(cycles, ndir=16, h=1e-7):
+    """Discrete adjoint case: Euler M=0.8, alpha=1.25 on the 64^3 bump box,
+    first-order HLLC, after `cycles` growing-V cycles; 16 random unit
+    directions (seed 1): tangent vs central differences of the residual,
+    the tangent/adjoint dot-product identity, and the drag state gradient
+    vs central differences of the drag."""
+    from flite3d_amd import device, mesh
+    lv = mesh.bump_levels(64, 3)
+    s = device.Solver(euler_params(0.8, 1.25, scheme=3), lv)
+    s.init_freestream()
+    r0 = None
+    for c in range(1, cycles + 1):
+        s.cycle(c)
+        m = s.monitor()
+        r0 = r0 or m["res"][0]
+    drop = float(np.log10(m["res"][0] / r0))
+    u = s.get(1, "U")
+    n = u.shape[1]
+
+    def resid(x):
+        s.set(1, "U", x)
+        s.apply(1, "PRESSURE")
+        s.apply(1, "RHS")
+        return s.get(1, "RHS")[:5]
+
+    rng = np.random.default_rng(1)
+    fd_err, dot_err, drag_err = [], [], []
+    t_tan = t_adj = 0.0
+    s.set(1, "U", u)
+    drag0, g = s.drag_gradient()
+    for _ in range(ndir):
+        du = rng.standard_normal((5, n))
+        du /= np.linalg.norm(du)
+        lam = rng.standard_normal((5, n))
+        lam /= np.linalg.norm(lam)
+        s.set(1, "U", u)
+        t0 = time.perf_counter()
+        jt = s.tangent(du)
+        t_tan += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ja = s.adjoint(lam)
+        t_adj += time.perf_counter() - t0
+        a, b = float(np.sum(lam * jt)), float(np.sum(ja * du))
+        dot_err.append(abs(a - b) / max(abs(a), abs(b)))
+        up, um = u.copy(), u.copy()
+        up[:5] += h * du
+        um[:5] -= h * du
+        fd = (resid(up) - resid(um)) / (2 * h)
+        fd_err.append(float(np.linalg.norm(fd - jt) / np.linalg.norm(jt)))
+        s.set(1, "U", up)
+        dp, _ = s.drag_gradient()
+        s.set(1, "U", um)
+        dm, _ = s.drag_gradient()
+        an = float(np.sum(g * du))
+        drag_err.append(abs((dp - dm) / (2 * h) - an) / max(abs(an), 1e-300))
+    s.close()
+    return {"metric": "discrete adjoint checks, Euler 64^3 bump box", "config": 3,
+            "cycles": cycles, "residual_drop_log10": round(drop, 3), "directions": ndir, "fd_step": h,
+            "tangent_vs_fd_max_rel_err": max(fd_err), "dot_product_max_rel_err": max(dot_err),
+            "drag_gradient_vs_fd_max_rel_err": max(drag_err),
+            "ms_per_tangent_apply": round(1e3 * t_tan / ndir, 3),
+            "ms_per_adjoint_apply": round(1e3 * t_adj / ndir, 3),
+            "note": "apply times include host<->device transfer of the vectors",
+            "drag": drag0, "nodes": lv[0].n_nodes, "sides": lv[0].n_sides}
+
+
 def cpu_baseline(levels, budget_s=30.0):
     """The FP64 CPU restatement (oracle, one core) on one cycle of the same
     workload, or fewer fine-level smoothing steps if a cycle exceeds the
@@ -144,11 +276,24 @@ def main():
     ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the timed steps: run each precision mode to a 1e-8 "
                          "monitor drop or this many seconds, and report the histories")
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 5),
+                    help="1: Euler 20^3 GPU vs CPU; 2: the RANS headline (default); "
+                         "3: adjoint checks at 64^3; 5: RANS-SA 200^3, 5 levels")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if a.config == 1:
+        print(json.dumps(bench_config1(200)), flush=True)
+        return
+    if a.config == 3:
+        print(json.dumps(bench_config3(300)), flush=True)
+        return
+    prm = None
+    if a.config == 5:
+        a.n, a.levels = 200, 5
+        prm = lambda fp32: params(fp32, re=1.0e7, mach=0.75, alpha=2.0)  # noqa: E731
     levels = build_case(a.n, a.levels)
     sides = [l.n_sides for l in levels]
     if a.converge > 0:
@@ -162,7 +307,7 @@ def main():
     results = {}
     for tag, fp32 in ((("fp64", False),) if a.fp64_only else (("mixed", True), ("fp64", False))):
         log(f"[bench] GPU run {tag}: warmup {a.warmup}, steps {a.steps}")
-        r = run_gpu(levels, fp32, a.steps, a.warmup)
+        r = run_gpu(levels, fp32, a.steps, a.warmup, prm=prm)
         edges = sum(e * n for e, n in zip(sides, r["evals"]))
         r["gedges"] = edges * a.steps / r["time"] / 1e9
         r["ms_per_cycle"] = 1e3 * r["time"] / a.steps
@@ -195,7 +340,7 @@ def main():
                    "residual_drop_log10": round(float(np.log10(head["res"] / head["res0"])), 3),
                    "time_to_1e-8_drop": None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": flux_traffic(),
                      "kernel": "k_flux<double,2,SA> (level 1)",
                      "avg_launch_ms": round(k_avg_ms, 4)},
         "cpu_baseline": cpu,
