@@ -247,6 +247,53 @@ def bench_config3(cycles, ndir=16, h=1e-7):
             "drag": drag0, "nodes": lv[0].n_nodes, "sides": lv[0].n_sides}
 
 
+def bench_config4(nbatch, steps, warmup):
+    """Batched conditions: nbatch Euler solves on the jittered 64^3 bump box
+    in one context (every launch covers all conditions), M ~ U[0.3, 0.8],
+    alpha ~ U[-4, 8] deg (seed 2), FP32 coarse / FP64 fine, HLLC-2, growing
+    V; against the same conditions one context at a time."""
+    from flite3d_amd import device, mesh
+    lv = mesh.bump_levels(64, 4, jitter=0.15, seed=0)
+    rng = np.random.default_rng(2)
+    mach = rng.uniform(0.3, 0.8, nbatch)
+    alpha = rng.uniform(-4.0, 8.0, nbatch)
+    from flite3d_amd.params import InputVariables
+    ps = [InputVariables(reynoldsnumber=0.0, machnumber=float(m), alpha=float(a), fluxtype=4,
+                         hllcfluxorder=2, automaticlipping=1, cflnumber=2.0, multigridscheme=3,
+                         restartnumber=0).to_struct(coarse_fp32=True) for m, a in zip(mach, alpha)]
+    E = [l.n_sides for l in lv]
+
+    def timed(plist):
+        s = device.Solver(plist, lv)
+        s.init_freestream()
+        for c in range(1, warmup + 1):
+            s.cycle(c)
+        evals = device.cycle_evaluations(s)
+        s.sync()
+        t0 = time.perf_counter()
+        for c in range(warmup + 1, warmup + steps + 1):
+            s.cycle(c)
+        s.sync()
+        t = time.perf_counter() - t0
+        s.close()
+        return t, evals
+
+    tb, evals = timed(ps)
+    ts = 0.0
+    nsingle = min(nbatch, 8)
+    for p in ps[:nsingle]:
+        ts += timed([p])[0]
+    ts *= nbatch / nsingle
+    ge = sum(e * n for e, n in zip(E, evals)) * steps * nbatch
+    return {"metric": "MG cycles/s for a batch of conditions, Euler 64^3 jittered bump box", "config": 4,
+            "n_conditions": nbatch, "steps": steps, "batched_ms_per_cycle": round(1e3 * tb / steps, 3),
+            "batched_condition_cycles_per_s": round(nbatch * steps / tb, 2),
+            "batched_gedges": round(ge / tb / 1e9, 4),
+            "one_at_a_time_ms_per_cycle": round(1e3 * ts / steps, 3),
+            "batching_speedup": round(ts / tb, 3), "single_runs_timed": nsingle,
+            "nodes": lv[0].n_nodes, "sides": E}
+
+
 def cpu_baseline(levels, budget_s=30.0):
     """The FP64 CPU restatement (oracle, one core) on one cycle of the same
     workload, or fewer fine-level smoothing steps if a cycle exceeds the
@@ -276,9 +323,10 @@ def main():
     ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the timed steps: run each precision mode to a 1e-8 "
                          "monitor drop or this many seconds, and report the histories")
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
                     help="1: Euler 20^3 GPU vs CPU; 2: the RANS headline (default); "
-                         "3: adjoint checks at 64^3; 5: RANS-SA 200^3, 5 levels")
+                         "3: adjoint checks at 64^3; 4: 64 batched conditions at 64^3; "
+                         "5: RANS-SA 200^3, 5 levels")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -289,6 +337,9 @@ def main():
         return
     if a.config == 3:
         print(json.dumps(bench_config3(300)), flush=True)
+        return
+    if a.config == 4:
+        print(json.dumps(bench_config4(64, a.steps, a.warmup)), flush=True)
         return
     prm = None
     if a.config == 5:

@@ -38,6 +38,16 @@ struct DevParams {
   double clip, max_inc, min_rho, max_turb, turb_k;
   double beta0;  // Davis wave-speed factor sqrt((gamma - 1) / (2 gamma))
   double prolong_relax, turb_prolong_relax, mg_bc_relax;
+  // batched conditions: per condition (Mach, inflowField(1:6,1), spare),
+  // 8 doubles each; null for a single condition
+  const double *cond;
+  __device__ __forceinline__ void to_batch(int b) {
+    if (!cond) return;
+    const double *q = cond + 8 * (size_t)b;
+    mach = q[0];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) inflow[0][k] = q[1 + k];
+  }
 };
 
 // node boundary class from nodeIndicatorArray ranges; bit 4 marks host
@@ -62,6 +72,33 @@ struct LevelDev {
   const int *ncls;
   R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
+  // batched conditions: nb copies of every state array, condition-major;
+  // gstride = gradient-record elements per condition (0 if none)
+  int nb;
+  long long gstride;
+  __device__ __forceinline__ void to_batch(int b) {
+    if (b == 0) return;
+    size_t bn = (size_t)b * (size_t)n;
+    u += 6 * bn;
+    uprev += 6 * bn;
+    rhs += 6 * bn;
+    diss += 5 * bn;
+    src += 6 * bn;
+    p += bn;
+    dt += bn;
+    dtt += bn;
+    mu += bn;
+    vort += bn;
+    divg += bn;
+    tdiff += bn;
+    rec0 += 8 * bn;
+    grec += (size_t)b * (size_t)gstride;
+    tau += 9 * bn;
+    dnu += 4 * bn;
+    fbuf += (size_t)b * 6 * (size_t)ne;
+    bbuf += (size_t)b * 10 * (size_t)nbs;
+    ws += 3 * bn;
+  }
 };
 
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -79,6 +116,16 @@ __device__ __forceinline__ int xcd_block() {
   return start + k;
 }
 __device__ __forceinline__ int gtid_x() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// Batched conditions: logical workgroup lb serves condition b = lb % nb on
+// block lb / nb of the nodes or sides, so consecutive workgroups gather the
+// same mesh data for successive conditions and find it in L2.  nb = 1 is
+// the plain index.
+__device__ __forceinline__ int btid(int nb, int &b) {
+  int lb = xcd_block();
+  b = lb % nb;
+  return (lb / nb) * blockDim.x + threadIdx.x;
+}
 
 template <typename R>
 __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
@@ -171,8 +218,8 @@ __device__ __forceinline__ void ldrec(const R *p, R *o) {
 // on small coarse levels SUB > 1 puts more lanes on the chip and shortens
 // each lane's chain of dependent gathers.
 template <int SUB>
-__device__ __forceinline__ void node_lanes(int &i, int &sub) {
-  int t = gtid_x();
+__device__ __forceinline__ void node_lanes(int nb, int &b, int &i, int &sub) {
+  int t = btid(nb, b);
   i = t / SUB;
   sub = t % SUB;
 }
@@ -226,7 +273,10 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, int sub
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_prep(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= L.n) return;
   int n = L.n;
   R r = L.u[i], m1 = L.u[n + i], m2 = L.u[2 * n + i], m3 = L.u[3 * n + i], E = L.u[4 * n + i];
@@ -247,7 +297,10 @@ __global__ void k_prep(LevelDev<R> L, DevParams P) {
 // gather records from the current state without recomputing p (parity ops)
 template <typename R>
 __global__ void k_pack(LevelDev<R> L, DevParams P) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= L.n) return;
   int n = L.n;
   R r = L.u[i];
@@ -268,7 +321,10 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 template <typename R, int SUB>
 __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   int i, sub;
-  node_lanes<SUB>(i, sub);
+  int bb_;
+  node_lanes<SUB>(L.nb, bb_, i, sub);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
@@ -485,7 +541,10 @@ __device__ __forceinline__ void side_flux(const LevelDev<R> &L, const DevParams 
 // fbuf[6*e + c]: flux leaving ea towards eb (+ to ea, - to eb).
 template <typename R, int ORDER, bool SA, int NSIG>
 __global__ void k_flux(LevelDev<R> L, DevParams P) {
-  int e = gtid_x();
+  int bb_;
+  int e = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (e >= L.ne) return;
   int ia = L.ea[e], ib = L.eb[e];
   R A[8], B[8];
@@ -546,7 +605,10 @@ __device__ __forceinline__ void farfield_roe(const double *inf, const R *q, R gm
 // bbuf[10*s + 5*end + c]
 template <typename R>
 __global__ void k_bflux(LevelDev<R> L, DevParams P) {
-  int s = gtid_x();
+  int bb_;
+  int s = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (s >= L.nbs) return;
   int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
   R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
@@ -1160,7 +1222,10 @@ struct SADiff2Part {
 template <typename R, int SUB, typename A, typename B, typename C>
 __global__ void k_gather(LevelDev<R> L, DevParams P) {
   int i, sub;
-  node_lanes<SUB>(i, sub);
+  int bb_;
+  node_lanes<SUB>(L.nb, bb_, i, sub);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= L.n) return;
   const R *qi = L.rec0 + 8 * (size_t)i;
   A a;
@@ -1201,7 +1266,9 @@ __global__ void k_gather(LevelDev<R> L, DevParams P) {
 // new laminar viscosity into the gather records after a fused pass
 template <typename R>
 __global__ void k_mu_record(LevelDev<R> L) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
   if (i >= L.n) return;
   L.rec0[8 * (size_t)i + 7] = L.mu[i];
 }
@@ -1360,7 +1427,10 @@ struct StageArgs {
 template <typename R, int MODE, bool SA, int SUB>
 __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
   int i, sub;
-  node_lanes<SUB>(i, sub);
+  int bb_;
+  node_lanes<SUB>(L.nb, bb_, i, sub);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= L.n) return;
   int n = L.n;
   R r[6];
@@ -1448,9 +1518,12 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
 // simple node kernels
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_copy_state(R *dst, const R *src, int n, int ncomp) {
-  int i = gtid_x();
+__global__ void k_copy_state(R *dst, const R *src, int n, int ncomp, int nb) {
+  int bb_;
+  int i = btid(nb, bb_);
   if (i >= n) return;
+  dst += (size_t)bb_ * 6 * (size_t)n;
+  src += (size_t)bb_ * 6 * (size_t)n;
   for (int c = 0; c < ncomp; ++c) dst[(size_t)c * n + i] = src[(size_t)c * n + i];
 }
 
@@ -1463,7 +1536,11 @@ __global__ void k_fill(R *dst, R v, size_t cnt) {
 // node BC application on a state array (setBCsOnSolutionField + outer)
 template <typename R>
 __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  a += (size_t)bb_ * 6 * (size_t)L.n;
   if (i >= L.n) return;
   int n = L.n;
   R q[6];
@@ -1478,10 +1555,13 @@ __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
 // nested device order the members of coarse node k are the contiguous fine
 // range [cptr[k], cptr[k+1]); SUB lanes per coarse node read it coalesced.
 template <typename RF, typename RC, int SUB>
-__global__ void k_restrict(const int *cptr, const double *rw, const RF *in, RC *out, int nc, int nf, int ncomp) {
-  int k, sub;
-  node_lanes<SUB>(k, sub);
+__global__ void k_restrict(const int *cptr, const double *rw, const RF *in, RC *out, int nc, int nf, int ncomp,
+                           int nb, long long in_bs, long long out_bs) {
+  int k, sub, bb_;
+  node_lanes<SUB>(nb, bb_, k, sub);
   if (k >= nc) return;
+  in += (size_t)bb_ * (size_t)in_bs;
+  out += (size_t)bb_ * (size_t)out_bs;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int i = cptr[k] + sub; i < cptr[k + 1]; i += SUB) {
     double r = rw[i];
@@ -1501,7 +1581,11 @@ __global__ void k_restrict(const int *cptr, const double *rw, const RF *in, RC *
 // formed: inject, relax, add, BCs, clamps on the fine level
 template <typename RF, typename RC>
 __global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, const RC *crhs, int nc, int ncomp_add) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(F.nb, bb_);
+  F.to_batch(bb_);
+  P.to_batch(bb_);
+  crhs += (size_t)bb_ * 6 * (size_t)nc;
   if (i >= F.n) return;
   int n = F.n;
   int k = f2c[i];
@@ -1525,7 +1609,10 @@ __global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, cons
 // coarse correction v_H = u_H - BC(R u_h), BC'd as an increment
 template <typename R>
 __global__ void k_coarse_correction(LevelDev<R> K, DevParams P) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(K.nb, bb_);
+  K.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= K.n) return;
   int n = K.n;
   R r[6];
@@ -1539,7 +1626,10 @@ __global__ void k_coarse_correction(LevelDev<R> K, DevParams P) {
 // u_H += rhs_H (DEF:12362-12428)
 template <typename R>
 __global__ void k_restrict_finish(LevelDev<R> K, DevParams P) {
-  int i = gtid_x();
+  int bb_;
+  int i = btid(K.nb, bb_);
+  K.to_batch(bb_);
+  P.to_batch(bb_);
   if (i >= K.n) return;
   int n = K.n;
   R r[6];
@@ -1559,7 +1649,8 @@ __global__ void k_restrict_finish(LevelDev<R> K, DevParams P) {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_monitor_partial(LevelDev<R> L, int nres, const int *host_index, double *part_sum, double *part_max,
-                                  int *part_idx) {
+                                  int *part_idx, int bsel) {
+  L.to_batch(bsel);
   __shared__ double ssum[NTHR];
   __shared__ double smax[NTHR];
   __shared__ int sidx[NTHR];
@@ -1601,7 +1692,9 @@ __global__ void k_monitor_partial(LevelDev<R> L, int nres, const int *host_index
 template <typename R>
 __global__ void k_force_terms(LevelDev<R> L, DevParams P, const int *wall_nodes, int nw, const R *warea, double *out,
                               double f0, double f1, double f2, double n10, double n11, double n12, double n20,
-                              double n21, double n22, double groundplane, double d0, double d1, double d2) {
+                              double n21, double n22, double groundplane, double d0, double d1, double d2, int bsel) {
+  L.to_batch(bsel);
+  P.to_batch(bsel);
   int k = gtid();
   if (k >= nw) return;
   int ip = wall_nodes[k];

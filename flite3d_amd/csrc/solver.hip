@@ -138,6 +138,8 @@ struct Level {
   bool fp32 = false;
   int n = 0, ne = 0, nbs = 0, nb = 0, gnum = 0, nfine = 0;
   int lanes = 1;  // lanes per node in the gather kernels (1, 4 or 16)
+  int nbatch = 1;         // conditions sharing this mesh (state arrays are nbatch-fold)
+  long long gstride = 0;  // gradient-record elements per condition
   bool has_coords = false;
   bool uploaded = false;
   std::vector<int> h2d, d2h;
@@ -174,6 +176,8 @@ struct Level {
     v.p = (R *)p; v.dt = (R *)dt; v.dtt = (R *)dtt; v.mu = (R *)mu; v.vort = (R *)vort;
     v.divg = (R *)divg; v.tdiff = (R *)tdiff; v.rec0 = (R *)rec0; v.grec = (R *)grec;
     v.tau = (R *)tau; v.dnu = (R *)dnu; v.fbuf = (R *)fbuf; v.bbuf = (R *)bbuf; v.ws = (R *)ws;
+    v.nb = nbatch;
+    v.gstride = gstride;
     return v;
   }
 };
@@ -183,6 +187,9 @@ struct flite_ctx {
   DevParams P;
   int nl = 0, nbatch = 1;
   int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
+  std::vector<flite_params> hp_b;  // per condition (batched contexts)
+  double *cond_dev = nullptr;      // per-condition Mach and inflow state on the device
+  int sel = 0;                     // condition that field/state/monitor calls act on
   std::vector<std::unique_ptr<Level>> lev;
   hipStream_t stream = nullptr;
   // kernel timing (flite_timing_enable)
@@ -488,6 +495,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   L.n = S.n;
   // enough lanes to give the chip >= ~1M threads on mid-size levels, and
   // short per-lane gather chains on the small coarse ones
+  L.nbatch = c->nbatch;
   L.lanes = c->lanes_override ? c->lanes_override : (S.n >= (1 << 20) ? 1 : S.n >= (1 << 16) ? 4 : 16);
   L.ne = S.ne;
   L.nbs = S.nbs;
@@ -630,6 +638,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   auto al_real = [&](void **out, size_t cnt) {
     return L.fp32 ? alloc_real<float>(P, out, cnt) : alloc_real<double>(P, out, cnt);
   };
+  const size_t NB = (size_t)L.nbatch;
   TRY(up_real(&L.w, w));
   TRY(up_real(&L.ellw, ellw));
   TRY(up_real(&L.wb, wb));
@@ -638,26 +647,27 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(up_real(&L.nrm, nrm));
   TRY(up_real(&L.warea, warea));
   if (L.has_coords) TRY(up_real(&L.xyz, xyz)); else L.xyz = nullptr;
-  TRY(al_real(&L.u, 6 * (size_t)n));
-  TRY(al_real(&L.uprev, 6 * (size_t)n));
-  TRY(al_real(&L.rhs, 6 * (size_t)n));
-  TRY(al_real(&L.diss, 5 * (size_t)n));
-  TRY(al_real(&L.src, 6 * (size_t)n));
-  TRY(al_real(&L.p, n));
-  TRY(al_real(&L.dt, n));
-  TRY(al_real(&L.dtt, n));
-  TRY(al_real(&L.mu, n));
-  TRY(al_real(&L.vort, n));
-  TRY(al_real(&L.divg, n));
-  TRY(al_real(&L.tdiff, n));
-  TRY(al_real(&L.rec0, 8 * (size_t)n));
+  TRY(al_real(&L.u, (size_t)(6 * (size_t)n) * NB));
+  TRY(al_real(&L.uprev, (size_t)(6 * (size_t)n) * NB));
+  TRY(al_real(&L.rhs, (size_t)(6 * (size_t)n) * NB));
+  TRY(al_real(&L.diss, (size_t)(5 * (size_t)n) * NB));
+  TRY(al_real(&L.src, (size_t)(6 * (size_t)n) * NB));
+  TRY(al_real(&L.p, (size_t)(n) * NB));
+  TRY(al_real(&L.dt, (size_t)(n) * NB));
+  TRY(al_real(&L.dtt, (size_t)(n) * NB));
+  TRY(al_real(&L.mu, (size_t)(n) * NB));
+  TRY(al_real(&L.vort, (size_t)(n) * NB));
+  TRY(al_real(&L.divg, (size_t)(n) * NB));
+  TRY(al_real(&L.tdiff, (size_t)(n) * NB));
+  TRY(al_real(&L.rec0, (size_t)(8 * (size_t)n) * NB));
   bool need_grad = L.has_coords && c->hp.hllc_flux_order > 1;
-  TRY(al_real(&L.grec, need_grad ? GREC * (size_t)n : 1));
-  TRY(al_real(&L.tau, 9 * (size_t)n));
-  TRY(al_real(&L.dnu, 4 * (size_t)n));
-  TRY(al_real(&L.fbuf, 6 * (size_t)ne));
-  TRY(al_real(&L.bbuf, 10 * (size_t)nbs));
-  TRY(al_real(&L.ws, 3 * (size_t)n));
+  L.gstride = need_grad ? GREC * (long long)n : 0;
+  TRY(al_real(&L.grec, need_grad ? GREC * (size_t)n * NB : 1));
+  TRY(al_real(&L.tau, (size_t)(9 * (size_t)n) * NB));
+  TRY(al_real(&L.dnu, (size_t)(4 * (size_t)n) * NB));
+  TRY(al_real(&L.fbuf, (size_t)(6 * (size_t)ne) * NB));
+  TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
+  TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
   int nbm = nblk(n);
   TRY(P.alloc(&L.m_sum, 7 * (size_t)nbm));
   TRY(P.alloc(&L.m_max, 7 * (size_t)nbm));
@@ -726,9 +736,9 @@ int op_prep(flite_ctx *c, Level &L, bool compute_p) {
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (compute_p) {
-      hipLaunchKernelGGL(k_prep<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+      hipLaunchKernelGGL(k_prep<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
     } else {
-      hipLaunchKernelGGL(k_pack<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P);
+      hipLaunchKernelGGL(k_pack<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
     }
     return check_launch();
   });
@@ -742,7 +752,7 @@ int op_grad(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_grad_minmax<R, SUB>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0, c->stream, V, P);
+      hipLaunchKernelGGL((k_grad_minmax<R, SUB>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
       return check_launch();
     });
   });
@@ -756,7 +766,7 @@ int op_flux(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (L.ne > 0) {
       Timed tm(c, "flux", L.gnum);
-      dim3 g(nblk(L.ne)), b(NTHR);
+      dim3 g(nblk(L.ne) * L.nbatch), b(NTHR);
       bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
       if (o2 && P.sa) {
         if (n4) hipLaunchKernelGGL((k_flux<R, 2, true, 4>), g, b, 0, c->stream, V, P);
@@ -772,7 +782,7 @@ int op_flux(flite_ctx *c, Level &L) {
     }
     if (L.nbs > 0) {
       Timed tm(c, "bflux", L.gnum);
-      hipLaunchKernelGGL(k_bflux<R>, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P);
+      hipLaunchKernelGGL(k_bflux<R>, dim3(nblk(L.nbs) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
     }
     return check_launch();
   });
@@ -785,7 +795,7 @@ int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      dim3 g(nblk((long long)L.n * SUB)), t(NTHR);
+      dim3 g(nblk((long long)L.n * SUB) * L.nbatch), t(NTHR);
 #define LAUNCH_STAGE(M)                                                                       \
   if (P.sa) hipLaunchKernelGGL((k_stage<R, M, true, SUB>), g, t, 0, c->stream, V, P, S);      \
   else hipLaunchKernelGGL((k_stage<R, M, false, SUB>), g, t, 0, c->stream, V, P, S);
@@ -808,7 +818,7 @@ int op_timesteps(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       return check_launch();
     });
@@ -818,7 +828,7 @@ int op_timesteps(flite_ctx *c, Level &L) {
 int op_zero_diss(flite_ctx *c, Level &L) {
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    size_t cnt = 5 * (size_t)L.n;
+    size_t cnt = 5 * (size_t)L.n * L.nbatch;
     hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.diss, R(0), cnt);
     return check_launch();
   });
@@ -838,9 +848,9 @@ int op_viscous(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       return check_launch();
     });
@@ -853,7 +863,7 @@ int op_sa_timesteps(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, SATimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, SATimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       return check_launch();
     });
@@ -866,9 +876,9 @@ int op_sa_diffusion(flite_ctx *c, Level &L) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, SADiff1Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, SADiff1Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
-      hipLaunchKernelGGL((k_gather<R, SUB, SADiff2Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, SADiff2Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       return check_launch();
     });
@@ -892,28 +902,28 @@ int op_step_terms(flite_ctx *c, Level &L, bool timesteps, bool sa_timesteps) {
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
       if (timesteps && sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       } else if (timesteps) {
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       } else if (sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       } else {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       }
       if (timesteps)
-        hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V);
+        hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V);
       if (sa && sa_timesteps) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SATimeStepPart<R>, SADiff2Part<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SATimeStepPart<R>, SADiff2Part<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       } else if (sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SADiff2Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SADiff2Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       } else {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB)), dim3(NTHR), 0,
+      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
                          c->stream, V, P);
       }
       return check_launch();
@@ -924,8 +934,8 @@ int op_step_terms(flite_ctx *c, Level &L, bool timesteps, bool sa_timesteps) {
 int op_copy(flite_ctx *c, Level &L, void *dst, const void *src, int ncomp) {
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_copy_state<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, (R *)dst, (const R *)src, L.n,
-                       ncomp);
+    hipLaunchKernelGGL(k_copy_state<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, (R *)dst, (const R *)src,
+                       L.n, ncomp, L.nbatch);
     return check_launch();
   });
 }
@@ -934,7 +944,7 @@ int op_bc_state(flite_ctx *c, Level &L, void *a, int outer) {
   DevParams P = c->P;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_bc_state<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (R *)a, outer);
+    hipLaunchKernelGGL(k_bc_state<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V, P, (R *)a, outer);
     return check_launch();
   });
 }
@@ -945,8 +955,9 @@ int op_restrict_array(flite_ctx *c, Level &F, Level &K, const void *fa, void *ka
     using RF = std::remove_const_t<std::remove_pointer_t<decltype(fin)>>;
     using RC = std::remove_pointer_t<decltype(kout)>;
     constexpr int SUB = 4;  // about 8 fine members per coarse node
-    hipLaunchKernelGGL((k_restrict<RF, RC, SUB>), dim3(nblk((long long)K.n * SUB)), dim3(NTHR), 0, c->stream, K.cptr,
-                       K.rw, fin, kout, K.n, F.n, ncomp);
+    long long per = ncomp == 1 ? 1 : 6;  // components allocated per node of the array
+    hipLaunchKernelGGL((k_restrict<RF, RC, SUB>), dim3(nblk((long long)K.n * SUB) * K.nbatch), dim3(NTHR), 0,
+                       c->stream, K.cptr, K.rw, fin, kout, K.n, F.n, ncomp, K.nbatch, per * F.n, per * K.n);
     return check_launch();
   };
   if (F.fp32 && K.fp32) return go((const float *)fa, (float *)ka);
@@ -1032,7 +1043,7 @@ int do_restrict(flite_ctx *c, int fine_level) {
   DevParams P = c->P;
   TRY(with_level(K, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_restrict_finish<R>, dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, V, P);
+    hipLaunchKernelGGL(k_restrict_finish<R>, dim3(nblk(K.n) * K.nbatch), dim3(NTHR), 0, c->stream, V, P);
     return check_launch();
   }));
   return 0;
@@ -1046,14 +1057,14 @@ int do_prolongate(flite_ctx *c, int coarse_level) {
   TRY(op_bc_state(c, K, K.uprev, 1));
   TRY(with_level(K, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_coarse_correction<R>, dim3(nblk(K.n)), dim3(NTHR), 0, c->stream, V, P);
+    hipLaunchKernelGGL(k_coarse_correction<R>, dim3(nblk(K.n) * K.nbatch), dim3(NTHR), 0, c->stream, V, P);
     return check_launch();
   }));
   int ncomp = (F.gnum < c->hp.number_of_turbulence_grids && P.sa) ? 6 : 5;
   auto go = [&](auto FV, auto *crhs) {
     using RF = std::remove_pointer_t<decltype(FV.u)>;
     using RC = std::remove_pointer_t<decltype(crhs)>;
-    hipLaunchKernelGGL((k_prolong_fine<RF, RC>), dim3(nblk(F.n)), dim3(NTHR), 0, c->stream, FV, P, K.f2c,
+    hipLaunchKernelGGL((k_prolong_fine<RF, RC>), dim3(nblk(F.n) * F.nbatch), dim3(NTHR), 0, c->stream, FV, P, K.f2c,
                        (const RC *)crhs, K.n, ncomp);
     return check_launch();
   };
@@ -1080,7 +1091,8 @@ int do_cycle(flite_ctx *c, int cycle) {
     Level &Kc = *c->lev[ng - 1];
     TRY(with_level(Kc, [&](auto V) {
       using R = std::remove_pointer_t<decltype(V.u)>;
-      hipLaunchKernelGGL(k_fill<R>, dim3(nblk(6LL * Kc.n)), dim3(NTHR), 0, c->stream, V.src, R(0), 6 * (size_t)Kc.n);
+      size_t cnt = 6 * (size_t)Kc.n * Kc.nbatch;
+      hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.src, R(0), cnt);
       return check_launch();
     }));
     TRY(do_time_iterations(c, ng, nrs, true));
@@ -1163,7 +1175,33 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   if (!p || !out) return fail(FLITE_E_ARG, "null argument");
   *out = nullptr;
   if (n_levels < 1 || n_levels > 16) return fail(FLITE_E_ARG, "n_levels must be 1..16");
-  if (n_batch != 1) return fail(FLITE_E_UNSUPPORTED, "batched conditions are not implemented yet");
+  if (n_batch < 1) return fail(FLITE_E_ARG, "n_batch must be >= 1");
+  // a batch varies only the freestream: every other setting must match
+  for (int b = 1; b < n_batch; ++b) {
+    const flite_params &q = p[b];
+    bool same = q.gamma == p->gamma && q.reynolds_number == p->reynolds_number &&
+                q.prandtl_number == p->prandtl_number && q.turbulent_prandtl_number == p->turbulent_prandtl_number &&
+                q.inflow_temperature == p->inflow_temperature && q.walls_are_isentropic == p->walls_are_isentropic &&
+                q.turbulence_model == p->turbulence_model && q.flux_type == p->flux_type &&
+                q.hllc_flux_order == p->hllc_flux_order && q.automatic_clipping == p->automatic_clipping &&
+                q.gradient_method == p->gradient_method && q.hllc_clipping == p->hllc_clipping &&
+                q.boundary_term == p->boundary_term && q.cfl_number == p->cfl_number &&
+                q.turbulent_cfl_number == p->turbulent_cfl_number && q.multigrid_scheme == p->multigrid_scheme &&
+                q.number_of_relaxation_steps == p->number_of_relaxation_steps &&
+                q.number_of_turbulence_grids == p->number_of_turbulence_grids &&
+                q.number_of_turbulence_steps == p->number_of_turbulence_steps &&
+                q.max_increment_factor == p->max_increment_factor && q.minimum_density == p->minimum_density &&
+                q.max_turbulence_value == p->max_turbulence_value && q.turb_k == p->turb_k &&
+                q.prolongation_relaxation == p->prolongation_relaxation &&
+                q.turb_prolongation_relaxation == p->turb_prolongation_relaxation &&
+                q.multigrid_bc_relaxation == p->multigrid_bc_relaxation && q.coarse_fp32 == p->coarse_fp32 &&
+                q.second_order_dissipation_factor == p->second_order_dissipation_factor &&
+                q.fourth_order_dissipation_factor == p->fourth_order_dissipation_factor;
+    for (int k = 0; k < 3; ++k) same = same && q.rk_coefficients[k] == p->rk_coefficients[k];
+    for (int k = 0; k < 6; ++k) same = same && q.inflow_field[1][k] == p->inflow_field[1][k];
+    if (!same)
+      return fail(FLITE_E_ARG, "batched conditions may differ only in Mach number, alpha, beta and the inflow state");
+  }
   if (p->flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "only fluxType=4 (HLLC) is implemented");
   if (p->turbulence_model != 0 && p->turbulence_model != 1)
     return fail(FLITE_E_UNSUPPORTED, "only turbulenceModel 0/1 are implemented");
@@ -1177,6 +1215,18 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   auto c = std::make_unique<flite_ctx>();
   c->hp = *p;
   c->P = make_dev_params(*p);
+  c->P.cond = nullptr;
+  c->hp_b.assign(p, p + n_batch);
+  if (n_batch > 1) {
+    std::vector<double> cond(8 * (size_t)n_batch, 0.0);
+    for (int b = 0; b < n_batch; ++b) {
+      cond[8 * (size_t)b] = p[b].mach_number;
+      for (int k = 0; k < 6; ++k) cond[8 * (size_t)b + 1 + k] = p[b].inflow_field[0][k];
+    }
+    HIPTRY(hipMalloc(&c->cond_dev, cond.size() * sizeof(double)));
+    HIPTRY(hipMemcpy(c->cond_dev, cond.data(), cond.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->P.cond = c->cond_dev;
+  }
   if (const char *e = std::getenv("FLITE_GATHER_LANES")) {
     int v = std::atoi(e);
     if (v != 1 && v != 4 && v != 16) {
@@ -1198,6 +1248,7 @@ void flite_ctx_destroy(flite_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->lev.clear();
+  if (c->cond_dev) (void)hipFree(c->cond_dev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1227,9 +1278,10 @@ int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
     return 0;
   };
   std::vector<double> h;
+  const size_t sb = (size_t)c->sel;  // selected condition
   if (field == FLITE_FIELD_GRADIENT) {
     if (!(L.has_coords && c->hp.hllc_flux_order > 1)) return fail(FLITE_E_STATE, "no gradient on this level");
-    TRY(to_host(L.grec, GREC * (size_t)L.n, h));
+    TRY(to_host((char *)L.grec + sb * (size_t)L.gstride * es, GREC * (size_t)L.n, h));
     for (int hh = 0; hh < L.n; ++hh) {
       const double *g = &h[GREC * (size_t)L.h2d[hh]];
       for (int k = 0; k < 15; ++k) out[(size_t)k * ld + hh] = g[k];
@@ -1241,7 +1293,7 @@ int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
     return 0;
   }
   if (field == FLITE_FIELD_WALL_STRESS) {
-    TRY(to_host(L.ws, 3 * (size_t)L.n, h));
+    TRY(to_host((char *)L.ws + sb * 3 * (size_t)L.n * es, 3 * (size_t)L.n, h));
     for (int hh = 0; hh < L.n; ++hh)
       for (int k = 0; k < 3; ++k) out[(size_t)k * ld + hh] = h[3 * (size_t)L.h2d[hh] + k];
     return 0;
@@ -1250,6 +1302,7 @@ int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
   size_t stride;
   void *p = field_ptr(L, field, &nc, &stride);
   if (!p) return fail(FLITE_E_ARG, "bad field");
+  p = (char *)p + sb * nc * stride * es;
   TRY(to_host(p, nc * stride, h));
   for (int k = 0; k < nc; ++k)
     for (int hh = 0; hh < L.n; ++hh) out[(size_t)k * ld + hh] = h[(size_t)k * stride + L.h2d[hh]];
@@ -1264,6 +1317,7 @@ int flite_field_set(flite_ctx *c, int level, int field, const double *in, int ld
   size_t stride;
   void *p = field_ptr(L, field, &nc, &stride);
   if (!p) return fail(FLITE_E_ARG, "field cannot be set");
+  p = (char *)p + (size_t)c->sel * nc * stride * (L.fp32 ? 4 : 8);
   TRY(sync(c));
   std::vector<double> h(nc * stride);
   for (int k = 0; k < nc; ++k)
@@ -1278,8 +1332,14 @@ int flite_field_set(flite_ctx *c, int level, int field, const double *in, int ld
 }
 
 int flite_state_set(flite_ctx *c, int level, const double *u, int ld, int ncomp, int batch) {
-  if (batch != 0) return fail(FLITE_E_UNSUPPORTED, "batch index must be 0");
   TRY(check_level(c, level));
+  if (batch < 0 || batch >= c->nbatch) return fail(FLITE_E_ARG, "batch index out of range");
+  struct Sel {
+    flite_ctx *c;
+    int old;
+    ~Sel() { c->sel = old; }
+  } keep{c, c->sel};
+  c->sel = batch;
   Level &L = *c->lev[level - 1];
   if (ncomp < 5 || ncomp > 6) return fail(FLITE_E_ARG, "ncomp must be 5 or 6");
   std::vector<double> full(6 * (size_t)L.n, 0.0);
@@ -1289,8 +1349,14 @@ int flite_state_set(flite_ctx *c, int level, const double *u, int ld, int ncomp,
 }
 
 int flite_state_get(flite_ctx *c, int level, double *u, int ld, int ncomp, int batch) {
-  if (batch != 0) return fail(FLITE_E_UNSUPPORTED, "batch index must be 0");
   TRY(check_level(c, level));
+  if (batch < 0 || batch >= c->nbatch) return fail(FLITE_E_ARG, "batch index out of range");
+  struct Sel {
+    flite_ctx *c;
+    int old;
+    ~Sel() { c->sel = old; }
+  } keep{c, c->sel};
+  c->sel = batch;
   Level &L = *c->lev[level - 1];
   if (ncomp < 1 || ncomp > 6) return fail(FLITE_E_ARG, "ncomp must be 1..6");
   std::vector<double> full(6 * (size_t)L.n);
@@ -1307,7 +1373,8 @@ int flite_init_from_state(flite_ctx *c) {
     Level &L = *c->lev[l];
     TRY(with_level(L, [&](auto V) {
       using R = std::remove_pointer_t<decltype(V.u)>;
-      hipLaunchKernelGGL(k_fill<R>, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), (size_t)L.n);
+      size_t cnt = (size_t)L.n * L.nbatch;
+      hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), cnt);
       return check_launch();
     }));
     if (l > 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
@@ -1319,11 +1386,13 @@ int flite_init_freestream(flite_ctx *c) {
   TRY(check_ctx(c));
   Level &L = *c->lev[0];
   std::vector<double> u(6 * (size_t)L.n, 0.0);
-  for (int k = 0; k < 5; ++k)
-    for (int i = 0; i < L.n; ++i) u[(size_t)k * L.n + i] = c->hp.inflow_field[0][k];
-  if (c->P.sa)
-    for (int i = 0; i < L.n; ++i) u[5 * (size_t)L.n + i] = 0.1;
-  TRY(flite_field_set(c, 1, FLITE_FIELD_U, u.data(), L.n));
+  for (int b = 0; b < c->nbatch; ++b) {
+    for (int k = 0; k < 5; ++k)
+      for (int i = 0; i < L.n; ++i) u[(size_t)k * L.n + i] = c->hp_b[b].inflow_field[0][k];
+    if (c->P.sa)
+      for (int i = 0; i < L.n; ++i) u[5 * (size_t)L.n + i] = 0.1;
+    TRY(flite_state_set(c, 1, u.data(), L.n, 6, b));
+  }
   TRY(op_prep(c, L, true));
   TRY(op_bc_state(c, L, L.u, 0));
   return flite_init_from_state(c);
@@ -1410,7 +1479,8 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
   int nres = c->P.sa ? 6 : 5;
   int nb = nblk(L.n);
   DevParams P = c->P;
-  const flite_params &hp = c->hp;
+  const int bsel = c->sel;  // selected condition
+  const flite_params &hp = c->hp_b[bsel];
   double PI = 4.0 * std::atan(1.0);
   double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
   double ft[3] = {std::cos(ar) * std::cos(br), std::sin(br), std::sin(ar)};
@@ -1420,11 +1490,11 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
   TRY(with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     hipLaunchKernelGGL(k_monitor_partial<R>, dim3(nb), dim3(NTHR), 0, c->stream, V, nres, L.host_index, L.m_sum,
-                       L.m_max, L.m_idx);
+                       L.m_max, L.m_idx, bsel);
     if (nw > 0)
       hipLaunchKernelGGL(k_force_terms<R>, dim3(nblk(nw)), dim3(NTHR), 0, c->stream, V, P, L.wall_nodes, nw,
                          (const R *)L.warea, L.m_force, ft[0], ft[1], ft[2], fn1[0], fn1[1], fn1[2], fn2[0], fn2[1],
-                         fn2[2], hp.groundplane, hp.datum[0], hp.datum[1], hp.datum[2]);
+                         fn2[2], hp.groundplane, hp.datum[0], hp.datum[1], hp.datum[2], bsel);
     return check_launch();
   }));
   std::vector<double> s(7 * (size_t)nb), m(7 * (size_t)nb), f(8 * (size_t)std::max(1, nw));
@@ -1474,6 +1544,7 @@ int check_linearisable(flite_ctx *c) {
   const Level &L = *c->lev[0];
   const flite_params &hp = c->hp;
   if (L.fp32) return fail(FLITE_E_UNSUPPORTED, "level 1 must be FP64");
+  if (c->nbatch != 1) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: single-condition contexts only");
   if (L.has_coords && hp.hllc_flux_order > 1)
     return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: first-order HLLC (HLLCFluxOrder=1) only");
   if (hp.reynolds_number > 0.0 || hp.turbulence_model != 0)
@@ -1552,6 +1623,13 @@ int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) 
                        (const double *)L.warea, 2.0 / hp.reference_area, t0, t1, t2, hp.groundplane, V.rhs);
   TRY(check_launch());
   return get_vector5(c, dJdU, ld);
+}
+
+int flite_batch_select(flite_ctx *c, int batch) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  if (batch < 0 || batch >= c->nbatch) return fail(FLITE_E_ARG, "batch index out of range");
+  c->sel = batch;
+  return 0;
 }
 
 int flite_sync(flite_ctx *c) {

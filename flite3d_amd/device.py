@@ -29,6 +29,7 @@ def lib():
         P = ctypes.POINTER
         d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
         L.flite_ctx_create.argtypes = [P(FliteParams), i, i, P(vp)]
+        L.flite_batch_select.argtypes = [vp, i]
         L.flite_level_upload.argtypes = [vp, i, P(FliteLevelView)]
         L.flite_ctx_destroy.argtypes = [vp]
         L.flite_ctx_destroy.restype = None
@@ -57,13 +58,17 @@ def lib():
 class Solver:
     """One flite_ctx: the multigrid levels of one case on one GPU."""
 
-    def __init__(self, params: FliteParams, levels, n_batch=1):
+    def __init__(self, params, levels):
+        """params: one FliteParams, or a list of them for a batch of flow
+        conditions on the same mesh (they may differ only in the freestream)."""
         self.L = lib()
-        self.params = params
+        plist = list(params) if isinstance(params, (list, tuple)) else [params]
+        self.n_batch = len(plist)
+        arr = (FliteParams * self.n_batch)(*plist)
+        self.params = arr
         self.levels = levels
         ctx = ctypes.c_void_p()
-        self._check(self.L.flite_ctx_create(ctypes.byref(params), len(levels),
-                                            n_batch, ctypes.byref(ctx)))
+        self._check(self.L.flite_ctx_create(arr, len(levels), self.n_batch, ctypes.byref(ctx)))
         self.ctx = ctx
         for k, lv in enumerate(levels):
             v = level_view(lv, k == 0)
@@ -132,6 +137,10 @@ class Solver:
     def cycle(self, number):
         """oneCycle (MG:907-1196)."""
         self._check(self.L.flite_cycle(self.ctx, number))
+
+    def select(self, batch):
+        """Condition that get/set/monitor act on (batched contexts)."""
+        self._check(self.L.flite_batch_select(self.ctx, batch))
 
     def sync(self):
         self._check(self.L.flite_sync(self.ctx))

@@ -230,6 +230,12 @@ module flite_gpu
        integer(c_int), value :: ld
      end function flite_drag_state_gradient
 
+     integer(c_int) function flite_batch_select(ctx, batch) bind(C, name="flite_batch_select")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: batch
+     end function flite_batch_select
+
      integer(c_int) function flite_sync(ctx) bind(C, name="flite_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -103,6 +103,12 @@ typedef struct flite_ctx flite_ctx;
 
 /* ---- lifecycle -------------------------------------------------------- */
 /* Replaces the allocation half of readComputationData (mgnsg3d_mg_par.F90:1198-1268). */
+/* n_batch >= 1 flow conditions share the mesh: p points to n_batch structs
+ * that may differ only in mach_number, alpha, beta and inflow_field[0]
+ * (config of many freestream conditions on one mesh; the reference runs
+ * one condition per process).  Every stepping call advances all conditions
+ * in the same launches; field, state and monitor calls act on the condition
+ * chosen by flite_batch_select (or the batch argument). */
 int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx **out);
 /* Replaces readGridComputationData's device side (mgnsg3d_def_par.F90:13812-14708). */
 int flite_level_upload(flite_ctx *ctx, int level, const flite_level_view *v);
@@ -194,6 +200,10 @@ int flite_apply(flite_ctx *ctx, int level, int op);
 int flite_tangent_apply(flite_ctx *ctx, const double *du, int ld_du, double *out, int ld_out);
 int flite_adjoint_apply(flite_ctx *ctx, const double *lam, int ld_lam, double *out, int ld_out);
 int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld);
+
+/* Condition that flite_field_get/set and flite_monitor act on (default 0);
+ * no reference counterpart (the reference has one condition per run). */
+int flite_batch_select(flite_ctx *ctx, int batch);
 
 /* Wait for all work queued by the context. */
 int flite_sync(flite_ctx *ctx);

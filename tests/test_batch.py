@@ -1,0 +1,71 @@
+"""Batched flow conditions on one mesh (one context, n_batch conditions,
+every launch covering all of them).
+
+Each condition of a batch must come out exactly as the same condition run
+alone: the batch shares the mesh data and the launches, not the arithmetic
+(same kernels, same summation order), so the comparison is bit for bit.
+"""
+import numpy as np
+import pytest
+
+from flite3d_amd.device import DeviceError, Solver
+
+from cases import euler, levels, rans
+
+pytestmark = pytest.mark.gpu
+
+
+def run(params, lv, cycles):
+    s = Solver(params, list(lv))
+    s.init_freestream()
+    mons = []
+    nb = len(params) if isinstance(params, list) else 1
+    for c in range(1, cycles + 1):
+        s.cycle(c)
+        row = []
+        for b in range(nb):
+            s.select(b)
+            row.append(s.monitor())
+        mons.append(row)
+    us = []
+    for b in range(nb):
+        s.select(b)
+        us.append(s.get(1, "U"))
+    s.close()
+    return us, mons
+
+
+@pytest.mark.parametrize("order,fp32", [(1, False), (2, True)])
+def test_euler_batch_equals_single_runs(order, fp32):
+    print(f"[case] euler batch order={order} fp32={fp32}", flush=True)
+    lv = levels(8, 3)
+    conds = [(0.5, 2.0), (0.75, -3.0), (0.35, 6.5)]
+    ps = [euler(order, machnumber=m, alpha=a, multigridscheme=3).to_struct(coarse_fp32=fp32) for m, a in conds]
+    ub, mb = run(ps, lv, 3)
+    for b, p in enumerate(ps):
+        us, ms = run(p, lv, 3)
+        assert np.array_equal(ub[b], us[0]), (b, np.abs(ub[b] - us[0]).max())
+        for c in range(3):
+            assert np.array_equal(mb[c][b]["res"], ms[c][0]["res"])
+            assert np.array_equal(mb[c][b]["forces"], ms[c][0]["forces"])
+    # the conditions really differ
+    assert not np.array_equal(ub[0], ub[1])
+
+
+def test_rans_batch_equals_single_runs():
+    print("[case] rans batch", flush=True)
+    lv = levels(8, 3, first_cell=0.005)
+    ps = [rans(machnumber=0.7, alpha=1.0, multigridscheme=3).to_struct(),
+          rans(machnumber=0.6, alpha=3.0, multigridscheme=3).to_struct()]
+    ub, mb = run(ps, lv, 2)
+    for b, p in enumerate(ps):
+        us, ms = run(p, lv, 2)
+        assert np.array_equal(ub[b], us[0]), (b, np.abs(ub[b] - us[0]).max())
+        assert np.array_equal(mb[-1][b]["forces"], ms[-1][0]["forces"])
+
+
+def test_batch_rejects_differing_settings():
+    lv = levels(8, 2)
+    ps = [euler(1).to_struct(), euler(2).to_struct()]
+    with pytest.raises(DeviceError):
+        Solver(ps, list(lv))
