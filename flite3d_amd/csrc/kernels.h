@@ -48,6 +48,13 @@ struct DevParams {
 #pragma unroll
     for (int k = 0; k < 6; ++k) inflow[0][k] = q[1 + k];
   }
+  // farfield state `row` (0: inflowField(:,1), 1: inflowField(:,2)) copied
+  // out by value: selecting rows with constant indices keeps the parameter
+  // block in registers (a run-time row index would force it into scratch)
+  __device__ __forceinline__ void inflow_row(int row, double *o) const {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = row == 0 ? inflow[0][k] : inflow[1][k];
+  }
 };
 
 // node boundary class from nodeIndicatorArray ranges; bit 4 marks host
@@ -629,8 +636,10 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
     phys_flux(B, rnx, rny, rnz, f2);
   } else {
     R gm1 = R(P.gamma - 1.0);
-    farfield_roe(P.inflow[ind - 3], A, gm1, rnx, rny, rnz, f1);
-    farfield_roe(P.inflow[ind - 3], B, gm1, rnx, rny, rnz, f2);
+    double inf[6];
+    P.inflow_row(ind - 3, inf);
+    farfield_roe(inf, A, gm1, rnx, rny, rnz, f1);
+    farfield_roe(inf, B, gm1, rnx, rny, rnz, f2);
   }
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
@@ -676,7 +685,9 @@ __device__ __forceinline__ void bside_end_flux(const DevParams &P, int ind, cons
   if (ind <= 2 || ind == 9) {
     phys_flux(A, T(rnx), T(rny), T(rnz), f);
   } else {
-    farfield_roe(P.inflow[ind - 3], A, T(P.gamma - 1.0), T(rnx), T(rny), T(rnz), f);
+    double inf[6];
+    P.inflow_row(ind - 3, inf);
+    farfield_roe(inf, A, T(P.gamma - 1.0), T(rnx), T(rny), T(rnz), f);
   }
 }
 
@@ -1364,7 +1375,8 @@ template <typename R>
 __device__ __forceinline__ void outer_bc(const LevelDev<R> &L, const DevParams &P, int i, R *q, R p) {
   int cls = L.ncls[i] & NC_MASK;
   if (cls != NC_FAR1 && cls != NC_FAR2) return;
-  const double *inf = P.inflow[cls - NC_FAR1];
+  double inf[6];
+  P.inflow_row(cls - NC_FAR1, inf);
   R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
   R rho0 = R(inf[0]), U1 = R(inf[1]), U2 = R(inf[2]), U3 = R(inf[3]);
   R gm1 = R(P.gamma - 1.0);
