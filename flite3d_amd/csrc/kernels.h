@@ -180,6 +180,12 @@ __device__ __forceinline__ Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a;
 __device__ __forceinline__ Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
 }  // namespace ad
 
+// The incidence lists and slot weights are streamed once per gather launch:
+// nontemporal loads keep them from evicting the node records that
+// neighbouring nodes re-read from L2.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
+
 // N consecutive values of a gather record in 16-byte loads (8-byte loads for
 // FP32 records whose stride is not a multiple of 16 bytes): fewer memory
 // instructions per lane for the scattered record reads.  p must be aligned
@@ -252,7 +258,7 @@ __device__ __forceinline__ void for_edges(const LevelDev<R> &L, int i, int sub, 
   int s = i / SLICE, lane = i % SLICE;
   int base = L.ell_ptr[s], K = L.ell_k[s];
   for (int k = sub; k < K; k += SUB) {
-    int v = L.ell_e[base + k * SLICE + lane];
+    int v = ldnt(L.ell_e + base + k * SLICE + lane);
     if (v < 0) continue;
     f(v >> 1, v & 1);
   }
@@ -269,9 +275,9 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, int sub
   const long long ns = L.nslot;
   for (int k = sub; k < K; k += SUB) {
     int slot = base + k * SLICE + lane;
-    int j = L.ell_nbr[slot];
+    int j = ldnt(L.ell_nbr + slot);
     if (j < 0) continue;
-    f(j, L.ellw[slot], L.ellw[ns + slot], L.ellw[2 * ns + slot]);
+    f(j, ldnt(L.ellw + slot), ldnt(L.ellw + ns + slot), ldnt(L.ellw + 2 * ns + slot));
   }
 }
 
