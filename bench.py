@@ -389,7 +389,10 @@ def main():
                    "fp64_gedges": round(results["fp64"]["gedges"], 4),
                    "fp64_ms_per_step": round(results["fp64"]["ms_per_cycle"], 3),
                    "residual_drop_log10": round(float(np.log10(head["res"] / head["res0"])), 3),
-                   "time_to_1e-8_drop": None},
+                   "time_to_1e-8_drop": None,
+                   "time_to_1e-8_drop_note": "not reached: from freestream both modes stall near a "
+                                             "10^-2.8 drop (bench.py --converge; "
+                                             "profiles/convergence_rans128_mixed_vs_fp64.txt)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": flux_traffic(),
                      "kernel": "k_flux<double,2,SA> (level 1)",
