@@ -28,6 +28,10 @@ namespace fk {
 
 constexpr int SLICE = 64;
 constexpr int GREC = 26;
+// viscous record per node: stress 6 + k grad T 3 | grad nu~ 3 + cb2 term | pad,
+// 16 values = one 128-byte line, so a neighbour's tau and grad nu~ come in
+// one cache line (tau at offset 0, dnu at offset 9)
+constexpr int VREC = 16;
 constexpr int NTHR = 256;
 
 struct DevParams {
@@ -78,7 +82,7 @@ struct LevelDev {
   const R *vol, *xyz, *wd, *nrm;  // nrm: 3 per node AoS
   const int *ncls;
   R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
-  R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
+  R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;  // tau, dnu: VREC-strided, one shared record
   // batched conditions: nb copies of every state array, condition-major;
   // gstride = gradient-record elements per condition (0 if none)
   int nb;
@@ -100,8 +104,8 @@ struct LevelDev {
     tdiff += bn;
     rec0 += 8 * bn;
     grec += (size_t)b * (size_t)gstride;
-    tau += 9 * bn;
-    dnu += 4 * bn;
+    tau += VREC * bn;
+    dnu += VREC * bn;
     fbuf += (size_t)b * 6 * (size_t)ne;
     bbuf += (size_t)b * 10 * (size_t)nbs;
     ws += 3 * bn;
@@ -959,7 +963,7 @@ struct TimeStepPart {
 
 // makeViscosityTerm (DEF:5198-5893), pass 1: gradients of (u,v,w,T) and the
 // node loop (Sutherland, SA eddy viscosity, stress, heat flux, vorticity,
-// divergence, adiabatic wall, wall stress).  tau[9*i]: stress 6 + k grad T 3.
+// divergence, adiabatic wall, wall stress).  tau: stress 6 + k grad T 3.
 // WRITE_REC also stores the new laminar viscosity in the gather record;
 // when fused with makeTimeSteps (which reads the neighbours' previous one)
 // that store is left to k_mu_record after the pass.
@@ -1044,7 +1048,7 @@ struct ViscNodePart {
       L.ws[3 * (size_t)i + 1] = t[1] * nx + t[3] * ny + t[4] * nz;
       L.ws[3 * (size_t)i + 2] = t[2] * nx + t[4] * ny + t[5] * nz;
     }
-    R *o = L.tau + 9 * (size_t)i;
+    R *o = L.tau + VREC * (size_t)i;
 #pragma unroll
     for (int c = 0; c < 9; ++c) o[c] = t[c];
   }
@@ -1055,7 +1059,7 @@ template <typename R>
 struct ViscFluxPart {
   R t1[9], u1, v1, w1, a[4];
   __device__ void init(const LevelDev<R> &L, const DevParams &, int i, const R *qi) {
-    const R *ti = L.tau + 9 * (size_t)i;
+    const R *ti = L.tau + VREC * (size_t)i;
 #pragma unroll
     for (int c = 0; c < 9; ++c) t1[c] = ti[c];
     u1 = qi[1];
@@ -1065,7 +1069,7 @@ struct ViscFluxPart {
     for (int c = 0; c < 4; ++c) a[c] = 0;
   }
   __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
-    const R *tj = L.tau + 9 * (size_t)j;
+    const R *tj = L.tau + VREC * (size_t)j;
     R u2 = qj[1], v2 = qj[2], w2 = qj[3];
     R d[9];
 #pragma unroll
@@ -1081,7 +1085,7 @@ struct ViscFluxPart {
   __device__ void bside(const LevelDev<R> &L, const DevParams &, const R *, int, int o, int ind, const R *qo, R wx,
                         R wy, R wz) {
     if (ind == 2 || ind == 9) return;  // symmetry / inviscid wall carry no viscous flux
-    const R *to = L.tau + 9 * (size_t)o;
+    const R *to = L.tau + VREC * (size_t)o;
     R u2 = qo[1], v2 = qo[2], w2 = qo[3];
     R d[9];
 #pragma unroll
@@ -1159,24 +1163,24 @@ struct SATimeStepPart {
   }
 };
 
-// makeSADiffusion (DEF:7436-7808), pass 1: grad nu~ (dnu[4*i]: 3 + nodal
-// cb2 term).  nu~ is read from the state u(:,6).
+// makeSADiffusion (DEF:7436-7808), pass 1: grad nu~ (dnu: 3 + nodal cb2
+// term).  nu~ = u(:,6) is read from the gather records.
 template <typename R>
 struct SADiff1Part {
   R ti, g[3];
-  __device__ void init(const LevelDev<R> &L, const DevParams &, int i, const R *) {
-    ti = L.u[5 * (size_t)L.n + i];
+  __device__ void init(const LevelDev<R> &, const DevParams &, int, const R *qi) {
+    ti = qi[6];
     g[0] = g[1] = g[2] = 0;
   }
-  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *, R wx, R wy, R wz) {
-    R s = ti + L.u[5 * (size_t)L.n + j];
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *qj, R wx, R wy, R wz) {
+    R s = ti + qj[6];
     g[0] += wx * s;
     g[1] += wy * s;
     g[2] += wz * s;
   }
-  __device__ void bside(const LevelDev<R> &L, const DevParams &P, const R *, int, int o, int, const R *, R wx, R wy,
+  __device__ void bside(const LevelDev<R> &, const DevParams &P, const R *, int, int, int, const R *qo, R wx, R wy,
                         R wz) {
-    R to = L.u[5 * (size_t)L.n + o];
+    R to = qo[6];
     R c = P.bterm == 1 ? (R(3) * ti + to) : R(4) * ti;
     g[0] += c * wx;
     g[1] += c * wy;
@@ -1191,7 +1195,7 @@ struct SADiff1Part {
     R vt = R(1) / L.vol[i];
     R dx = g[0] * vt, dy = g[1] * vt, dz = g[2] * vt;
     R fact = R(1.0 / P.Re * 0.622 / (2.0 / 3.0));
-    R *o = L.dnu + 4 * (size_t)i;
+    R *o = L.dnu + VREC * (size_t)i;
     o[0] = dx;
     o[1] = dy;
     o[2] = dz;
@@ -1204,7 +1208,7 @@ template <typename R>
 struct SADiff2Part {
   R Re, fact, v1, acc, d0, d1, d2, d3;
   __device__ void init(const LevelDev<R> &L, const DevParams &P, int i, const R *qi) {
-    const R *di = L.dnu + 4 * (size_t)i;
+    const R *di = L.dnu + VREC * (size_t)i;
     d0 = di[0];
     d1 = di[1];
     d2 = di[2];
@@ -1215,14 +1219,14 @@ struct SADiff2Part {
     acc = 0;
   }
   __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
-    const R *dj = L.dnu + 4 * (size_t)j;
+    const R *dj = L.dnu + VREC * (size_t)j;
     R v2 = Re * qj[7] / qj[0] + qj[6];
     R fd = v1 * (wx * d0 + wy * d1 + wz * d2) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
     acc -= fact * fd;
   }
   __device__ void bside(const LevelDev<R> &L, const DevParams &P, const R *, int, int o, int, const R *qo, R wx, R wy,
                         R wz) {
-    const R *dq = L.dnu + 4 * (size_t)o;
+    const R *dq = L.dnu + VREC * (size_t)o;
     R vo = Re * qo[7] / qo[0] + qo[6];
     R fd1 = v1 * (wx * d0 + wy * d1 + wz * d2);
     R fd2 = vo * (wx * dq[0] + wy * dq[1] + wz * dq[2]);
@@ -1440,6 +1444,10 @@ struct StageArgs {
   int calc_turb;             // SA terms active in this stage
   int fas;                   // subtract alpha*sourceTerm (coarse levels)
   int has_src;               // level has a sourceTerm (gridNumber > 1)
+  int next_record;           // MODE_STAGE: also form p and the gather record of the new
+                             // state for the next stage (not after the last stage: the
+                             // pressure field then stays that of its start, as the
+                             // reference's makePressureField leaves it)
 };
 
 template <typename R, int MODE, bool SA, int SUB>
@@ -1510,6 +1518,21 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
     }
     outer_bc(L, P, i, q, pi);
     for (int c = 0; c < 6; ++c) L.u[(size_t)c * n + i] = q[c];
+    if (!S.next_record) return;
+    // pressure and gather record of the new state, as k_prep forms them, so
+    // the next stage starts without a separate pass
+    R pp = R(P.gamma - 1.0) * (q[4] - R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]);
+    pp = pp > R(0.1) ? pp : R(0.1);
+    L.p[i] = pp;
+    R *g = L.rec0 + 8 * (size_t)i;
+    g[0] = q[0];
+    g[1] = q[1] / q[0];
+    g[2] = q[2] / q[0];
+    g[3] = q[3] / q[0];
+    g[4] = q[4];
+    g[5] = pp;
+    g[6] = P.sa ? q[5] : R(0);
+    g[7] = L.mu[i];
   } else if (MODE == MODE_COARSE) {
     outer_bc(L, P, i, q, pi);
     for (int c = 0; c < 6; ++c) {

@@ -663,8 +663,8 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   bool need_grad = L.has_coords && c->hp.hllc_flux_order > 1;
   L.gstride = need_grad ? GREC * (long long)n : 0;
   TRY(al_real(&L.grec, need_grad ? GREC * (size_t)n * NB : 1));
-  TRY(al_real(&L.tau, (size_t)(9 * (size_t)n) * NB));
-  TRY(al_real(&L.dnu, (size_t)(4 * (size_t)n) * NB));
+  TRY(al_real(&L.tau, (size_t)(VREC * (size_t)n) * NB));  // tau and dnu share the record
+  L.dnu = (char *)L.tau + 9 * (L.fp32 ? 4 : 8);
   TRY(al_real(&L.fbuf, (size_t)(6 * (size_t)ne) * NB));
   TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
   TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
@@ -979,7 +979,9 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
     }
   }
   for (int k = 1; k <= 3; ++k) {
-    TRY(op_prep(c, L, true));
+    // stage k > 1 starts from the pressure and records the previous stage
+    // kernel wrote with the new state; stage 1 after the frozen terms too
+    if (k == 1 && !calc_diss) TRY(op_prep(c, L, true));
     TRY(op_grad(c, L));
     TRY(op_flux(c, L));
     StageArgs S;
@@ -989,6 +991,7 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
     S.calc_turb = 1;
     S.fas = L.gnum > 1;
     S.has_src = L.gnum > 1;
+    S.next_record = k < 3;
     TRY(op_stage(c, L, MODE_STAGE, S));
   }
   return 0;
