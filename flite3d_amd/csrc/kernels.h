@@ -79,6 +79,27 @@ struct LevelDev {
   const R *ellw;       // signed side weights per ELL slot, SoA: [c * nslot + slot]
   long long nslot;
   const int *bs_ptr, *bs_list;
+  // side blocks of k_flux_blk: block k serves sides fb_start[k] .. fb_start[k+1]-1
+  // and stages nodes fb_node[fb_nptr[k] ..]; fb_loc[e] = both ends' positions
+  // in that list (first end in the low 16 bits)
+  int nfb;
+  const int *fb_start, *fb_nptr, *fb_node;
+  const unsigned *fb_loc;
+  // per-node partial sums of the side fluxes (k_flux_blk): entry q =
+  // fb_nptr position; fb_inc[fb_iptr[q] ..] = the entry's sides within its
+  // block (local side << 1 | 1 when the node is the side's second end);
+  // node i's entries are pn_ent[pn_ptr[i] ..], in block order
+  long long nent;
+  const int *fb_iptr, *pn_ptr, *pn_ent;
+  const unsigned short *fb_inc;
+  R *pbuf;
+  // node blocks of the LDS-staged gathers: block k serves nodes
+  // gb_first[k] .. gb_first[k+1]-1 and stages their neighbours
+  // gb_node[gb_nptr[k] ..]; ell_loc[slot] = the slot's neighbour in that list
+  // (0xffff for an empty slot)
+  int ngb;
+  const int *gb_first, *gb_nptr, *gb_node;
+  const unsigned short *ell_loc;
   const R *vol, *xyz, *wd, *nrm;  // nrm: 3 per node AoS
   const int *ncls;
   R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
@@ -107,6 +128,7 @@ struct LevelDev {
     tau += VREC * bn;
     dnu += VREC * bn;
     fbuf += (size_t)b * 6 * (size_t)ne;
+    pbuf += (size_t)b * 6 * (size_t)nent;
     bbuf += (size_t)b * 10 * (size_t)nbs;
     ws += 3 * bn;
   }
@@ -286,6 +308,98 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, int sub
 }
 
 // ---------------------------------------------------------------------------
+// LDS staging of node records for blocks of sides or nodes
+// ---------------------------------------------------------------------------
+template <typename R> struct Pair;
+template <> struct Pair<double> { using type = double2; };
+template <> struct Pair<float> { using type = float2; };
+// one staging pass: NW words of each of the nn (<= CAP) listed nodes'
+// records src + STRIDE*g, in pairs when the record allows it, into LDS
+// words OFF.. of the node's S-word slot
+template <int CAP, int NW, int STRIDE, int OFF, int S, typename R>
+__device__ __forceinline__ void fb_stage(R *sn, const R *src, const int *nodes, int nn) {
+  constexpr bool PAIRS = NW % 2 == 0 && STRIDE % 2 == 0;
+  constexpr int U = PAIRS ? NW / 2 : NW;  // load units per node
+  constexpr int PER = (CAP * U + NTHR - 1) / NTHR;
+  using P2 = typename Pair<R>::type;
+  P2 pv[PAIRS ? PER : 1];
+  R sv[PAIRS ? 1 : PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    int t = threadIdx.x + q * NTHR, k = t / U, j = t - k * U;
+    if (k < nn) {
+      size_t g = (size_t)nodes[k];
+      if constexpr (PAIRS) pv[q] = reinterpret_cast<const P2 *>(src + STRIDE * g)[j];
+      else sv[q] = src[STRIDE * g + j];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    int t = threadIdx.x + q * NTHR, k = t / U, j = t - k * U;
+    if (k < nn) {
+      if constexpr (PAIRS) {
+        sn[k * S + OFF + 2 * j] = pv[q].x;
+        sn[k * S + OFF + 2 * j + 1] = pv[q].y;
+      } else {
+        sn[k * S + OFF + j] = sv[q];
+      }
+    }
+  }
+}
+
+// Neighbour walks of node i: f(qj, tj, wx, wy, wz) per interior side with
+// qj the neighbour's gather record (8 words, in registers), tj its viscous
+// record and w the node-signed side weights.  walk_global reads the
+// neighbours' records from memory; walk_staged from the LDS slots a node
+// block staged (words 0..7 gather record, 8.. viscous record).
+template <int SUB, typename R, typename F>
+__device__ __forceinline__ void walk_global(const LevelDev<R> &L, int i, int sub, F f) {
+  for_edges_w<SUB>(L, i, sub, [&](int j, R wx, R wy, R wz) {
+    R qj[8];
+    ldrec<8>(L.rec0 + 8 * (size_t)j, qj);
+    f(qj, L.tau + VREC * (size_t)j, wx, wy, wz);
+  });
+}
+template <int SUB, int S, typename R, typename F>
+__device__ __forceinline__ void walk_staged(const LevelDev<R> &L, const R *sn, int i, int sub, F f) {
+  int s = i / SLICE, lane = i % SLICE;
+  int base = L.ell_ptr[s], K = L.ell_k[s];
+  const long long ns = L.nslot;
+  for (int k = sub; k < K; k += SUB) {
+    int slot = base + k * SLICE + lane;
+    unsigned loc = ldnt(L.ell_loc + slot);
+    if (loc == 0xffffu) continue;
+    const R *q = sn + loc * S;
+    R qj[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) qj[c] = q[c];
+    f(qj, q + 8, ldnt(L.ellw + slot), ldnt(L.ellw + ns + slot), ldnt(L.ellw + 2 * ns + slot));
+  }
+}
+
+// Node blocks with their neighbours' records staged in LDS.  A block serves
+// up to GB_NODES consecutive nodes, GB_SUB lanes each, whose neighbours are
+// at most GB_CAP distinct nodes (tables built at upload).  Its threads copy
+// those records into LDS with loads in which consecutive lanes read
+// consecutive words of one record, so a wavefront touches a few cache lines
+// per load instead of one per lane; the walks then read the neighbours from
+// LDS.  The arithmetic and its order are those of k_gather with SUB = GB_SUB.
+constexpr int GB_SUB = 4;
+constexpr int GB_NODES = NTHR / GB_SUB;
+constexpr int GB_CAP = 288;
+constexpr int GB_AUX = 12;  // viscous-record words staged: stress 6 + k grad T 3 + grad nu~ 3
+
+template <typename R, int S, bool AUX>
+__device__ __forceinline__ bool gb_stage(const LevelDev<R> &L, R *sn, int blk) {
+  if (blk >= L.ngb) return false;
+  const int n0 = L.gb_nptr[blk], nn = L.gb_nptr[blk + 1] - n0;
+  const int *nodes = L.gb_node + n0;
+  fb_stage<GB_CAP, 8, 8, 0, S>(sn, L.rec0, nodes, nn);
+  if constexpr (AUX) fb_stage<GB_CAP, GB_AUX, VREC, 8, S>(sn, L.tau, nodes, nn);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // node preparation: makePressureField (DEF:13612-13630) + gather records
 // ---------------------------------------------------------------------------
 template <typename R>
@@ -335,25 +449,15 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 // computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459), gathered
 // ---------------------------------------------------------------------------
-template <typename R, int SUB>
-__global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
-  int i, sub;
-  int bb_;
-  node_lanes<SUB>(L.nb, bb_, i, sub);
-  L.to_batch(bb_);
-  P.to_batch(bb_);
-  if (i >= L.n) return;
+template <typename R, int SUB, typename Walk>
+__device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub, Walk walk) {
   const R *qi = L.rec0 + 8 * (size_t)i;
   R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
   R g[15];
   R mn[5], mx[5];
   for (int c = 0; c < 15; ++c) g[c] = 0;
   for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
-  bool any = false;
-  for_edges_w<SUB>(L, i, sub, [&](int j, R wx, R wy, R wz) {
-    R qj[8];
-    ldrec<8>(L.rec0 + 8 * (size_t)j, qj);
-    any = true;
+  walk([&](const R *qj, const R *, R wx, R wy, R wz) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       R s = q0[c] + qj[c];
@@ -366,7 +470,6 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
       mx[c] = rmin(mx[c], hi);
     }
   });
-  (void)any;
   for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
     int v = L.bs_list[k], s = v >> 1;
     int o = (v & 1) ? L.ba[s] : L.bb[s];
@@ -404,6 +507,17 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   out[25] = 0;
 }
 
+template <typename R, int SUB>
+__global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
+  int i, sub;
+  int bb_;
+  node_lanes<SUB>(L.nb, bb_, i, sub);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (i >= L.n) return;
+  grad_minmax_body<R, SUB>(L, i, sub, [&](auto f) { walk_global<SUB>(L, i, sub, f); });
+}
+
 // LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds.
 // Branch-free: the bound is selected by the sign of a, so a wavefront runs
 // the root and the division once whatever the signs of its lanes.
@@ -421,12 +535,14 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
 // HLLC interior flux of one side (makeRHS, DEF:1794-2100) from the two
 // ends' gather records A, B (rho, u, v, w, rhoE, p, nu~, mu) and the side
 // weights; writes o[0..5] (component 5: SA convection, DEF:6873-6880).
+// Second order also reads both ends' gradient records G1, G2 (grad 15 |
+// 1/lo 5 | 1/hi 5) and coordinates X1, X2; first order ignores them.
 // T is the arithmetic type: the level's R for the residual, a dual number
 // for its tangent and adjoint (first order only: the MUSCL branch reads the
 // level's gradient records and is built for T = R).
 template <typename T, typename R, int ORDER, bool SA, int NSIG>
-__device__ __forceinline__ void side_flux(const LevelDev<R> &L, const DevParams &P, int ia, int ib, const T *A,
-                                          const T *B, T wx, T wy, T wz, T *o) {
+__device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const R *G2, const R *X1, const R *X2,
+                                          const T *A, const T *B, T wx, T wy, T wz, T *o) {
   T gam = T(P.gamma), gm1 = gam - T(1);
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
   T roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
@@ -455,9 +571,6 @@ __device__ __forceinline__ void side_flux(const LevelDev<R> &L, const DevParams 
   T inv = rnorm1 > T(0) ? T(1) / rnorm1 : rnorm1;
   T rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
   if constexpr (ORDER > 1) {
-    const T *G1 = L.grec + GREC * (size_t)ia;
-    const T *G2 = L.grec + GREC * (size_t)ib;
-    const T *X1 = L.xyz + 3 * (size_t)ia, *X2 = L.xyz + 3 * (size_t)ib;
     T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
     T hl = T(0.5) * T(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
     T d[5];
@@ -568,7 +681,113 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
   ldrec<8>(L.rec0 + 8 * (size_t)ia, A);
   ldrec<8>(L.rec0 + 8 * (size_t)ib, B);
   R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
-  side_flux<R, R, ORDER, SA, NSIG>(L, P, ia, ib, A, B, wx, wy, wz, L.fbuf + 6 * (size_t)e);
+  const R *G1 = nullptr, *G2 = nullptr, *X1 = nullptr, *X2 = nullptr;
+  if constexpr (ORDER > 1) {
+    G1 = L.grec + GREC * (size_t)ia;
+    G2 = L.grec + GREC * (size_t)ib;
+    X1 = L.xyz + 3 * (size_t)ia;
+    X2 = L.xyz + 3 * (size_t)ib;
+  }
+  side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, L.fbuf + 6 * (size_t)e);
+}
+
+// The same flux with the two ends' records staged in LDS per block of
+// sides, and the sides' fluxes summed per node inside the block.  A block
+// serves up to FB_SIDES consecutive sides that touch at most FB_NODES
+// distinct nodes (tables built at upload).  Its threads first copy those
+// nodes' records into LDS with loads in which consecutive lanes read
+// consecutive words of one record, so a wavefront touches a few cache lines
+// per load instead of one line per lane; each thread then evaluates its side
+// from LDS.  The block then sums, for each of its nodes, +f over the sides
+// the node starts and -f over the sides it ends, in side order, and writes
+// that partial sum to pbuf (one 6-word entry per block and node, in the
+// order of the block's node list).  A node's residual is the sum of its
+// entries in block order (k_stage), which replaces the gather of one flux
+// record per incident side by a few partial sums per node.
+constexpr int FB_SIDES = NTHR;
+constexpr int FB_NODES = 128;
+template <typename R, int ORDER, bool SA, int NSIG>
+__global__ __launch_bounds__(NTHR) void k_flux_blk(LevelDev<R> L, DevParams P) {
+  // LDS words per node: gather record 8 | grad + bounds + pad 26 | coordinates 3
+  constexpr int S = ORDER > 1 ? 8 + GREC + 3 : 9;  // odd: lanes reading different nodes spread over banks
+  constexpr int SN = FB_NODES * S > FB_SIDES * 6 ? FB_NODES * S : FB_SIDES * 6;
+  __shared__ R sn[SN];
+  __shared__ unsigned short sinc[2 * FB_SIDES];
+  int lb = xcd_block();
+  int bb_ = lb % L.nb, blk = lb / L.nb;
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (blk >= L.nfb) return;
+  const int n0 = L.fb_nptr[blk], nn = L.fb_nptr[blk + 1] - n0;
+  const int *nodes = L.fb_node + n0;
+  // this thread's node (if any) and the block's incidence lists
+  const int ibase = L.fb_iptr[n0];
+  int q0 = 0, q1 = 0;
+  if ((int)threadIdx.x < nn) {
+    q0 = L.fb_iptr[n0 + threadIdx.x] - ibase;
+    q1 = L.fb_iptr[n0 + threadIdx.x + 1] - ibase;
+  }
+  {
+    const int ni = L.fb_iptr[n0 + nn] - ibase;  // 2 x sides of the block
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int t = threadIdx.x + q * NTHR;
+      if (t < ni) sinc[t] = L.fb_inc[ibase + t];
+    }
+  }
+  fb_stage<FB_NODES, 8, 8, 0, S>(sn, L.rec0, nodes, nn);
+  if constexpr (ORDER > 1) {
+    fb_stage<FB_NODES, GREC, GREC, 8, S>(sn, L.grec, nodes, nn);
+    fb_stage<FB_NODES, 3, 3, 8 + GREC, S>(sn, L.xyz, nodes, nn);
+  }
+  __syncthreads();
+  const int e = L.fb_start[blk] + (int)threadIdx.x;
+  const bool active = e < L.fb_start[blk + 1];
+  R fo[6];
+  if (active) {
+    unsigned loc = L.fb_loc[e];
+    const R *na = sn + (loc & 0xffffu) * S, *nb = sn + (loc >> 16) * S;
+    R A[8], B[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      A[c] = na[c];
+      B[c] = nb[c];
+    }
+    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+    const R *G1 = nullptr, *G2 = nullptr, *X1 = nullptr, *X2 = nullptr;
+    if constexpr (ORDER > 1) {
+      G1 = na + 8;
+      G2 = nb + 8;
+      X1 = na + 8 + GREC;
+      X2 = nb + 8 + GREC;
+    }
+    side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, fo);
+  }
+  __syncthreads();  // node records no longer needed: the side fluxes take their place
+  R *sf = sn;
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sf[6 * threadIdx.x + c] = fo[c];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nn) {
+    const int ent = n0 + (int)threadIdx.x;
+    R r[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = q0; q < q1; ++q) {
+      unsigned v = sinc[q];
+      const R *f = sf + 6 * (v >> 1);
+      if (v & 1u) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[c] -= f[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[c] += f[c];
+      }
+    }
+    R *o = L.pbuf + 6 * (size_t)ent;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[c] = r[c];
+  }
 }
 
 // physical flux of one state through unit normal
@@ -717,8 +936,8 @@ __global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du
   }
   state_record(Ua, P.gamma, A);
   state_record(Ub, P.gamma, B);
-  side_flux<Dual, double, 1, false, 2>(L, P, ia, ib, A, B, Dual(L.w[3 * e]), Dual(L.w[3 * e + 1]),
-                                       Dual(L.w[3 * e + 2]), o);
+  side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
+                                       Dual(L.w[3 * e + 1]), Dual(L.w[3 * e + 2]), o);
   double *f = L.fbuf + 6 * (size_t)e;
 #pragma unroll
   for (int c = 0; c < 5; ++c) f[c] = o[c].d;
@@ -787,7 +1006,7 @@ __global__ void k_flux_adjoint(LevelDev<double> L, DevParams P, const double *la
     }
     state_record(Ua, P.gamma, A);
     state_record(Ub, P.gamma, B);
-    side_flux<Dual, double, 1, false, 2>(L, P, ia, ib, A, B, wx, wy, wz, o);
+    side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, wx, wy, wz, o);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
@@ -900,8 +1119,9 @@ __global__ void k_drag_state_gradient(LevelDev<double> L, DevParams P, const int
 // ---------------------------------------------------------------------------
 template <typename R>
 struct NoPart {
+  static constexpr bool AUX = false;  // reads the neighbours' viscous records
   __device__ void init(const LevelDev<R> &, const DevParams &, int, const R *) {}
-  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *, R, R, R) {}
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *, const R *, R, R, R) {}
   __device__ void bside(const LevelDev<R> &, const DevParams &, const R *, int, int, int, const R *, R, R, R) {}
   template <int SUB>
   __device__ void reduce() {}
@@ -911,6 +1131,7 @@ struct NoPart {
 // makeTimeSteps (DEF:13335-13610)
 template <typename R>
 struct TimeStepPart {
+  static constexpr bool AUX = false;
   R lam, lamv, cc, am, gam;
   bool visc;
   __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *qi) {
@@ -922,7 +1143,8 @@ struct TimeStepPart {
     visc = P.Re > 0.0;
     am = visc ? R(1.0 / P.Re) : R(0);
   }
-  __device__ void edge(const LevelDev<R> &, const DevParams &P, const R *qi, int, const R *qj, R wx, R wy, R wz) {
+  __device__ void edge(const LevelDev<R> &, const DevParams &P, const R *qi, const R *, const R *qj, R wx, R wy,
+                       R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += fabs(vn) + sqrt(cc * al);
@@ -969,6 +1191,7 @@ struct TimeStepPart {
 // that store is left to k_mu_record after the pass.
 template <typename R, bool WRITE_REC>
 struct ViscNodePart {
+  static constexpr bool AUX = false;
   R gog1, ai[4], g[12];
   __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *qi) {
     gog1 = R(P.gamma / (P.gamma - 1.0));
@@ -979,7 +1202,7 @@ struct ViscNodePart {
 #pragma unroll
     for (int c = 0; c < 12; ++c) g[c] = 0;
   }
-  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *qj, R wx, R wy, R wz) {
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *, const R *qj, R wx, R wy, R wz) {
     R aj[4] = {qj[1], qj[2], qj[3], gog1 * qj[5] / qj[0]};
     R ww[3] = {wx, wy, wz};
 #pragma unroll
@@ -1057,6 +1280,7 @@ struct ViscNodePart {
 // makeViscosityTerm pass 2 (DEF:5640-5893): dissipation(:,2:5) = -sum
 template <typename R>
 struct ViscFluxPart {
+  static constexpr bool AUX = true;
   R t1[9], u1, v1, w1, a[4];
   __device__ void init(const LevelDev<R> &L, const DevParams &, int i, const R *qi) {
     const R *ti = L.tau + VREC * (size_t)i;
@@ -1068,8 +1292,8 @@ struct ViscFluxPart {
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = 0;
   }
-  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
-    const R *tj = L.tau + VREC * (size_t)j;
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *tj, const R *qj, R wx, R wy,
+                       R wz) {
     R u2 = qj[1], v2 = qj[2], w2 = qj[3];
     R d[9];
 #pragma unroll
@@ -1118,6 +1342,7 @@ struct ViscFluxPart {
 // makeTurbulentTimeSteps (DEF:12878-13099)
 template <typename R>
 struct SATimeStepPart {
+  static constexpr bool AUX = false;
   R ioRe, oos, cb2, tk, lam, lamv;
   __device__ void init(const LevelDev<R> &, const DevParams &P, int, const R *) {
     ioRe = R(1.0 / P.Re);
@@ -1127,7 +1352,8 @@ struct SATimeStepPart {
     lam = 0;
     lamv = 0;
   }
-  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *qi, int, const R *qj, R wx, R wy, R wz) {
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *qi, const R *, const R *qj, R wx, R wy,
+                       R wz) {
     R al = wx * wx + wy * wy + wz * wz;
     R vn = wx * qi[1] + wy * qi[2] + wz * qi[3];
     lam += tk * sqrt(al) + fabs(vn);
@@ -1167,12 +1393,13 @@ struct SATimeStepPart {
 // term).  nu~ = u(:,6) is read from the gather records.
 template <typename R>
 struct SADiff1Part {
+  static constexpr bool AUX = false;
   R ti, g[3];
   __device__ void init(const LevelDev<R> &, const DevParams &, int, const R *qi) {
     ti = qi[6];
     g[0] = g[1] = g[2] = 0;
   }
-  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, int, const R *qj, R wx, R wy, R wz) {
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *, const R *qj, R wx, R wy, R wz) {
     R s = ti + qj[6];
     g[0] += wx * s;
     g[1] += wy * s;
@@ -1206,6 +1433,7 @@ struct SADiff1Part {
 // makeSADiffusion pass 2
 template <typename R>
 struct SADiff2Part {
+  static constexpr bool AUX = true;
   R Re, fact, v1, acc, d0, d1, d2, d3;
   __device__ void init(const LevelDev<R> &L, const DevParams &P, int i, const R *qi) {
     const R *di = L.dnu + VREC * (size_t)i;
@@ -1218,8 +1446,9 @@ struct SADiff2Part {
     v1 = Re * qi[7] / qi[0] + qi[6];
     acc = 0;
   }
-  __device__ void edge(const LevelDev<R> &L, const DevParams &, const R *, int j, const R *qj, R wx, R wy, R wz) {
-    const R *dj = L.dnu + VREC * (size_t)j;
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *tj, const R *qj, R wx, R wy,
+                       R wz) {
+    const R *dj = tj + 9;  // grad nu~ follows the 9 stress words in the viscous record
     R v2 = Re * qj[7] / qj[0] + qj[6];
     R fd = v1 * (wx * d0 + wy * d1 + wz * d2) + v2 * (wx * dj[0] + wy * dj[1] + wz * dj[2]);
     acc -= fact * fd;
@@ -1240,14 +1469,8 @@ struct SADiff2Part {
 };
 
 // one walk of node i's interior and boundary sides feeding up to three parts
-template <typename R, int SUB, typename A, typename B, typename C>
-__global__ void k_gather(LevelDev<R> L, DevParams P) {
-  int i, sub;
-  int bb_;
-  node_lanes<SUB>(L.nb, bb_, i, sub);
-  L.to_batch(bb_);
-  P.to_batch(bb_);
-  if (i >= L.n) return;
+template <typename R, int SUB, typename A, typename B, typename C, typename Walk>
+__device__ __forceinline__ void gather_body(LevelDev<R> &L, const DevParams &P, int i, int sub, Walk walk) {
   const R *qi = L.rec0 + 8 * (size_t)i;
   A a;
   B b;
@@ -1255,12 +1478,10 @@ __global__ void k_gather(LevelDev<R> L, DevParams P) {
   a.init(L, P, i, qi);
   b.init(L, P, i, qi);
   c.init(L, P, i, qi);
-  for_edges_w<SUB>(L, i, sub, [&](int j, R wx, R wy, R wz) {
-    R qj[8];
-    ldrec<8>(L.rec0 + 8 * (size_t)j, qj);
-    a.edge(L, P, qi, j, qj, wx, wy, wz);
-    b.edge(L, P, qi, j, qj, wx, wy, wz);
-    c.edge(L, P, qi, j, qj, wx, wy, wz);
+  walk([&](const R *qj, const R *tj, R wx, R wy, R wz) {
+    a.edge(L, P, qi, tj, qj, wx, wy, wz);
+    b.edge(L, P, qi, tj, qj, wx, wy, wz);
+    c.edge(L, P, qi, tj, qj, wx, wy, wz);
   });
   for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
     int v = L.bs_list[k], s = v >> 1;
@@ -1282,6 +1503,33 @@ __global__ void k_gather(LevelDev<R> L, DevParams P) {
   a.finish(L, P, i, qi);
   b.finish(L, P, i, qi);
   c.finish(L, P, i, qi);
+}
+
+template <typename R, int SUB, typename A, typename B, typename C>
+__global__ void k_gather(LevelDev<R> L, DevParams P) {
+  int i, sub;
+  int bb_;
+  node_lanes<SUB>(L.nb, bb_, i, sub);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (i >= L.n) return;
+  gather_body<R, SUB, A, B, C>(L, P, i, sub, [&](auto f) { walk_global<SUB>(L, i, sub, f); });
+}
+
+template <typename R, typename A, typename B, typename C>
+__global__ __launch_bounds__(NTHR) void k_gather_blk(LevelDev<R> L, DevParams P) {
+  constexpr bool AUX = A::AUX || B::AUX || C::AUX;
+  constexpr int S = AUX ? 8 + GB_AUX + 1 : 9;  // odd LDS stride in words
+  __shared__ R sn[GB_CAP * S];
+  int lb = xcd_block();
+  int bb_ = lb % L.nb, blk = lb / L.nb;
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (!gb_stage<R, S, AUX>(L, sn, blk)) return;
+  __syncthreads();
+  int i = L.gb_first[blk] + (int)threadIdx.x / GB_SUB, sub = (int)threadIdx.x % GB_SUB;
+  if (i >= L.gb_first[blk + 1]) return;
+  gather_body<R, GB_SUB, A, B, C>(L, P, i, sub, [&](auto f) { walk_staged<GB_SUB, S>(L, sn, i, sub, f); });
 }
 
 // new laminar viscosity into the gather records after a fused pass
@@ -1405,11 +1653,20 @@ __device__ __forceinline__ void outer_bc(const LevelDev<R> &L, const DevParams &
   if (P.sa) q[5] = R(0.1);
 }
 
-// gather of the convective residual of node i (interior fbuf + boundary)
+// gather of the convective residual of node i: interior fluxes from the
+// node's partial sums (pbuf) or from one record per incident side (fbuf),
+// then the boundary-side shares
 template <typename R, bool SA, int SUB>
-__device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, int sub, R *r) {
+__device__ __forceinline__ void gather_rhs(const LevelDev<R> &L, int i, int sub, bool partials, R *r) {
   for (int c = 0; c < 6; ++c) r[c] = 0;
-  for_edges<SUB>(L, i, sub, [&](int e, int role) {
+  if (partials) {
+    for (int k = L.pn_ptr[i] + sub; k < L.pn_ptr[i + 1]; k += SUB) {
+      R f[6];
+      ldrec<6>(L.pbuf + 6 * (size_t)L.pn_ent[k], f);
+#pragma unroll
+      for (int c = 0; c < (SA ? 6 : 5); ++c) r[c] += f[c];
+    }
+  } else for_edges<SUB>(L, i, sub, [&](int e, int role) {
     R f[6];
     ldrec<6>(L.fbuf + 6 * (size_t)e, f);
     if (role) {
@@ -1444,6 +1701,7 @@ struct StageArgs {
   int calc_turb;             // SA terms active in this stage
   int fas;                   // subtract alpha*sourceTerm (coarse levels)
   int has_src;               // level has a sourceTerm (gridNumber > 1)
+  int partials;              // interior fluxes are per-node partial sums (pbuf), else per-side records (fbuf)
   int next_record;           // MODE_STAGE: also form p and the gather record of the new
                              // state for the next stage (not after the last stage: the
                              // pressure field then stays that of its start, as the
@@ -1460,7 +1718,7 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
   if (i >= L.n) return;
   int n = L.n;
   R r[6];
-  gather_rhs<R, SA, SUB>(L, i, sub, r);
+  gather_rhs<R, SA, SUB>(L, i, sub, S.partials, r);
   if (sub) return;
   bool sa_on = SA && S.calc_turb;
   if (sa_on) r[5] += R(sa_source(L, P, i));

@@ -154,6 +154,19 @@ struct Level {
   int *ell_e, *ell_nbr;
   void *ellw;
   long long nslot = 0;
+  // side blocks of the LDS-staged flux kernel
+  int nfb = 0;
+  int *fb_start = nullptr, *fb_nptr = nullptr, *fb_node = nullptr;
+  unsigned *fb_loc = nullptr;
+  long long nent = 0;  // per-node partial-sum entries of the side blocks
+  int *fb_iptr = nullptr, *pn_ptr = nullptr, *pn_ent = nullptr;
+  unsigned short *fb_inc = nullptr;
+  void *pbuf = nullptr;
+  bool flux_partials = false;  // the last interior fluxes went to pbuf (k_flux_blk), not fbuf
+  // node blocks of the LDS-staged gathers (ngb = 0: the per-node kernels)
+  int ngb = 0;
+  int *gb_first = nullptr, *gb_nptr = nullptr, *gb_node = nullptr;
+  unsigned short *ell_loc = nullptr;
   // transfer from the next finer level (coarse levels)
   int *f2c = nullptr, *cptr = nullptr, *clist = nullptr;
   double *rw = nullptr;
@@ -170,6 +183,9 @@ struct Level {
     v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
     v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell_e = ell_e; v.ell_nbr = ell_nbr; v.ellw = (const R *)ellw; v.nslot = nslot;
     v.bs_ptr = bs_ptr; v.bs_list = bs_list;
+    v.nfb = nfb; v.fb_start = fb_start; v.fb_nptr = fb_nptr; v.fb_node = fb_node; v.fb_loc = fb_loc;
+    v.nent = nent; v.fb_iptr = fb_iptr; v.fb_inc = fb_inc; v.pn_ptr = pn_ptr; v.pn_ent = pn_ent; v.pbuf = (R *)pbuf;
+    v.ngb = ngb; v.gb_first = gb_first; v.gb_nptr = gb_nptr; v.gb_node = gb_node; v.ell_loc = ell_loc;
     v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
     v.wd = (const R *)wd; v.nrm = (const R *)nrm; v.ncls = ncls;
     v.u = (R *)u; v.uprev = (R *)uprev; v.rhs = (R *)rhs; v.diss = (R *)diss; v.src = (R *)src;
@@ -187,6 +203,8 @@ struct flite_ctx {
   DevParams P;
   int nl = 0, nbatch = 1;
   int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
+  bool flux_per_side = false;  // FLITE_FLUX_KERNEL=side: one lane per side with direct record loads
+  bool gather_per_node = false;  // FLITE_GATHER_KERNEL=node: no LDS-staged node blocks
   std::vector<flite_params> hp_b;  // per condition (batched contexts)
   double *cond_dev = nullptr;      // per-condition Mach and inflow state on the device
   int sel = 0;                     // condition that field/state/monitor calls act on
@@ -228,6 +246,25 @@ int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return 0;
+}
+
+// one node walk feeding parts A, B, C.  Walks that read the neighbours'
+// viscous records run on the LDS-staged node blocks where the level has
+// them; walks that read only the 64-byte gather records stay one thread
+// group of L.lanes lanes per node, which measured faster for them.
+template <typename A, typename B, typename C, typename V>
+void launch_gather(flite_ctx *c, const Level &L, const V &v, const DevParams &P) {
+  using R = std::remove_pointer_t<decltype(v.u)>;
+  if (L.ngb > 0 && (A::AUX || B::AUX || C::AUX)) {
+    hipLaunchKernelGGL((k_gather_blk<R, A, B, C>), dim3((unsigned)L.ngb * L.nbatch), dim3(NTHR), 0, c->stream, v, P);
+    return;
+  }
+  with_lanes(L.lanes, [&](auto sc) {
+    constexpr int SUB = decltype(sc)::value;
+    hipLaunchKernelGGL((k_gather<R, SUB, A, B, C>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
+                       c->stream, v, P);
+    return 0;
+  });
 }
 
 int sync(flite_ctx *c) {
@@ -555,6 +592,63 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     eb[k] = db[e];
     for (int q = 0; q < 3; ++q) w[3 * (size_t)k + q] = S.w[(size_t)q * ne + e];
   }
+  // side blocks for the LDS-staged flux: consecutive sides, at most
+  // FB_SIDES of them touching at most FB_NODES distinct nodes; each block's
+  // node list is ascending (neighbouring records share cache lines)
+  {
+    std::vector<int> fb_start{0}, fb_nptr{0}, fb_node, fb_iptr{0};
+    std::vector<unsigned> fb_loc(ne);
+    std::vector<unsigned short> fb_inc;
+    std::vector<int> stamp(n, -1), local(n, 0), cur;
+    std::vector<std::vector<unsigned short>> inc;
+    int start = 0, blk = 0;
+    auto close = [&](int end) {
+      std::sort(cur.begin(), cur.end());
+      for (size_t q = 0; q < cur.size(); ++q) local[cur[q]] = (int)q;
+      for (int k = start; k < end; ++k) fb_loc[k] = (unsigned)local[ea[k]] | ((unsigned)local[eb[k]] << 16);
+      // each node's sides within the block, in side order
+      inc.assign(cur.size(), {});
+      for (int k = start; k < end; ++k) {
+        inc[local[ea[k]]].push_back((unsigned short)((k - start) << 1));
+        inc[local[eb[k]]].push_back((unsigned short)(((k - start) << 1) | 1));
+      }
+      for (auto &l : inc) {
+        fb_inc.insert(fb_inc.end(), l.begin(), l.end());
+        fb_iptr.push_back((int)fb_inc.size());
+      }
+      fb_node.insert(fb_node.end(), cur.begin(), cur.end());
+      fb_nptr.push_back((int)fb_node.size());
+      fb_start.push_back(end);
+      cur.clear();
+      start = end;
+      ++blk;
+    };
+    for (int k = 0; k < ne; ++k) {
+      int add = (stamp[ea[k]] != blk) + (stamp[eb[k]] != blk && eb[k] != ea[k]);
+      if (k - start == FB_SIDES || (int)cur.size() + add > FB_NODES) close(k);
+      for (int v : {ea[k], eb[k]})
+        if (stamp[v] != blk) { stamp[v] = blk; cur.push_back(v); }
+    }
+    if (start < ne) close(ne);
+    L.nfb = blk;
+    L.nent = (long long)fb_node.size();
+    // node -> its entries, in block order
+    std::vector<int> pn_ptr(n + 1, 0), pn_ent(fb_node.size());
+    for (int g : fb_node) pn_ptr[g + 1]++;
+    for (int i = 0; i < n; ++i) pn_ptr[i + 1] += pn_ptr[i];
+    {
+      std::vector<int> pos(pn_ptr.begin(), pn_ptr.end() - 1);
+      for (size_t q = 0; q < fb_node.size(); ++q) pn_ent[pos[fb_node[q]]++] = (int)q;
+    }
+    TRY(L.pool.upload(&L.fb_start, fb_start));
+    TRY(L.pool.upload(&L.fb_nptr, fb_nptr));
+    TRY(L.pool.upload(&L.fb_node, fb_node));
+    TRY(L.pool.upload(&L.fb_loc, fb_loc));
+    TRY(L.pool.upload(&L.fb_iptr, fb_iptr));
+    TRY(L.pool.upload(&L.fb_inc, fb_inc));
+    TRY(L.pool.upload(&L.pn_ptr, pn_ptr));
+    TRY(L.pool.upload(&L.pn_ent, pn_ent));
+  }
   // sliced ELL incidence
   std::vector<int> deg(n, 0);
   for (int k = 0; k < ne; ++k) { deg[ea[k]]++; deg[eb[k]]++; }
@@ -582,6 +676,51 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     }
   }
   L.nslot = (long long)nslot;
+  // node blocks for the LDS-staged gathers, on levels gathered one lane per
+  // node: consecutive nodes, at most GB_NODES of them, whose neighbours are
+  // at most GB_CAP distinct nodes; each block's neighbour list is ascending
+  if (n > 0 && L.lanes == 1 && !c->gather_per_node && *std::max_element(deg.begin(), deg.end()) <= GB_CAP) {
+    std::vector<int> gb_first{0}, gb_nptr{0}, gb_node;
+    std::vector<unsigned short> ell_loc(nslot, 0xffff);
+    std::vector<int> stamp(n, -1), seen(n, -1), local(n, 0), cur;
+    int start = 0, blk = 0;
+    auto nbr_of = [&](int i, int k) {
+      return ell_nbr[ell_ptr[i / SLICE] + (size_t)k * SLICE + i % SLICE];
+    };
+    auto close = [&](int end) {
+      std::sort(cur.begin(), cur.end());
+      for (size_t q = 0; q < cur.size(); ++q) local[cur[q]] = (int)q;
+      for (int i = start; i < end; ++i)
+        for (int k = 0; k < ell_k[i / SLICE]; ++k) {
+          size_t slot = ell_ptr[i / SLICE] + (size_t)k * SLICE + i % SLICE;
+          if (ell_nbr[slot] >= 0) ell_loc[slot] = (unsigned short)local[ell_nbr[slot]];
+        }
+      gb_node.insert(gb_node.end(), cur.begin(), cur.end());
+      gb_nptr.push_back((int)gb_node.size());
+      gb_first.push_back(end);
+      cur.clear();
+      start = end;
+      ++blk;
+    };
+    for (int i = 0; i < n; ++i) {
+      int add = 0;
+      for (int k = 0; k < deg[i]; ++k) {
+        int j = nbr_of(i, k);
+        if (stamp[j] != blk && seen[j] != i) { seen[j] = i; ++add; }
+      }
+      if (i - start == GB_NODES || (int)cur.size() + add > GB_CAP) close(i);
+      for (int k = 0; k < deg[i]; ++k) {
+        int j = nbr_of(i, k);
+        if (stamp[j] != blk) { stamp[j] = blk; cur.push_back(j); }
+      }
+    }
+    if (start < n) close(n);
+    L.ngb = blk;
+    TRY(L.pool.upload(&L.gb_first, gb_first));
+    TRY(L.pool.upload(&L.gb_nptr, gb_nptr));
+    TRY(L.pool.upload(&L.gb_node, gb_node));
+    TRY(L.pool.upload(&L.ell_loc, ell_loc));
+  }
   // boundary sides in device numbering, per-node CSR
   std::vector<int> ba(nbs), bb(nbs);
   std::vector<double> wb(3 * (size_t)nbs);
@@ -666,6 +805,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(al_real(&L.tau, (size_t)(VREC * (size_t)n) * NB));  // tau and dnu share the record
   L.dnu = (char *)L.tau + 9 * (L.fp32 ? 4 : 8);
   TRY(al_real(&L.fbuf, (size_t)(6 * (size_t)ne) * NB));
+  TRY(al_real(&L.pbuf, (size_t)(6 * (size_t)L.nent) * NB));
   TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
   TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
   int nbm = nblk(n);
@@ -768,7 +908,21 @@ int op_flux(flite_ctx *c, Level &L) {
       Timed tm(c, "flux", L.gnum);
       dim3 g(nblk(L.ne) * L.nbatch), b(NTHR);
       bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
-      if (o2 && P.sa) {
+      L.flux_partials = !c->flux_per_side;
+      if (L.flux_partials) {
+        dim3 gb((unsigned)L.nfb * L.nbatch);
+        if (o2 && P.sa) {
+          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, true, 4>), gb, b, 0, c->stream, V, P);
+          else hipLaunchKernelGGL((k_flux_blk<R, 2, true, 2>), gb, b, 0, c->stream, V, P);
+        } else if (o2) {
+          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, false, 4>), gb, b, 0, c->stream, V, P);
+          else hipLaunchKernelGGL((k_flux_blk<R, 2, false, 2>), gb, b, 0, c->stream, V, P);
+        } else if (P.sa) {
+          hipLaunchKernelGGL((k_flux_blk<R, 1, true, 2>), gb, b, 0, c->stream, V, P);
+        } else {
+          hipLaunchKernelGGL((k_flux_blk<R, 1, false, 2>), gb, b, 0, c->stream, V, P);
+        }
+      } else if (o2 && P.sa) {
         if (n4) hipLaunchKernelGGL((k_flux<R, 2, true, 4>), g, b, 0, c->stream, V, P);
         else hipLaunchKernelGGL((k_flux<R, 2, true, 2>), g, b, 0, c->stream, V, P);
       } else if (o2) {
@@ -788,8 +942,10 @@ int op_flux(flite_ctx *c, Level &L) {
   });
 }
 
-int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S) {
+int op_stage(flite_ctx *c, Level &L, int mode, const StageArgs &S0) {
   DevParams P = c->P;
+  StageArgs S = S0;
+  S.partials = L.flux_partials;
   Timed tm(c, "stage", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
@@ -816,12 +972,8 @@ int op_timesteps(flite_ctx *c, Level &L) {
   Timed tm(c, "timesteps", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    return with_lanes(L.lanes, [&](auto sc) {
-      constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      return check_launch();
-    });
+    launch_gather<TimeStepPart<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    return check_launch();
   });
 }
 
@@ -846,14 +998,9 @@ int op_viscous(flite_ctx *c, Level &L) {
   Timed tm(c, "viscous", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    return with_lanes(L.lanes, [&](auto sc) {
-      constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      return check_launch();
-    });
+    launch_gather<ViscNodePart<R, true>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    launch_gather<ViscFluxPart<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    return check_launch();
   });
 }
 
@@ -861,12 +1008,8 @@ int op_sa_timesteps(flite_ctx *c, Level &L) {
   DevParams P = c->P;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    return with_lanes(L.lanes, [&](auto sc) {
-      constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, SATimeStepPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      return check_launch();
-    });
+    launch_gather<SATimeStepPart<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    return check_launch();
   });
 }
 
@@ -874,14 +1017,9 @@ int op_sa_diffusion(flite_ctx *c, Level &L) {
   DevParams P = c->P;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    return with_lanes(L.lanes, [&](auto sc) {
-      constexpr int SUB = decltype(sc)::value;
-      hipLaunchKernelGGL((k_gather<R, SUB, SADiff1Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      hipLaunchKernelGGL((k_gather<R, SUB, SADiff2Part<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      return check_launch();
-    });
+    launch_gather<SADiff1Part<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    launch_gather<SADiff2Part<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    return check_launch();
   });
 }
 
@@ -899,35 +1037,25 @@ int op_step_terms(flite_ctx *c, Level &L, bool timesteps, bool sa_timesteps) {
   Timed tm(c, "viscous", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    return with_lanes(L.lanes, [&](auto sc) {
-      constexpr int SUB = decltype(sc)::value;
-      if (timesteps && sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      } else if (timesteps) {
-      hipLaunchKernelGGL((k_gather<R, SUB, TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      } else if (sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      } else {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscNodePart<R, true>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      }
-      if (timesteps)
-        hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V);
-      if (sa && sa_timesteps) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SATimeStepPart<R>, SADiff2Part<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      } else if (sa) {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, SADiff2Part<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      } else {
-      hipLaunchKernelGGL((k_gather<R, SUB, ViscFluxPart<R>, NoPart<R>, NoPart<R>>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0,
-                         c->stream, V, P);
-      }
-      return check_launch();
-    });
+    if (timesteps && sa) {
+    launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>(c, L, V, P);
+    } else if (timesteps) {
+    launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>(c, L, V, P);
+    } else if (sa) {
+    launch_gather<ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>(c, L, V, P);
+    } else {
+    launch_gather<ViscNodePart<R, true>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    }
+    if (timesteps)
+      hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V);
+    if (sa && sa_timesteps) {
+    launch_gather<ViscFluxPart<R>, SATimeStepPart<R>, SADiff2Part<R>>(c, L, V, P);
+    } else if (sa) {
+    launch_gather<ViscFluxPart<R>, SADiff2Part<R>, NoPart<R>>(c, L, V, P);
+    } else {
+    launch_gather<ViscFluxPart<R>, NoPart<R>, NoPart<R>>(c, L, V, P);
+    }
+    return check_launch();
   });
 }
 
@@ -1236,6 +1364,16 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
       return fail(FLITE_E_ARG, "FLITE_GATHER_LANES must be 1, 4 or 16");
     }
     c->lanes_override = v;
+  }
+  if (const char *e = std::getenv("FLITE_FLUX_KERNEL")) {
+    std::string v(e);
+    if (v != "side" && v != "block") return fail(FLITE_E_ARG, "FLITE_FLUX_KERNEL must be side or block");
+    c->flux_per_side = v == "side";
+  }
+  if (const char *e = std::getenv("FLITE_GATHER_KERNEL")) {
+    std::string v(e);
+    if (v != "node" && v != "block") return fail(FLITE_E_ARG, "FLITE_GATHER_KERNEL must be node or block");
+    c->gather_per_node = v == "node";
   }
   c->nl = n_levels;
   c->nbatch = n_batch;
@@ -1583,6 +1721,7 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   if (L.ne > 0) hipLaunchKernelGGL(k_flux_tangent, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  L.flux_partials = false;  // the tangent fluxes are per-side records
   if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_tangent, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
   TRY(check_launch());
   StageArgs S{};
