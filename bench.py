@@ -37,7 +37,7 @@ def flux_traffic():
     summary of the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over
     this benchmark, scaled by the copy-kernel calibration it records; None
     when no summary is present."""
-    path = os.path.join(ROOT, "profiles", "k_flux_hllc2_sa_rans128_pmc.txt")
+    path = os.path.join(ROOT, "profiles", "k_flux_blk_hllc2_sa_rans128_pmc.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -397,7 +397,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      # the committed PMC figure is for this kernel on the 128^3, 4-level case only
                      "traffic": flux_traffic() if (a.n, a.levels) == (128, 4) else None,
-                     "kernel": "k_flux<double,2,SA> (level 1)",
+                     "kernel": "k_flux_blk<double,2,SA> (level 1)",
                      "avg_launch_ms": round(k_avg_ms, 4)},
         "cpu_baseline": cpu,
     }

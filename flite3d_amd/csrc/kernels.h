@@ -518,16 +518,47 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   grad_minmax_body<R, SUB>(L, i, sub, [&](auto f) { walk_global<SUB>(L, i, sub, f); });
 }
 
+// Reciprocal and reciprocal square root in the FP64 flux: the hardware
+// estimates (v_rcp_f64, v_rsq_f64) refined by two Newton-Raphson steps,
+// y <- y + y (1 - b y) and y <- y + y (1/2 - x y^2 / 2), as fused
+// multiply-adds.  The results stay within a few ulp of the correctly rounded
+// quotient and root at about half the instructions of the IEEE sequences.
+// frcp needs a finite nonzero argument (every call site guarantees it);
+// frsqrt maps +inf to 0 as 1/sqrt does.  Other arithmetic types (FP32
+// levels, dual numbers) use the plain operations.
+template <typename T>
+__device__ __forceinline__ T frcp(T b) { return T(1) / b; }
+template <>
+__device__ __forceinline__ double frcp<double>(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  return fma(y, e, y);
+}
+template <typename T>
+__device__ __forceinline__ T frsqrt(T x) { return T(1) / sqrt(x); }
+template <>
+__device__ __forceinline__ double frsqrt<double>(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  return x == __builtin_inf() ? 0.0 : y;
+}
+
 // LED local clipping (DEF:1852-1889) with precomputed reciprocal bounds.
-// Branch-free: the bound is selected by the sign of a, so a wavefront runs
-// the root and the division once whatever the signs of its lanes.
+// Branch-free: the bound is selected by the sign of a, so a wavefront
+// evaluates the reciprocal root once whatever the signs of its lanes.
 template <int NSIG, typename R>
 __device__ __forceinline__ R led(R a, R ilo, R ihi) {
   R a1 = R(2) * a;
   R iv = a1 > R(0) ? ihi : ilo;
   R t = a1 * iv, t2 = t * t;
-  R den = NSIG == 4 ? sqrt(sqrt(R(1) + t2 * t2)) : sqrt(R(1) + t2);
-  R r = R(0.5) * a1 / den;
+  R iden = NSIG == 4 ? frsqrt(sqrt(R(1) + t2 * t2)) : frsqrt(R(1) + t2);
+  R r = R(0.5) * a1 * iden;
   r = iv == R(0) ? R(0) : r;
   return (a1 > R(0) || a1 < R(0)) ? r : a;  // a == 0 (or NaN) passes through
 }
@@ -552,7 +583,7 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const
     fsa = wx * (u1 * t1 + u2 * t2) + wy * (v1 * t1 + v2 * t2) + wz * (w1 * t1 + w2 * t2);
     T dtt = t1 - t2;
     if (fabs(dtt) > T(1e-20)) {
-      T id = T(1) / dtt;
+      T id = frcp(dtt);
       T ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
       fsa = fsa + fabs(wx * ax + wy * ay + wz * az) * dtt;
     }
@@ -567,8 +598,9 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const
   fc[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
   fc[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
   T nx = T(2) * wx, ny = T(2) * wy, nz = T(2) * wz;
-  T rnorm1 = sqrt(nx * nx + ny * ny + nz * nz);
-  T inv = rnorm1 > T(0) ? T(1) / rnorm1 : rnorm1;
+  T n2 = nx * nx + ny * ny + nz * nz;
+  T inv = n2 > T(0) ? frsqrt(n2) : T(0);
+  T rnorm1 = n2 * inv;  // |2w|
   T rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
   if constexpr (ORDER > 1) {
     T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
@@ -606,12 +638,12 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const
     rh2 = re2 + pr2;
     ru2 = roe2 * u2; rv2 = roe2 * v2; rw2 = roe2 * w2;
   }
-  T rinv1 = T(1) / roe1, rinv2 = T(1) / roe2;
+  T rinv1 = frcp(roe1), rinv2 = frcp(roe2);
   T nv1 = u1 * rnx + v1 * rny + w1 * rnz;
   T nv2 = u2 * rnx + v2 * rny + w2 * rnz;
   T beta0 = T(P.beta0);
   T rroe = sqrt(roe2 * rinv1);
-  T iro = T(1) / (T(1) + rroe);
+  T iro = frcp(T(1) + rroe);
   T ut = (u1 + u2 * rroe) * iro, vt = (v1 + v2 * rroe) * iro, wt = (w1 + w2 * rroe) * iro;
   T qt = ut * rnx + vt * rny + wt * rnz;
   T ht = (rh1 * rinv1 + rh2 * rinv2 * rroe) * iro;
@@ -622,7 +654,7 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const
   T sr = fmax(fmax(nv2 + beta0 * c2, qt + ct), nv1 + beta0 * c1);
   T sm = roe2 * nv2 * (sr - nv2) - roe1 * nv1 * (sl - nv1) + pr1 - pr2;
   T fact = roe2 * (sr - nv2) - roe1 * (sl - nv1);
-  if (fact != T(0)) fact = T(1) / fact;
+  if (fact != T(0)) fact = frcp(fact);
   sm = fact * sm;
   // upwind side K: left when sl > 0 or (sl <= 0 < sm), right when
   // sm <= 0 <= sr or sr < 0; neither only for NaN wave speeds, where the
@@ -649,7 +681,7 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const
   f[4] = rnx * (rhk * uk) + rny * (rhk * vk) + rnz * (rhk * wk);
   T fk = sk - sm;
   if (star && fk != T(0)) {
-    fk = T(1) / fk;
+    fk = frcp(fk);
     T a = sk - nvk;
     T pks = rk * (nvk - sk) * (nvk - sm) + pk;
     T ps = pks - pk;
