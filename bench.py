@@ -181,14 +181,24 @@ def bench_config1(cycles):
             "drag_cpu": float(mo["forces"][1]), "nodes": lv[0].n_nodes, "sides": E}
 
 
-def bench_config3(cycles, ndir=16, h=1e-7):
+def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6):
     """Discrete adjoint case: Euler M=0.8, alpha=1.25 on the 64^3 bump box,
     first-order HLLC, after `cycles` growing-V cycles; 16 random unit
     directions (seed 1): tangent vs central differences of the residual,
     the tangent/adjoint dot-product identity, and the drag state gradient
-    vs central differences of the drag."""
+    vs central differences of the drag (step hd: the drag depends on the
+    wall nodes only, so along a unit direction over all nodes its change is
+    small and a 1e-7 step leaves mostly rounding).  Then the coordinate gradient of
+    drag - lam^T R at the same state for a random lam (metric adjoint +
+    drag metric gradient, chained through the preprocessor's transpose) vs
+    central differences with the metrics rebuilt from perturbed coordinates,
+    along `nxdir` random directions that move the interior and wall nodes;
+    nodes on farfield faces stay fixed, since the farfield Roe flux's |u.n|
+    has a kink on the side faces, where the freestream has no normal
+    velocity."""
     from flite3d_amd import device, mesh
-    lv = mesh.bump_levels(64, 3)
+    raw = mesh.bump_box(64)
+    lv, _ = mesh.build_levels(*raw, 3)
     s = device.Solver(euler_params(0.8, 1.25, scheme=3), lv)
     s.init_freestream()
     r0 = None
@@ -230,17 +240,53 @@ def bench_config3(cycles, ndir=16, h=1e-7):
         um[:5] -= h * du
         fd = (resid(up) - resid(um)) / (2 * h)
         fd_err.append(float(np.linalg.norm(fd - jt) / np.linalg.norm(jt)))
+        up, um = u.copy(), u.copy()
+        up[:5] += hd * du
+        um[:5] -= hd * du
         s.set(1, "U", up)
         dp, _ = s.drag_gradient()
         s.set(1, "U", um)
         dm, _ = s.drag_gradient()
         an = float(np.sum(g * du))
-        drag_err.append(abs((dp - dm) / (2 * h) - an) / max(abs(an), 1e-300))
+        drag_err.append(abs((dp - dm) / (2 * hd) - an) / max(abs(an), 1e-300))
+    # coordinate gradient of drag - lam^T R through the metrics
+    lam = rng.standard_normal((5, n))
+    s.set(1, "U", u)
+    t0 = time.perf_counter()
+    gw, gwb = s.metric_adjoint(lam)
+    _, gjwb = s.drag_metric_gradient()
+    xyz, tets, btri, bind = raw
+    gx = mesh.metric_vjp(xyz, tets, btri, bind, -gw, gjwb - gwb)
+    t_shape = time.perf_counter() - t0
     s.close()
+    movable = np.ones(xyz.shape[0], bool)
+    movable[np.unique(btri[bind != 1])] = False
+
+    def objective(x):
+        lvx, _ = mesh.build_levels(x, tets, btri, bind, 2)
+        sx = device.Solver(euler_params(0.8, 1.25, scheme=3), lvx)
+        sx.set(1, "U", u)
+        sx.apply(1, "PRESSURE")
+        sx.apply(1, "RHS")
+        r = sx.get(1, "RHS")[:5]
+        dr, _ = sx.drag_gradient()
+        sx.close()
+        return dr - float(np.sum(lam * r))
+
+    x_err = []
+    for _ in range(nxdir):
+        dx = rng.standard_normal(xyz.shape) * movable[:, None]
+        dx /= np.linalg.norm(dx)
+        fd = (objective(xyz + hx * dx) - objective(xyz - hx * dx)) / (2 * hx)
+        an = float(np.sum(gx * dx))
+        x_err.append(abs(fd - an) / max(abs(an), 1e-300))
     return {"metric": "discrete adjoint checks, Euler 64^3 bump box", "config": 3,
             "cycles": cycles, "residual_drop_log10": round(drop, 3), "directions": ndir, "fd_step": h,
             "tangent_vs_fd_max_rel_err": max(fd_err), "dot_product_max_rel_err": max(dot_err),
-            "drag_gradient_vs_fd_max_rel_err": max(drag_err),
+            "drag_fd_step": hd, "drag_gradient_vs_fd_max_rel_err": max(drag_err),
+            "coordinate_gradient_directions": nxdir, "coordinate_fd_step": hx,
+            "coordinate_gradient_vs_fd_max_rel_err": max(x_err),
+            "s_per_coordinate_gradient": round(t_shape, 3),
             "ms_per_tangent_apply": round(1e3 * t_tan / ndir, 3),
             "ms_per_adjoint_apply": round(1e3 * t_adj / ndir, 3),
             "note": "apply times include host<->device transfer of the vectors",

@@ -41,6 +41,7 @@ def prepro():
                                       P(d), P(d), P(d), P(d), P(i), P(i),
                                       P(i), P(i), P(i), P(i), P(d)]
         lib.fp_fine_old_index.argtypes = [ctypes.c_void_p, P(i)]
+        lib.fp_metric_vjp.argtypes = [i, P(d), i, P(i), i, P(i), P(i), P(d), P(d), P(d)]
         _prepro = lib
     return _prepro
 

@@ -938,9 +938,8 @@ __device__ __forceinline__ void state_record(const T *U, double gamma, T *A) {
 }
 
 // boundary flux of one end (the per-end flux of k_bflux): F(U)·n̂
-template <typename T>
-__device__ __forceinline__ void bside_end_flux(const DevParams &P, int ind, const T *U, double rnx, double rny,
-                                               double rnz, T *f) {
+template <typename T, typename N>
+__device__ __forceinline__ void bside_end_flux(const DevParams &P, int ind, const T *U, N rnx, N rny, N rnz, T *f) {
   T A[8];
   state_record(U, P.gamma, A);
   if (ind <= 2 || ind == 9) {
@@ -1109,6 +1108,91 @@ __global__ void k_adjoint_gather(LevelDev<double> L, const double *abuf, double 
 #pragma unroll
   for (int c = 0; c < 5; ++c) out[c * n + i] = r[c];
   out[5 * n + i] = 0;
+}
+
+// Metric adjoint: the derivative of lam^T R with respect to the side
+// weights at the current state, for the residual of the tangent and adjoint
+// (first-order HLLC, inviscid).  The interior flux depends on w through
+// n = 2w, |n| and n/|n|, the boundary flux through |w_b| and w_b/|w_b|;
+// three dual passes per side, seeded on the weight components, give the
+// three components of the derivative.  Side e adds +f to R(ea) and -f to
+// R(eb), so its share is (lam_a - lam_b) . df/dw.
+__global__ void k_flux_metric_adjoint(LevelDev<double> L, DevParams P, const double *lam, double *gw) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  Dual Ua[5], Ub[5], A[8], B[8];
+  double d[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib]);
+    d[c] = lam[c * n + ia] - lam[c * n + ib];
+  }
+  state_record(Ua, P.gamma, A);
+  state_record(Ub, P.gamma, B);
+  const double w0 = L.w[3 * e], w1 = L.w[3 * e + 1], w2 = L.w[3 * e + 2];
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    Dual o[6];
+    side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
+                                         Dual(w1, k == 1 ? 1.0 : 0.0), Dual(w2, k == 2 ? 1.0 : 0.0), o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
+    gw[3 * (size_t)e + k] = acc;
+  }
+}
+
+// boundary sides: R(a) += |w|(3F_a + F_b), R(b) += |w|(F_a + 3F_b)
+// (boundaryTerm 1; 4F_a and 4F_b for 2) with F evaluated along w/|w|
+__global__ void k_bflux_metric_adjoint(LevelDev<double> L, DevParams P, const double *lam, double *gwb) {
+  using ad::Dual;
+  int s = gtid_x();
+  if (s >= L.nbs) return;
+  size_t n = L.n;
+  int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
+  const double w0 = L.wb[3 * s], w1 = L.wb[3 * s + 1], w2 = L.wb[3 * s + 2];
+  double *o = gwb + 3 * (size_t)s;
+  double rn = ::sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+  if (!(rn > 1e-10) || !(ind <= 4 || ind == 9)) {
+    o[0] = o[1] = o[2] = 0;
+    return;
+  }
+  Dual Ua[5], Ub[5];
+  double la[5], lb[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib]);
+    la[c] = lam[c * n + ia];
+    lb[c] = lam[c * n + ib];
+  }
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    Dual wx(w0, k == 0 ? 1.0 : 0.0), wy(w1, k == 1 ? 1.0 : 0.0), wz(w2, k == 2 ? 1.0 : 0.0);
+    Dual rnorm = sqrt(wx * wx + wy * wy + wz * wz);
+    Dual irn = Dual(1.0) / rnorm;
+    Dual f1[5], f2[5];
+    bside_end_flux(P, ind, Ua, wx * irn, wy * irn, wz * irn, f1);
+    bside_end_flux(P, ind, Ub, wx * irn, wy * irn, wz * irn, f2);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Dual ra, rb;
+      if (P.bterm == 1) {
+        ra = rnorm * (Dual(3.0) * f1[c] + f2[c]);
+        rb = rnorm * (f1[c] + Dual(3.0) * f2[c]);
+      } else {
+        ra = rnorm * Dual(4.0) * f1[c];
+        rb = rnorm * Dual(4.0) * f2[c];
+      }
+      acc += la[c] * ra.d + lb[c] * rb.d;
+    }
+    o[k] = acc;
+  }
 }
 
 // partial derivative of the pressure drag (calculateLiftAndDrag,

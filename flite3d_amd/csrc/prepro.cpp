@@ -40,6 +40,11 @@ struct Level {
 struct fp_mesh {
   std::vector<Level> lev;
   std::vector<int> old_index;  // fine new -> input index
+  // kept for fp_metric_vjp only: positively oriented tets and, per boundary
+  // triangle, its three boundary sides (fine numbering)
+  bool keep_cells = false;
+  std::vector<std::array<int, 4>> tets;
+  std::vector<int> bface, bface_side;  // 3 vertices, 3 boundary sides per triangle
 };
 
 namespace {
@@ -252,6 +257,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
     for (int f = 0; f < nb; ++f) {
       int p[3];
       for (int c = 0; c < 3; ++c) p[c] = o2n[btri_in[3 * f + c]];
+      if (m.keep_cells) m.bface.insert(m.bface.end(), p, p + 3);
       const double *x1 = X + 3 * p[0], *x2 = X + 3 * p[1], *x3 = X + 3 * p[2];
       double xm[3], xb1[3], xb2[3], xb3[3];
       for (int c = 0; c < 3; ++c) {
@@ -284,6 +290,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
           L.wb.push_back(0.0); L.wb.push_back(0.0); L.wb.push_back(0.0);
         }
         for (int c = 0; c < 3; ++c) L.wb[3 * size_t(sid) + c] += 0.25 * (0.5 * cx[c]);
+        if (m.keep_cells) m.bface_side.push_back(sid);
       }
     }
   }
@@ -316,7 +323,106 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
   }
   L.seed.resize(n);
   std::iota(L.seed.begin(), L.seed.end(), 0);
+  if (m.keep_cells) m.tets = std::move(tets);
   m.lev.push_back(std::move(L));
+}
+
+// Transpose of the metric computation of build_fine: given gw = dF/dw per
+// side and gwb = dF/dw_b per boundary side (column-major (E,3), (Eb,3),
+// fine numbering), accumulate gx = dF/dx (3 per node, fine numbering).  The
+// median-dual coefficients are sums of cross products of differences of
+// vertex, edge-midpoint, face-centroid and tet-centroid positions; the
+// transpose of G.(u x v) is (v x G) for u and (G x u) for v.
+void metric_vjp(const fp_mesh &m, const double *gw, const double *gwb, double *gx) {
+  const Level &L = m.lev[0];
+  const double *X = L.xyz.data();
+  const size_t ne = L.e1.size(), nb = L.b1.size();
+  static const int perm[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2},
+                                 {1, 2, 0, 3}, {1, 3, 2, 0}, {2, 3, 0, 1}};
+  // sides are sorted by (e1, e2): the same lookup as build_fine's table
+  std::vector<long long> start(size_t(L.n_nodes) + 1, 0);
+  for (size_t e = 0; e < ne; ++e) start[L.e1[e] + 1]++;
+  for (int i = 0; i < L.n_nodes; ++i) start[i + 1] += start[i];
+  auto find = [&](int lo, int hi) {
+    auto first = L.e2.begin() + start[lo], last = L.e2.begin() + start[lo + 1];
+    return int(std::lower_bound(first, last, hi) - L.e2.begin());
+  };
+  for (const auto &q : m.tets) {
+    double g[3];
+    for (int c = 0; c < 3; ++c)
+      g[c] = 0.25 * (X[3 * q[0] + c] + X[3 * q[1] + c] + X[3 * q[2] + c] + X[3 * q[3] + c]);
+    double dg[3] = {0, 0, 0};
+    for (int e = 0; e < 6; ++e) {
+      int a = q[perm[e][0]], b = q[perm[e][1]], c3 = q[perm[e][2]], d = q[perm[e][3]];
+      int lo = std::min(a, b), hi = std::max(a, b);
+      int eid = find(lo, hi);
+      double sg = (a == lo) ? 1.0 : -1.0;
+      // w_e += sg C / 2 with C = (r1 x r2 + r3 x r4) / 2,
+      // r1 = r4 = g - m, r2 = f_abd - m, r3 = f_abc - m
+      double G[3], r1[3], r2[3], r3[3];
+      for (int c = 0; c < 3; ++c) {
+        G[c] = sg * 0.25 * gw[c * ne + eid];
+        double mm = 0.5 * (X[3 * a + c] + X[3 * b + c]);
+        r1[c] = g[c] - mm;
+        r2[c] = (X[3 * a + c] + X[3 * b + c] + X[3 * d + c]) / 3. - mm;
+        r3[c] = (X[3 * a + c] + X[3 * b + c] + X[3 * c3 + c]) / 3. - mm;
+      }
+      double d1[3], d2[3], d3[3], d4[3];
+      cross(r2, G, d1);  // d/dr1
+      cross(G, r1, d2);  // d/dr2
+      cross(r1, G, d3);  // d/dr3 (r4 = r1)
+      cross(G, r3, d4);  // d/dr4
+      for (int c = 0; c < 3; ++c) {
+        double dm = -(d1[c] + d2[c] + d3[c] + d4[c]);
+        dg[c] += d1[c] + d4[c];
+        gx[3 * size_t(a) + c] += 0.5 * dm + (d2[c] + d3[c]) / 3.;
+        gx[3 * size_t(b) + c] += 0.5 * dm + (d2[c] + d3[c]) / 3.;
+        gx[3 * size_t(c3) + c] += d3[c] / 3.;
+        gx[3 * size_t(d) + c] += d2[c] / 3.;
+      }
+    }
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) gx[3 * size_t(q[k]) + c] += 0.25 * dg[c];
+  }
+  // boundary triangles: w_b(side) += (r1 x r2) / 8 for the three
+  // (vertex, edge midpoint, centroid) sub-triangles, as build_fine
+  const size_t nf = m.bface.size() / 3;
+  for (size_t f = 0; f < nf; ++f) {
+    const int p[3] = {m.bface[3 * f], m.bface[3 * f + 1], m.bface[3 * f + 2]};
+    const double *x1 = X + 3 * p[0], *x2 = X + 3 * p[1], *x3 = X + 3 * p[2];
+    double xm[3], xb1[3], xb2[3], xb3[3];
+    for (int c = 0; c < 3; ++c) {
+      xm[c] = (x1[c] + x2[c] + x3[c]) / 3.;
+      xb1[c] = 0.5 * (x1[c] + x2[c]);
+      xb2[c] = 0.5 * (x2[c] + x3[c]);
+      xb3[c] = 0.5 * (x3[c] + x1[c]);
+    }
+    // gradients of the sub-triangles' points: vertex x1, x2, x3, centroid, midpoints b1, b2, b3
+    double dx1[3] = {0, 0, 0}, dx2[3] = {0, 0, 0}, dxm[3] = {0, 0, 0};
+    double db1[3] = {0, 0, 0}, db2[3] = {0, 0, 0}, db3[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+      int sid = m.bface_side[3 * f + k];
+      double G[3], r1[3], r2[3], d1[3], d2[3];
+      for (int c = 0; c < 3; ++c) G[c] = 0.125 * gwb[c * nb + sid];
+      for (int c = 0; c < 3; ++c) {
+        if (k == 0) { r1[c] = xm[c] - x1[c]; r2[c] = xb1[c] - x1[c]; }
+        else if (k == 1) { r1[c] = xb3[c] - x1[c]; r2[c] = xm[c] - x1[c]; }
+        else { r1[c] = xm[c] - x2[c]; r2[c] = xb2[c] - x2[c]; }
+      }
+      cross(r2, G, d1);
+      cross(G, r1, d2);
+      for (int c = 0; c < 3; ++c) {
+        if (k == 0) { dxm[c] += d1[c]; db1[c] += d2[c]; dx1[c] -= d1[c] + d2[c]; }
+        else if (k == 1) { db3[c] += d1[c]; dxm[c] += d2[c]; dx1[c] -= d1[c] + d2[c]; }
+        else { dxm[c] += d1[c]; db2[c] += d2[c]; dx2[c] -= d1[c] + d2[c]; }
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      gx[3 * size_t(p[0]) + c] += dx1[c] + dxm[c] / 3. + 0.5 * (db1[c] + db3[c]);
+      gx[3 * size_t(p[1]) + c] += dx2[c] + dxm[c] / 3. + 0.5 * (db1[c] + db2[c]);
+      gx[3 * size_t(p[2]) + c] += dxm[c] / 3. + 0.5 * (db2[c] + db3[c]);
+    }
+  }
 }
 
 // One agglomeration step, mergeNodes (gen3d_grid.f90:2358-2901) with
@@ -616,6 +722,26 @@ int fp_level_copy(const fp_mesh *m, int level, int *side_index, double *side_w,
   if (f2c) { if (level == 0) { g_err = "fine level has no transfer"; return -2; } std::copy(L.f2c.begin(), L.f2c.end(), f2c); }
   if (rweight) { if (level == 0) { g_err = "fine level has no transfer"; return -2; } std::copy(L.rweight.begin(), L.rweight.end(), rweight); }
   return 0;
+}
+
+int fp_metric_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
+                  int n_btri, const int *btri, const int *btri_ind,
+                  const double *gw, const double *gwb, double *gx) {
+  try {
+    if (n_nodes <= 0 || n_tets <= 0 || n_btri <= 0) throw std::runtime_error("bad sizes");
+    if (!gw || !gwb || !gx) throw std::runtime_error("null array");
+    fp_mesh m;
+    m.keep_cells = true;
+    build_fine(m, n_nodes, xyz, n_tets, tets, n_btri, btri, btri_ind);
+    std::vector<double> g(3 * size_t(n_nodes), 0.0);
+    metric_vjp(m, gw, gwb, g.data());
+    for (int i = 0; i < n_nodes; ++i)
+      for (int c = 0; c < 3; ++c) gx[3 * size_t(m.old_index[i]) + c] = g[3 * size_t(i) + c];
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 int fp_fine_old_index(const fp_mesh *m, int *old_index) {

@@ -173,6 +173,7 @@ struct Level {
   // monitor scratch
   double *m_sum = nullptr, *m_max = nullptr, *m_force = nullptr;
   double *abuf = nullptr;  // adjoint side shares, 10 per side (allocated on first use)
+  std::vector<int> side_host;  // device side -> uploaded side index
   int *m_idx = nullptr;
 
   template <typename R>
@@ -584,6 +585,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     if (lx != ly) return lx < ly;
     return std::max(da[x], db[x]) < std::max(da[y], db[y]);
   });
+  L.side_host = eorder;
   std::vector<int> ea(ne), eb(ne);
   std::vector<double> w(3 * (size_t)ne);
   for (int k = 0; k < ne; ++k) {
@@ -1765,6 +1767,99 @@ int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) 
                        (const double *)L.warea, 2.0 / hp.reference_area, t0, t1, t2, hp.groundplane, V.rhs);
   TRY(check_launch());
   return get_vector5(c, dJdU, ld);
+}
+
+int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb,
+                         int ld_wb) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output arrays");
+  TRY(put_vector5(c, lam, ld_lam));
+  if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  // side shares into abuf (3 per side), boundary shares into bbuf (3 per side)
+  if (L.ne > 0) hipLaunchKernelGGL(k_flux_metric_adjoint, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf);
+  if (L.nbs > 0)
+    hipLaunchKernelGGL(k_bflux_metric_adjoint, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src, V.bbuf);
+  TRY(check_launch());
+  std::vector<double> hw(3 * (size_t)L.ne), hb(3 * (size_t)L.nbs);
+  if (L.ne > 0) HIPTRY(hipMemcpyAsync(hw.data(), L.abuf, hw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (L.nbs > 0) HIPTRY(hipMemcpyAsync(hb.data(), V.bbuf, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < L.ne; ++k)
+    for (int q = 0; q < 3; ++q) gw[(size_t)q * ld_w + L.side_host[k]] = hw[3 * (size_t)k + q];
+  for (int s = 0; s < L.nbs; ++s)
+    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] = hb[3 * (size_t)s + q];
+  return 0;
+}
+
+// pressure drag of calculateLiftAndDrag (DEF:14757-14803) as a function of
+// the boundary-side weights at a fixed state: the wall node areas
+// A_i = sum 4|w_b| and normals n_i = m_i/|m_i|, m_i = -sum w_b, over the
+// boundary sides at the node (as build_level forms warea and nrm)
+int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_wb) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  const Staged &S = c->staged[0];
+  if (!gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output array");
+  const flite_params &hp = c->hp;
+  TRY(op_prep(c, L, true));
+  double res[7], mx[7], forces[9];
+  int idx[7];
+  TRY(flite_monitor(c, res, mx, idx, forces));
+  if (drag) *drag = forces[1];
+  const int n = L.n, nbs = L.nbs;
+  std::vector<double> u(6 * (size_t)n);
+  TRY(flite_state_get(c, 1, u.data(), n, 5, 0));
+  double PI = 4.0 * std::atan(1.0);
+  double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
+  const double t[3] = {std::cos(ar) * std::cos(br), std::sin(br), std::sin(ar)};
+  const double f = 2.0 / hp.reference_area, g1 = hp.gamma - 1.0;
+  const double pinf = 1.0 / (hp.gamma * hp.mach_number * hp.mach_number);
+  // node sums m_i and A_i over all boundary sides, as build_level forms them
+  std::vector<double> m(3 * (size_t)n, 0.0), area(n, 0.0);
+  for (int s = 0; s < nbs; ++s) {
+    double w[3], a = 0;
+    for (int q = 0; q < 3; ++q) { w[q] = S.wb[(size_t)q * nbs + s]; a += w[q] * w[q]; }
+    a = 4.0 * std::sqrt(a);
+    for (int q = 0; q < 3; ++q) { m[3 * (size_t)S.b1[s] + q] -= w[q]; m[3 * (size_t)S.b2[s] + q] -= w[q]; }
+    area[S.b1[s]] += a;
+    area[S.b2[s]] += a;
+  }
+  // per counted wall node: dJ/dA_i and dJ/dm_i
+  std::vector<double> gA(n, 0.0), gM(3 * (size_t)n, 0.0);
+  for (int d : L.wall_dev) {
+    int i = L.d2h[d];
+    double z = S.xyz.empty() ? 0.0 : S.xyz[2 * (size_t)n + i];
+    if (!(z > hp.groundplane + 0.01)) continue;
+    double r = u[i], m1 = u[(size_t)n + i], m2 = u[2 * (size_t)n + i], m3 = u[3 * (size_t)n + i],
+           E = u[4 * (size_t)n + i];
+    double p = g1 * (E - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r);
+    p = p > 0.1 ? p : 0.1;
+    const double *mi = &m[3 * (size_t)i];
+    double mn = std::sqrt(mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]);
+    if (!(mn > 1e-10)) continue;  // zero normal: no dependence
+    double nh[3] = {mi[0] / mn, mi[1] / mn, mi[2] / mn};
+    double nt = nh[0] * t[0] + nh[1] * t[1] + nh[2] * t[2];
+    gA[i] = -f * (p - pinf) * nt;
+    double gN[3], gNn = 0;
+    for (int q = 0; q < 3; ++q) { gN[q] = -f * area[i] * (p - pinf) * t[q]; gNn += gN[q] * nh[q]; }
+    for (int q = 0; q < 3; ++q) gM[3 * (size_t)i + q] = (gN[q] - gNn * nh[q]) / mn;
+  }
+  for (int s = 0; s < nbs; ++s) {
+    double w[3], a = 0;
+    for (int q = 0; q < 3; ++q) { w[q] = S.wb[(size_t)q * nbs + s]; a += w[q] * w[q]; }
+    a = std::sqrt(a);
+    double g[3] = {0, 0, 0};
+    for (int i : {S.b1[s], S.b2[s]}) {
+      for (int q = 0; q < 3; ++q) g[q] -= gM[3 * (size_t)i + q];
+      if (a > 0)
+        for (int q = 0; q < 3; ++q) g[q] += gA[i] * 4.0 * w[q] / a;
+    }
+    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] = g[q];
+  }
+  return 0;
 }
 
 int flite_batch_select(flite_ctx *c, int batch) {

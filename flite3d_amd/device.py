@@ -51,6 +51,8 @@ def lib():
         L.flite_tangent_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
+        L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
+        L.flite_drag_metric_gradient.argtypes = [vp, P(d), P(d), i]
         _bound = L
     return _bound
 
@@ -189,6 +191,34 @@ class Solver:
         D = ctypes.POINTER(ctypes.c_double)
         self._check(self.L.flite_drag_state_gradient(self.ctx, ctypes.byref(drag), out.ctypes.data_as(D), n))
         return drag.value, out
+
+
+    # ---- derivatives with respect to the level-1 metrics
+    def _sizes1(self):
+        lv = self.levels[0]
+        return lv.side_index.shape[1], lv.bside_index.shape[1]
+
+    def metric_adjoint(self, lam):
+        """(d(lam^T R)/d side_w, d(lam^T R)/d bside_w) at the current state,
+        shaped like the uploaded level's side_w (3, E) and bside_w (3, Eb)."""
+        n = self.n(1)
+        ne, nbs = self._sizes1()
+        x = self._vec5(lam)
+        gw = np.zeros((3, ne))
+        gwb = np.zeros((3, nbs))
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_metric_adjoint(self.ctx, x.ctypes.data_as(D), n, gw.ctypes.data_as(D), ne,
+                                                gwb.ctypes.data_as(D), nbs))
+        return gw, gwb
+
+    def drag_metric_gradient(self):
+        """(pressure drag, dDrag/d bside_w (3, Eb)) at the current state."""
+        ne, nbs = self._sizes1()
+        gwb = np.zeros((3, nbs))
+        drag = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_drag_metric_gradient(self.ctx, ctypes.byref(drag), gwb.ctypes.data_as(D), nbs))
+        return drag.value, gwb
 
 
 def _bind_extra():

@@ -247,6 +247,31 @@ def build_levels(xyz, tets, btri, bind, n_levels):
         lib.fp_mesh_free(m)
 
 
+def metric_vjp(xyz, tets, btri, bind, gw, gwb):
+    """Transpose of the fine level's metrics with respect to the coordinates:
+    (d side_w/d xyz)^T gw + (d bside_w/d xyz)^T gwb for the fine level that
+    build_levels makes from the same mesh.  gw is (3, n_sides) and gwb
+    (3, n_bsides) in that level's side order (the layout of side_w,
+    bside_w); returns an (n_nodes, 3) array in the input node order."""
+    lib = _native.prepro()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    btri = np.ascontiguousarray(btri, dtype=np.int32)
+    bind = np.ascontiguousarray(bind, dtype=np.int32)
+    gw = np.ascontiguousarray(gw, dtype=np.float64)
+    gwb = np.ascontiguousarray(gwb, dtype=np.float64)
+    gx = np.zeros((xyz.shape[0], 3))
+    rc = lib.fp_metric_vjp(xyz.shape[0], _ptr(xyz, ctypes.c_double),
+                           tets.shape[0], _ptr(tets, ctypes.c_int),
+                           btri.shape[0], _ptr(btri, ctypes.c_int),
+                           _ptr(bind, ctypes.c_int),
+                           _ptr(gw, ctypes.c_double), _ptr(gwb, ctypes.c_double),
+                           _ptr(gx, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("metric_vjp failed: " + lib.fp_last_error().decode())
+    return gx
+
+
 def bump_levels(n, n_levels, jitter=0.0, seed=0, first_cell=None,
                 ratio=1.15, h=0.05, wall_indicator=1):
     xyz, tets, btri, bind = bump_box(n, jitter=jitter, seed=seed,

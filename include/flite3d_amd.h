@@ -201,6 +201,25 @@ int flite_tangent_apply(flite_ctx *ctx, const double *du, int ld_du, double *out
 int flite_adjoint_apply(flite_ctx *ctx, const double *lam, int ld_lam, double *out, int ld_out);
 int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld);
 
+/* Derivatives with respect to the level-1 mesh metrics, for gradients with
+ * respect to the node coordinates (no counterpart in the reference; for
+ * shape-sensitivity callers).  Same residual and restrictions as
+ * flite_adjoint_apply.  Outputs use the uploaded side order and layout,
+ * column-major (E,3) like flite_level_view.side_weights and (Eb,3) like
+ * bside_weights, with leading dimensions ld_w >= n_sides, ld_wb >= n_bsides:
+ *   flite_metric_adjoint:  gw  = d(lam^T R)/d side_weights,
+ *                          gwb = d(lam^T R)/d bside_weights, at the current
+ *                          state (dual numbers seeded on the weights);
+ *   flite_drag_metric_gradient: the pressure drag (as flite_drag_state_gradient)
+ *                          and its partial derivative with respect to the
+ *                          boundary-side weights, through the wall-node areas
+ *                          and normals the library forms from them.
+ * With lam solving (dR/dU)^T lam = dJ/dU, the coordinate gradient of J is
+ * the chain of (gJwb - gwb, -gw) through the metrics' dependence on the
+ * coordinates (fp_metric_vjp in flite3d_prepro.h). */
+int flite_metric_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb);
+int flite_drag_metric_gradient(flite_ctx *ctx, double *drag, double *gwb, int ld_wb);
+
 /* Condition that flite_field_get/set and flite_monitor act on (default 0);
  * no reference counterpart (the reference has one condition per run). */
 int flite_batch_select(flite_ctx *ctx, int batch);

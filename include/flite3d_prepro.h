@@ -77,6 +77,18 @@ int fp_level_copy(const fp_mesh *m, int level, int *side_index,
 
 /* fine-level node i (new numbering) -> index in the xyz passed in */
 int fp_fine_old_index(const fp_mesh *m, int *old_index);
+/* Transpose of the fine level's metrics with respect to the coordinates
+ * (no counterpart in the reference; for coordinate gradients):
+ *   gx = (d side_w / d xyz)^T gw + (d bside_w / d xyz)^T gwb
+ * for the mesh (xyz, tets, btri, btri_ind) as fp_mesh_build(..., 1) builds
+ * it.  gw [3*n_sides] and gwb [3*n_bsides] are column-major in that build's
+ * side order (the side_w / bside_w layout of fp_level_copy); gx
+ * [n_nodes*3] is row-major in the input node order, like xyz.  The dual
+ * volumes are not differentiated (the first-order inviscid residual and
+ * the pressure drag do not depend on them). */
+int fp_metric_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
+                  int n_btri, const int *btri, const int *btri_ind,
+                  const double *gw, const double *gwb, double *gx);
 
 #ifdef __cplusplus
 }

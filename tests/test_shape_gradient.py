@@ -1,0 +1,132 @@
+"""Gradients with respect to the mesh metrics and the node coordinates on
+the GPU (flite_metric_adjoint, flite_drag_metric_gradient, and the
+preprocessor's fp_metric_vjp).
+
+The reference has no differentiated code, so these are pinned by central
+finite differences of what the library and the preprocessor compute:
+  * d(lam^T R)/d(side weights): re-upload the level with perturbed weights;
+  * d(pressure drag)/d(boundary-side weights) at a fixed state;
+  * the chain to the coordinates, L(X) = drag(U, X) - lam^T R(U, X) at a
+    fixed state U and vector lam, with the metrics rebuilt by the
+    preprocessor from X +- h dX (the node and side numbering depend on the
+    topology only, so U and lam keep their meaning).
+Euler, first-order HLLC, M=0.8, alpha=1.25 deg on a jittered bump box after
+a few cycles, with 2 deg of sideslip.  R and drag are piecewise smooth: the
+farfield Roe flux (SolverSource/mgnsg3d_def_par.F90:2311-2346) carries
+|u.n|, which has a kink where the normal velocity through a farfield face
+vanishes.  Without sideslip the side faces y = 0 and y = 1 sit on that
+kink (v ~ 1e-5 there), the one-sided derivatives differ, and central
+differences over any step straddle it; with sideslip every farfield face
+has |u.n| >= sin(1.25 deg) |u|.  The bar is 1e-6 relative for a step of 1e-6.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from flite3d_amd import mesh
+from flite3d_amd.device import Solver
+
+from cases import euler
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+
+
+def params():
+    return euler(1, machnumber=0.8, alpha=1.25, beta=2.0, cflnumber=2.0, multigridscheme=3).to_struct()
+
+
+@pytest.fixture(scope="module")
+def base():
+    print("[case] shape gradient base state n=6", flush=True)
+    raw = mesh.bump_box(6, jitter=0.15, seed=0)
+    lv, _ = mesh.build_levels(*raw, 2)
+    s = Solver(params(), list(lv))
+    s.init_freestream()
+    for c in range(1, 5):
+        s.cycle(c)
+    u = s.get(1, "U")
+    s.close()
+    rng = np.random.default_rng(11)
+    lam = rng.standard_normal((5, u.shape[1]))
+    return raw, lv, u, lam
+
+
+def evaluate(lv, u, lam):
+    """(lam^T R, drag) at state u on levels lv."""
+    s = Solver(params(), list(lv))
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    s.apply(1, "RHS")
+    r = s.get(1, "RHS")[:5]
+    drag, _ = s.drag_gradient()
+    s.close()
+    return float(np.sum(lam * r)), drag
+
+
+def gradients(lv, u, lam):
+    s = Solver(params(), list(lv))
+    s.set(1, "U", u)
+    gw, gwb = s.metric_adjoint(lam)
+    drag, gjwb = s.drag_metric_gradient()
+    s.close()
+    return gw, gwb, drag, gjwb
+
+
+def with_metrics(lv, w, wb):
+    return [dataclasses.replace(lv[0], side_w=np.ascontiguousarray(w), bside_w=np.ascontiguousarray(wb))] + list(lv[1:])
+
+
+def test_metric_adjoint_matches_central_differences(base):
+    _, lv, u, lam = base
+    gw, gwb, _, _ = gradients(lv, u, lam)
+    rng = np.random.default_rng(2)
+    w0, wb0 = lv[0].side_w, lv[0].bside_w
+    for part in ("interior", "boundary"):
+        dw = rng.standard_normal(w0.shape) * np.abs(w0).max() * (part == "interior")
+        dwb = rng.standard_normal(wb0.shape) * np.abs(wb0).max() * (part == "boundary")
+        lp, _ = evaluate(with_metrics(lv, w0 + H * dw, wb0 + H * dwb), u, lam)
+        lm, _ = evaluate(with_metrics(lv, w0 - H * dw, wb0 - H * dwb), u, lam)
+        fd = (lp - lm) / (2 * H)
+        an = float(np.sum(gw * dw) + np.sum(gwb * dwb))
+        print(part, fd, an, flush=True)
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-12), (part, fd, an)
+
+
+def test_drag_metric_gradient_matches_central_differences(base):
+    _, lv, u, lam = base
+    _, _, drag, gjwb = gradients(lv, u, lam)
+    _, d0 = evaluate(lv, u, lam)
+    assert drag == d0
+    rng = np.random.default_rng(4)
+    wb0 = lv[0].bside_w
+    for _ in range(2):
+        dwb = rng.standard_normal(wb0.shape) * np.abs(wb0).max()
+        _, dp = evaluate(with_metrics(lv, lv[0].side_w, wb0 + H * dwb), u, lam)
+        _, dm = evaluate(with_metrics(lv, lv[0].side_w, wb0 - H * dwb), u, lam)
+        fd = (dp - dm) / (2 * H)
+        an = float(np.sum(gjwb * dwb))
+        print(fd, an, flush=True)
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-12), (fd, an)
+
+
+def test_coordinate_gradient_matches_central_differences(base):
+    """d/dX of drag - lam^T R through the metrics the preprocessor builds."""
+    raw, lv, u, lam = base
+    xyz, tets, btri, bind = raw
+    gw, gwb, _, gjwb = gradients(lv, u, lam)
+    gx = mesh.metric_vjp(xyz, tets, btri, bind, -gw, gjwb - gwb)
+    rng = np.random.default_rng(6)
+    for _ in range(3):
+        d = rng.standard_normal(xyz.shape)
+        vals = []
+        for sgn in (1.0, -1.0):
+            lvp, _ = mesh.build_levels(xyz + sgn * H * d, tets, btri, bind, 2)
+            r, dr = evaluate(lvp, u, lam)
+            vals.append(dr - r)
+        fd = (vals[0] - vals[1]) / (2 * H)
+        an = float(np.sum(gx * d))
+        print(fd, an, flush=True)
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-12), (fd, an)
