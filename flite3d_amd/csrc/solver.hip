@@ -210,6 +210,10 @@ struct flite_ctx {
   double *cond_dev = nullptr;      // per-condition Mach and inflow state on the device
   int sel = 0;                     // condition that field/state/monitor calls act on
   std::vector<std::unique_ptr<Level>> lev;
+  // coarse levels of the other precision (flite_set_coarse_precision),
+  // built on first use and swapped with lev[1..] on every switch
+  std::vector<std::unique_ptr<Level>> lev_alt;
+  std::vector<std::vector<int>> orders;  // device numbering per level
   hipStream_t stream = nullptr;
   // kernel timing (flite_timing_enable)
   std::string timed;
@@ -525,11 +529,12 @@ std::vector<std::vector<int>> nested_orders(const std::vector<Staged> &st) {
   return h2d;
 }
 
-int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int> &h2d) {
+int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int> &h2d, bool coarse_fp32,
+                std::unique_ptr<Level> &out) {
   auto Lp = std::make_unique<Level>();
   Level &L = *Lp;
   L.gnum = level;
-  L.fp32 = c->hp.coarse_fp32 && level > 1;
+  L.fp32 = coarse_fp32 && level > 1;
   L.n = S.n;
   // enough lanes to give the chip >= ~1M threads on mid-size levels, and
   // short per-lane gather chains on the small coarse ones
@@ -839,7 +844,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     TRY(P.upload(&L.clist, clist));
   }
   L.uploaded = true;
-  c->lev[level - 1] = std::move(Lp);
+  out = std::move(Lp);
   return 0;
 }
 
@@ -855,6 +860,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   c->staged_ok[level - 1] = 1;
   c->built = false;
   for (auto &l : c->lev) l.reset();
+  c->lev_alt.clear();
   return 0;
 }
 
@@ -863,8 +869,10 @@ int ensure_built(flite_ctx *c) {
   if (c->built) return 0;
   for (int l = 0; l < c->nl; ++l)
     if (!c->staged_ok[l]) return fail(FLITE_E_STATE, "not all levels uploaded");
-  auto orders = nested_orders(c->staged);
-  for (int l = 0; l < c->nl; ++l) TRY(build_level(c, l + 1, c->staged[l], orders[l]));
+  c->orders = nested_orders(c->staged);
+  c->lev_alt.clear();
+  for (int l = 0; l < c->nl; ++l)
+    TRY(build_level(c, l + 1, c->staged[l], c->orders[l], c->hp.coarse_fp32 != 0, c->lev[l]));
   c->built = true;
   return 0;
 }
@@ -1859,6 +1867,21 @@ int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_w
     }
     for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] = g[q];
   }
+  return 0;
+}
+
+int flite_set_coarse_precision(flite_ctx *c, int fp32) {
+  TRY(check_ctx(c));
+  if (fp32 != 0 && fp32 != 1) return fail(FLITE_E_ARG, "fp32 must be 0 or 1");
+  if (c->nl < 2 || c->lev[1]->fp32 == (fp32 == 1)) return 0;
+  TRY(sync(c));
+  if (c->lev_alt.empty()) {
+    c->lev_alt.resize(c->nl);
+    for (int l = 1; l < c->nl; ++l) TRY(build_level(c, l + 1, c->staged[l], c->orders[l], fp32 == 1, c->lev_alt[l]));
+  }
+  for (int l = 1; l < c->nl; ++l) std::swap(c->lev[l], c->lev_alt[l]);
+  c->hp.coarse_fp32 = fp32;
+  for (auto &q : c->hp_b) q.coarse_fp32 = fp32;
   return 0;
 }
 

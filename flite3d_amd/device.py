@@ -52,6 +52,7 @@ def lib():
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
+        L.flite_set_coarse_precision.argtypes = [vp, i]
         L.flite_drag_metric_gradient.argtypes = [vp, P(d), P(d), i]
         _bound = L
     return _bound
@@ -139,6 +140,10 @@ class Solver:
     def cycle(self, number):
         """oneCycle (MG:907-1196)."""
         self._check(self.L.flite_cycle(self.ctx, number))
+
+    def set_coarse_precision(self, fp32):
+        """Run the coarse levels in FP32 (True) or FP64 (False) from now on."""
+        self._check(self.L.flite_set_coarse_precision(self.ctx, 1 if fp32 else 0))
 
     def select(self, batch):
         """Condition that get/set/monitor act on (batched contexts)."""

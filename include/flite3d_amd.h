@@ -220,6 +220,17 @@ int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld
 int flite_metric_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb);
 int flite_drag_metric_gradient(flite_ctx *ctx, double *drag, double *gwb, int ld_wb);
 
+/* Precision of the coarse levels from the next call on: 1 = FP32, 0 = FP64
+ * (flite_params.coarse_fp32 sets the initial one).  No counterpart in the
+ * reference.  FP32 coarse corrections are cheaper, but the FAS forcing
+ * on a coarse level is the difference of O(1) residuals, rounded to FP32, so
+ * the fine residual stalls near 1e-5 of its start; a driver that needs a
+ * deeper drop runs FP32 until the stall and then switches to FP64, which
+ * converges to the FP64 solution.  The first switch builds the other
+ * precision's coarse levels (as at upload); later switches swap them.  The
+ * coarse states are rebuilt from the fine state at the next restriction. */
+int flite_set_coarse_precision(flite_ctx *ctx, int fp32);
+
 /* Condition that flite_field_get/set and flite_monitor act on (default 0);
  * no reference counterpart (the reference has one condition per run). */
 int flite_batch_select(flite_ctx *ctx, int batch);

@@ -134,3 +134,48 @@ def test_fp32_coarse_mode():
     e = cycles_match(rans(multigridscheme=3), levels(8, 3, first_cell=0.005), tol=1e-5, mon_tol=1e-4,
                      fp32=True)
     assert e > 0.0  # the FP32 path really differs from the FP64 oracle
+
+
+def converge(s, r_stop, max_cycles, start=1):
+    """Cycles from `start` until the monitor residual is at most r_stop or
+    max_cycles have run; returns (last cycle, first residual, last residual)."""
+    r0 = r = None
+    c = start
+    while c < start + max_cycles:
+        s.cycle(c)
+        r = s.monitor()["res"][0]
+        r0 = r0 or r
+        if r <= r_stop:
+            break
+        c += 1
+    return c, r0, r
+
+
+def test_coarse_precision_switch():
+    """FP32 coarse levels stall (the FAS forcing is a difference of O(1)
+    residuals rounded to FP32); switching the coarse levels to FP64
+    (flite_set_coarse_precision) converges to the FP64 run's solution.
+    Switching to the current precision changes nothing."""
+    print("[case] coarse precision switch, euler HLLC-2 8^3", flush=True)
+    iv, lv = euler(2, multigridscheme=3), levels(8, 3)
+    ref = Solver(iv.to_struct(), list(lv))
+    ref.init_freestream()
+    ref.cycle(1)
+    r1 = ref.monitor()["res"][0]
+    _, _, r = converge(ref, 1e-12 * r1, 2000, start=2)
+    assert r <= 1e-12 * r1
+    u_ref, drag_ref = ref.get(1, "U")[:5], ref.monitor()["forces"][1]
+    ref.close()
+    s = Solver(iv.to_struct(coarse_fp32=True), list(lv))
+    s.init_freestream()
+    c, _, r32 = converge(s, 0.0, 300)
+    assert r32 > 1e-8 * r1  # the FP32 coarse corrections stall well above this
+    s.set_coarse_precision(True)
+    s.set_coarse_precision(False)
+    s.set_coarse_precision(False)
+    _, _, r = converge(s, 1e-12 * r1, 2000, start=c + 1)
+    assert r <= 1e-12 * r1
+    u, drag = s.get(1, "U")[:5], s.monitor()["forces"][1]
+    s.close()
+    assert rel_err(u, u_ref) < 1e-9, rel_err(u, u_ref)
+    assert abs(drag - drag_ref) <= 1e-10 * abs(drag_ref), (drag, drag_ref)
