@@ -191,3 +191,27 @@ def test_gather_lanes(lanes, monkeypatch):
     assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
     e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
     assert e.max() < 1e-9, e
+
+
+@pytest.mark.parametrize("env", [
+    {"FLITE_GATHER_LANES": "1"},                                   # LDS-staged node blocks on every level
+    {"FLITE_GATHER_LANES": "1", "FLITE_GATHER_KERNEL": "node"},    # per-node gathers only
+    {"FLITE_FLUX_KERNEL": "side"},                                 # per-side flux records and gather
+    {"FLITE_FLUX_KERNEL": "side", "FLITE_GATHER_LANES": "4"},
+])
+def test_kernel_variants(env, monkeypatch):
+    """Every kernel path the library selects (side blocks with per-node
+    partial sums or per-side flux records; LDS-staged or per-node gathers)
+    against the oracle over three growing-V RANS cycles."""
+    print(f"[case] kernel variant {env}", flush=True)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    lv = levels(8, 3, first_cell=0.005)
+    s, o = pair(rans(multigridscheme=3), lv)
+    s.init_freestream(); o.init_freestream()
+    for c in (1, 2, 3):
+        s.cycle(c); o.cycle(c)
+    ms, mo = s.monitor(), o.monitor()
+    assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
+    e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
+    assert e.max() < 1e-9, e
