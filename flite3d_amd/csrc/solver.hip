@@ -206,6 +206,13 @@ struct flite_ctx {
   int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
   bool flux_per_side = false;  // FLITE_FLUX_KERNEL=side: one lane per side with direct record loads
   bool gather_per_node = false;  // FLITE_GATHER_KERNEL=node: no LDS-staged node blocks
+  // One cycle's launches form the same sequence with the same arguments
+  // every cycle, so flite_cycle captures them once into a HIP graph and
+  // replays it (FLITE_CYCLE_GRAPH=0: launch them one by one).  Anything that
+  // changes the levels or the launch sequence drops the graph.
+  bool graph_off = false;
+  hipGraphExec_t cycle_exec = nullptr;
+  std::vector<int> cycle_evals;  // evaluations per level counted while capturing
   std::vector<flite_params> hp_b;  // per condition (batched contexts)
   double *cond_dev = nullptr;      // per-condition Mach and inflow state on the device
   int sel = 0;                     // condition that field/state/monitor calls act on
@@ -848,6 +855,11 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   return 0;
 }
 
+void drop_cycle_graph(flite_ctx *c) {
+  if (c->cycle_exec) (void)hipGraphExecDestroy(c->cycle_exec);
+  c->cycle_exec = nullptr;
+}
+
 int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   if (level > 1) {
     if (c->staged.size() < (size_t)level - 1 || !c->staged_ok[level - 2])
@@ -859,6 +871,7 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   c->staged[level - 1] = std::move(S);
   c->staged_ok[level - 1] = 1;
   c->built = false;
+  drop_cycle_graph(c);
   for (auto &l : c->lev) l.reset();
   c->lev_alt.clear();
   return 0;
@@ -1380,6 +1393,11 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
     if (v != "side" && v != "block") return fail(FLITE_E_ARG, "FLITE_FLUX_KERNEL must be side or block");
     c->flux_per_side = v == "side";
   }
+  if (const char *e = std::getenv("FLITE_CYCLE_GRAPH")) {
+    std::string v(e);
+    if (v != "0" && v != "1") return fail(FLITE_E_ARG, "FLITE_CYCLE_GRAPH must be 0 or 1");
+    c->graph_off = v == "0";
+  }
   if (const char *e = std::getenv("FLITE_GATHER_KERNEL")) {
     std::string v(e);
     if (v != "node" && v != "block") return fail(FLITE_E_ARG, "FLITE_GATHER_KERNEL must be node or block");
@@ -1398,7 +1416,14 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
 void flite_ctx_destroy(flite_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_cycle_graph(c);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (auto &pr : c->ev_used) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
   c->lev.clear();
+  c->lev_alt.clear();
   if (c->cond_dev) (void)hipFree(c->cond_dev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1582,8 +1607,35 @@ int flite_map_u_fine_to_coarse(flite_ctx *c, int fine_level) {
 
 int flite_cycle(flite_ctx *c, int cycle_number) {
   TRY(check_ctx(c));
-  c->evals.assign(c->nl, 0);
-  TRY(do_cycle(c, cycle_number));
+  if (cycle_number > c->hp.number_of_turbulence_steps && c->P.sa)
+    return fail(FLITE_E_UNSUPPORTED, "frozen turbulence (cycle > numberOfTurbulenceSteps) is not implemented");
+  // kernel timing records events around single launches: launched one by one
+  // (events recorded while capturing cannot be timed on this runtime)
+  if (c->graph_off || !c->timed.empty()) {
+    c->evals.assign(c->nl, 0);
+    TRY(do_cycle(c, cycle_number));
+    return sync(c);
+  }
+  if (!c->cycle_exec) {
+    c->evals.assign(c->nl, 0);
+    HIPTRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = do_cycle(c, cycle_number);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != 0 || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc != 0 ? rc : fail(FLITE_E_HIP, std::string("cycle capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&c->cycle_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->cycle_exec = nullptr;
+      return fail(FLITE_E_HIP, std::string("cycle graph: ") + hipGetErrorString(e));
+    }
+    c->cycle_evals = c->evals;
+  }
+  c->evals = c->cycle_evals;
+  HIPTRY(hipGraphLaunch(c->cycle_exec, c->stream));
   return sync(c);
 }
 
@@ -1879,6 +1931,7 @@ int flite_set_coarse_precision(flite_ctx *c, int fp32) {
     c->lev_alt.resize(c->nl);
     for (int l = 1; l < c->nl; ++l) TRY(build_level(c, l + 1, c->staged[l], c->orders[l], fp32 == 1, c->lev_alt[l]));
   }
+  drop_cycle_graph(c);
   for (int l = 1; l < c->nl; ++l) std::swap(c->lev[l], c->lev_alt[l]);
   c->hp.coarse_fp32 = fp32;
   for (auto &q : c->hp_b) q.coarse_fp32 = fp32;

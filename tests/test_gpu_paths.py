@@ -179,3 +179,40 @@ def test_coarse_precision_switch():
     s.close()
     assert rel_err(u, u_ref) < 1e-9, rel_err(u, u_ref)
     assert abs(drag - drag_ref) <= 1e-10 * abs(drag_ref), (drag, drag_ref)
+
+
+def test_cycle_graph_replays_match_eager(monkeypatch):
+    """flite_cycle captures one cycle's launches into a HIP graph and replays
+    it.  Every replay, with eager work and synchronisation between replays
+    (monitor, field reads, a pressure recomputation), gives the bits of the
+    same cycle launched kernel by kernel (FLITE_CYCLE_GRAPH=0); a precision
+    switch drops the graph and the next cycle captures the new levels; with
+    kernel timing on, cycles run launch by launch and every launch is timed."""
+    print("[case] cycle graph replays vs eager, rans 8^3", flush=True)
+    from flite3d_amd import device
+    lv = levels(8, 3, first_cell=0.005)
+    iv = rans(multigridscheme=3)
+    monkeypatch.setenv("FLITE_CYCLE_GRAPH", "0")
+    e = Solver(iv.to_struct(coarse_fp32=True), list(lv))
+    monkeypatch.setenv("FLITE_CYCLE_GRAPH", "1")
+    g = Solver(iv.to_struct(coarse_fp32=True), list(lv))
+    for x in (e, g):
+        x.init_freestream()
+    for c in range(1, 7):
+        if c == 6:
+            device.timing_enable(g, "flux_l1")
+        if c == 4:
+            for x in (e, g):
+                x.set_coarse_precision(False)
+        e.cycle(c)
+        g.cycle(c)
+        me, mg = e.monitor(), g.monitor()
+        assert np.array_equal(e.get(1, "U"), g.get(1, "U")), c
+        assert np.array_equal(e.get(2, "U"), g.get(2, "U")), c
+        assert np.array_equal(me["res"], mg["res"]) and np.array_equal(me["forces"], mg["forces"]), c
+        for x in (e, g):
+            x.apply(1, "PRESSURE")
+    ms, n = device.timing_get(g)
+    assert n == device.cycle_evaluations(g)[0] and ms > 0, (ms, n)
+    e.close()
+    g.close()
