@@ -230,6 +230,32 @@ module flite_gpu
        integer(c_int), value :: ld
      end function flite_drag_state_gradient
 
+     integer(c_int) function flite_metric_adjoint(ctx, lam, ld_lam, gw, ld_w, gwb, ld_wb) &
+          bind(C, name="flite_metric_adjoint")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: lam(*)
+       integer(c_int), value :: ld_lam
+       real(c_double), intent(out) :: gw(*)
+       integer(c_int), value :: ld_w
+       real(c_double), intent(out) :: gwb(*)
+       integer(c_int), value :: ld_wb
+     end function flite_metric_adjoint
+
+     integer(c_int) function flite_drag_metric_gradient(ctx, drag, gwb, ld_wb) bind(C, name="flite_drag_metric_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: drag
+       real(c_double), intent(out) :: gwb(*)
+       integer(c_int), value :: ld_wb
+     end function flite_drag_metric_gradient
+
+     integer(c_int) function flite_set_coarse_precision(ctx, fp32) bind(C, name="flite_set_coarse_precision")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: fp32
+     end function flite_set_coarse_precision
+
      integer(c_int) function flite_batch_select(ctx, batch) bind(C, name="flite_batch_select")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -15,7 +15,10 @@
 !
 ! Usage: mgnsg3d_gpu [--coarse-fp32]
 !   --coarse-fp32  smooth and transfer on the coarse levels in FP32
-!                  (the fine level, the solution update and the forces stay FP64)
+!                  (the fine level, the solution update and the forces stay FP64);
+!                  FP32 coarse corrections stall near a 1e-5 residual drop, so
+!                  a run to a deeper drop (numberOfMGIterations < -4) switches
+!                  the coarse levels to FP64 once the residual has fallen 1e-4
 program mgnsg3d_gpu
   use, intrinsic :: iso_c_binding
   use flite_gpu
@@ -29,7 +32,7 @@ program mgnsg3d_gpu
   type(flite_level_view) :: v
   character(len=256) :: fileName, arg
   integer :: numberOfGrids, i, l, nvar, nres, k
-  logical :: coarseFp32
+  logical :: coarseFp32, switched
   real(c_double), allocatable :: u(:, :)
   real(c_double) :: res(7), maxres(7), forces(9)
   integer(c_int) :: maxind(7)
@@ -94,6 +97,7 @@ program mgnsg3d_gpu
   call system_clock(count_rate=rate)
 
   write (*, '(A)') "Starting timestepping..."
+  switched = .false.
   i = 1
   do
      if (stopAtResidualReduction < 0.0_c_double) then
@@ -108,6 +112,13 @@ program mgnsg3d_gpu
      averageTimePerCycle = averageTimePerCycle + dble(t1 - t0)/dble(rate)
 
      call writeResiduals(i)
+     if (coarseFp32 .and. .not. switched .and. stopAtResidualReduction > 4.0_c_double) then
+        if (log(initialResidual) - log(currentResidual) >= 4.0_c_double*log(10.0_c_double)) then
+           call flite_check(flite_set_coarse_precision(gpu_ctx, 0_c_int), "coarse precision")
+           switched = .true.
+           write (*, '(A,I7)') " coarse levels FP32 -> FP64 from cycle ", i + 1
+        end if
+     end if
      if (mod(i, ivd%writeToFileInterval) == 0) call writeResultsToFile()
      i = i + 1
      if (stopAtResidualReduction >= 0.0_c_double .and. i > ivd%maximumnumberofcycles) exit
