@@ -55,7 +55,8 @@ def log(*a):
 def build_case(n, n_levels, first_cell=1e-6):
     from flite3d_amd import mesh
     t = time.time()
-    lv = mesh.bump_levels(n, n_levels, first_cell=first_cell, ratio=1.15)
+    lv = mesh.bump_levels(n, n_levels, first_cell=first_cell, ratio=1.15,
+                          progress=lambda m: log(f"[bench] mesh: {m}"))
     log(f"[bench] mesh {n}^3: {[l.n_nodes for l in lv]} nodes, "
         f"{[l.n_sides for l in lv]} sides, built in {time.time() - t:.1f}s")
     return lv

@@ -19,6 +19,12 @@ namespace {
 
 thread_local std::string g_err;
 
+// optional progress report after each phase of a build (fp_set_progress)
+void (*g_progress)(const char *) = nullptr;
+void progress(const std::string &msg) {
+  if (g_progress) g_progress(msg.c_str());
+}
+
 struct Level {
   int n_nodes = 0, n_bnodes = 0;
   std::vector<int> e1, e2;       // sides, e1 < e2
@@ -162,6 +168,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
     tets[t] = q;
   }
 
+  progress("fine level: " + std::to_string(nt) + " tets oriented");
   // unique edges: out-neighbour lists from node->tet incidence
   std::vector<long long> tstart(n + 1, 0);
   for (auto &q : tets) for (int c = 0; c < 4; ++c) tstart[q[c] + 1]++;
@@ -201,6 +208,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
   L.w.assign(3 * size_t(ne), 0.0);
   L.vol.assign(n, 0.0);
 
+  progress("fine level: " + std::to_string(ne) + " sides");
   // median-dual coefficients, gen3d_grid.f90:2092-2332 with the per-side
   // formula of registerSideGCL (gen3d_grid.f90:786-855): for edge (a,b) of
   // a positively oriented tet with (a,b,c,d) an even permutation,
@@ -240,6 +248,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
     }
   }
 
+  progress("fine level: median-dual coefficients and volumes");
   // boundary node indicators in triangle order (gen3d_boundary.f90:1602-1627)
   L.node_ind.assign(nbn, 0);
   for (int f = 0; f < nb; ++f)
@@ -295,6 +304,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
     }
   }
 
+  progress("fine level: " + std::to_string(L.b1.size()) + " boundary sides");
   // wall distance: distance to the nearest node of a wall (indicator 1) face
   {
     std::vector<char> iswall(n, 0);
@@ -667,6 +677,8 @@ extern "C" {
 
 const char *fp_last_error(void) { return g_err.c_str(); }
 
+void fp_set_progress(void (*cb)(const char *msg)) { g_progress = cb; }
+
 fp_mesh *fp_mesh_build(int n_nodes, const double *xyz, int n_tets,
                        const int *tets, int n_btri, const int *btri,
                        const int *btri_ind, int n_levels) {
@@ -675,7 +687,11 @@ fp_mesh *fp_mesh_build(int n_nodes, const double *xyz, int n_tets,
       throw std::runtime_error("bad sizes");
     auto *m = new fp_mesh;
     build_fine(*m, n_nodes, xyz, n_tets, tets, n_btri, btri, btri_ind);
-    for (int l = 1; l < n_levels; ++l) m->lev.push_back(agglomerate(m->lev.back()));
+    progress("fine level: wall distance");
+    for (int l = 1; l < n_levels; ++l) {
+      m->lev.push_back(agglomerate(m->lev.back()));
+      progress("level " + std::to_string(l + 1) + ": " + std::to_string(m->lev.back().n_nodes) + " nodes");
+    }
     return m;
   } catch (const std::exception &e) {
     g_err = e.what();

@@ -184,9 +184,15 @@ def _ptr(a, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
-def build_levels(xyz, tets, btri, bind, n_levels):
-    """Run the preprocessor; returns (levels, old_index)."""
+_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_char_p)
+
+
+def build_levels(xyz, tets, btri, bind, n_levels, progress=None):
+    """Run the preprocessor; returns (levels, old_index).  progress(msg),
+    if given, is called after each phase of the build."""
     lib = _native.prepro()
+    cb = _PROGRESS(lambda m: progress(m.decode())) if progress else None
+    lib.fp_set_progress(ctypes.cast(cb, ctypes.c_void_p) if cb else None)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     tets = np.ascontiguousarray(tets, dtype=np.int32)
     btri = np.ascontiguousarray(btri, dtype=np.int32)
@@ -195,6 +201,7 @@ def build_levels(xyz, tets, btri, bind, n_levels):
                           tets.shape[0], _ptr(tets, ctypes.c_int),
                           btri.shape[0], _ptr(btri, ctypes.c_int),
                           _ptr(bind, ctypes.c_int), n_levels)
+    lib.fp_set_progress(None)
     if not m:
         raise RuntimeError("preprocessor failed: " +
                            lib.fp_last_error().decode())
@@ -273,10 +280,12 @@ def metric_vjp(xyz, tets, btri, bind, gw, gwb):
 
 
 def bump_levels(n, n_levels, jitter=0.0, seed=0, first_cell=None,
-                ratio=1.15, h=0.05, wall_indicator=1):
+                ratio=1.15, h=0.05, wall_indicator=1, progress=None):
     xyz, tets, btri, bind = bump_box(n, jitter=jitter, seed=seed,
                                      first_cell=first_cell, ratio=ratio,
                                      h=h, wall_indicator=wall_indicator)
-    levels, _ = build_levels(xyz, tets, btri, bind, n_levels)
+    if progress:
+        progress(f"bump box {n}^3: {xyz.shape[0]} nodes, {tets.shape[0]} tets")
+    levels, _ = build_levels(xyz, tets, btri, bind, n_levels, progress)
     return levels
 

@@ -47,6 +47,10 @@ fp_mesh *fp_mesh_build(int n_nodes, const double *xyz, int n_tets,
                        const int *btri_ind, int n_levels);
 void fp_mesh_free(fp_mesh *m);
 const char *fp_last_error(void);
+/* Progress callback: called with a short message after each phase of a
+ * build (fine-level sides, coefficients, boundary sides, wall distance,
+ * each coarse level); NULL (the default) reports nothing. */
+void fp_set_progress(void (*cb)(const char *msg));
 
 int fp_mesh_nlevels(const fp_mesh *m);
 
