@@ -690,10 +690,11 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     }
   }
   L.nslot = (long long)nslot;
-  // node blocks for the LDS-staged gathers, on levels gathered one lane per
-  // node: consecutive nodes, at most GB_NODES of them, whose neighbours are
-  // at most GB_CAP distinct nodes; each block's neighbour list is ascending
-  if (n > 0 && L.lanes == 1 && !c->gather_per_node && *std::max_element(deg.begin(), deg.end()) <= GB_CAP) {
+  // node blocks for the LDS-staged gathers, on levels large enough for one
+  // or four lanes per node (enough blocks to fill the chip): consecutive
+  // nodes, at most GB_NODES of them, whose neighbours are at most GB_CAP
+  // distinct nodes; each block's neighbour list is ascending
+  if (n > 0 && L.lanes <= 4 && !c->gather_per_node && *std::max_element(deg.begin(), deg.end()) <= GB_CAP) {
     std::vector<int> gb_first{0}, gb_nptr{0}, gb_node;
     std::vector<unsigned short> ell_loc(nslot, 0xffff);
     std::vector<int> stamp(n, -1), seen(n, -1), local(n, 0), cur;
