@@ -72,10 +72,12 @@ def params(fp32, re=6.0e6, mach=0.7, alpha=1.0):
     return iv.to_struct(coarse_fp32=fp32)
 
 
-def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None):
+def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autotune=False):
     from flite3d_amd import device
     s = device.Solver(prm(fp32) if prm else params(fp32), levels)
     s.init_freestream()
+    lanes = s.autotune(3) if autotune else [s.lanes(k + 1) for k in range(len(levels))]
+    log(f"[bench] lanes per node per level: {lanes}{' (autotuned)' if autotune else ''}")
     c = 1
     r0 = None
     for _ in range(warmup):
@@ -99,7 +101,7 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None):
     res = m["res"]
     s.close()
     return dict(time=t1 - t0, evals=evals, kernel_ms=kms, kernel_launches=kcnt,
-                res0=r0, res=res[0], forces=m["forces"][:4].tolist())
+                res0=r0, res=res[0], forces=m["forces"][:4].tolist(), lanes=lanes)
 
 
 def run_converge(levels, fp32, drop, cap_s, every=250, switch_drop=4.0):
@@ -376,6 +378,8 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the lanes per node of each level by timing (flite_autotune) before the run")
     ap.add_argument("--first-cell", type=float, default=1e-6,
                     help="first wall-normal spacing of the graded box (the benchmark case: 1e-6)")
     ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
@@ -417,7 +421,7 @@ def main():
     results = {}
     for tag, fp32 in ((("fp64", False),) if a.fp64_only else (("mixed", True), ("fp64", False))):
         log(f"[bench] GPU run {tag}: warmup {a.warmup}, steps {a.steps}")
-        r = run_gpu(levels, fp32, a.steps, a.warmup, prm=prm)
+        r = run_gpu(levels, fp32, a.steps, a.warmup, prm=prm, autotune=a.autotune)
         edges = sum(e * n for e, n in zip(sides, r["evals"]))
         r["gedges"] = edges * a.steps / r["time"] / 1e9
         r["ms_per_cycle"] = 1e3 * r["time"] / a.steps
@@ -445,6 +449,7 @@ def main():
         "config": {"workload": f"RANS-SA bump box {a.n}^3 hexes ({levels[0].n_nodes} nodes, "
                                f"{sides[0]} sides), first cell {a.first_cell:g}, {a.levels} MG levels, growing-V",
                    "evaluations_per_level": head["evals"],
+                   "lanes_per_level": head["lanes"], "autotuned": a.autotune,
                    "fp64_gedges": round(results["fp64"]["gedges"], 4),
                    "fp64_ms_per_step": round(results["fp64"]["ms_per_cycle"], 3),
                    "residual_drop_log10": round(float(np.log10(head["res"] / head["res0"])), 3),

@@ -267,7 +267,7 @@ int check_launch() {
 template <typename A, typename B, typename C, typename V>
 void launch_gather(flite_ctx *c, const Level &L, const V &v, const DevParams &P) {
   using R = std::remove_pointer_t<decltype(v.u)>;
-  if (L.ngb > 0 && (A::AUX || B::AUX || C::AUX)) {
+  if (L.ngb > 0 && L.lanes <= 4 && (A::AUX || B::AUX || C::AUX)) {
     hipLaunchKernelGGL((k_gather_blk<R, A, B, C>), dim3((unsigned)L.ngb * L.nbatch), dim3(NTHR), 0, c->stream, v, P);
     return;
   }
@@ -690,11 +690,11 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     }
   }
   L.nslot = (long long)nslot;
-  // node blocks for the LDS-staged gathers, on levels large enough for one
-  // or four lanes per node (enough blocks to fill the chip): consecutive
-  // nodes, at most GB_NODES of them, whose neighbours are at most GB_CAP
-  // distinct nodes; each block's neighbour list is ascending
-  if (n > 0 && L.lanes <= 4 && !c->gather_per_node && *std::max_element(deg.begin(), deg.end()) <= GB_CAP) {
+  // node blocks for the LDS-staged gathers (used while the level runs one or
+  // four lanes per node, when they give enough blocks to fill the chip):
+  // consecutive nodes, at most GB_NODES of them, whose neighbours are at
+  // most GB_CAP distinct nodes; each block's neighbour list is ascending
+  if (n > 0 && !c->gather_per_node && *std::max_element(deg.begin(), deg.end()) <= GB_CAP) {
     std::vector<int> gb_first{0}, gb_nptr{0}, gb_node;
     std::vector<unsigned short> ell_loc(nslot, 0xffff);
     std::vector<int> stamp(n, -1), seen(n, -1), local(n, 0), cur;
@@ -1936,6 +1936,99 @@ int flite_set_coarse_precision(flite_ctx *c, int fp32) {
   for (int l = 1; l < c->nl; ++l) std::swap(c->lev[l], c->lev_alt[l]);
   c->hp.coarse_fp32 = fp32;
   for (auto &q : c->hp_b) q.coarse_fp32 = fp32;
+  return 0;
+}
+
+// Per level, the lanes per node (1, 4 or 16) of the gather and stage
+// kernels that run fastest on this context's current state: each candidate
+// runs the level's lane-dependent kernels (pressure and records, time steps
+// and viscous/SA passes, gradient, residual gather) `repeats` times after one
+// untimed run, timed with HIP events on the context's stream.
+int flite_autotune(flite_ctx *c, int repeats, int *lanes_out) {
+  TRY(check_ctx(c));
+  if (repeats < 1 || !lanes_out) return fail(FLITE_E_ARG, "repeats must be >= 1 and lanes_out non-null");
+  TRY(sync(c));
+  TRY(harvest_timing(c));
+  std::string timed;
+  std::swap(timed, c->timed);  // autotuning launches are not counted by kernel timing
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = 0;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventCreate");
+  const bool visc = c->hp.reynolds_number > 0.0;
+  auto run = [&](Level &L) -> int {
+    TRY(op_prep(c, L, true));
+    if (visc) TRY(op_step_terms(c, L, true, c->P.sa));
+    else TRY(op_timesteps(c, L));
+    TRY(op_grad(c, L));
+    StageArgs S{};
+    S.calc_turb = 1;
+    return op_stage(c, L, MODE_RHS, S);
+  };
+  for (int l = 0; l < c->nl && rc == 0; ++l) {
+    Level &L = *c->lev[l];
+    int best = L.lanes;
+    float best_ms = 1e30f;
+    // the timed kernels overwrite the level's derived fields, some of which
+    // a cycle keeps from an earlier smoothing step: save and restore them
+    const size_t es = L.fp32 ? 4 : 8, nb = (size_t)L.n * L.nbatch;
+    const std::pair<void *, size_t> kept[] = {
+        {L.p, nb * es},        {L.rec0, 8 * nb * es},  {L.dt, nb * es},         {L.dtt, nb * es},
+        {L.mu, nb * es},       {L.vort, nb * es},      {L.divg, nb * es},       {L.tdiff, nb * es},
+        {L.diss, 5 * nb * es}, {L.tau, VREC * nb * es}, {L.ws, 3 * nb * es},     {L.rhs, 6 * nb * es},
+        {L.grec, (size_t)L.gstride * L.nbatch * es}};
+    size_t total = 0;
+    for (auto &k : kept) total += k.second;
+    char *save = nullptr;
+    if (hipMalloc(&save, std::max<size_t>(total, 1)) != hipSuccess) {
+      rc = fail(FLITE_E_HIP, "hipMalloc (autotune save)");
+      break;
+    }
+    size_t off = 0;
+    for (auto &k : kept) {
+      if (k.second && hipMemcpyAsync(save + off, k.first, k.second, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        rc = fail(FLITE_E_HIP, "hipMemcpyAsync (autotune save)");
+      off += k.second;
+    }
+    if (rc == 0) rc = op_prep(c, L, true);
+    if (rc == 0) rc = op_flux(c, L);
+    for (int lanes : {1, 4, 16}) {
+      if (rc != 0) break;
+      L.lanes = lanes;
+      rc = run(L);
+      if (rc == 0 && hipEventRecord(a, c->stream) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventRecord");
+      for (int r = 0; r < repeats && rc == 0; ++r) rc = run(L);
+      if (rc == 0 && hipEventRecord(b, c->stream) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventRecord");
+      if (rc == 0) rc = sync(c);
+      float ms = 0;
+      if (rc == 0 && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventElapsedTime");
+      if (rc == 0 && ms < best_ms) {
+        best_ms = ms;
+        best = lanes;
+      }
+    }
+    L.lanes = best;
+    lanes_out[l] = best;
+    off = 0;
+    for (auto &k : kept) {
+      if (k.second && hipMemcpyAsync(k.first, save + off, k.second, hipMemcpyDeviceToDevice, c->stream) != hipSuccess &&
+          rc == 0)
+        rc = fail(FLITE_E_HIP, "hipMemcpyAsync (autotune restore)");
+      off += k.second;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == 0) rc = fail(FLITE_E_HIP, "autotune restore");
+    (void)hipFree(save);
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  std::swap(timed, c->timed);
+  drop_cycle_graph(c);
+  return rc;
+}
+
+int flite_level_lanes(flite_ctx *c, int level, int *lanes) {
+  TRY(check_level(c, level));
+  if (!lanes) return fail(FLITE_E_ARG, "null output");
+  *lanes = c->lev[level - 1]->lanes;
   return 0;
 }
 

@@ -53,6 +53,8 @@ def lib():
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
         L.flite_set_coarse_precision.argtypes = [vp, i]
+        L.flite_autotune.argtypes = [vp, i, P(i)]
+        L.flite_level_lanes.argtypes = [vp, i, P(i)]
         L.flite_drag_metric_gradient.argtypes = [vp, P(d), P(d), i]
         _bound = L
     return _bound
@@ -144,6 +146,17 @@ class Solver:
     def set_coarse_precision(self, fp32):
         """Run the coarse levels in FP32 (True) or FP64 (False) from now on."""
         self._check(self.L.flite_set_coarse_precision(self.ctx, 1 if fp32 else 0))
+
+    def autotune(self, repeats=3):
+        """Pick each level's lanes per node by timing; returns them."""
+        out = (ctypes.c_int * len(self.levels))()
+        self._check(self.L.flite_autotune(self.ctx, repeats, out))
+        return list(out)
+
+    def lanes(self, level):
+        v = ctypes.c_int()
+        self._check(self.L.flite_level_lanes(self.ctx, level, ctypes.byref(v)))
+        return v.value
 
     def select(self, batch):
         """Condition that get/set/monitor act on (batched contexts)."""

@@ -256,6 +256,13 @@ module flite_gpu
        integer(c_int), value :: fp32
      end function flite_set_coarse_precision
 
+     integer(c_int) function flite_autotune(ctx, repeats, lanes_out) bind(C, name="flite_autotune")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: repeats
+       integer(c_int), intent(out) :: lanes_out(*)
+     end function flite_autotune
+
      integer(c_int) function flite_batch_select(ctx, batch) bind(C, name="flite_batch_select")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

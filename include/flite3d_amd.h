@@ -231,6 +231,19 @@ int flite_drag_metric_gradient(flite_ctx *ctx, double *drag, double *gwb, int ld
  * coarse states are rebuilt from the fine state at the next restriction. */
 int flite_set_coarse_precision(flite_ctx *ctx, int fp32);
 
+/* Autotuning of the launch shape (no counterpart in the reference): per
+ * level, the lanes per node (1, 4 or 16) of the gather and stage kernels
+ * that run fastest on the context's current state, each candidate timed
+ * with HIP events over `repeats` runs of the level's lane-dependent
+ * kernels.  The choice holds for the context's life and is written to
+ * lanes_out[0 .. n_levels-1]; without autotuning a level gets 1 lane from
+ * 2^20 nodes, 4 from 2^16, else 16 (FLITE_GATHER_LANES forces one value).
+ * The lanes change the order of the gather sums (results agree to
+ * rounding), and timing noise can change the choice from run to run.
+ * flite_level_lanes reads a level's current setting. */
+int flite_autotune(flite_ctx *ctx, int repeats, int *lanes_out);
+int flite_level_lanes(flite_ctx *ctx, int level, int *lanes);
+
 /* Condition that flite_field_get/set and flite_monitor act on (default 0);
  * no reference counterpart (the reference has one condition per run). */
 int flite_batch_select(flite_ctx *ctx, int batch);

@@ -215,3 +215,21 @@ def test_kernel_variants(env, monkeypatch):
     assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
     e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
     assert e.max() < 1e-9, e
+
+
+def test_autotune():
+    """flite_autotune picks 1, 4 or 16 lanes per level by timing; the cycles
+    that follow, on the chosen launch shapes, match the oracle."""
+    print("[case] autotune lanes, rans 8^3", flush=True)
+    lv = levels(8, 3, first_cell=0.005)
+    s, o = pair(rans(multigridscheme=3), lv)
+    s.init_freestream(); o.init_freestream()
+    lanes = s.autotune(2)
+    assert len(lanes) == 3 and all(x in (1, 4, 16) for x in lanes), lanes
+    assert [s.lanes(k + 1) for k in range(3)] == lanes
+    for c in (1, 2, 3):
+        s.cycle(c); o.cycle(c)
+    ms, mo = s.monitor(), o.monitor()
+    assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
+    e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
+    assert e.max() < 1e-9, e
