@@ -193,9 +193,97 @@ def bench_config1(cycles):
             "drag_cpu": float(mo["forces"][1]), "nodes": lv[0].n_nodes, "sides": E}
 
 
-def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6):
+def smooth_state(coords, u):
+    """The freestream state modulated by smooth functions of all three
+    coordinates (no two nodes share a value, no velocity component is 0)."""
+    x, y, z = coords
+    f = [np.sin(2.1 * x + 0.4 * z + 0.31) * np.cos(1.7 * y + 0.23),
+         np.cos(1.3 * x - 0.5 * z + 0.8 * y + 0.17),
+         np.sin(1.9 * y + z - 0.3 * x + 0.57),
+         np.cos(2.3 * z + 0.7 * x + 0.2 * y + 0.41),
+         np.sin(1.1 * x + 1.4 * y - 0.6 * z + 0.73)]
+    out = u.copy()
+    for c in range(5):
+        out[c] = u[c] * (1 + 0.05 * f[c]) + (0.02 * (1.5 + f[c]) if c in (2, 3) else 0.0)
+    return out
+
+
+def second_order_checks(lv, cycles, ndir, h, smooth=False):
+    """Second-order HLLC (MUSCL + LED) linearisation at the config-3
+    conditions, about that scheme's own state after `cycles` cycles (or a
+    smooth non-uniform state): the dot-product identity and the tangent
+    against finite differences.  The residual is piecewise smooth, so each
+    entry of J du is classed as equal to the central difference, or to a
+    one-sided one (a limiter switch within h of the state), to 1e-6 of
+    max |J du|.  Where neighbouring values agree to far less than the step
+    (the converged field away from the bump), the LED bounds
+    1/(q_min - q_i) vary on that smaller scale and no difference quotient
+    with this step resolves the derivative."""
+    from flite3d_amd import device
+    s = device.Solver(euler_params(0.8, 1.25, order=2, scheme=3), lv)
+    s.init_freestream()
+    drop = None
+    if smooth:
+        s.set(1, "U", smooth_state(lv[0].coords, s.get(1, "U")))
+        s.apply(1, "PRESSURE")
+    else:
+        r0 = None
+        for c in range(1, cycles + 1):
+            s.cycle(c)
+            m = s.monitor()
+            r0 = r0 or m["res"][0]
+        drop = round(float(np.log10(m["res"][0] / r0)), 3)
+    u = s.get(1, "U")
+    n = u.shape[1]
+
+    def resid(x):
+        s.set(1, "U", x)
+        s.apply(1, "PRESSURE")
+        s.apply(1, "RHS")
+        return s.get(1, "RHS")[:5]
+
+    rng = np.random.default_rng(11)
+    dot_err, central, one_sided, neither = [], 0, 0, 0
+    t_tan = t_adj = 0.0
+    for _ in range(ndir):
+        du = rng.standard_normal((5, n))
+        du /= np.linalg.norm(du)
+        lam = rng.standard_normal((5, n))
+        lam /= np.linalg.norm(lam)
+        s.set(1, "U", u)
+        t0 = time.perf_counter()
+        jt = s.tangent(du)
+        t_tan += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ja = s.adjoint(lam)
+        t_adj += time.perf_counter() - t0
+        a, b = float(np.sum(lam * jt)), float(np.sum(ja * du))
+        dot_err.append(abs(a - b) / max(abs(a), abs(b)))
+        rz = resid(u)
+        up, um = u.copy(), u.copy()
+        up[:5] += h * du
+        um[:5] -= h * du
+        rp, rm = resid(up), resid(um)
+        tol = 1e-6 * np.abs(jt).max()
+        c_ok = np.abs((rp - rm) / (2 * h) - jt) <= tol
+        s_ok = (np.abs((rp - rz) / h - jt) <= tol) | (np.abs((rz - rm) / h - jt) <= tol)
+        central += int(c_ok.sum())
+        one_sided += int((~c_ok & s_ok).sum())
+        neither += int((~c_ok & ~s_ok).sum())
+    s.close()
+    return {"order": 2, "state": "smooth" if smooth else f"{cycles} cycles",
+            "residual_drop_log10": drop, "directions": ndir,
+            "dot_product_max_rel_err": max(dot_err), "fd_step": h,
+            "entries_equal_to_central_fd": central, "entries_equal_to_one_sided_fd": one_sided,
+            "entries_equal_to_neither": neither,
+            "ms_per_tangent_apply": round(1e3 * t_tan / ndir, 3),
+            "ms_per_adjoint_apply": round(1e3 * t_adj / ndir, 3)}
+
+
+def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6, ndir2=4):
     """Discrete adjoint case: Euler M=0.8, alpha=1.25 on the 64^3 bump box,
-    first-order HLLC, after `cycles` growing-V cycles; 16 random unit
+    first-order HLLC (second order in second_order_checks), after `cycles`
+    growing-V cycles; 16 random unit
     directions (seed 1): tangent vs central differences of the residual,
     the tangent/adjoint dot-product identity, and the drag state gradient
     vs central differences of the drag (step hd: the drag depends on the
@@ -302,7 +390,9 @@ def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6):
             "ms_per_tangent_apply": round(1e3 * t_tan / ndir, 3),
             "ms_per_adjoint_apply": round(1e3 * t_adj / ndir, 3),
             "note": "apply times include host<->device transfer of the vectors",
-            "drag": drag0, "nodes": lv[0].n_nodes, "sides": lv[0].n_sides}
+            "drag": drag0, "nodes": lv[0].n_nodes, "sides": lv[0].n_sides,
+            "second_order": second_order_checks(lv, cycles, ndir2, h),
+            "second_order_smooth_state": second_order_checks(lv, cycles, ndir2, h, smooth=True)}
 
 
 def bench_config4(nbatch, steps, warmup):

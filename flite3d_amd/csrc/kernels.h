@@ -567,12 +567,13 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
 // ends' gather records A, B (rho, u, v, w, rhoE, p, nu~, mu) and the side
 // weights; writes o[0..5] (component 5: SA convection, DEF:6873-6880).
 // Second order also reads both ends' gradient records G1, G2 (grad 15 |
-// 1/lo 5 | 1/hi 5) and coordinates X1, X2; first order ignores them.
+// 1/lo 5 | 1/hi 5), of type TG (R, or dual numbers for the linearisation),
+// and coordinates X1, X2; first order ignores them.
 // T is the arithmetic type: the level's R for the residual, a dual number
-// for its tangent and adjoint (first order only: the MUSCL branch reads the
-// level's gradient records and is built for T = R).
-template <typename T, typename R, int ORDER, bool SA, int NSIG>
-__device__ __forceinline__ void side_flux(const DevParams &P, const R *G1, const R *G2, const R *X1, const R *X2,
+// for its tangent and adjoint; with T a dual number the second-order branch
+// takes the gradient records as dual numbers too (TG = T).
+template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R>
+__device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const R *X1, const R *X2,
                                           const T *A, const T *B, T wx, T wy, T wz, T *o) {
   T gam = T(P.gamma), gm1 = gam - T(1);
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
@@ -967,7 +968,7 @@ __global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du
   }
   state_record(Ua, P.gamma, A);
   state_record(Ub, P.gamma, B);
-  side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
+  side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
                                        Dual(L.w[3 * e + 1]), Dual(L.w[3 * e + 2]), o);
   double *f = L.fbuf + 6 * (size_t)e;
 #pragma unroll
@@ -1037,7 +1038,7 @@ __global__ void k_flux_adjoint(LevelDev<double> L, DevParams P, const double *la
     }
     state_record(Ua, P.gamma, A);
     state_record(Ub, P.gamma, B);
-    side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, wx, wy, wz, o);
+    side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, wx, wy, wz, o);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
@@ -1137,7 +1138,7 @@ __global__ void k_flux_metric_adjoint(LevelDev<double> L, DevParams P, const dou
 #pragma unroll 1
   for (int k = 0; k < 3; ++k) {
     Dual o[6];
-    side_flux<Dual, double, 1, false, 2>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
+    side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
                                          Dual(w1, k == 1 ? 1.0 : 0.0), Dual(w2, k == 2 ? 1.0 : 0.0), o);
     double acc = 0;
 #pragma unroll
@@ -1193,6 +1194,289 @@ __global__ void k_bflux_metric_adjoint(LevelDev<double> L, DevParams P, const do
     }
     o[k] = acc;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Tangent and adjoint of the second-order (MUSCL + LED) inviscid residual.
+// Side (a, b)'s flux depends on both ends' gather records q, their
+// Green-Gauss gradients G and reciprocal limiter bounds B (grad_minmax_body),
+// and G and B on the neighbours' q.  The tangent runs that chain forward on
+// dual numbers.  The adjoint seeds each side's flux on its 60 inputs (both
+// ends' conserved states, gradients and bounds), sums the shares per node,
+// and transposes the gradient (a linear node gather) and the bounds (a
+// bound 1/(q_k* - q_i) moves with the neighbour k* that attains the
+// extremum and with the node itself).
+// ---------------------------------------------------------------------------
+constexpr int LIN_G = 25;                   // gradient 15 | 1/lo 5 | 1/hi 5
+constexpr int LIN_SIDE = 10 + 2 * LIN_G;    // adjoint shares per side: U_a 5, U_b 5, (G,B)_a 25, (G,B)_b 25
+
+// tangent of the gather record (rho, u, v, w, rhoE, p) along du
+__global__ void k_record_tangent(LevelDev<double> L, DevParams P, const double *du, double *dq) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  size_t n = L.n;
+  Dual U[5], A[8];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) U[c] = Dual(L.u[c * n + i], du[c * n + i]);
+  state_record(U, P.gamma, A);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dq[6 * (size_t)i + c] = A[c].d;
+}
+
+// gradient and bounds of grad_minmax_body on dual numbers: gd[26 i + k].
+// The values are the level's own gradient records (grec), so that every
+// branch of the side flux is taken as in the residual and the adjoint; the
+// tangents follow the same walk in slot order, the bounds' with the extremal
+// neighbour that k_minmax_arg names.
+__global__ void k_grad_tangent(LevelDev<double> L, const double *dq, ad::Dual *gd) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  Dual q0[5], g[15], mn[5], mx[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    q0[c] = Dual(L.rec0[8 * (size_t)i + c], dq[6 * (size_t)i + c]);
+    mn[c] = Dual(1e15);
+    mx[c] = Dual(1e15);
+  }
+#pragma unroll
+  for (int c = 0; c < 15; ++c) g[c] = Dual(0.0);
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Dual qj(L.rec0[8 * (size_t)j + c], dq[6 * (size_t)j + c]);
+      Dual sq = q0[c] + qj;
+      g[c] += sq * Dual(wx);
+      g[5 + c] += sq * Dual(wy);
+      g[10 + c] += sq * Dual(wz);
+      mn[c] = rmin(mn[c], rmin(q0[c], qj));
+      mx[c] = rmin(mx[c], rmin(-q0[c], -qj));
+    }
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Dual qo(L.rec0[8 * (size_t)o + c], dq[6 * (size_t)o + c]);
+      Dual t = Dual(2.0) * q0[c] + (q0[c] + qo);
+      g[c] += t * Dual(wx);
+      g[5 + c] += t * Dual(wy);
+      g[10 + c] += t * Dual(wz);
+    }
+  }
+  Dual vt = Dual(1.0) / Dual(L.vol[i]);
+  const double *gv = L.grec + GREC * (size_t)i;
+  Dual *out = gd + 26 * (size_t)i;
+#pragma unroll
+  for (int c = 0; c < 15; ++c) out[c] = Dual(gv[c], (g[c] * vt).d);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Dual lo = mn[c] - q0[c];
+    Dual hi = -mx[c] - q0[c];
+    out[15 + c] = Dual(gv[15 + c], lo < Dual(0.0) ? (Dual(1.0) / lo).d : 0.0);
+    out[20 + c] = Dual(gv[20 + c], hi > Dual(0.0) ? (Dual(1.0) / hi).d : 0.0);
+  }
+  out[25] = Dual(0.0);
+}
+
+// second-order side fluxes along (du, dG, dB): fbuf[6 e + c] = d f_c
+template <int NSIG>
+__global__ void k_flux_tangent2(LevelDev<double> L, DevParams P, const double *dq, const ad::Dual *gd) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  int ia = L.ea[e], ib = L.eb[e];
+  Dual A[8], B[8], o[6];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    A[c] = Dual(L.rec0[8 * (size_t)ia + c], c < 6 ? dq[6 * (size_t)ia + c] : 0.0);
+    B[c] = Dual(L.rec0[8 * (size_t)ib + c], c < 6 ? dq[6 * (size_t)ib + c] : 0.0);
+  }
+  side_flux<Dual, double, 2, false, NSIG, Dual>(P, gd + 26 * (size_t)ia, gd + 26 * (size_t)ib, L.xyz + 3 * (size_t)ia,
+                                                L.xyz + 3 * (size_t)ib, A, B, Dual(L.w[3 * e]), Dual(L.w[3 * e + 1]),
+                                                Dual(L.w[3 * e + 2]), o);
+  double *f = L.fbuf + 6 * (size_t)e;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) f[c] = o[c].d;
+  f[5] = 0;
+}
+
+// node i's extremal neighbours: amin[10 i + c] = the node whose q_c sets
+// the lower bound of component c (-1 if none: the bound is 0), amin[10 i +
+// 5 + c] the upper one, with grad_minmax_body's comparisons in slot order
+__global__ void k_minmax_arg(LevelDev<double> L, int *amin) {
+  int i = gtid_x();
+  if (i >= L.n) return;
+  double q0[5], mn[5], mx[5];
+  int kn[5], kx[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    q0[c] = L.rec0[8 * (size_t)i + c];
+    mn[c] = 1e15;
+    mx[c] = 1e15;
+    kn[c] = kx[c] = -1;
+  }
+  for_edges_w<1>(L, i, 0, [&](int j, double, double, double) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      // rmin(a, b) = a < b ? a : b keeps b on ties, as grad_minmax_body does
+      double qj = L.rec0[8 * (size_t)j + c];
+      double lo = rmin(q0[c], qj);
+      int klo = q0[c] < qj ? i : j;
+      if (!(mn[c] < lo)) { mn[c] = lo; kn[c] = klo; }
+      double hi = rmin(-q0[c], -qj);
+      int khi = -q0[c] < -qj ? i : j;
+      if (!(mx[c] < hi)) { mx[c] = hi; kx[c] = khi; }
+    }
+  });
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    amin[10 * (size_t)i + c] = (mn[c] - q0[c] < 0.0 && kn[c] != i) ? kn[c] : -1;
+    amin[10 * (size_t)i + 5 + c] = (-mx[c] - q0[c] > 0.0 && kx[c] != i) ? kx[c] : -1;
+  }
+}
+
+// adjoint shares of side e: d = lam_a - lam_b against df/d(U_a, U_b,
+// G_a, B_a, G_b, B_b), one dual pass per input
+template <int NSIG>
+__global__ void k_flux_adjoint2(LevelDev<double> L, DevParams P, const double *lam, double *abuf2) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  double ua[5], ub[5], d[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    ua[c] = L.u[c * n + ia];
+    ub[c] = L.u[c * n + ib];
+    d[c] = lam[c * n + ia] - lam[c * n + ib];
+  }
+  const double *ga = L.grec + GREC * (size_t)ia, *gb = L.grec + GREC * (size_t)ib;
+  const double w0 = L.w[3 * e], w1 = L.w[3 * e + 1], w2 = L.w[3 * e + 2];
+  double *out = abuf2 + LIN_SIDE * (size_t)e;
+#pragma unroll 1
+  for (int k = 0; k < LIN_SIDE; ++k) {
+    Dual Ua[5], Ub[5], A[8], B[8], GA[LIN_G], GB[LIN_G], o[6];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Ua[c] = Dual(ua[c], k == c ? 1.0 : 0.0);
+      Ub[c] = Dual(ub[c], k == 5 + c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int c = 0; c < LIN_G; ++c) {
+      GA[c] = Dual(ga[c], k == 10 + c ? 1.0 : 0.0);
+      GB[c] = Dual(gb[c], k == 10 + LIN_G + c ? 1.0 : 0.0);
+    }
+    state_record(Ua, P.gamma, A);
+    state_record(Ub, P.gamma, B);
+    side_flux<Dual, double, 2, false, NSIG, Dual>(P, GA, GB, L.xyz + 3 * (size_t)ia, L.xyz + 3 * (size_t)ib, A, B,
+                                                  Dual(w0), Dual(w1), Dual(w2), o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
+    out[k] = acc;
+  }
+}
+
+// per node: ub = its conserved-state shares (sides and boundary sides),
+// gbn = its gradient and bound shares
+__global__ void k_adjoint_gather2(LevelDev<double> L, const double *abuf2, double *ub, double *gbn) {
+  int i = gtid_x();
+  if (i >= L.n) return;
+  double r[5] = {0, 0, 0, 0, 0}, g[LIN_G];
+#pragma unroll
+  for (int c = 0; c < LIN_G; ++c) g[c] = 0;
+  for_edges<1>(L, i, 0, [&](int e, int role) {
+    const double *a = abuf2 + LIN_SIDE * (size_t)e;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += a[5 * role + c];
+#pragma unroll
+    for (int c = 0; c < LIN_G; ++c) g[c] += a[10 + LIN_G * role + c];
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k];
+    const double *b = L.bbuf + 10 * (size_t)(v >> 1) + 5 * (v & 1);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += b[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) ub[5 * (size_t)i + c] = r[c];
+#pragma unroll
+  for (int c = 0; c < LIN_G; ++c) gbn[LIN_G * (size_t)i + c] = g[c];
+}
+
+// transpose of the gradient and bounds: qb[5 j + c] = sum over the G and B
+// that q_j(c) enters of their shares times d(G, B)/dq_j(c)
+__global__ void k_grad_adjoint(LevelDev<double> L, const double *gbn, const int *amin, double *qb) {
+  int j = gtid_x();
+  if (j >= L.n) return;
+  double r[5] = {0, 0, 0, 0, 0}, sw[3] = {0, 0, 0};
+  const double *gj = gbn + LIN_G * (size_t)j;
+  // neighbours i: G_i holds (q_i + q_j) w_s(i -> j) / V_i with w_s(i -> j) = -w_s(j -> i)
+  for_edges_w<1>(L, j, 0, [&](int i, double wx, double wy, double wz) {
+    sw[0] += wx;
+    sw[1] += wy;
+    sw[2] += wz;
+    const double *gi = gbn + LIN_G * (size_t)i;
+    const double *Bi = L.grec + GREC * (size_t)i;
+    double iv = 1.0 / L.vol[i];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      r[c] -= (gi[c] * wx + gi[5 + c] * wy + gi[10 + c] * wz) * iv;
+      if (amin[10 * (size_t)i + c] == j) r[c] -= gi[15 + c] * Bi[15 + c] * Bi[15 + c];
+      if (amin[10 * (size_t)i + 5 + c] == j) r[c] -= gi[20 + c] * Bi[20 + c] * Bi[20 + c];
+    }
+  });
+  // boundary sides: G_j holds (3 q_j + q_o) w_b / V_j and G_o (3 q_o + q_j) w_b / V_o
+  double sb[3] = {0, 0, 0};
+  for (int k = L.bs_ptr[j]; k < L.bs_ptr[j + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    sb[0] += wx;
+    sb[1] += wy;
+    sb[2] += wz;
+    const double *go = gbn + LIN_G * (size_t)o;
+    double iv = 1.0 / L.vol[o];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] += (go[c] * wx + go[5 + c] * wy + go[10 + c] * wz) * iv;
+  }
+  const double *Bj = L.grec + GREC * (size_t)j;
+  double iv = 1.0 / L.vol[j];
+  double s0 = sw[0] + 3.0 * sb[0], s1 = sw[1] + 3.0 * sb[1], s2 = sw[2] + 3.0 * sb[2];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    r[c] += (gj[c] * s0 + gj[5 + c] * s1 + gj[10 + c] * s2) * iv;
+    r[c] += gj[15 + c] * Bj[15 + c] * Bj[15 + c] + gj[20 + c] * Bj[20 + c] * Bj[20 + c];
+    qb[5 * (size_t)j + c] = r[c];
+  }
+}
+
+// out(i) = ub(i) + (dq/dU)^T qb(i): the record's dependence on the state
+__global__ void k_record_adjoint(LevelDev<double> L, DevParams P, const double *ub, const double *qb, double *out) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  size_t n = L.n;
+  double u[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) u[c] = L.u[c * n + i];
+#pragma unroll 1
+  for (int k = 0; k < 5; ++k) {
+    Dual U[5], A[8];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) U[c] = Dual(u[c], k == c ? 1.0 : 0.0);
+    state_record(U, P.gamma, A);
+    double acc = ub[5 * (size_t)i + k];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += qb[5 * (size_t)i + c] * A[c].d;
+    out[k * n + i] = acc;
+  }
+  out[5 * n + i] = 0;
 }
 
 // partial derivative of the pressure drag (calculateLiftAndDrag,

@@ -60,6 +60,11 @@ struct Pool {
     if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     e = hipMemset(p, 0, bytes);
     if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    // the null-stream memset is asynchronous to the host for device memory
+    // and unordered with the context's non-blocking stream: finish it before
+    // a kernel there writes the buffer (lazily allocated scratch)
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
     ptrs.push_back(p);
     *out = static_cast<T *>(p);
     return 0;
@@ -173,6 +178,10 @@ struct Level {
   // monitor scratch
   double *m_sum = nullptr, *m_max = nullptr, *m_force = nullptr;
   double *abuf = nullptr;  // adjoint side shares, 10 per side (allocated on first use)
+  // second-order linearisation (allocated on first use): per-side adjoint
+  // shares, per-node scratch, extremal neighbours of the limiter bounds
+  double *abuf2 = nullptr, *lin = nullptr;
+  int *amin = nullptr;
   std::vector<int> side_host;  // device side -> uploaded side index
   int *m_idx = nullptr;
 
@@ -1743,14 +1752,14 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
 
 // ---- tangent / adjoint of the level-1 residual ---------------------------
 namespace {
-int check_linearisable(flite_ctx *c) {
+int check_linearisable(flite_ctx *c, bool second_order_ok = true) {
   TRY(check_ctx(c));
   const Level &L = *c->lev[0];
   const flite_params &hp = c->hp;
   if (L.fp32) return fail(FLITE_E_UNSUPPORTED, "level 1 must be FP64");
   if (c->nbatch != 1) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: single-condition contexts only");
-  if (L.has_coords && hp.hllc_flux_order > 1)
-    return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: first-order HLLC (HLLCFluxOrder=1) only");
+  if (!second_order_ok && L.has_coords && hp.hllc_flux_order > 1)
+    return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
   if (hp.reynolds_number > 0.0 || hp.turbulence_model != 0)
     return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: inviscid flow (ReynoldsNumber=0) only");
   return 0;
@@ -1775,6 +1784,17 @@ int get_vector5(flite_ctx *c, double *y, int ld) {
     for (int i = 0; i < L.n; ++i) y[(size_t)k * ld + i] = h[(size_t)k * L.n + i];
   return 0;
 }
+
+bool second_order(const flite_ctx *c, const Level &L) { return L.has_coords && c->hp.hllc_flux_order > 1; }
+
+// second-order linearisation scratch: per node 58 doubles (tangent: dq 6 +
+// dual (G, B) 52; adjoint: state shares 5, (G, B) shares 25, record shares 5)
+// and 10 extremal-neighbour indices
+int alloc_lin(Level &L) {
+  if (!L.lin) TRY(L.pool.alloc(&L.lin, 58 * (size_t)std::max(1, L.n)));
+  if (!L.amin) TRY(L.pool.alloc(&L.amin, 10 * (size_t)std::max(1, L.n)));
+  return 0;
+}
 }  // namespace
 
 int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
@@ -1783,7 +1803,27 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
   TRY(put_vector5(c, du, ld_du));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
-  if (L.ne > 0) hipLaunchKernelGGL(k_flux_tangent, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  if (second_order(c, L)) {
+    // records, gradients and bounds of the current state, then dq,
+    // (dG, dB) and the side fluxes' tangents
+    TRY(op_prep(c, L, true));
+    TRY(op_grad(c, L));
+    TRY(alloc_lin(L));
+    double *dq = L.lin;
+    ad::Dual *gd = reinterpret_cast<ad::Dual *>(L.lin + 6 * (size_t)L.n);
+    hipLaunchKernelGGL(k_record_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, dq);
+    hipLaunchKernelGGL(k_grad_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, (const double *)dq, gd);
+    if (L.ne > 0) {
+      if (P.nsig == 4)
+        hipLaunchKernelGGL(k_flux_tangent2<4>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
+                           (const ad::Dual *)gd);
+      else
+        hipLaunchKernelGGL(k_flux_tangent2<2>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
+                           (const ad::Dual *)gd);
+    }
+  } else if (L.ne > 0) {
+    hipLaunchKernelGGL(k_flux_tangent, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  }
   L.flux_partials = false;  // the tangent fluxes are per-side records
   if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_tangent, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
   TRY(check_launch());
@@ -1795,6 +1835,33 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
 int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
   TRY(check_linearisable(c));
   Level &L = *c->lev[0];
+  if (second_order(c, L)) {
+    // records, gradients and bounds of the current state, the bounds'
+    // extremal neighbours, then the per-side shares and their transposes
+    TRY(op_prep(c, L, true));
+    TRY(op_grad(c, L));
+    TRY(alloc_lin(L));
+    if (!L.abuf2) TRY(L.pool.alloc(&L.abuf2, (size_t)LIN_SIDE * std::max(1, L.ne)));
+    TRY(put_vector5(c, lam, ld_lam));
+    DevParams P = c->P;
+    LevelDev<double> V = L.view<double>();
+    double *ub = L.lin, *gbn = L.lin + 5 * (size_t)L.n, *qb = L.lin + 30 * (size_t)L.n;
+    hipLaunchKernelGGL(k_minmax_arg, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, L.amin);
+    if (L.ne > 0) {
+      if (P.nsig == 4)
+        hipLaunchKernelGGL(k_flux_adjoint2<4>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf2);
+      else
+        hipLaunchKernelGGL(k_flux_adjoint2<2>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf2);
+    }
+    if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_adjoint, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
+    hipLaunchKernelGGL(k_adjoint_gather2, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, (const double *)L.abuf2, ub, gbn);
+    hipLaunchKernelGGL(k_grad_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, (const double *)gbn,
+                       (const int *)L.amin, qb);
+    hipLaunchKernelGGL(k_record_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)ub,
+                       (const double *)qb, V.rhs);
+    TRY(check_launch());
+    return get_vector5(c, out, ld_out);
+  }
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
   TRY(put_vector5(c, lam, ld_lam));
   DevParams P = c->P;
@@ -1832,7 +1899,7 @@ int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) 
 
 int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb,
                          int ld_wb) {
-  TRY(check_linearisable(c));
+  TRY(check_linearisable(c, false));
   Level &L = *c->lev[0];
   if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output arrays");
   TRY(put_vector5(c, lam, ld_lam));
@@ -1860,7 +1927,7 @@ int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw
 // A_i = sum 4|w_b| and normals n_i = m_i/|m_i|, m_i = -sum w_b, over the
 // boundary sides at the node (as build_level forms warea and nrm)
 int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_wb) {
-  TRY(check_linearisable(c));
+  TRY(check_linearisable(c, false));
   Level &L = *c->lev[0];
   const Staged &S = c->staged[0];
   if (!gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output array");

@@ -178,7 +178,7 @@ class Solver:
                     forces=np.array(forces[:]), rc=rc)
 
 
-    # ---- linearisation of the level-1 residual (first-order HLLC, inviscid)
+    # ---- linearisation of the level-1 residual (HLLC-1/2, inviscid)
     def _vec5(self, x):
         n = self.n(1)
         return np.ascontiguousarray(np.asarray(x, dtype=np.float64)[:5].reshape(5, n))

@@ -22,9 +22,9 @@ pytestmark = pytest.mark.gpu
 H = 1e-7
 
 
-def make(n, cycles):
+def make(n, cycles, order=1):
     lv = levels(n, 2, jitter=0.15)
-    iv = euler(1, machnumber=0.8, alpha=1.25, cflnumber=2.0)
+    iv = euler(order, machnumber=0.8, alpha=1.25, cflnumber=2.0)
     s = Solver(iv.to_struct(), list(lv))
     s.init_freestream()
     for c in range(1, cycles + 1):
@@ -120,6 +120,157 @@ def test_drag_gradient_matches_central_differences():
         assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-12), (fd, an)
     s.set(1, "U", u)
     s.close()
+
+
+# Second order (MUSCL reconstruction with LED clipping): the residual is
+# piecewise smooth, with kinks where a limiter switch flips (the minmod
+# choice, the clip to the side half-length, the neighbour that sets a
+# bound) or where a Roe eigenvalue |u.n| crosses zero.  A state a few
+# cycles from freestream is nearly uniform away from the bump, so equal
+# neighbour values put bound switches right at the state; the
+# linearisation is therefore checked about a smooth non-uniform state with
+# no equal neighbour values and no zero velocity component.
+def smooth_state(lv, u):
+    x, y, z = lv[0].coords
+    f = [np.sin(2.1 * x + 0.4 * z + 0.31) * np.cos(1.7 * y + 0.23),
+         np.cos(1.3 * x - 0.5 * z + 0.8 * y + 0.17),
+         np.sin(1.9 * y + z - 0.3 * x + 0.57),
+         np.cos(2.3 * z + 0.7 * x + 0.2 * y + 0.41),
+         np.sin(1.1 * x + 1.4 * y - 0.6 * z + 0.73)]
+    out = u.copy()
+    for c in range(5):
+        out[c] = u[c] * (1 + 0.05 * f[c]) + (0.02 * (1.5 + f[c]) if c in (2, 3) else 0.0)
+    return out
+
+
+def make_smooth(n, order):
+    lv = levels(n, 2, jitter=0.15)
+    s = Solver(euler(order, machnumber=0.8, alpha=1.25, cflnumber=2.0).to_struct(), list(lv))
+    s.init_freestream()
+    u = smooth_state(lv, s.get(1, "U"))
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    return s, lv, u
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_second_order_tangent_matches_finite_differences(n):
+    """Every entry of J du equals the central difference, or, where a
+    switch lies within H of the state (the residual is only piecewise
+    smooth), the one-sided difference taken on the state's side of it;
+    such entries are rare."""
+    print(f"[case] second-order tangent FD n={n}", flush=True)
+    s, lv, u = make_smooth(n, 2)
+    rng = np.random.default_rng(5)
+    one_sided = 0
+    for du in directions(rng, u, 4):
+        jt = s.tangent(du)
+        r0 = residual(s, u)
+        up, um = u.copy(), u.copy()
+        up[:5] += H * du
+        um[:5] -= H * du
+        rp, rm = residual(s, up), residual(s, um)
+        s.set(1, "U", u)
+        tol = 1e-6 * np.abs(jt).max()
+        central = np.abs((rp - rm) / (2 * H) - jt) <= tol
+        fwd = np.abs((rp - r0) / H - jt) <= tol
+        bwd = np.abs((r0 - rm) / H - jt) <= tol
+        assert np.all(central | fwd | bwd), np.argwhere(~(central | fwd | bwd))[:5]
+        one_sided += int((~central).sum())
+    assert one_sided <= 0.001 * jt.size * 4, one_sided
+    s.close()
+
+
+def test_second_order_tangent_differs_from_first_order():
+    """The reconstruction's derivatives contribute: on the same state the
+    second-order Jacobian is not the first-order one, and the LED clipping
+    changes it again."""
+    print("[case] second- vs first-order tangent", flush=True)
+    s2, lv, u = make_smooth(8, 2)
+    du = directions(np.random.default_rng(6), u, 1)[0]
+    j2 = s2.tangent(du)
+    s2.close()
+    for kw in (dict(hllcfluxorder=1), dict(automaticlipping=0)):
+        args = dict(machnumber=0.8, alpha=1.25, cflnumber=2.0)
+        args.update(kw)
+        s = Solver(euler(2, **args).to_struct(), list(lv))
+        s.set(1, "U", u)
+        s.init_from_state()
+        j = s.tangent(du)
+        s.close()
+        assert np.abs(j2 - j).max() > 1e-3 * np.abs(j2).max(), kw
+
+
+@pytest.mark.parametrize("n,smooth", [(8, True), (12, True), (12, False)])
+def test_second_order_adjoint_dot_product(n, smooth):
+    """<lam, J du> = <J^T lam, du>; the identity holds at any state, also
+    one with limiter switches at the state (a few cycles from freestream)."""
+    print(f"[case] second-order adjoint dot product n={n} smooth={smooth}", flush=True)
+    if smooth:
+        s, lv, u = make_smooth(n, 2)
+    else:
+        s, lv = make(n, 5, order=2)
+        u = s.get(1, "U")
+    rng = np.random.default_rng(7)
+    for du, lam in zip(directions(rng, u, 4), directions(rng, u, 4)):
+        a = np.sum(lam * s.tangent(du))
+        b = np.sum(s.adjoint(lam) * du)
+        assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)) + 1e-300, (a, b)
+    s.close()
+
+
+def test_second_order_adjoint_columns_are_jacobian_rows():
+    """J^T e_k equals row k of J, over the nodes of k's two-ring (the
+    second-order stencil: the neighbours' gradients and bounds)."""
+    print("[case] second-order adjoint vs explicit Jacobian rows", flush=True)
+    s, lv, u = make_smooth(8, 2)
+    n = u.shape[1]
+    ea, eb = lv[0].side_index
+    nbr = [set() for _ in range(n)]
+    for a, b in zip(ea, eb):
+        nbr[a].add(b)
+        nbr[b].add(a)
+    rng = np.random.default_rng(8)
+    for node in rng.choice(n, 2, replace=False):
+        ring = set(nbr[node])
+        for m in list(ring):
+            ring |= nbr[m]
+        cols = rng.choice(sorted(ring), 6, replace=False)
+        for comp in (0, 1, 4):
+            e = np.zeros((5, n))
+            e[comp, node] = 1.0
+            row = s.adjoint(e)
+            for j in cols:
+                for c in range(5):
+                    d = np.zeros((5, n))
+                    d[c, j] = 1.0
+                    t = s.tangent(d)[comp, node]
+                    assert abs(t - row[c, j]) <= 1e-12 * (1 + abs(row[c, j])), (node, comp, j, c, t, row[c, j])
+        # the row reaches the second ring
+        far = sorted(j for j in ring if j not in nbr[node] and j != node)
+        e = np.zeros((5, n))
+        e[1, node] = 1.0
+        assert np.abs(s.adjoint(e)[:, far]).max() > 0
+    s.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_first_apply_on_a_fresh_context_equals_the_second(order):
+    """The scratch of the tangent and adjoint is allocated (and zeroed) on
+    the first call: that call's result must equal a repeat's."""
+    print(f"[case] fresh-context tangent/adjoint order={order}", flush=True)
+    for k in range(2):
+        s, lv, u = make_smooth(12, order)
+        rng = np.random.default_rng(9 + k)
+        du, lam = directions(rng, u, 2)
+        if k == 0:
+            a1, t1 = s.adjoint(lam), s.tangent(du)
+            a2, t2 = s.adjoint(lam), s.tangent(du)
+        else:
+            t1, a1 = s.tangent(du), s.adjoint(lam)
+            t2, a2 = s.tangent(du), s.adjoint(lam)
+        assert np.array_equal(a1, a2) and np.array_equal(t1, t2)
+        s.close()
 
 
 def test_linearisation_rejects_unsupported_settings():
