@@ -41,6 +41,11 @@ struct DevParams {
   int isentropic, sa, order, autoclip, nsig, bterm;
   double clip, max_inc, min_rho, max_turb, turb_k;
   double beta0;  // Davis wave-speed factor sqrt((gamma - 1) / (2 gamma))
+  // interior flux (1 central, 2 LLF, 3 Roe, 4 HLLC) and the JST dissipation
+  // of the first three: second-order factor, fourth-order factor on grid 1
+  // and below it, matrix scaling and its entropy fix
+  int ftype, matdiss;
+  double k2, k4, k4c, hcf;
   double prolong_relax, turb_prolong_relax, mg_bc_relax;
   // batched conditions: per condition (Mach, inflowField(1:6,1), spare),
   // 8 doubles each; null for a single condition
@@ -104,6 +109,7 @@ struct LevelDev {
   const int *ncls;
   R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;  // tau, dnu: VREC-strided, one shared record
+  R *jst;  // JST node record (8 per node): L(U) 5, pressure switch, side count, pad
   // batched conditions: nb copies of every state array, condition-major;
   // gstride = gradient-record elements per condition (0 if none)
   int nb;
@@ -131,6 +137,7 @@ struct LevelDev {
     pbuf += (size_t)b * 6 * (size_t)nent;
     bbuf += (size_t)b * 10 * (size_t)nbs;
     ws += 3 * bn;
+    jst += 8 * bn;
   }
 };
 
@@ -572,6 +579,22 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
 // T is the arithmetic type: the level's R for the residual, a dual number
 // for its tangent and adjoint; with T a dual number the second-order branch
 // takes the gradient records as dual numbers too (TG = T).
+// upwinded SA convection of one side (makeSARHS, DEF:6873-6880) from the
+// ends' gather records
+template <typename T>
+__device__ __forceinline__ T sa_side_convection(const T *A, const T *B, T wx, T wy, T wz) {
+  T u1 = A[1], v1 = A[2], w1 = A[3], t1 = A[6];
+  T u2 = B[1], v2 = B[2], w2 = B[3], t2 = B[6];
+  T fsa = wx * (u1 * t1 + u2 * t2) + wy * (v1 * t1 + v2 * t2) + wz * (w1 * t1 + w2 * t2);
+  T dtt = t1 - t2;
+  if (fabs(dtt) > T(1e-20)) {
+    T id = frcp(dtt);
+    T ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
+    fsa = fsa + fabs(wx * ax + wy * ay + wz * az) * dtt;
+  }
+  return fsa;
+}
+
 template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R>
 __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const R *X1, const R *X2,
                                           const T *A, const T *B, T wx, T wy, T wz, T *o) {
@@ -579,16 +602,7 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
   T roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
   T fsa = 0;
-  if (SA) {
-    T t1 = A[6], t2 = B[6];
-    fsa = wx * (u1 * t1 + u2 * t2) + wy * (v1 * t1 + v2 * t2) + wz * (w1 * t1 + w2 * t2);
-    T dtt = t1 - t2;
-    if (fabs(dtt) > T(1e-20)) {
-      T id = frcp(dtt);
-      T ax = (u2 * t2 - u1 * t1) * id, ay = (v2 * t2 - v1 * t1) * id, az = (w2 * t2 - w1 * t1) * id;
-      fsa = fsa + fabs(wx * ax + wy * ay + wz * az) * dtt;
-    }
-  }
+  if (SA) fsa = sa_side_convection(A, B, wx, wy, wz);
   T ru1 = roe1 * u1, rv1 = roe1 * v1, rw1 = roe1 * w1, rh1 = re1 + pr1;
   T ru2 = roe2 * u2, rv2 = roe2 * v2, rw2 = roe2 * w2, rh2 = re2 + pr2;
   // central part: only kept when no HLLC branch is taken (NaN wave speeds)
@@ -913,6 +927,231 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
       o[5 + c] = rnorm * R(4) * f2[c];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Interior fluxes 1-3 and their JST dissipation
+// ---------------------------------------------------------------------------
+// Interior side flux of fluxType 1 (central, DEF:1664-1669), 2 (local
+// Lax-Friedrichs, DEF:1670-1724) or 3 (Roe, DEF:1725-1792) from the ends'
+// conserved states and pressures, without reconstruction.  LLF and Roe are
+// the textbook fluxes |w| (F_1 + F_2 + D) along w/|w|: the reference reads
+// their densities (and Roe's enthalpies) before assigning them and subtracts
+// the upwind term D, which reverses its sign; both are repaired here.
+// fbuf[6 e + c] as k_flux (component 5: SA convection from the records).
+template <typename R, int FT, bool SA>
+__global__ void k_flux_simple(LevelDev<R> L, DevParams P) {
+  int bb_;
+  int e = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (e >= L.ne) return;
+  const size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
+  R r1 = L.u[ia], ru1 = L.u[n + ia], rv1 = L.u[2 * n + ia], rw1 = L.u[3 * n + ia], E1 = L.u[4 * n + ia];
+  R r2 = L.u[ib], ru2 = L.u[n + ib], rv2 = L.u[2 * n + ib], rw2 = L.u[3 * n + ib], E2 = L.u[4 * n + ib];
+  R pr1 = L.p[ia], pr2 = L.p[ib];
+  R rinv1 = R(1) / r1, rinv2 = R(1) / r2;
+  R u1 = rinv1 * ru1, v1 = rinv1 * rv1, w1 = rinv1 * rw1, rh1 = E1 + pr1;
+  R u2 = rinv2 * ru2, v2 = rinv2 * rv2, w2 = rinv2 * rw2, rh2 = E2 + pr2;
+  R f[5];
+  if constexpr (FT == 1) {
+    f[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
+    f[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
+    f[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
+    f[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
+    f[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[c] = 0;
+    R rnorm = sqrt(wx * wx + wy * wy + wz * wz);
+    if (rnorm > R(1.0e-10)) {
+      R rnx = wx / rnorm, rny = wy / rnorm, rnz = wz / rnorm;
+      R nv1 = u1 * rnx + v1 * rny + w1 * rnz, nv2 = u2 * rnx + v2 * rny + w2 * rnz;
+      R F1[5] = {r1 * nv1, pr1 * rnx + ru1 * nv1, pr1 * rny + rv1 * nv1, pr1 * rnz + rw1 * nv1, rh1 * nv1};
+      R F2[5] = {r2 * nv2, pr2 * rnx + ru2 * nv2, pr2 * rny + rv2 * nv2, pr2 * rnz + rw2 * nv2, rh2 * nv2};
+      R du[5] = {r1 - r2, ru1 - ru2, rv1 - rv2, rw1 - rw2, E1 - E2};
+      R D[5];
+      if constexpr (FT == 2) {
+        R c1 = sqrt(R(P.gamma) * pr1 * rinv1), c2 = sqrt(R(P.gamma) * pr2 * rinv2);
+        R l1 = fabs(nv1 + c1), l12 = fabs(nv2 + c2);
+        if (l1 < l12) l1 = l12;
+        R l2 = fabs(nv1 - c1), l22 = fabs(nv2 - c2);
+        if (l2 < l22) l2 = l22;
+        R lam = l1;
+        if (lam < l2) lam = l2;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) D[c] = lam * du[c];
+      } else {
+        R gam1 = R(P.gamma - 1.0);
+        R h1 = rh1 * rinv1, h2 = rh2 * rinv2;
+        R di = sqrt(r1 / r2), d1 = R(1) / (di + R(1));
+        R ui = (di * u1 + u2) * d1, vi = (di * v1 + v2) * d1, wi = (di * w1 + w2) * d1;
+        R hi = (di * h1 + h2) * d1;
+        R ci2 = gam1 * (hi - R(0.5) * (ui * ui + vi * vi + wi * wi));
+        ci2 = ci2 > R(1.0e-5) ? ci2 : R(1.0e-5);
+        R ci = sqrt(ci2), af = R(0.5) * (ui * ui + vi * vi + wi * wi), ucp = ui * rnx + vi * rny + wi * rnz;
+        R l1 = fabs(ucp + ci), l2 = fabs(ucp - ci), l3 = fabs(ucp);
+        R s1 = R(0.5) * (l1 + l2), s2 = R(0.5) * (l1 - l2);
+        R a1 = gam1 * (af * du[0] - ui * du[1] - vi * du[2] - wi * du[3] + du[4]);
+        R a2 = -ucp * du[0] + du[1] * rnx + du[2] * rny + du[3] * rnz;
+        R cc1 = ((s1 - l3) * a1 / ci2) + (s2 * a2 / ci);
+        R cc2 = (s2 * a1 / ci) + (s1 - l3) * a2;
+        D[0] = l3 * du[0] + cc1;
+        D[1] = l3 * du[1] + cc1 * ui + cc2 * rnx;
+        D[2] = l3 * du[2] + cc1 * vi + cc2 * rny;
+        D[3] = l3 * du[3] + cc1 * wi + cc2 * rnz;
+        D[4] = l3 * du[4] + cc1 * hi + cc2 * ucp;
+      }
+#pragma unroll
+      for (int c = 0; c < 5; ++c) f[c] = rnorm * ((F1[c] + F2[c]) + D[c]);
+    }
+  }
+  R *o = L.fbuf + 6 * (size_t)e;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) o[c] = f[c];
+  if (SA) {
+    R A[8], B[8];
+    ldrec<8>(L.rec0 + 8 * (size_t)ia, A);
+    ldrec<8>(L.rec0 + 8 * (size_t)ib, B);
+    o[5] = sa_side_convection(A, B, wx, wy, wz);
+  } else {
+    o[5] = 0;
+  }
+}
+
+// JST passes 1, 2 and the node step of makeArtificialDissipation2
+// (DEF:11531-11563, 11659-11672, 11746-11751), gathered per node over its
+// sides in their stored orientation (d = U_a - U_b as the side loop forms
+// it, taken with - at a and + at b):
+//   L_i = (1/conn_i) sum (U_j - U_i), energy on E + p;
+//   nu_i = 12 |sum (p_j - p_i) / sum (p_i + p_j)|;  conn_i = sides at i.
+template <typename R>
+__global__ void k_jst_sums(LevelDev<R> L, DevParams P) {
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  R h[5] = {0, 0, 0, 0, 0}, h7 = 0, h8 = 0, conn = 0;
+  for_edges<1>(L, i, 0, [&](int e, int role) {
+    int a = L.ea[e], b = L.eb[e];
+    R d[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) d[c] = L.u[c * n + a] - L.u[c * n + b];
+    d[4] = d[4] + L.p[a] - L.p[b];
+    R dp = L.p[a] - L.p[b], sp = L.p[a] + L.p[b];
+    if (role == 0) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) h[c] = h[c] - d[c];
+      h7 = h7 - dp;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) h[c] = h[c] + d[c];
+      h7 = h7 + dp;
+    }
+    h8 = h8 + sp;
+    conn += R(1);
+  });
+  R fact = R(1) / conn;
+  R *o = L.jst + 8 * (size_t)i;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) o[c] = fact * h[c];
+  o[5] = R(12) * fabs(h7 / h8);
+  o[6] = conn;
+  o[7] = 0;
+}
+
+// JST pass 3 (DEF:11788-11939): per side (a, b) with dr4 = L_a - L_b and
+// dr2 = U_a - U_b,
+//   scalar: d0 = min(V_a/dt_a, V_b/dt_b) / (conn_a + conn_b),
+//           d2 = d0 k2 max(nu_a, nu_b), d4 = dim(d0 k4, d2);
+//   matrix: dr4, dr2 <- |w| |A| dr4, |w| |A| dr2 (Roe matrix along w in
+//           Turkel's form, Harten's fix), d2 = k2 max(nu), d4 = dim(k4, d2);
+// D_a += -dr4 d4 + dr2 d2 and D_b += dr4 d4 - dr2 d2, gathered per node and
+// added to the dissipation array (k4: coarse-grid factor below grid 1).
+template <typename R, bool MATRIX>
+__global__ void k_jst_flux(LevelDev<R> L, DevParams P) {
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  const R k2 = R(P.k2), k4 = R(L.gnum == 1 ? P.k4 : P.k4c);
+  R acc[5] = {0, 0, 0, 0, 0};
+  for_edges<1>(L, i, 0, [&](int e, int role) {
+    int a = L.ea[e], b = L.eb[e];
+    const R *ja = L.jst + 8 * (size_t)a, *jb = L.jst + 8 * (size_t)b;
+    R d4v[5], d2v[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      d4v[c] = ja[c] - jb[c];
+      d2v[c] = L.u[c * n + a] - L.u[c * n + b];
+    }
+    R nu = ja[5] > jb[5] ? ja[5] : jb[5];
+    R d2, d4;
+    if constexpr (!MATRIX) {
+      R va = L.vol[a] / L.dt[a], vb = L.vol[b] / L.dt[b];
+      R d0 = (va < vb ? va : vb) / (ja[6] + jb[6]);
+      d2 = d0 * k2 * nu;
+      d4 = d0 * k4;
+    } else {
+      const R hcf = R(P.hcf), gam1 = R(P.gamma - 1.0);
+      const R eps1 = R(1) / (hcf > R(1.e-5) ? hcf : R(1.e-5));
+      R rnx = L.w[3 * e], rny = L.w[3 * e + 1], rnz = L.w[3 * e + 2];
+      R al = sqrt(rnx * rnx + rny * rny + rnz * rnz), vmo1 = R(1) / al;
+      rnx *= vmo1;
+      rny *= vmo1;
+      rnz *= vmo1;
+      R rhol = L.u[a], ra = R(1) / rhol;
+      R uxl = L.u[n + a] * ra, vyl = L.u[2 * n + a] * ra, wzl = L.u[3 * n + a] * ra;
+      R hl = (L.u[4 * n + a] + L.p[a]) * ra;
+      R rhor = L.u[b], rb = R(1) / rhor;
+      R uxr = L.u[n + b] * rb, vyr = L.u[2 * n + b] * rb, wzr = L.u[3 * n + b] * rb;
+      R hr = (L.u[4 * n + b] + L.p[b]) * rb;
+      R di = sqrt(rhor / rhol), d1 = R(1) / (di + R(1));
+      R ui = (di * uxr + uxl) * d1, vi = (di * vyr + vyl) * d1, wi = (di * wzr + wzl) * d1;
+      R hi = (di * hr + hl) * d1;
+      R ci2 = gam1 * (hi - R(0.5) * (ui * ui + vi * vi + wi * wi)), ci = sqrt(ci2);
+      R af = R(0.5) * (ui * ui + vi * vi + wi * wi), ucp = ui * rnx + vi * rny + wi * rnz;
+      R l1 = fabs(ucp + ci), l2 = fabs(ucp - ci), l3 = fabs(ucp);
+      if (l1 < hcf) l1 = R(0.5) * (l1 * l1 * eps1 + hcf);
+      if (l2 < hcf) l2 = R(0.5) * (l2 * l2 * eps1 + hcf);
+      if (l3 < hcf) l3 = R(0.5) * (l3 * l3 * eps1 + hcf);
+      R s1 = R(0.5) * (l1 + l2), s2 = R(0.5) * (l1 - l2);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        R *q = k == 0 ? d4v : d2v;
+        R a1 = gam1 * (af * q[0] - ui * q[1] - vi * q[2] - wi * q[3] + q[4]);
+        R a2 = -ucp * q[0] + q[1] * rnx + q[2] * rny + q[3] * rnz;
+        R cc1 = ((s1 - l3) * a1 / ci2) + (s2 * a2 / ci);
+        R cc2 = (s2 * a1 / ci) + (s1 - l3) * a2;
+        R t0 = al * (l3 * q[0] + cc1), t1 = al * (l3 * q[1] + cc1 * ui + cc2 * rnx);
+        R t2 = al * (l3 * q[2] + cc1 * vi + cc2 * rny), t3 = al * (l3 * q[3] + cc1 * wi + cc2 * rnz);
+        R t4 = al * (l3 * q[4] + cc1 * hi + cc2 * ucp);
+        q[0] = t0;
+        q[1] = t1;
+        q[2] = t2;
+        q[3] = t3;
+        q[4] = t4;
+      }
+      d2 = k2 * nu;
+      d4 = k4;
+    }
+    d4 = d4 > d2 ? d4 - d2 : R(0);
+    if (role == 0) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[c] = acc[c] - d4v[c] * d4 + d2v[c] * d2;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[c] = acc[c] + d4v[c] * d4 - d2v[c] * d2;
+    }
+  });
+#pragma unroll
+  for (int c = 0; c < 5; ++c) L.diss[c * n + i] += acc[c];
 }
 
 // ---------------------------------------------------------------------------

@@ -155,6 +155,7 @@ struct Level {
   void *w, *wb, *vol, *xyz, *wd, *nrm, *warea;
   void *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
+  void *jst = nullptr;  // JST node records (fluxType 1-3 only)
   int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
   int *ell_e, *ell_nbr;
   void *ellw;
@@ -202,6 +203,7 @@ struct Level {
     v.p = (R *)p; v.dt = (R *)dt; v.dtt = (R *)dtt; v.mu = (R *)mu; v.vort = (R *)vort;
     v.divg = (R *)divg; v.tdiff = (R *)tdiff; v.rec0 = (R *)rec0; v.grec = (R *)grec;
     v.tau = (R *)tau; v.dnu = (R *)dnu; v.fbuf = (R *)fbuf; v.bbuf = (R *)bbuf; v.ws = (R *)ws;
+    v.jst = (R *)jst;
     v.nb = nbatch;
     v.gstride = gstride;
     return v;
@@ -364,6 +366,12 @@ DevParams make_dev_params(const flite_params &p) {
   P.prolong_relax = p.prolongation_relaxation;
   P.turb_prolong_relax = p.turb_prolongation_relaxation;
   P.mg_bc_relax = p.multigrid_bc_relaxation;
+  P.ftype = p.flux_type;
+  P.matdiss = p.use_matrix_dissipation;
+  P.k2 = p.second_order_dissipation_factor;
+  P.k4 = p.fourth_order_dissipation_factor;
+  P.k4c = p.coarse_grid_dissipation_factor;
+  P.hcf = p.hartens_correction_factor;
   return P;
 }
 
@@ -832,6 +840,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(al_real(&L.pbuf, (size_t)(6 * (size_t)L.nent) * NB));
   TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
   TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
+  TRY(al_real(&L.jst, c->hp.flux_type != 4 ? (size_t)(8 * (size_t)n) * NB : 1));
   int nbm = nblk(n);
   TRY(P.alloc(&L.m_sum, 7 * (size_t)nbm));
   TRY(P.alloc(&L.m_max, 7 * (size_t)nbm));
@@ -918,7 +927,7 @@ int op_prep(flite_ctx *c, Level &L, bool compute_p) {
 }
 
 int op_grad(flite_ctx *c, Level &L) {
-  if (!(L.has_coords && c->hp.hllc_flux_order > 1)) return 0;
+  if (!(L.has_coords && c->hp.hllc_flux_order > 1 && c->hp.flux_type == 4)) return 0;
   DevParams P = c->P;
   Timed tm(c, "grad", L.gnum);
   return with_level(L, [&](auto V) {
@@ -937,7 +946,23 @@ int op_flux(flite_ctx *c, Level &L) {
   if (L.gnum <= (int)c->evals.size()) c->evals[L.gnum - 1] += 1;
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    if (L.ne > 0) {
+    if (L.ne > 0 && P.ftype != 4) {
+      // central / LLF / Roe: one flux per side into fbuf
+      Timed tm(c, "flux", L.gnum);
+      dim3 g(nblk(L.ne) * L.nbatch), b(NTHR);
+      L.flux_partials = false;
+#define LAUNCH_SIMPLE(FT)                                                                        \
+  if (P.sa) hipLaunchKernelGGL((k_flux_simple<R, FT, true>), g, b, 0, c->stream, V, P);          \
+  else hipLaunchKernelGGL((k_flux_simple<R, FT, false>), g, b, 0, c->stream, V, P);
+      if (P.ftype == 1) {
+        LAUNCH_SIMPLE(1);
+      } else if (P.ftype == 2) {
+        LAUNCH_SIMPLE(2);
+      } else {
+        LAUNCH_SIMPLE(3);
+      }
+#undef LAUNCH_SIMPLE
+    } else if (L.ne > 0) {
       Timed tm(c, "flux", L.gnum);
       dim3 g(nblk(L.ne) * L.nbatch), b(NTHR);
       bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
@@ -1019,11 +1044,33 @@ int op_zero_diss(flite_ctx *c, Level &L) {
   });
 }
 
-int op_dissipation(flite_ctx *c, Level &L) {
-  if (!(c->hp.flux_type == 4 && L.gnum == 1) &&
-      !(c->hp.second_order_dissipation_factor == 0.0 && c->hp.fourth_order_dissipation_factor == 0.0))
-    return fail(FLITE_E_UNSUPPORTED, "JST artificial dissipation is not implemented");
-  return op_zero_diss(c, L);
+// makeArtificialDissipation2 (DEF:11484-11993) in two steps: op_dissipation
+// zeroes the array (as the routine does first), op_jst adds the JST terms
+// once the local time steps exist.  The reference computes the time steps
+// before the dissipation and subtracts the viscous term after it
+// (DEF:1023-1047).  Here the viscous gather writes dissipation = -viscous
+// (ViscFluxPart), so op_jst always runs after it and adds its node totals:
+// -viscous + JST is the reference's JST - viscous, each a single addition.
+int op_dissipation(flite_ctx *c, Level &L) { return op_zero_diss(c, L); }
+
+bool jst_active(const flite_ctx *c, const Level &L) {
+  const flite_params &hp = c->hp;
+  if (hp.flux_type == 4 && L.gnum == 1) return false;  // DEF:11506
+  return !(hp.fourth_order_dissipation_factor == 0.0 && hp.second_order_dissipation_factor == 0.0);  // DEF:11508
+}
+
+int op_jst(flite_ctx *c, Level &L) {
+  if (!jst_active(c, L) || L.ne == 0) return 0;
+  DevParams P = c->P;
+  Timed tm(c, "dissipation", L.gnum);
+  return with_level(L, [&](auto V) {
+    using R = std::remove_pointer_t<decltype(V.u)>;
+    dim3 g(nblk(L.n) * L.nbatch), b(NTHR);
+    hipLaunchKernelGGL(k_jst_sums<R>, g, b, 0, c->stream, V, P);
+    if (P.matdiss) hipLaunchKernelGGL((k_jst_flux<R, true>), g, b, 0, c->stream, V, P);
+    else hipLaunchKernelGGL((k_jst_flux<R, false>), g, b, 0, c->stream, V, P);
+    return check_launch();
+  });
 }
 
 int op_viscous(flite_ctx *c, Level &L) {
@@ -1138,6 +1185,7 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
     } else {
       TRY(op_timesteps(c, L));
     }
+    TRY(op_jst(c, L));
   }
   for (int k = 1; k <= 3; ++k) {
     // stage k > 1 starts from the pressure and records the previous stage
@@ -1185,6 +1233,7 @@ int do_restrict(flite_ctx *c, int fine_level) {
   } else {
     TRY(op_timesteps(c, K));
   }
+  TRY(op_jst(c, K));
   TRY(op_flux(c, K));
   StageArgs S;
   S.alpha = 1.0;
@@ -1198,6 +1247,7 @@ int do_restrict(flite_ctx *c, int fine_level) {
   TRY(op_prep(c, F, true));
   TRY(op_dissipation(c, F));
   if (visc) TRY(op_step_terms(c, F, false, false));
+  TRY(op_jst(c, F));  // with the time steps of the last smoothing step
   TRY(op_grad(c, F));
   TRY(op_flux(c, F));
   S.has_src = F.gnum > 1;
@@ -1360,13 +1410,20 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
                 q.turb_prolongation_relaxation == p->turb_prolongation_relaxation &&
                 q.multigrid_bc_relaxation == p->multigrid_bc_relaxation && q.coarse_fp32 == p->coarse_fp32 &&
                 q.second_order_dissipation_factor == p->second_order_dissipation_factor &&
-                q.fourth_order_dissipation_factor == p->fourth_order_dissipation_factor;
+                q.fourth_order_dissipation_factor == p->fourth_order_dissipation_factor &&
+                q.coarse_grid_dissipation_factor == p->coarse_grid_dissipation_factor &&
+                q.dissipation_scheme == p->dissipation_scheme &&
+                q.coarse_grid_dissipation_scheme == p->coarse_grid_dissipation_scheme &&
+                q.use_matrix_dissipation == p->use_matrix_dissipation &&
+                q.hartens_correction_factor == p->hartens_correction_factor;
     for (int k = 0; k < 3; ++k) same = same && q.rk_coefficients[k] == p->rk_coefficients[k];
     for (int k = 0; k < 6; ++k) same = same && q.inflow_field[1][k] == p->inflow_field[1][k];
     if (!same)
       return fail(FLITE_E_ARG, "batched conditions may differ only in Mach number, alpha, beta and the inflow state");
   }
-  if (p->flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "only fluxType=4 (HLLC) is implemented");
+  if (p->flux_type < 1 || p->flux_type > 4) return fail(FLITE_E_UNSUPPORTED, "fluxType must be 1-4");
+  if (p->flux_type != 4 && (p->dissipation_scheme != 2 || p->coarse_grid_dissipation_scheme != 2))
+    return fail(FLITE_E_UNSUPPORTED, "only dissipation scheme 2 (JST) is implemented");
   if (p->turbulence_model != 0 && p->turbulence_model != 1)
     return fail(FLITE_E_UNSUPPORTED, "only turbulenceModel 0/1 are implemented");
   if (p->hllc_flux_order > 1 && p->gradient_method != 1)
@@ -1656,7 +1713,7 @@ int flite_apply(flite_ctx *c, int level, int op) {
     case FLITE_OP_PRESSURE:
     case FLITE_OP_PRESSURE_ABS: TRY(op_prep(c, L, true)); break;
     case FLITE_OP_TIMESTEPS: TRY(op_prep(c, L, false)); TRY(op_timesteps(c, L)); break;
-    case FLITE_OP_DISSIPATION: TRY(op_dissipation(c, L)); break;
+    case FLITE_OP_DISSIPATION: TRY(op_dissipation(c, L)); TRY(op_jst(c, L)); break;
     case FLITE_OP_VISCOUS:
       if (!(c->hp.reynolds_number > 0.0)) return fail(FLITE_E_ARG, "viscous term needs ReynoldsNumber > 0");
       TRY(op_prep(c, L, false));
@@ -1758,6 +1815,7 @@ int check_linearisable(flite_ctx *c, bool second_order_ok = true) {
   const flite_params &hp = c->hp;
   if (L.fp32) return fail(FLITE_E_UNSUPPORTED, "level 1 must be FP64");
   if (c->nbatch != 1) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: single-condition contexts only");
+  if (hp.flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: HLLC (fluxType=4) only");
   if (!second_order_ok && L.has_coords && hp.hllc_flux_order > 1)
     return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
   if (hp.reynolds_number > 0.0 || hp.turbulence_model != 0)
@@ -1785,7 +1843,9 @@ int get_vector5(flite_ctx *c, double *y, int ld) {
   return 0;
 }
 
-bool second_order(const flite_ctx *c, const Level &L) { return L.has_coords && c->hp.hllc_flux_order > 1; }
+bool second_order(const flite_ctx *c, const Level &L) {
+  return L.has_coords && c->hp.hllc_flux_order > 1 && c->hp.flux_type == 4;
+}
 
 // second-order linearisation scratch: per node 58 doubles (tangent: dq 6 +
 // dual (G, B) 52; adjoint: state shares 5, (G, B) shares 25, record shares 5)

@@ -70,6 +70,11 @@ module flite_gpu
      real(c_double) :: groundplane
      real(c_double) :: datum(3)
      integer(c_int) :: coarse_fp32
+     real(c_double) :: coarse_grid_dissipation_factor
+     integer(c_int) :: dissipation_scheme
+     integer(c_int) :: coarse_grid_dissipation_scheme
+     integer(c_int) :: use_matrix_dissipation
+     real(c_double) :: hartens_correction_factor
   end type flite_params
 
   ! flite_level_view: non-owning pointers to one level's arrays as the

@@ -278,7 +278,11 @@ contains
     if (ivd%flowType /= 0) error stop "ERROR: only steady flow (flowType=0) runs on the GPU path"
     if (ivd%turbulenceModel /= 0 .and. ivd%turbulenceModel /= 1) &
          error stop "ERROR: only turbulenceModel 0 and 1 (Spalart-Allmaras) run on the GPU path"
-    if (ivd%fluxType /= 4) error stop "ERROR: only fluxType=4 (HLLC) runs on the GPU path"
+    if (ivd%fluxType < 1 .or. ivd%fluxType > 4) error stop "ERROR: only fluxType 1-4 run on the GPU path"
+    if (ivd%fluxType /= 4 .and. (ivd%dissipationScheme /= 2 .or. ivd%coarseGridDissipationScheme /= 2)) &
+         error stop "ERROR: only dissipationScheme 2 (JST) runs on the GPU path"
+    if (ivd%fluxType /= 4 .and. ivd%useDissipationWeighting) &
+         error stop "ERROR: useDissipationWeighting is not on the GPU path"
     if (ivd%HLLCFluxOrder > 1 .and. ivd%GradientMethod /= 1) &
          error stop "ERROR: only Green-Gauss gradients (GradientMethod=1) run on the GPU path"
     if (ivd%movingwall) error stop "ERROR: moving walls are not on the GPU path"
@@ -330,6 +334,11 @@ contains
     p%groundplane = ivd%groundplane
     p%datum = ivd%datum
     p%coarse_fp32 = merge(1, 0, coarse_fp32)
+    p%coarse_grid_dissipation_factor = ivd%coarseGridDissipationFactor
+    p%dissipation_scheme = ivd%dissipationScheme
+    p%coarse_grid_dissipation_scheme = ivd%coarseGridDissipationScheme
+    p%use_matrix_dissipation = merge(1, 0, ivd%useMatrixDissipation)
+    p%hartens_correction_factor = ivd%HartensCorrectionFactor
   end function to_flite_params
 
 end module flite_input

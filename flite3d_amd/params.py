@@ -40,6 +40,9 @@ DEFAULTS = {
     "secondorderdissipationfactor": 0.4,
     "fourthorderdissipationfactor": 0.2,
     "coarsegriddissipationfactor": 0.75,
+    "hartenscorrectionfactor": 0.0,
+    "usematrixdissipation": False,
+    "usedissipationweighting": False,
     "residualsmoothingfactor": 0.0,
     "multigridscheme": 3,
     "numberofrelaxationsteps": 1,
@@ -171,8 +174,12 @@ class InputVariables:
             bad.append("flowType != 0 (unsteady)")
         if v["turbulencemodel"] not in (0, 1):
             bad.append("turbulenceModel other than 0/1")
-        if v["fluxtype"] != 4:
-            bad.append("fluxType other than 4 (HLLC)")
+        if v["fluxtype"] not in (1, 2, 3, 4):
+            bad.append("fluxType other than 1-4")
+        if v["fluxtype"] != 4 and (v["dissipationscheme"] != 2 or v["coarsegriddissipationscheme"] != 2):
+            bad.append("dissipationScheme / coarseGridDissipationScheme other than 2 (JST)")
+        if v["fluxtype"] != 4 and v["usedissipationweighting"]:
+            bad.append("useDissipationWeighting")
         if v["hllcfluxorder"] == 2 and v["gradientmethod"] != 1:
             bad.append("least-squares gradients")
         if v["viscosityscheme"] != 1:
@@ -232,6 +239,11 @@ class InputVariables:
             d = [d.get(f"({i})", 0.0) for i in (1, 2, 3)]
         p.datum[:] = list(d)
         p.coarse_fp32 = int(bool(coarse_fp32))
+        p.coarse_grid_dissipation_factor = v["coarsegriddissipationfactor"]
+        p.dissipation_scheme = v["dissipationscheme"]
+        p.coarse_grid_dissipation_scheme = v["coarsegriddissipationscheme"]
+        p.use_matrix_dissipation = int(bool(v["usematrixdissipation"]))
+        p.hartens_correction_factor = v["hartenscorrectionfactor"]
         return p
 
 
@@ -275,6 +287,11 @@ class FliteParams(ctypes.Structure):
         ("groundplane", ctypes.c_double),
         ("datum", ctypes.c_double * 3),
         ("coarse_fp32", ctypes.c_int),
+        ("coarse_grid_dissipation_factor", ctypes.c_double),
+        ("dissipation_scheme", ctypes.c_int),
+        ("coarse_grid_dissipation_scheme", ctypes.c_int),
+        ("use_matrix_dissipation", ctypes.c_int),
+        ("hartens_correction_factor", ctypes.c_double),
     ]
 
 

@@ -47,7 +47,7 @@ typedef struct flite_params {
   double inflow_temperature;
   int walls_are_isentropic;
   int turbulence_model;            /* 0 none, 1 Spalart-Allmaras */
-  int flux_type;                   /* 4 = HLLC */
+  int flux_type;                   /* 1 central, 2 LLF, 3 Roe, 4 HLLC (DEF:1664-2100) */
   int hllc_flux_order;             /* 1 or 2 */
   int automatic_clipping;          /* 1 = LED local clipping */
   int gradient_method;             /* 1 = Green-Gauss (Galerkin) */
@@ -73,6 +73,13 @@ typedef struct flite_params {
   double groundplane;
   double datum[3];
   int coarse_fp32;                 /* 1: levels >= 2 smooth and transfer in FP32 */
+  /* artificial dissipation of the non-HLLC fluxes (makeArtificialDissipation2,
+   * mgnsg3d_def_par.F90:11484-11993; defaults DEF:263-267, 316) */
+  double coarse_grid_dissipation_factor;  /* fourth-order factor on levels >= 2 */
+  int dissipation_scheme;                 /* 2 = JST (scheme 2 only) */
+  int coarse_grid_dissipation_scheme;     /* 2 = JST (scheme 2 only) */
+  int use_matrix_dissipation;             /* 1: Roe-matrix scaling instead of the scalar d0 */
+  double hartens_correction_factor;       /* entropy fix of the matrix scaling */
 } flite_params;
 
 /* Non-owning view of one level as read from <case>.sol_1

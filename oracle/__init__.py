@@ -44,6 +44,7 @@ def lib():
         L.or_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
         L.or_hllc_flux.argtypes = [P(FliteParams), P(d), d, P(d), d, P(d), P(d)]
         L.or_farfield_flux.argtypes = [P(FliteParams), i, P(d), d, P(d), P(d)]
+        L.or_simple_flux.argtypes = [P(FliteParams), i, P(d), d, P(d), d, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -146,6 +147,18 @@ def hllc_flux(params, ul, pl, ur, pr, w):
     b = (ctypes.c_double * 5)(*ur)
     ww = (ctypes.c_double * 3)(*w)
     lib().or_hllc_flux(ctypes.byref(params), a, pl, b, pr, ww, f)
+    return np.array(f[:])
+
+
+def simple_flux(params, flux_type, ul, pl, ur, pr, w):
+    """Interior flux of fluxType 1 (central), 2 (LLF) or 3 (Roe)."""
+    f = (ctypes.c_double * 5)()
+    a = (ctypes.c_double * 5)(*ul)
+    b = (ctypes.c_double * 5)(*ur)
+    ww = (ctypes.c_double * 3)(*w)
+    rc = lib().or_simple_flux(ctypes.byref(params), flux_type, a, pl, b, pr, ww, f)
+    if rc != 0:
+        raise OracleError(lib().or_last_error().decode())
     return np.array(f[:])
 
 

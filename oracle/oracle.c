@@ -22,9 +22,13 @@
  *   - MG: restriction conserves sum V u, injection prolongation, a
  *     converged fine solution gives a zero coarse correction;
  *   - RK: the 3-stage amplification polynomial on a linear problem.
+ *   - central / LLF / Roe interior fluxes: consistency F(U,U) = F(U).n,
+ *     the Roe flux against the textbook Roe solver, upwinding of a contact;
+ *   - JST dissipation: zero for uniform states, conservative (sums to zero
+ *     over the nodes), the scalar and matrix forms on hand-computed sides.
  * Branches not reached by those checks (engine faces, inviscid-wall
- * indicator 9, JST dissipation, k-omega/SST) are not restated: the calls
- * that would need them return FLITE_E_UNSUPPORTED.
+ * indicator 9, dissipation schemes 1 and 3, k-omega/SST) are not restated:
+ * the calls that would need them return FLITE_E_UNSUPPORTED.
  *
  * Known reference defects and how they are mirrored (SURVEY Appendix B
  * decisions, restated here):
@@ -472,14 +476,128 @@ static void timesteps(or_ctx *C, orl *L) {
     dt[i] = 0.5 / (dt[i] / L->vol[i] + H(C, 0, i) / (L->vol[i] * L->vol[i]));
 }
 
-/* makeArtificialDissipation2 (DEF:11484-11511): zeroes dissipation and the
- * help array; returns early for HLLC or when both factors are zero. */
+/* makeArtificialDissipation2 (DEF:11484-11993): zeroes dissipation and the
+ * help array; returns early for HLLC on grid 1 or when both factors are
+ * zero (DEF:11506-11511).  Otherwise the JST blend of second and fourth
+ * differences:
+ *   pass 1 (DEF:11531-11563): L_i = sum_j (U_j - U_i) with the energy
+ *     difference taken on E + p (help 1-5; help 6 sums the weights, unused);
+ *   pass 2 (DEF:11659-11672): sum_j (p_j - p_i) and sum_j (p_i + p_j)
+ *     (help 7, 8);
+ *   node (DEF:11746-11751): switch nu_i = 12 |help7 / help8| (help 9),
+ *     L_i /= nodeConnectivity_i (the number of sides at the node, DEF:14283);
+ *   pass 3 (DEF:11788-11939): per side, with dr4 = L_1 - L_2, dr2 = U_1 - U_2,
+ *     scalar: d0 = min(V_1/dt_1, V_2/dt_2) / (conn_1 + conn_2),
+ *             d2 = d0 k2 max(nu_1, nu_2), d4 = dim(d0 k4, d2);
+ *     matrix: dr4 and dr2 replaced by |w| |A| dr4 and |w| |A| dr2 with the
+ *             Roe matrix along w (Turkel's form, Harten's entropy fix) and
+ *             d2 = k2 max(nu_1, nu_2), d4 = dim(k4, d2);
+ *     D_1 += -dr4 d4 + dr2 d2, D_2 += dr4 d4 - dr2 d2.
+ * k4 is the coarse-grid factor below grid 1.  Side-length weighting
+ * (useDissipationWeighting) is not restated. */
 static int dissipation2(or_ctx *C, orl *L) {
-  memset(L->diss, 0, sizeof(double) * 5 * (size_t)L->n);
+  const flite_params *P = &C->P;
+  int n = L->n;
+  memset(L->diss, 0, sizeof(double) * 5 * (size_t)n);
   memset(C->help, 0, sizeof(double) * 25 * (size_t)C->nh);
-  if (C->P.flux_type == 4 && L->gnum == 1) return 0;
-  if (C->P.fourth_order_dissipation_factor == 0.0 && C->P.second_order_dissipation_factor == 0.0) return 0;
-  return fail(FLITE_E_UNSUPPORTED, "JST dissipation is not restated");
+  if (P->flux_type == 4 && L->gnum == 1) return 0;
+  if (P->fourth_order_dissipation_factor == 0.0 && P->second_order_dissipation_factor == 0.0) return 0;
+  int scheme = L->gnum == 1 ? P->dissipation_scheme : P->coarse_grid_dissipation_scheme;
+  if (scheme != 2) return fail(FLITE_E_UNSUPPORTED, "only dissipation scheme 2 (JST) is restated");
+  double *conn = zalloc(n);
+  for (int e = 0; e < L->ne; ++e) {
+    conn[L->e1[e]] += 1.0;
+    conn[L->e2[e]] += 1.0;
+  }
+  for (int e = 0; e < L->ne; ++e) {
+    int i1 = L->e1[e], i2 = L->e2[e];
+    double d[5];
+    for (int c = 0; c < 5; ++c) d[c] = U(L, c, i1) - U(L, c, i2);
+    d[4] = d[4] + L->p[i1] - L->p[i2];
+    double df = 1.0;
+    for (int c = 0; c < 5; ++c) {
+      H(C, c, i1) = H(C, c, i1) - d[c] * df;
+      H(C, c, i2) = H(C, c, i2) + d[c] * df;
+    }
+    H(C, 5, i1) += df;
+    H(C, 5, i2) += df;
+  }
+  for (int e = 0; e < L->ne; ++e) {
+    int i1 = L->e1[e], i2 = L->e2[e];
+    double dp = L->p[i1] - L->p[i2], sp = L->p[i1] + L->p[i2];
+    H(C, 6, i1) = H(C, 6, i1) - dp;
+    H(C, 7, i1) = H(C, 7, i1) + sp;
+    H(C, 6, i2) = H(C, 6, i2) + dp;
+    H(C, 7, i2) = H(C, 7, i2) + sp;
+  }
+  for (int i = 0; i < n; ++i) {
+    double fact = 1. / conn[i];
+    H(C, 8, i) = 12. * fabs(H(C, 6, i) / H(C, 7, i));
+    for (int c = 0; c < 5; ++c) H(C, c, i) = fact * H(C, c, i);
+  }
+  double k2 = P->second_order_dissipation_factor;
+  double k4 = L->gnum == 1 ? P->fourth_order_dissipation_factor : P->coarse_grid_dissipation_factor;
+  for (int e = 0; e < L->ne; ++e) {
+    int i1 = L->e1[e], i2 = L->e2[e];
+    double d4v[5], d2v[5], d2, d4;
+    for (int c = 0; c < 5; ++c) {
+      d4v[c] = H(C, c, i1) - H(C, c, i2);
+      d2v[c] = U(L, c, i1) - U(L, c, i2);
+    }
+    double nu = H(C, 8, i1) > H(C, 8, i2) ? H(C, 8, i1) : H(C, 8, i2);
+    if (!P->use_matrix_dissipation) {
+      double a = L->vol[i1] / L->dt[i1], b = L->vol[i2] / L->dt[i2];
+      double d0 = (a < b ? a : b) / (conn[i1] + conn[i2]);
+      d2 = d0 * k2 * nu;
+      d4 = d0 * k4;
+      d4 = d4 > d2 ? d4 - d2 : 0.0;
+    } else {
+      double hcf = P->hartens_correction_factor;
+      double eps1 = 1. / (hcf > 1.e-5 ? hcf : 1.e-5), gam1 = P->gamma - 1.;
+      double rnx = L->w[e], rny = L->w[L->ne + e], rnz = L->w[2 * (size_t)L->ne + e];
+      double al = sqrt(rnx * rnx + rny * rny + rnz * rnz), vmo1 = 1. / al;
+      rnx *= vmo1;
+      rny *= vmo1;
+      rnz *= vmo1;
+      double rhol = U(L, 0, i1), r1 = 1. / rhol;
+      double uxl = U(L, 1, i1) * r1, vyl = U(L, 2, i1) * r1, wzl = U(L, 3, i1) * r1;
+      double hl = (U(L, 4, i1) + L->p[i1]) * r1;
+      double rhor = U(L, 0, i2), r2 = 1. / rhor;
+      double uxr = U(L, 1, i2) * r2, vyr = U(L, 2, i2) * r2, wzr = U(L, 3, i2) * r2;
+      double hr = (U(L, 4, i2) + L->p[i2]) * r2;
+      double di = sqrt(rhor / rhol), d1 = 1.0 / (di + 1.0);
+      double ui = (di * uxr + uxl) * d1, vi = (di * vyr + vyl) * d1, wi = (di * wzr + wzl) * d1;
+      double hi = (di * hr + hl) * d1;
+      double ci2 = gam1 * (hi - 0.5 * (ui * ui + vi * vi + wi * wi)), ci = sqrt(ci2);
+      double af = 0.5 * (ui * ui + vi * vi + wi * wi), ucp = ui * rnx + vi * rny + wi * rnz;
+      double l1 = fabs(ucp + ci), l2 = fabs(ucp - ci), l3 = fabs(ucp);
+      if (l1 < hcf) l1 = 0.5 * (l1 * l1 * eps1 + hcf);
+      if (l2 < hcf) l2 = 0.5 * (l2 * l2 * eps1 + hcf);
+      if (l3 < hcf) l3 = 0.5 * (l3 * l3 * eps1 + hcf);
+      double s1 = 0.5 * (l1 + l2), s2 = 0.5 * (l1 - l2);
+      double *dv[2] = {d4v, d2v};
+      for (int k = 0; k < 2; ++k) {
+        double *q = dv[k];
+        double a1 = gam1 * (af * q[0] - ui * q[1] - vi * q[2] - wi * q[3] + q[4]);
+        double a2 = -ucp * q[0] + q[1] * rnx + q[2] * rny + q[3] * rnz;
+        double cc1 = ((s1 - l3) * a1 / ci2) + (s2 * a2 / ci);
+        double cc2 = (s2 * a1 / ci) + (s1 - l3) * a2;
+        double t0 = al * (l3 * q[0] + cc1), t1 = al * (l3 * q[1] + cc1 * ui + cc2 * rnx);
+        double t2 = al * (l3 * q[2] + cc1 * vi + cc2 * rny), t3 = al * (l3 * q[3] + cc1 * wi + cc2 * rnz);
+        double t4 = al * (l3 * q[4] + cc1 * hi + cc2 * ucp);
+        q[0] = t0; q[1] = t1; q[2] = t2; q[3] = t3; q[4] = t4;
+      }
+      d2 = k2 * nu;
+      d4 = k4;
+      d4 = d4 > d2 ? d4 - d2 : 0.0;
+    }
+    for (int c = 0; c < 5; ++c) {
+      L->diss[(size_t)c * n + i1] = L->diss[(size_t)c * n + i1] - d4v[c] * d4 + d2v[c] * d2;
+      L->diss[(size_t)c * n + i2] = L->diss[(size_t)c * n + i2] + d4v[c] * d4 - d2v[c] * d2;
+    }
+  }
+  free(conn);
+  return 0;
 }
 
 /* makeViscosityTerm, scheme 1 (DEF:5198-5893); result subtracted from
@@ -915,11 +1033,86 @@ int or_farfield_flux(const flite_params *p, int indicator, const double s[5], do
 
 /* makeRHS (DEF:1525-2590), HLLC interior flux + boundary loop; rhs must be
  * zeroed by the caller (the reference does grp%rhs = 0 before the call). */
+/* Interior side flux of fluxType 1-3 (makeRHS, DEF:1664-1792), no
+ * reconstruction (the JST dissipation supplies the higher-order damping):
+ *   1 central (DEF:1664-1669): w . (F(U_1) + F(U_2));
+ *   2 local Lax-Friedrichs (DEF:1670-1724) and 3 Roe (DEF:1725-1792), both
+ *     |w| (F_1 + F_2 + D) along w/|w|.  The reference reads rho1/rho2 before
+ *     any assignment in these branches (DEF:1683,1694; 1744,1752), reads the
+ *     Roe enthalpies h1/h2 unassigned (DEF:1763), and subtracts the upwind
+ *     term D (lambda (U_1 - U_2), or the Roe matrix times U_1 - U_2), which
+ *     makes it anti-dissipative.  Restated patched: rho = u(:,1),
+ *     h = (E + p)/rho and D added, which is the textbook flux.  Roe's
+ *     entropy fix is inactive (epslm = 0, DEF:1580).  Sides with
+ *     |w| <= 1e-10 keep no flux (the reference keeps the previous side's). */
+static void simple_side(const flite_params *P, int ft, const double *s1, double pr1, const double *s2, double pr2,
+                        double wx, double wy, double wz, double *f) {
+  double r1 = s1[0], rinv1 = 1. / s1[0], ru1 = s1[1], rv1 = s1[2], rw1 = s1[3], E1 = s1[4];
+  double u1 = rinv1 * ru1, v1 = rinv1 * rv1, w1 = rinv1 * rw1, rh1 = E1 + pr1;
+  double r2 = s2[0], rinv2 = 1. / s2[0], ru2 = s2[1], rv2 = s2[2], rw2 = s2[3], E2 = s2[4];
+  double u2 = rinv2 * ru2, v2 = rinv2 * rv2, w2 = rinv2 * rw2, rh2 = E2 + pr2;
+  if (ft == 1) {
+    f[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
+    f[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
+    f[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
+    f[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
+    f[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
+    return;
+  }
+  for (int c = 0; c < 5; ++c) f[c] = 0.0;
+  double rnorm = sqrt(wx * wx + wy * wy + wz * wz);
+  if (!(rnorm > 1.0e-10)) return;
+  double rnx = wx / rnorm, rny = wy / rnorm, rnz = wz / rnorm;
+  double nv1 = u1 * rnx + v1 * rny + w1 * rnz, nv2 = u2 * rnx + v2 * rny + w2 * rnz;
+  double F1[5] = {r1 * nv1, pr1 * rnx + ru1 * nv1, pr1 * rny + rv1 * nv1, pr1 * rnz + rw1 * nv1, rh1 * nv1};
+  double F2[5] = {r2 * nv2, pr2 * rnx + ru2 * nv2, pr2 * rny + rv2 * nv2, pr2 * rnz + rw2 * nv2, rh2 * nv2};
+  double du[5] = {r1 - r2, ru1 - ru2, rv1 - rv2, rw1 - rw2, E1 - E2};
+  double D[5];
+  if (ft == 2) {
+    double c1 = sqrt(P->gamma * pr1 * rinv1), c2 = sqrt(P->gamma * pr2 * rinv2);
+    double l1 = fabs(nv1 + c1), l12 = fabs(nv2 + c2);
+    if (l1 < l12) l1 = l12;
+    double l2 = fabs(nv1 - c1), l22 = fabs(nv2 - c2);
+    if (l2 < l22) l2 = l22;
+    double lam = l1;
+    if (lam < l2) lam = l2;
+    for (int c = 0; c < 5; ++c) D[c] = lam * du[c];
+  } else {
+    double gam1 = P->gamma - 1.0;
+    double h1 = rh1 * rinv1, h2 = rh2 * rinv2;
+    double di = sqrt(r1 / r2), d1 = 1.0 / (di + 1.0);
+    double ui = (di * u1 + u2) * d1, vi = (di * v1 + v2) * d1, wi = (di * w1 + w2) * d1;
+    double hi = (di * h1 + h2) * d1;
+    double ci2 = gam1 * (hi - 0.5 * (ui * ui + vi * vi + wi * wi));
+    ci2 = ci2 > 1.0e-5 ? ci2 : 1.0e-5;
+    double ci = sqrt(ci2), af = 0.5 * (ui * ui + vi * vi + wi * wi), ucp = ui * rnx + vi * rny + wi * rnz;
+    double l1 = fabs(ucp + ci), l2 = fabs(ucp - ci), l3 = fabs(ucp);
+    double s1 = 0.5 * (l1 + l2), s2 = 0.5 * (l1 - l2);
+    double a1 = gam1 * (af * du[0] - ui * du[1] - vi * du[2] - wi * du[3] + du[4]);
+    double a2 = -ucp * du[0] + du[1] * rnx + du[2] * rny + du[3] * rnz;
+    double cc1 = ((s1 - l3) * a1 / ci2) + (s2 * a2 / ci);
+    double cc2 = (s2 * a1 / ci) + (s1 - l3) * a2;
+    D[0] = l3 * du[0] + cc1;
+    D[1] = l3 * du[1] + cc1 * ui + cc2 * rnx;
+    D[2] = l3 * du[2] + cc1 * vi + cc2 * rny;
+    D[3] = l3 * du[3] + cc1 * wi + cc2 * rnz;
+    D[4] = l3 * du[4] + cc1 * hi + cc2 * ucp;
+  }
+  for (int c = 0; c < 5; ++c) f[c] = rnorm * ((F1[c] + F2[c]) + D[c]);
+}
+
+int or_simple_flux(const flite_params *p, int flux_type, const double ul[5], double pl,
+                   const double ur[5], double pr, const double w[3], double f[5]) {
+  if (!p || flux_type < 1 || flux_type > 3) return fail(FLITE_E_ARG, "flux_type must be 1-3");
+  simple_side(p, flux_type, ul, pl, ur, pr, w[0], w[1], w[2], f);
+  return 0;
+}
+
 static int make_rhs(or_ctx *C, orl *L) {
   const flite_params *P = &C->P;
   int n = L->n;
-  if (P->flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "only fluxType=4 (HLLC) is restated");
-  int order = P->hllc_flux_order, autoclip = P->automatic_clipping;
+  if (P->flux_type < 1 || P->flux_type > 4) return fail(FLITE_E_UNSUPPORTED, "fluxType must be 1-4");
+  int order = P->flux_type == 4 ? P->hllc_flux_order : 1, autoclip = P->automatic_clipping;
   int nsig = P->reynolds_number < 1 ? 4 : 2;
   if (order > 1) {
     if (P->gradient_method != 1) return fail(FLITE_E_UNSUPPORTED, "least-squares gradients are not restated");
@@ -936,8 +1129,12 @@ static int make_rhs(or_ctx *C, orl *L) {
     if (order > 1)
       for (int c = 0; c < 25; ++c) { gr1[c] = H(C, c, i1); gr2[c] = H(C, c, i2); }
     double f[5];
-    hllc_side(P, order, autoclip, nsig, s1, L->p[i1], s2, L->p[i2], L->w[e], L->w[L->ne + e],
-              L->w[2 * (size_t)L->ne + e], dx, gr1, gr2, f);
+    if (P->flux_type == 4)
+      hllc_side(P, order, autoclip, nsig, s1, L->p[i1], s2, L->p[i2], L->w[e], L->w[L->ne + e],
+                L->w[2 * (size_t)L->ne + e], dx, gr1, gr2, f);
+    else
+      simple_side(P, P->flux_type, s1, L->p[i1], s2, L->p[i2], L->w[e], L->w[L->ne + e],
+                  L->w[2 * (size_t)L->ne + e], f);
     for (int c = 0; c < 5; ++c) {
       RHS(L, c, i1) += f[c];
       RHS(L, c, i2) -= f[c];

@@ -50,6 +50,11 @@ int or_monitor(or_ctx *c, double res[7], double maxres[7], int maxidx[7],
 int or_hllc_flux(const flite_params *p, const double ul[5], double pl,
                  const double ur[5], double pr, const double w[3], double f[5]);
 
+/* interior flux of fluxType 1-3 (DEF:1664-1792, LLF and Roe patched as
+ * documented in oracle.c) between two given states, side weights w[3] */
+int or_simple_flux(const flite_params *p, int flux_type, const double ul[5], double pl,
+                   const double ur[5], double pr, const double w[3], double f[5]);
+
 /* farfield Roe flux of one boundary-side end (DEF:2258-2446), unit normal n */
 int or_farfield_flux(const flite_params *p, int indicator, const double s[5], double pr,
                      const double n[3], double f[5]);

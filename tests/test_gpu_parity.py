@@ -233,3 +233,79 @@ def test_autotune():
     assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (ms["res"], mo["res"])
     e = rel_err(s.get(1, "U")[:6], o.get(1, "U")[:6])
     assert e.max() < 1e-9, e
+
+
+# ---- fluxType 1-3 with the JST dissipation (makeArtificialDissipation2) ----
+JST = {
+    "central": dict(fluxtype=1),
+    "llf": dict(fluxtype=2),
+    "roe": dict(fluxtype=3),
+    "roe_matrix": dict(fluxtype=3, usematrixdissipation=True, hartenscorrectionfactor=0.1),
+}
+
+
+def jst_case(name, **kw):
+    base = dict(secondorderdissipationfactor=0.5, fourthorderdissipationfactor=1 / 64.0, cflnumber=1.5)
+    base.update(JST[name])
+    base.update(kw)
+    return euler(1, **base)
+
+
+@pytest.mark.parametrize("name", list(JST))
+@pytest.mark.parametrize("level", [1, 2])
+def test_jst_dissipation_and_rhs(name, level):
+    print(f"[case] jst dissipation/rhs {name} level={level}", flush=True)
+    lv = levels(8, 3)
+    s, o = pair(jst_case(name), lv)
+    s.init_freestream(); o.init_freestream()
+    perturb(s, o, level, np.random.default_rng(20 + level), amp=0.05)
+    for x in (s, o):
+        x.apply(level, "PRESSURE_ABS")
+        x.apply(level, "TIMESTEPS")
+        x.apply(level, "DISSIPATION")
+        x.apply(level, "RHS")
+    assert rel_err(s.get(level, "DT"), o.get(level, "DT")).max() < 1e-11
+    d = o.get(level, "DISSIPATION")
+    assert np.abs(d).max() > 1e-8  # the JST terms are really there
+    e = rel_err(s.get(level, "DISSIPATION"), d)
+    assert e.max() < 1e-11, e
+    e = rel_err(s.get(level, "RHS")[:5], o.get(level, "RHS")[:5])
+    assert e.max() < 1e-11, e
+
+
+@pytest.mark.parametrize("name", list(JST))
+def test_jst_time_iterations(name):
+    print(f"[case] jst time iterations {name}", flush=True)
+    lv = levels(8, 3)
+    s, o = pair(jst_case(name), lv)
+    s.init_freestream(); o.init_freestream()
+    for x in (s, o):
+        x.do_time_iterations(1, 2, True)
+        x.do_time_iterations(1, 1, False)
+    e = rel_err(s.get(1, "U")[:5], o.get(1, "U")[:5])
+    assert e.max() < 1e-11, e
+
+
+@pytest.mark.parametrize("name,scheme,visc", [("central", 1, False), ("llf", 3, False), ("roe", 1, False),
+                                               ("roe_matrix", 3, False), ("roe", 1, True)])
+def test_jst_cycles(name, scheme, visc):
+    """Whole V-cycles: the JST runs on every level (coarse factor below
+    grid 1), in the restriction's coarse and fine parts too."""
+    print(f"[case] jst cycles {name} scheme {scheme} viscous={visc}", flush=True)
+    if visc:
+        lv = levels(8, 3, first_cell=0.005)
+        iv = rans(multigridscheme=scheme, secondorderdissipationfactor=0.5,
+                  fourthorderdissipationfactor=1 / 64.0, **JST[name])
+    else:
+        lv = levels(8, 3)
+        iv = jst_case(name, multigridscheme=scheme)
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    for c in range(1, 6):
+        s.cycle(c); o.cycle(c)
+        ms, mo = s.monitor(), o.monitor()
+        assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (c, ms["res"], mo["res"])
+        assert np.allclose(ms["forces"], mo["forces"], rtol=1e-8, atol=1e-14), (c, ms["forces"], mo["forces"])
+    nc = 6 if visc else 5
+    e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
+    assert e.max() < 1e-9, e
