@@ -1187,7 +1187,7 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
     }
     TRY(op_jst(c, L));
   }
-  for (int k = 1; k <= 3; ++k) {
+  for (int k = 1; k <= hp.rk_stages; ++k) {
     // stage k > 1 starts from the pressure and records the previous stage
     // kernel wrote with the new state; stage 1 after the frozen terms too
     if (k == 1 && !calc_diss) TRY(op_prep(c, L, true));
@@ -1200,7 +1200,7 @@ int do_single_iteration(flite_ctx *c, Level &L, bool calc_diss) {
     S.calc_turb = 1;
     S.fas = L.gnum > 1;
     S.has_src = L.gnum > 1;
-    S.next_record = k < 3;
+    S.next_record = k < hp.rk_stages;
     TRY(op_stage(c, L, MODE_STAGE, S));
   }
   return 0;
@@ -1416,12 +1416,14 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
                 q.coarse_grid_dissipation_scheme == p->coarse_grid_dissipation_scheme &&
                 q.use_matrix_dissipation == p->use_matrix_dissipation &&
                 q.hartens_correction_factor == p->hartens_correction_factor;
-    for (int k = 0; k < 3; ++k) same = same && q.rk_coefficients[k] == p->rk_coefficients[k];
+    same = same && q.rk_stages == p->rk_stages;
+    for (int k = 0; k < 5; ++k) same = same && q.rk_coefficients[k] == p->rk_coefficients[k];
     for (int k = 0; k < 6; ++k) same = same && q.inflow_field[1][k] == p->inflow_field[1][k];
     if (!same)
       return fail(FLITE_E_ARG, "batched conditions may differ only in Mach number, alpha, beta and the inflow state");
   }
   if (p->flux_type < 1 || p->flux_type > 4) return fail(FLITE_E_UNSUPPORTED, "fluxType must be 1-4");
+  if (p->rk_stages < 1 || p->rk_stages > 5) return fail(FLITE_E_ARG, "rk_stages must be 1..5");
   if (p->flux_type != 4 && (p->dissipation_scheme != 2 || p->coarse_grid_dissipation_scheme != 2))
     return fail(FLITE_E_UNSUPPORTED, "only dissipation scheme 2 (JST) is implemented");
   if (p->turbulence_model != 0 && p->turbulence_model != 1)

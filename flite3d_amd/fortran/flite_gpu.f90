@@ -52,7 +52,8 @@ module flite_gpu
      integer(c_int) :: boundary_term
      real(c_double) :: cfl_number
      real(c_double) :: turbulent_cfl_number
-     real(c_double) :: rk_coefficients(3)
+     real(c_double) :: rk_coefficients(5)
+     integer(c_int) :: rk_stages
      real(c_double) :: max_increment_factor
      real(c_double) :: minimum_density
      real(c_double) :: max_turbulence_value

@@ -316,7 +316,8 @@ contains
     p%boundary_term = ivd%boundaryTerm
     p%cfl_number = ivd%CFLNumber
     p%turbulent_cfl_number = ivd%turbulentCFLNumber
-    p%rk_coefficients = [0.6_rk, 0.6_rk, 1.0_rk]
+    p%rk_coefficients = [0.6_rk, 0.6_rk, 1.0_rk, 0.0_rk, 0.0_rk]
+    p%rk_stages = 3
     p%max_increment_factor = ivd%maxIncrementFactor
     p%minimum_density = ivd%minimumDensity
     p%max_turbulence_value = ivd%maxTurbulenceValue

@@ -194,7 +194,9 @@ class InputVariables:
             raise NotImplementedError("outside the implemented path: " +
                                       ", ".join(bad))
 
-    def to_struct(self, coarse_fp32=False):
+    def to_struct(self, coarse_fp32=False, rk=None):
+        """flite_params; rk = stage factors (alpha_1..alpha_s, s <= 5) to
+        replace the reference's 3-stage (0.6, 0.6, 1.0)."""
         v = self.v
         p = FliteParams()
         p.gamma = v["gamma"]
@@ -218,7 +220,11 @@ class InputVariables:
         p.boundary_term = v["boundaryterm"]
         p.cfl_number = v["cflnumber"]
         p.turbulent_cfl_number = v["turbulentcflnumber"]
-        p.rk_coefficients[:] = [0.6, 0.6, 1.0]
+        alphas = list(rk) if rk is not None else [0.6, 0.6, 1.0]
+        if not 1 <= len(alphas) <= 5:
+            raise ValueError("1 to 5 Runge-Kutta stages")
+        p.rk_coefficients[:] = alphas + [0.0] * (5 - len(alphas))
+        p.rk_stages = len(alphas)
         p.max_increment_factor = v["maxincrementfactor"]
         p.minimum_density = v["minimumdensity"]
         p.max_turbulence_value = v["maxturbulencevalue"]
@@ -269,7 +275,8 @@ class FliteParams(ctypes.Structure):
         ("boundary_term", ctypes.c_int),
         ("cfl_number", ctypes.c_double),
         ("turbulent_cfl_number", ctypes.c_double),
-        ("rk_coefficients", ctypes.c_double * 3),
+        ("rk_coefficients", ctypes.c_double * 5),
+        ("rk_stages", ctypes.c_int),
         ("max_increment_factor", ctypes.c_double),
         ("minimum_density", ctypes.c_double),
         ("max_turbulence_value", ctypes.c_double),

@@ -55,7 +55,8 @@ typedef struct flite_params {
   int boundary_term;               /* 1 = FE (3:1), 2 = FV */
   double cfl_number;
   double turbulent_cfl_number;
-  double rk_coefficients[3];       /* 0.6, 0.6, 1.0 (mgnsg3d_def_par.F90:14240-14244) */
+  double rk_coefficients[5];       /* stage factors alpha_k: 0.6, 0.6, 1.0 (mgnsg3d_def_par.F90:14240-14244) */
+  int rk_stages;                   /* 3 as the reference; up to 5 (e.g. 1/4, 1/6, 3/8, 1/2, 1) */
   double max_increment_factor;
   double minimum_density;
   double max_turbulence_value;

@@ -1424,7 +1424,9 @@ static void update(or_ctx *C, orl *L, double *u, const double *uprev, double *de
     if (U(L, 0, i) < P->minimum_density) U(L, 0, i) = P->minimum_density;
 }
 
-/* singleIteration (DEF:1002-1181) */
+/* singleIteration (DEF:1002-1181); the reference's 3 stages with alpha =
+ * (0.6, 0.6, 1.0) are flite_params' defaults, other stage counts (up to 5)
+ * run the same stage with the given factors */
 static int single_iteration(or_ctx *C, orl *L, int calc_diss, int calc_turb, int iteration) {
   const flite_params *P = &C->P;
   int n = L->n, rc;
@@ -1443,7 +1445,7 @@ static int single_iteration(or_ctx *C, orl *L, int calc_diss, int calc_turb, int
       }
     }
   }
-  for (int k = 1; k <= 3; ++k) {
+  for (int k = 1; k <= P->rk_stages; ++k) {
     pressure(C, L, L->u);
     memset(L->rhs, 0, sizeof(double) * 6 * (size_t)n);
     if ((rc = make_rhs(C, L))) return rc;
