@@ -15,7 +15,7 @@ module flite_gridio
   use flite_gpu, only: flite_level_view
   implicit none
   private
-  public :: GridLevelData, readComputationData, level_view, writeGridResults, readStartupField
+  public :: GridLevelData, readComputationData, level_view, writeGridResults, readStartupField, makePlot
 
   type GridLevelData
      integer(c_int) :: numberOfNodes = 0, numberOfSides = 0
@@ -204,5 +204,49 @@ contains
        u(:, j) = u(:, 1)*u(:, j)
     end do
   end subroutine readStartupField
+
+  ! makeplot (mgnsg3d_mg_par.F90:1599-1707): merge the per-domain result
+  ! files <case>.res_<d> into <case>.unk through the preprocessor's plot
+  ! register plotreg.reg, a record (N, numberOfDomains), then per domain its
+  ! local node count and one (local, global) record per node.  The values
+  ! are copied as the result files hold them.
+  subroutine makePlot(caseName, nvar)
+    character(len=*), intent(in) :: caseName
+    integer, intent(in) :: nvar
+    integer :: numberOfNodes, numberOfDomains, numberOfLocalNodes, numberOfToNodes, i, j, k
+    integer, allocatable :: transArray(:, :)
+    real(c_double), allocatable :: u(:, :), unk(:, :)
+    character(len=16) :: dom
+    open (15, file='plotreg.reg', form='unformatted', status='old')
+    read (15) numberOfNodes, numberOfDomains
+    allocate (u(numberOfNodes, nvar))
+    u = 0.0_c_double
+    do i = 1, numberOfDomains
+       read (15) numberOfLocalNodes
+       allocate (transArray(numberOfLocalNodes, 2))
+       do j = 1, numberOfLocalNodes
+          read (15) transArray(j, :)
+       end do
+       write (dom, '(I0)') i
+       open (16, file=trim(caseName)//".res_"//trim(dom), form='unformatted', status='old')
+       read (16) numberOfToNodes
+       allocate (unk(numberOfToNodes, nvar))
+       read (16) ((unk(j, k), j=1, numberOfToNodes), k=1, nvar)
+       close (16)
+       do j = 1, numberOfLocalNodes
+          if (transArray(j, 1) < 1 .or. transArray(j, 1) > numberOfToNodes .or. &
+              transArray(j, 2) < 1 .or. transArray(j, 2) > numberOfNodes) &
+               error stop "ERROR: plotreg.reg does not match the result files"
+          u(transArray(j, 2), :) = unk(transArray(j, 1), :)
+       end do
+       deallocate (unk, transArray)
+    end do
+    close (15)
+    open (17, file=trim(caseName)//".unk", form='unformatted', status='replace')
+    write (17) numberOfNodes
+    write (17) ((u(i, j), i=1, numberOfNodes), j=1, nvar)
+    close (17)
+    deallocate (u)
+  end subroutine makePlot
 
 end module flite_gridio

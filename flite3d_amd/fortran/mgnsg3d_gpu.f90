@@ -9,8 +9,9 @@
 ! until the monitor residual has fallen by that many decades; every cycle
 ! appends a line to <case>.rsd (writeResiduals, MG:1478-1514), every
 ! writeToFileInterval cycles and at the end the solution goes to <case>.res_1
-! (writeResultsToFile, MG:1518-1531), and liftdrag.res gets the final forces
-! (MG:847-880).  Only single-domain runs exist here: the MPI partitioning
+! (writeResultsToFile, MG:1518-1531), liftdrag.res gets the final forces
+! (MG:847-880), and with a plot register (plotreg.reg) present the result is
+! merged into <case>.unk (makeplot, MG:1599-1707).  Only single-domain runs exist here: the MPI partitioning
 ! collapses to one domain on one GPU.
 !
 ! Usage: mgnsg3d_gpu [--coarse-fp32]
@@ -32,7 +33,7 @@ program mgnsg3d_gpu
   type(flite_level_view) :: v
   character(len=256) :: fileName, arg
   integer :: numberOfGrids, i, l, nvar, nres, k
-  logical :: coarseFp32, switched
+  logical :: coarseFp32, switched, havePlotReg
   real(c_double), allocatable :: u(:, :)
   real(c_double) :: res(7), maxres(7), forces(9)
   integer(c_int) :: maxind(7)
@@ -124,6 +125,13 @@ program mgnsg3d_gpu
      if (stopAtResidualReduction >= 0.0_c_double .and. i > ivd%maximumnumberofcycles) exit
   end do
   call writeResultsToFile()
+  ! merging results to the .unk file (MG:776-781), when the preprocessor
+  ! left its plot register here
+  inquire (file='plotreg.reg', exist=havePlotReg)
+  if (havePlotReg) then
+     write (*, '(A)') " Merging results to .unk file ..."
+     call makePlot(fileName, nvar)
+  end if
 
   call gpu_monitor(res, maxres, maxind, forces)
   open (28, file='liftdrag.res', form='formatted', status='unknown')

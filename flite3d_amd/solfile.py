@@ -50,6 +50,9 @@ _I4 = np.dtype("<i4")
 _F8 = np.dtype("<f8")
 
 
+_TRANS = np.dtype([("local", "<i4"), ("global", "<i4")])  # one plot-register record
+
+
 class _Writer:
     def __init__(self, f):
         self.f = f
@@ -274,6 +277,49 @@ def read_sol(path) -> list[Level]:
     if R.o != len(R.b):
         raise ValueError(f"{len(R.b) - R.o} trailing bytes after the last level")
     return levels
+
+
+def write_plotreg(path, n_nodes, domains):
+    """The preprocessor's plot register (gen3d_grid.f90:4293-4294,
+    4807-4815): (N, numberOfDomains), then per domain its node count and one
+    (local, global) record per node, 1-based.  `domains` lists each domain's
+    global node numbers in local order."""
+    with open(path, "wb") as f:
+        W = _Writer(f)
+        W.rec(n_nodes, len(domains))
+        for g in domains:
+            g = np.asarray(g, np.int64)
+            W.rec(len(g))
+            t = np.empty(len(g), _TRANS)
+            t["local"] = np.arange(1, len(g) + 1)
+            t["global"] = g
+            W.table(_TRANS, t)
+
+
+def makeplot(plotreg, result_paths, out_path, nvar):
+    """makeplot (mgnsg3d_mg_par.F90:1599-1707): the per-domain result files
+    scattered into one N x nvar field through the plot register, written as
+    <case>.unk: a record N, then one record u(N, nvar) column-major."""
+    with open(plotreg, "rb") as f:
+        R = _Reader(f.read())
+    n, nd = (int(x) for x in R.ints(2))
+    if nd != len(result_paths):
+        raise ValueError(f"plot register has {nd} domains, {len(result_paths)} result files given")
+    u = np.zeros((nvar, n))
+    for d in range(nd):
+        nl = R.ints()
+        t = R.table(_TRANS, nl)
+        trans = np.stack([t["local"], t["global"]], axis=1).astype(np.int64)
+        with open(result_paths[d], "rb") as f:
+            Q = _Reader(f.read())
+        nt = Q.ints()
+        unk = np.frombuffer(Q._payload(), _F8).reshape(nvar, nt)
+        u[:, trans[:, 1] - 1] = unk[:, trans[:, 0] - 1]
+    with open(out_path, "wb") as f:
+        W = _Writer(f)
+        W.rec(n)
+        W.raw(u.astype(_F8).tobytes())
+    return u
 
 
 def write_results(path, u, turbulence_model):

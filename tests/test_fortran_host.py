@@ -4,10 +4,11 @@ GPU path and the oracle.
 
 CPU tests: .sol_1 / .res_1 round trips (the record layout of
 readComputationData, SolverSource/mgnsg3d_mg_par.F90:1198-1268, and
-writeGridResults, SolverSource/mgnsg3d_def_par.F90:15331-15345), and the
+writeGridResults, SolverSource/mgnsg3d_def_par.F90:15331-15345), the .unk
+merge of makeplot (mgnsg3d_mg_par.F90:1599-1707), and the
 Fortran driver's input path (run.inp -> <case>.inp -> <case>.sol_1) up to
 the first GPU call.  GPU tests: the driver's .res_1, .rsd and liftdrag.res
-after a few V-cycles.
+after a few V-cycles, and its .unk merge.
 """
 import os
 import subprocess
@@ -116,6 +117,29 @@ def test_res_round_trip(tmp_path):
         assert np.allclose(back, u, rtol=1e-15, atol=0)
 
 
+def test_makeplot_merges_domains(tmp_path):
+    """makeplot (MG:1599-1707): two domains' result files scattered through
+    the plot register into one field, values as the files hold them."""
+    rng = np.random.default_rng(5)
+    n = 40
+    perm = rng.permutation(n) + 1
+    doms = [perm[:17], perm[17:]]
+    u = rng.uniform(0.5, 1.5, size=(6, n))
+    paths = []
+    for d, g in enumerate(doms, start=1):
+        p = tmp_path / f"c.res_{d}"
+        solfile.write_results(p, u[:, g - 1], 1)
+        paths.append(p)
+    solfile.write_plotreg(tmp_path / "plotreg.reg", n, doms)
+    m = solfile.makeplot(tmp_path / "plotreg.reg", paths, tmp_path / "c.unk", 6)
+    stored = np.concatenate([u[:1], u[1:] / u[0]])
+    assert np.allclose(m, stored, rtol=1e-15, atol=0)
+    raw = (tmp_path / "c.unk").read_bytes()
+    assert int(np.frombuffer(raw[4:8], "<i4")[0]) == n
+    body = np.frombuffer(raw[16:16 + 8 * 6 * n], "<f8").reshape(6, n)
+    assert np.array_equal(body, m)
+
+
 def test_driver_reads_inputs(tmp_path):
     """The driver parses run.inp, the namelist (with array elements,
     strings and logicals) and the .sol_1 file; without a GPU it stops at
@@ -195,6 +219,28 @@ def test_driver_matches_gpu_and_oracle(tmp_path):
     assert abs(drag - mo["forces"][1]) <= 1e-4 * abs(mo["forces"][1]) + 1e-12
     ld = open(os.path.join(d, "liftdrag.res")).read().split()
     assert len(ld) == 7
+
+
+@pytest.mark.gpu
+def test_driver_writes_unk(tmp_path):
+    """With a plot register present the driver merges its result into
+    <case>.unk as makeplot does (MG:776-781, 1599-1707)."""
+    print("[case] fortran driver .unk merge, euler 8^3", flush=True)
+    lv = levels(8, 2)
+    d = str(tmp_path)
+    write_case(d, lv, iters=1, re=0.0, turb=0, order=1, mach=0.5, alpha=2.0, cfl=2.0)
+    n = lv[0].n_nodes
+    perm = np.random.default_rng(6).permutation(n) + 1
+    solfile.write_plotreg(os.path.join(d, "plotreg.reg"), n, [perm])
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Merging results to .unk file" in r.stdout
+    ref = solfile.makeplot(os.path.join(d, "plotreg.reg"), [os.path.join(d, "bump.res_1")],
+                           os.path.join(d, "check.unk"), 5)
+    raw = open(os.path.join(d, "bump.unk"), "rb").read()
+    assert int(np.frombuffer(raw[4:8], "<i4")[0]) == n
+    body = np.frombuffer(raw[16:16 + 8 * 5 * n], "<f8").reshape(5, n)
+    assert np.array_equal(body, ref)
 
 
 @pytest.mark.gpu
