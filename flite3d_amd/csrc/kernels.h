@@ -46,6 +46,10 @@ struct DevParams {
   // and below it, matrix scaling and its entropy fix
   int ftype, matdiss;
   double k2, k4, k4c, hcf;
+  // dual time stepping on a fixed mesh: unsteady (flowType != 0), time
+  // differences added (addTimeStep), BDF2 on grid 1, physical time step
+  int unsteady, add_time, bdf2;
+  double dtp;
   double prolong_relax, turb_prolong_relax, mg_bc_relax;
   // batched conditions: per condition (Mach, inflowField(1:6,1), spare),
   // 8 doubles each; null for a single condition
@@ -110,6 +114,7 @@ struct LevelDev {
   R *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;  // tau, dnu: VREC-strided, one shared record
   R *jst;  // JST node record (8 per node): L(U) 5, pressure switch, side count, pad
+  R *uold, *uold2;  // solutions at the last two physical times (6 per node, unsteady only)
   // batched conditions: nb copies of every state array, condition-major;
   // gstride = gradient-record elements per condition (0 if none)
   int nb;
@@ -138,6 +143,8 @@ struct LevelDev {
     bbuf += (size_t)b * 10 * (size_t)nbs;
     ws += 3 * bn;
     jst += 8 * bn;
+    uold += 6 * bn;
+    uold2 += 6 * bn;
   }
 };
 
@@ -2365,11 +2372,27 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
     for (int c = 0; c < 6; ++c) L.rhs[(size_t)c * n + i] = r[c];
     return;
   }
+  if (MODE == MODE_STAGE && P.unsteady && P.add_time) {
+    // addTimeDifferences (DEF:1184-1241) on a fixed mesh: BDF2 on grid 1,
+    // BDF1 elsewhere, over components 1:5 (1:6 with SA)
+    const R V = L.vol[i], dtp = R(P.dtp);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (c == 5 && !SA) break;
+      const size_t k = (size_t)c * n + i;
+      if (L.gnum == 1 && P.bdf2)
+        r[c] = r[c] + (R(1.5) * V * L.u[k] - R(2.0) * V * L.uold[k] + R(0.5) * V * L.uold2[k]) / dtp;
+      else
+        r[c] = r[c] + V * (L.u[k] - L.uold[k]) / dtp;
+    }
+  }
   for (int c = 0; c < 5; ++c) r[c] += L.diss[(size_t)c * n + i];
   if (SA) r[5] += L.tdiff[i];
   if (!SA) r[5] = 0;
-  // solveSystem (DEF:1407-1479)
+  // solveSystem (DEF:1407-1479); unsteady: dt / (1 + 1.5 dt / dtp) for the
+  // mean flow (DEF:1427-1436)
   R V = L.vol[i], dt = L.dt[i];
+  if (P.unsteady) dt = dt / (R(1.0) + R(1.5) * dt / R(P.dtp));
   for (int c = 0; c < 5; ++c) r[c] = dt * r[c] / V;
   if (SA) r[5] = L.dtt[i] * r[5] / V;
   bc_increment(L, P, i, r);

@@ -156,6 +156,7 @@ struct Level {
   void *u, *uprev, *rhs, *diss, *src, *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
   void *jst = nullptr;  // JST node records (fluxType 1-3 only)
+  void *uold = nullptr, *uold2 = nullptr;  // unsteady only
   int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
   int *ell_e, *ell_nbr;
   void *ellw;
@@ -204,6 +205,8 @@ struct Level {
     v.divg = (R *)divg; v.tdiff = (R *)tdiff; v.rec0 = (R *)rec0; v.grec = (R *)grec;
     v.tau = (R *)tau; v.dnu = (R *)dnu; v.fbuf = (R *)fbuf; v.bbuf = (R *)bbuf; v.ws = (R *)ws;
     v.jst = (R *)jst;
+    v.uold = (R *)uold;
+    v.uold2 = (R *)uold2;
     v.nb = nbatch;
     v.gstride = gstride;
     return v;
@@ -372,6 +375,10 @@ DevParams make_dev_params(const flite_params &p) {
   P.k4 = p.fourth_order_dissipation_factor;
   P.k4c = p.coarse_grid_dissipation_factor;
   P.hcf = p.hartens_correction_factor;
+  P.unsteady = p.flow_type != 0;
+  P.add_time = p.add_time_step;
+  P.bdf2 = p.time_scheme == 2;
+  P.dtp = p.physical_time_step;
   return P;
 }
 
@@ -841,6 +848,8 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
   TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
   TRY(al_real(&L.jst, c->hp.flux_type != 4 ? (size_t)(8 * (size_t)n) * NB : 1));
+  TRY(al_real(&L.uold, c->hp.flow_type != 0 ? (size_t)(6 * (size_t)n) * NB : 1));
+  TRY(al_real(&L.uold2, c->hp.flow_type != 0 ? (size_t)(6 * (size_t)n) * NB : 1));
   int nbm = nblk(n);
   TRY(P.alloc(&L.m_sum, 7 * (size_t)nbm));
   TRY(P.alloc(&L.m_max, 7 * (size_t)nbm));
@@ -1415,7 +1424,9 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
                 q.dissipation_scheme == p->dissipation_scheme &&
                 q.coarse_grid_dissipation_scheme == p->coarse_grid_dissipation_scheme &&
                 q.use_matrix_dissipation == p->use_matrix_dissipation &&
-                q.hartens_correction_factor == p->hartens_correction_factor;
+                q.hartens_correction_factor == p->hartens_correction_factor &&
+                q.flow_type == p->flow_type && q.time_scheme == p->time_scheme &&
+                q.add_time_step == p->add_time_step && q.physical_time_step == p->physical_time_step;
     same = same && q.rk_stages == p->rk_stages;
     for (int k = 0; k < 5; ++k) same = same && q.rk_coefficients[k] == p->rk_coefficients[k];
     for (int k = 0; k < 6; ++k) same = same && q.inflow_field[1][k] == p->inflow_field[1][k];
@@ -1424,6 +1435,9 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   }
   if (p->flux_type < 1 || p->flux_type > 4) return fail(FLITE_E_UNSUPPORTED, "fluxType must be 1-4");
   if (p->rk_stages < 1 || p->rk_stages > 5) return fail(FLITE_E_ARG, "rk_stages must be 1..5");
+  if (p->flow_type != 0 && p->flow_type != 1)
+    return fail(FLITE_E_UNSUPPORTED, "only steady flow and unsteady flow on a fixed mesh (flowType 0, 1)");
+  if (p->flow_type == 1 && !(p->physical_time_step > 0.0)) return fail(FLITE_E_ARG, "physical_time_step must be > 0");
   if (p->flux_type != 4 && (p->dissipation_scheme != 2 || p->coarse_grid_dissipation_scheme != 2))
     return fail(FLITE_E_UNSUPPORTED, "only dissipation scheme 2 (JST) is implemented");
   if (p->turbulence_model != 0 && p->turbulence_model != 1)
@@ -1611,8 +1625,9 @@ int flite_state_get(flite_ctx *c, int level, double *u, int ld, int ncomp, int b
   return 0;
 }
 
-int flite_init_from_state(flite_ctx *c) {
-  TRY(check_ctx(c));
+// multiGridCycles preamble (MG:586-602): laminar viscosity, u and (for
+// unsteady runs) uold restricted to every coarse level
+int preamble(flite_ctx *c) {
   double mu0 = c->hp.reynolds_number > 0.0 ? 1.0 / c->hp.reynolds_number : 0.0;
   for (int l = 0; l < c->nl; ++l) {
     Level &L = *c->lev[l];
@@ -1622,8 +1637,34 @@ int flite_init_from_state(flite_ctx *c) {
       hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), cnt);
       return check_launch();
     }));
-    if (l > 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
+    if (l > 0) {
+      TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
+      if (c->hp.flow_type != 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->uold, L.uold, 6));
+    }
   }
+  return sync(c);
+}
+
+// uold2 <- src_old, uold <- src on level 1 (unsteady runs)
+int set_old_states(flite_ctx *c, const void *old_src, const void *src) {
+  if (c->hp.flow_type == 0) return 0;
+  Level &L = *c->lev[0];
+  TRY(op_copy(c, L, L.uold2, old_src, 6));
+  return op_copy(c, L, L.uold, src, 6);
+}
+
+int flite_init_from_state(flite_ctx *c) {
+  TRY(check_ctx(c));
+  // a restart (DEF:3866-3873): uold and uold2 take the given state
+  TRY(set_old_states(c, c->lev[0]->u, c->lev[0]->u));
+  return preamble(c);
+}
+
+int flite_advance_physical_time(flite_ctx *c) {
+  TRY(check_ctx(c));
+  if (c->hp.flow_type == 0) return fail(FLITE_E_STATE, "steady run (flowType 0): no physical time");
+  // a captured cycle reads uold and uold2 where they are, so it stays valid
+  TRY(set_old_states(c, c->lev[0]->uold, c->lev[0]->u));
   return sync(c);
 }
 
@@ -1638,9 +1679,12 @@ int flite_init_freestream(flite_ctx *c) {
       for (int i = 0; i < L.n; ++i) u[5 * (size_t)L.n + i] = 0.1;
     TRY(flite_state_set(c, 1, u.data(), L.n, 6, b));
   }
+  // uold and uold2 take the freestream before the boundary conditions act
+  // on u (DEF:3980-3994)
+  TRY(set_old_states(c, L.u, L.u));
   TRY(op_prep(c, L, true));
   TRY(op_bc_state(c, L, L.u, 0));
-  return flite_init_from_state(c);
+  return preamble(c);
 }
 
 int flite_do_time_iterations(flite_ctx *c, int level, int steps, int calc_diss, int calc_turb, int iteration) {

@@ -41,6 +41,7 @@ def lib():
         L.flite_apply.argtypes = [vp, i, i]
         L.flite_init_freestream.argtypes = [vp]
         L.flite_init_from_state.argtypes = [vp]
+        L.flite_advance_physical_time.argtypes = [vp]
         L.flite_do_time_iterations.argtypes = [vp, i, i, i, i, i]
         L.flite_restrict.argtypes = [vp, i, i]
         L.flite_prolongate.argtypes = [vp, i]
@@ -121,6 +122,10 @@ class Solver:
 
     def init_from_state(self):
         self._check(self.L.flite_init_from_state(self.ctx))
+
+    def advance_physical_time(self):
+        """End of a physical time step (MG:889-890): uold2 <- uold <- u."""
+        self._check(self.L.flite_advance_physical_time(self.ctx))
 
     def do_time_iterations(self, level, steps, calc_diss, calc_turb=True, iteration=1):
         """doTimeIterations (DEF:966-1000)."""

@@ -76,6 +76,10 @@ module flite_gpu
      integer(c_int) :: coarse_grid_dissipation_scheme
      integer(c_int) :: use_matrix_dissipation
      real(c_double) :: hartens_correction_factor
+     integer(c_int) :: flow_type
+     integer(c_int) :: time_scheme
+     integer(c_int) :: add_time_step
+     real(c_double) :: physical_time_step
   end type flite_params
 
   ! flite_level_view: non-owning pointers to one level's arrays as the
@@ -147,6 +151,11 @@ module flite_gpu
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function flite_init_from_state
+
+     integer(c_int) function flite_advance_physical_time(ctx) bind(C, name="flite_advance_physical_time")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function flite_advance_physical_time
 
      integer(c_int) function flite_do_time_iterations(ctx, level, relaxation_steps, calculate_dissipation, &
           calculate_turbulence, iteration_number) bind(C, name="flite_do_time_iterations")

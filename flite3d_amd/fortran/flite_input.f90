@@ -275,7 +275,9 @@ contains
   ! silently running something else
   subroutine check_supported(ivd)
     type(InputVariablesData), intent(in) :: ivd
-    if (ivd%flowType /= 0) error stop "ERROR: only steady flow (flowType=0) runs on the GPU path"
+    if (ivd%flowType /= 0 .and. ivd%flowType /= 1) &
+         error stop "ERROR: only steady flow and unsteady flow on a fixed mesh (flowType 0, 1) run on the GPU path"
+    if (ivd%flowType /= 0 .and. ivd%explicit) error stop "ERROR: explicit time stepping is not on the GPU path"
     if (ivd%turbulenceModel /= 0 .and. ivd%turbulenceModel /= 1) &
          error stop "ERROR: only turbulenceModel 0 and 1 (Spalart-Allmaras) run on the GPU path"
     if (ivd%fluxType < 1 .or. ivd%fluxType > 4) error stop "ERROR: only fluxType 1-4 run on the GPU path"
@@ -340,6 +342,10 @@ contains
     p%coarse_grid_dissipation_scheme = ivd%coarseGridDissipationScheme
     p%use_matrix_dissipation = merge(1, 0, ivd%useMatrixDissipation)
     p%hartens_correction_factor = ivd%HartensCorrectionFactor
+    p%flow_type = ivd%flowType
+    p%time_scheme = ivd%timeScheme
+    p%add_time_step = merge(1, 0, ivd%addTimeStep)
+    p%physical_time_step = ivd%physicalTimeStep
   end function to_flite_params
 
 end module flite_input

@@ -9,9 +9,12 @@
 ! until the monitor residual has fallen by that many decades; every cycle
 ! appends a line to <case>.rsd (writeResiduals, MG:1478-1514), every
 ! writeToFileInterval cycles and at the end the solution goes to <case>.res_1
-! (writeResultsToFile, MG:1518-1531), liftdrag.res gets the final forces
-! (MG:847-880), and with a plot register (plotreg.reg) present the result is
-! merged into <case>.unk (makeplot, MG:1599-1707).  Only single-domain runs exist here: the MPI partitioning
+! (writeResultsToFile, MG:1518-1531), liftdrag.res gets the forces of each
+! physical step (MG:847-884), and with a plot register (plotreg.reg) present
+! the result is merged into <case>.unk (makeplot, MG:1599-1707).  Unsteady
+! runs on a fixed mesh (flowType 1) loop over the physical time steps
+! restartNumber .. numberOfPhysicalTimesteps around multiGridCycles' loop
+! (MG:486-890); a moving mesh (flowType 2, a new .sol per step) is not run.  Only single-domain runs exist here: the MPI partitioning
 ! collapses to one domain on one GPU.
 !
 ! Usage: mgnsg3d_gpu [--coarse-fp32]
@@ -32,7 +35,7 @@ program mgnsg3d_gpu
   type(flite_params) :: p
   type(flite_level_view) :: v
   character(len=256) :: fileName, arg
-  integer :: numberOfGrids, i, l, nvar, nres, k
+  integer :: numberOfGrids, i, l, nvar, nres, k, physicalTimestepNumber, firstStep, lastStep
   logical :: coarseFp32, switched, havePlotReg
   real(c_double), allocatable :: u(:, :)
   real(c_double) :: res(7), maxres(7), forces(9)
@@ -91,14 +94,24 @@ program mgnsg3d_gpu
   end if
 
   initialResidual = ivd%initialResidual
+  averageTimePerCycle = 0.0_c_double
+  call system_clock(count_rate=rate)
+  switched = .false.
+  ! physical time steps (MG:477-486): one for steady flow
+  firstStep = ivd%restartNumber
+  lastStep = ivd%numberofphysicaltimesteps
+  if (ivd%flowType == 0) then
+     firstStep = 1
+     lastStep = 1
+  end if
+
+  do physicalTimestepNumber = firstStep, lastStep
+  if (ivd%flowType /= 0) write (*, '(A,I7)') " >>>>>>>>>----> number of timestep = ", physicalTimestepNumber
   currentResidual = 10000.0_c_double
   stopAtResidualReduction = -1.0_c_double
   if (ivd%numberOfMGIterations < 0) stopAtResidualReduction = abs(dble(ivd%numberOfMGIterations))
-  averageTimePerCycle = 0.0_c_double
-  call system_clock(count_rate=rate)
 
   write (*, '(A)') "Starting timestepping..."
-  switched = .false.
   i = 1
   do
      if (stopAtResidualReduction < 0.0_c_double) then
@@ -125,6 +138,21 @@ program mgnsg3d_gpu
      if (stopAtResidualReduction >= 0.0_c_double .and. i > ivd%maximumnumberofcycles) exit
   end do
   call writeResultsToFile()
+
+  ! lift file of the step (MG:855-884): a new file on the first step
+  call gpu_monitor(res, maxres, maxind, forces)
+  if (physicalTimestepNumber > 1) then
+     open (28, file='liftdrag.res', form='formatted', status='unknown', position='append')
+  else
+     open (28, file='liftdrag.res', form='formatted', status='unknown')
+  end if
+  write (28, '(7E14.5)') physicalTimestepNumber*ivd%physicalTimeStep, forces(1:6)
+  close (28)
+
+  ! end of the physical step: uold2 <- uold <- u (MG:889-890)
+  if (ivd%flowType /= 0) call flite_check(flite_advance_physical_time(gpu_ctx), "physical time step")
+  end do
+
   ! merging results to the .unk file (MG:776-781), when the preprocessor
   ! left its plot register here
   inquire (file='plotreg.reg', exist=havePlotReg)
@@ -132,11 +160,6 @@ program mgnsg3d_gpu
      write (*, '(A)') " Merging results to .unk file ..."
      call makePlot(fileName, nvar)
   end if
-
-  call gpu_monitor(res, maxres, maxind, forces)
-  open (28, file='liftdrag.res', form='formatted', status='unknown')
-  write (28, '(7E14.5)') ivd%physicalTimeStep, forces(1:6)
-  close (28)
   if (i > 1) write (*, *) "Average time per cycle: ", averageTimePerCycle/dble(i - 1)
 
   call flite_ctx_destroy(gpu_ctx)

@@ -170,8 +170,10 @@ class InputVariables:
     def check_supported(self):
         v = self.v
         bad = []
-        if v["flowtype"] != 0:
-            bad.append("flowType != 0 (unsteady)")
+        if v["flowtype"] not in (0, 1):
+            bad.append("flowType 2 (moving meshes)")
+        if v["flowtype"] != 0 and v["explicit"]:
+            bad.append("explicit time stepping")
         if v["turbulencemodel"] not in (0, 1):
             bad.append("turbulenceModel other than 0/1")
         if v["fluxtype"] not in (1, 2, 3, 4):
@@ -250,6 +252,10 @@ class InputVariables:
         p.coarse_grid_dissipation_scheme = v["coarsegriddissipationscheme"]
         p.use_matrix_dissipation = int(bool(v["usematrixdissipation"]))
         p.hartens_correction_factor = v["hartenscorrectionfactor"]
+        p.flow_type = v["flowtype"]
+        p.time_scheme = v["timescheme"]
+        p.add_time_step = int(bool(v["addtimestep"]))
+        p.physical_time_step = v["physicaltimestep"]
         return p
 
 
@@ -299,6 +305,10 @@ class FliteParams(ctypes.Structure):
         ("coarse_grid_dissipation_scheme", ctypes.c_int),
         ("use_matrix_dissipation", ctypes.c_int),
         ("hartens_correction_factor", ctypes.c_double),
+        ("flow_type", ctypes.c_int),
+        ("time_scheme", ctypes.c_int),
+        ("add_time_step", ctypes.c_int),
+        ("physical_time_step", ctypes.c_double),
     ]
 
 

@@ -36,6 +36,7 @@ def lib():
         L.or_apply.argtypes = [vp, i, i]
         L.or_init_freestream.argtypes = [vp]
         L.or_init_from_state.argtypes = [vp]
+        L.or_advance_physical_time.argtypes = [vp]
         L.or_do_time_iterations.argtypes = [vp, i, i, i, i, i]
         L.or_restrict.argtypes = [vp, i, i]
         L.or_prolongate.argtypes = [vp, i]
@@ -128,6 +129,9 @@ class Oracle:
 
     def cycle(self, number):
         self._check(self.L.or_cycle(self.ctx, number))
+
+    def advance_physical_time(self):
+        self._check(self.L.or_advance_physical_time(self.ctx))
 
     def monitor(self):
         res = (ctypes.c_double * 7)()

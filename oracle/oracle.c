@@ -73,6 +73,7 @@ typedef struct {
   double *rw;
   /* state, component-major [c*n+i] */
   double *u, *uprev, *rhs, *diss, *src;
+  double *uold, *uold2; /* solutions at the last two physical times (unsteady) */
   double *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   double *ws; /* wallStress(Nb,4): ws[c*nb+i] */
   int has_src;
@@ -119,7 +120,7 @@ static void free_level(orl *L) {
   void *ptrs[] = {L->e1, L->e2, L->w, L->b1, L->b2, L->bi, L->wb, L->vol,
                   L->xyz, L->wd, L->nia, L->nrm, L->warea, L->f2c, L->rw,
                   L->u, L->uprev, L->rhs, L->diss, L->src, L->p, L->dt,
-                  L->dtt, L->mu, L->vort, L->divg, L->tdiff, L->ws};
+                  L->dtt, L->mu, L->vort, L->divg, L->tdiff, L->ws, L->uold, L->uold2};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
   memset(L, 0, sizeof *L);
 }
@@ -215,6 +216,8 @@ int or_level_upload(or_ctx *c, int level, const flite_level_view *v) {
   L->uprev = zalloc(6 * (size_t)n);
   L->rhs = zalloc(6 * (size_t)n);
   L->diss = zalloc(5 * (size_t)n);
+  L->uold = zalloc(6 * (size_t)n);
+  L->uold2 = zalloc(6 * (size_t)n);
   L->has_src = level > 1; /* sourceTerm is nullified on grid 1 (DEF:14212-14213) */
   L->src = zalloc(6 * (size_t)n);
   L->p = zalloc(n);
@@ -1384,10 +1387,34 @@ static void sa_timesteps(or_ctx *C, orl *L) {
  * ------------------------------------------------------------------------ */
 
 /* solveSystem + solveTurbulenceSystem (DEF:1407-1522), steady flow */
+/* addTimeDifferences (DEF:1184-1241): the dual-time source of a fixed
+ * mesh (previous volumes = volumes), BDF2 on grid 1 with timeScheme 2,
+ * BDF1 otherwise; components 1:5, 1:6 with Spalart-Allmaras */
+static void time_differences(or_ctx *C, orl *L) {
+  const flite_params *P = &C->P;
+  if (!P->add_time_step || P->flow_type == 0) return;
+  int n = L->n, nc = P->turbulence_model == 1 ? 6 : 5;
+  double dtp = P->physical_time_step;
+  for (int i = 0; i < n; ++i) {
+    double V = L->vol[i];
+    for (int c = 0; c < nc; ++c) {
+      size_t k = (size_t)c * n + i;
+      if (L->gnum == 1 && P->time_scheme == 2)
+        L->rhs[k] = L->rhs[k] + (1.5 * V * L->u[k] - 2.0 * V * L->uold[k] + 0.5 * V * L->uold2[k]) / dtp;
+      else
+        L->rhs[k] = L->rhs[k] + V * (L->u[k] - L->uold[k]) / dtp;
+    }
+  }
+}
+
+/* solveSystem (DEF:1407-1479); unsteady: dt <- dt / (1 + 1.5 dt / dtp)
+ * for the mean flow (DEF:1427-1436), the SA time step unchanged */
 static void solve_system(or_ctx *C, orl *L, double *r) {
   int n = L->n;
+  double tdc = C->P.flow_type != 0 ? 1.5 : 0.0;
   for (int i = 0; i < n; ++i) {
     double dt = L->dt[i];
+    if (C->P.flow_type != 0) dt = dt / (1.0 + tdc * dt / C->P.physical_time_step);
     for (int c = 0; c < 5; ++c) r[(size_t)c * n + i] = dt * r[(size_t)c * n + i] / L->vol[i];
   }
   if (C->P.turbulence_model == 1)
@@ -1449,6 +1476,7 @@ static int single_iteration(or_ctx *C, orl *L, int calc_diss, int calc_turb, int
     pressure(C, L, L->u);
     memset(L->rhs, 0, sizeof(double) * 6 * (size_t)n);
     if ((rc = make_rhs(C, L))) return rc;
+    time_differences(C, L);
     if (sa) {
       sa_rhs(C, L);
       sa_source(C, L);
@@ -1683,8 +1711,22 @@ int or_cycle(or_ctx *C, int cycle) {
   return 0;
 }
 
-/* setUpInitialField2 freestream start (DEF:3940-3996) and the
- * multiGridCycles preamble (MG:586-602) */
+/* multiGridCycles preamble (MG:586-602): laminar viscosity, and u and
+ * uold restricted to every coarse level */
+static void preamble(or_ctx *C) {
+  double mu0 = C->P.reynolds_number > 0.0 ? 1. / C->P.reynolds_number : 0.0;
+  for (int l = 0; l < C->nl; ++l) {
+    orl *L = &C->L[l];
+    for (int i = 0; i < L->n; ++i) L->mu[i] = mu0;
+    if (l > 0) {
+      map_f2c(&C->L[l - 1], L, C->L[l - 1].u, L->u, 6);
+      map_f2c(&C->L[l - 1], L, C->L[l - 1].uold, L->uold, 6);
+    }
+  }
+}
+
+/* setUpInitialField2 freestream start (DEF:3940-3996): uold and uold2 take
+ * the freestream before the boundary conditions act on u (DEF:3980-3994) */
 int or_init_freestream(or_ctx *C) {
   orl *L = &C->L[0];
   const flite_params *P = &C->P;
@@ -1693,19 +1735,31 @@ int or_init_freestream(or_ctx *C) {
     for (int i = 0; i < n; ++i) U(L, c, i) = P->inflow_field[0][c];
   if (P->turbulence_model == 1)
     for (int i = 0; i < n; ++i) U(L, 5, i) = 0.1;
+  memcpy(L->uold, L->u, sizeof(double) * 6 * (size_t)n);
+  memcpy(L->uold2, L->u, sizeof(double) * 6 * (size_t)n);
   pressure(C, L, L->u);
   pressure(C, L, L->u);
   if ((rc = bc_solution(C, L, L->u))) return rc;
-  return or_init_from_state(C);
+  preamble(C);
+  return 0;
 }
 
+/* a start from a given level-1 state (restart, DEF:3866-3873: uold and
+ * uold2 take it too) followed by the preamble */
 int or_init_from_state(or_ctx *C) {
-  double mu0 = C->P.reynolds_number > 0.0 ? 1. / C->P.reynolds_number : 0.0;
-  for (int l = 0; l < C->nl; ++l) {
-    orl *L = &C->L[l];
-    for (int i = 0; i < L->n; ++i) L->mu[i] = mu0;
-    if (l > 0) map_f2c(&C->L[l - 1], L, C->L[l - 1].u, L->u, 6);
-  }
+  orl *L = &C->L[0];
+  memcpy(L->uold, L->u, sizeof(double) * 6 * (size_t)L->n);
+  memcpy(L->uold2, L->u, sizeof(double) * 6 * (size_t)L->n);
+  preamble(C);
+  return 0;
+}
+
+/* end of a physical time step (MG:889-890): grid 1 only */
+int or_advance_physical_time(or_ctx *C) {
+  if (!C || !C->L[0].uploaded) return fail(FLITE_E_STATE, "level 1 not uploaded");
+  orl *L = &C->L[0];
+  memcpy(L->uold2, L->uold, sizeof(double) * 6 * (size_t)L->n);
+  memcpy(L->uold, L->u, sizeof(double) * 6 * (size_t)L->n);
   return 0;
 }
 

@@ -42,6 +42,8 @@ int or_restrict(or_ctx *c, int fine_level, int calc_turb);
 int or_prolongate(or_ctx *c, int coarse_level);
 int or_map_u_fine_to_coarse(or_ctx *c, int fine_level);
 int or_cycle(or_ctx *c, int cycle_number);
+/* end of a physical time step: uold2 <- uold <- u on level 1 (MG:889-890) */
+int or_advance_physical_time(or_ctx *c);
 int or_monitor(or_ctx *c, double res[7], double maxres[7], int maxidx[7],
                double forces[9]);
 

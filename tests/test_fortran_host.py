@@ -159,7 +159,7 @@ def test_driver_reads_inputs(tmp_path):
 
 @pytest.mark.parametrize("extra,msg", [
     (",\n ivd%movingwall = .true.", "moving walls"),
-    (",\n ivd%flowType = 1", "steady flow"),
+    (",\n ivd%flowType = 2", "fixed mesh"),
 ])
 def test_driver_rejects_options_outside_the_path(tmp_path, extra, msg):
     lv = levels(8, 2)
@@ -219,6 +219,45 @@ def test_driver_matches_gpu_and_oracle(tmp_path):
     assert abs(drag - mo["forces"][1]) <= 1e-4 * abs(mo["forces"][1]) + 1e-12
     ld = open(os.path.join(d, "liftdrag.res")).read().split()
     assert len(ld) == 7
+
+
+@pytest.mark.gpu
+def test_driver_unsteady_matches_python_path(tmp_path):
+    """flowType 1: the driver loops over the physical steps around the
+    multigrid loop and advances uold between them (MG:486-890); its .res_1
+    equals the same steps driven from Python, and liftdrag.res has one line
+    per step at t = step * dt_p."""
+    print("[case] fortran driver unsteady euler 8^3, 3 steps x 2 cycles", flush=True)
+    from flite3d_amd.device import Solver
+    from cases import euler
+    lv = levels(8, 2)
+    d = str(tmp_path)
+    extra = (",\n ivd%flowType = 1,\n ivd%physicalTimeStep = 0.05,\n ivd%numberOfPhysicalTimesteps = 3")
+    write_case(d, lv, iters=2, re=0.0, turb=0, order=1, mach=0.5, alpha=2.0, cfl=2.0, restart=1, extra=extra)
+    u0 = np.zeros((5, lv[0].n_nodes))
+    inf = euler(1, machnumber=0.5, alpha=2.0).inflowfield[0]
+    for c in range(5):
+        u0[c] = inf[c]
+    solfile.write_results(os.path.join(d, "bump.rst_1"), u0, 0)
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u_f = solfile.read_results(os.path.join(d, "bump.res_1"), 0)
+    iv = euler(1, machnumber=0.5, alpha=2.0, cflnumber=2.0, flowtype=1, physicaltimestep=0.05,
+               wallsareisentropic=True)
+    s = Solver(iv.to_struct(), list(lv))
+    full = np.zeros((6, lv[0].n_nodes))
+    full[:5] = solfile.read_results(os.path.join(d, "bump.rst_1"), 0)
+    s.set(1, "U", full)
+    s.init_from_state()
+    for step in range(3):
+        for c in (1, 2):
+            s.cycle(c)
+        s.advance_physical_time()
+    u_s = s.get(1, "U")[:5]
+    s.close()
+    assert np.allclose(u_f, u_s, rtol=1e-14, atol=0), np.abs(u_f - u_s).max()
+    lines = [x.split() for x in open(os.path.join(d, "liftdrag.res")).read().split("\n") if x.strip()]
+    assert [float(x[0]) for x in lines] == pytest.approx([0.05, 0.10, 0.15])
 
 
 @pytest.mark.gpu

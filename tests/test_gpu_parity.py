@@ -334,3 +334,78 @@ def test_five_stage_runge_kutta_cycles():
         s3.cycle(c)
     assert rel_err(s3.get(1, "U")[:5], o.get(1, "U")[:5]).max() > 1e-6
     s.close(); o.close(); s3.close()
+
+
+# ---- dual time stepping on a fixed mesh (flowType 1) ----
+@pytest.mark.parametrize("case", ["euler_bdf2", "euler_bdf1", "rans_bdf2"])
+def test_unsteady_time_iterations(case):
+    print(f"[case] unsteady time iterations {case}", flush=True)
+    scheme = 1 if case.endswith("bdf1") else 2
+    if case.startswith("rans"):
+        lv = levels(8, 3, first_cell=0.005)
+        iv = rans(flowtype=1, physicaltimestep=0.05, timescheme=scheme)
+        nc = 6
+    else:
+        lv = levels(8, 3)
+        iv = euler(1, flowtype=1, physicaltimestep=0.05, timescheme=scheme)
+        nc = 5
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    for step in range(2):
+        for x in (s, o):
+            x.do_time_iterations(1, 2, True)
+            x.do_time_iterations(2, 1, True)
+        for l in (1, 2):
+            e = rel_err(s.get(l, "U")[:nc], o.get(l, "U")[:nc])
+            assert e.max() < 1e-11, (step, l, e)
+        s.advance_physical_time(); o.advance_physical_time()
+
+
+@pytest.mark.parametrize("case,scheme", [("euler", 3), ("rans", 1)])
+def test_unsteady_cycles(case, scheme):
+    """Two physical steps of three V-cycles each: BDF2 on level 1, BDF1 with
+    the restricted initial uold on the coarse levels, the modified local
+    time step in the smoother and in the restriction."""
+    print(f"[case] unsteady cycles {case} scheme {scheme}", flush=True)
+    if case == "rans":
+        lv = levels(8, 3, first_cell=0.005)
+        iv = rans(flowtype=1, physicaltimestep=0.05, multigridscheme=scheme)
+        nc = 6
+    else:
+        lv = levels(8, 3)
+        iv = euler(2, flowtype=1, physicaltimestep=0.05, multigridscheme=scheme)
+        nc = 5
+    s, o = pair(iv, lv)
+    s.init_freestream(); o.init_freestream()
+    c = 0
+    for step in range(2):
+        for _ in range(3):
+            c += 1
+            s.cycle(c); o.cycle(c)
+            ms, mo = s.monitor(), o.monitor()
+            assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (c, ms["res"], mo["res"])
+            assert np.allclose(ms["forces"], mo["forces"], rtol=1e-8, atol=1e-14), (c, ms["forces"], mo["forces"])
+        s.advance_physical_time(); o.advance_physical_time()
+    e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
+    assert e.max() < 1e-9, e
+    # the time term acts: the same cycles of a steady run end elsewhere
+    st = Solver(euler(2, multigridscheme=scheme).to_struct() if case == "euler" else
+                rans(multigridscheme=scheme).to_struct(), list(lv))
+    st.init_freestream()
+    for k in range(1, 7):
+        st.cycle(k)
+    assert rel_err(st.get(1, "U")[:nc], o.get(1, "U")[:nc]).max() > 1e-6
+    s.close(); o.close(); st.close()
+
+
+def test_unsteady_rejects_moving_meshes():
+    from flite3d_amd.device import DeviceError
+    lv = levels(8, 2)
+    P = euler(1).to_struct()
+    P.flow_type = 2
+    with pytest.raises(DeviceError):
+        Solver(P, list(lv))
+    s = Solver(euler(1).to_struct(), list(lv))
+    with pytest.raises(DeviceError):
+        s.advance_physical_time()  # steady run: no physical time
+    s.close()
