@@ -2,8 +2,9 @@
 with torch.autograd.gradcheck (central differences of the forward against
 the backward), on the GPU through the library:
 
-  Residual: U -> R(U), the level-1 first-order inviscid residual
-            (FLITE_OP_RHS); backward = flite_adjoint_apply.
+  Residual: U -> R(U), the level-1 inviscid residual (FLITE_OP_RHS),
+            first order after a few cycles and second order (MUSCL + LED)
+            about a smooth non-uniform state; backward = flite_adjoint_apply.
   ShapeObjective: X -> drag(U, X) - lam^T R(U, X) at a fixed state U and
             vector lam, with the metrics rebuilt by the preprocessor from X;
             backward = flite_metric_adjoint + flite_drag_metric_gradient,
@@ -24,8 +25,8 @@ from cases import euler
 pytestmark = pytest.mark.gpu
 
 
-def params():
-    return euler(1, machnumber=0.8, alpha=1.25, cflnumber=2.0, multigridscheme=3).to_struct()
+def params(order=1):
+    return euler(order, machnumber=0.8, alpha=1.25, cflnumber=2.0, multigridscheme=3).to_struct()
 
 
 class Residual(torch.autograd.Function):
@@ -88,6 +89,21 @@ def test_residual_function_gradcheck(small):
     _, lv, u = small
     s = Solver(params(), lv)
     x = torch.from_numpy(u[:5].copy()).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda t: Residual.apply(t, s), (x,), eps=1e-6, atol=1e-7, rtol=1e-5)
+    s.close()
+
+
+def test_second_order_residual_function_gradcheck(small):
+    """HLLC-2 with LED clipping, about a smooth state with no equal
+    neighbour values (a limiter switch within eps would make the central
+    differences straddle a kink)."""
+    print("[case] autograd second-order residual gradcheck n=3", flush=True)
+    from test_adjoint import smooth_state
+    _, lv, u = small
+    s = Solver(params(2), lv)
+    s.init_freestream()
+    u2 = smooth_state(lv, s.get(1, "U"))
+    x = torch.from_numpy(u2[:5].copy()).requires_grad_(True)
     assert torch.autograd.gradcheck(lambda t: Residual.apply(t, s), (x,), eps=1e-6, atol=1e-7, rtol=1e-5)
     s.close()
 
