@@ -42,6 +42,7 @@ def prepro():
                                       P(i), P(i), P(i), P(i), P(d)]
         lib.fp_fine_old_index.argtypes = [ctypes.c_void_p, P(i)]
         lib.fp_metric_vjp.argtypes = [i, P(d), i, P(i), i, P(i), P(i), P(d), P(d), P(d)]
+        lib.fp_mesh_vjp.argtypes = [i, P(d), i, P(i), i, P(i), P(i), P(d), P(d), P(d), P(d), P(d)]
         lib.fp_set_progress.argtypes = [ctypes.c_void_p]
         lib.fp_set_progress.restype = None
         _prepro = lib

@@ -585,7 +585,8 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
 // and coordinates X1, X2; first order ignores them.
 // T is the arithmetic type: the level's R for the residual, a dual number
 // for its tangent and adjoint; with T a dual number the second-order branch
-// takes the gradient records as dual numbers too (TG = T).
+// takes the gradient records (TG) and the coordinates (TX) as dual numbers
+// too when their derivatives are wanted.
 // upwinded SA convection of one side (makeSARHS, DEF:6873-6880) from the
 // ends' gather records
 template <typename T>
@@ -602,8 +603,8 @@ __device__ __forceinline__ T sa_side_convection(const T *A, const T *B, T wx, T 
   return fsa;
 }
 
-template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R>
-__device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const R *X1, const R *X2,
+template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R, typename TX = R>
+__device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const TX *X1, const TX *X2,
                                           const T *A, const T *B, T wx, T wy, T wz, T *o) {
   T gam = T(P.gamma), gm1 = gam - T(1);
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
@@ -1214,7 +1215,7 @@ __global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du
   }
   state_record(Ua, P.gamma, A);
   state_record(Ub, P.gamma, B);
-  side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
+  side_flux<Dual, double, 1, false, 2, double, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
                                        Dual(L.w[3 * e + 1]), Dual(L.w[3 * e + 2]), o);
   double *f = L.fbuf + 6 * (size_t)e;
 #pragma unroll
@@ -1284,7 +1285,7 @@ __global__ void k_flux_adjoint(LevelDev<double> L, DevParams P, const double *la
     }
     state_record(Ua, P.gamma, A);
     state_record(Ub, P.gamma, B);
-    side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, wx, wy, wz, o);
+    side_flux<Dual, double, 1, false, 2, double, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, wx, wy, wz, o);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
@@ -1384,7 +1385,7 @@ __global__ void k_flux_metric_adjoint(LevelDev<double> L, DevParams P, const dou
 #pragma unroll 1
   for (int k = 0; k < 3; ++k) {
     Dual o[6];
-    side_flux<Dual, double, 1, false, 2, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
+    side_flux<Dual, double, 1, false, 2, double, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
                                          Dual(w1, k == 1 ? 1.0 : 0.0), Dual(w2, k == 2 ? 1.0 : 0.0), o);
     double acc = 0;
 #pragma unroll
@@ -1723,6 +1724,101 @@ __global__ void k_record_adjoint(LevelDev<double> L, DevParams P, const double *
     out[k * n + i] = acc;
   }
   out[5 * n + i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh adjoint of the second-order residual: d(lam^T R) with respect to the
+// side weights, boundary-side weights, dual volumes and coordinates.  The
+// side flux reads w directly, the coordinates through dx = X_b - X_a (the
+// reconstruction and its clip length), and the Green-Gauss gradients G,
+// which hold (q_a + q_b) w_s / V (sides, node-signed w_s) and
+// (3 q_i + q_o) w_b / V (boundary sides).  gbn = d(lam^T R)/d(G, B) per node
+// from the state adjoint's side shares (k_flux_adjoint2, k_adjoint_gather2).
+// ---------------------------------------------------------------------------
+// per side: gw[3 e + k] (direct and through both ends' gradients) and
+// gdx[3 e + k] = d(lam^T R)/d dx_k
+template <int NSIG>
+__global__ void k_flux_mesh_adjoint2(LevelDev<double> L, DevParams P, const double *lam, const double *gbn,
+                                     double *gw, double *gdx) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  size_t n = L.n;
+  int ia = L.ea[e], ib = L.eb[e];
+  Dual Ua[5], Ub[5], A[8], B[8], GA[LIN_G], GB[LIN_G];
+  double d[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib]);
+    d[c] = lam[c * n + ia] - lam[c * n + ib];
+  }
+  state_record(Ua, P.gamma, A);
+  state_record(Ub, P.gamma, B);
+  const double *ga = L.grec + GREC * (size_t)ia, *gb = L.grec + GREC * (size_t)ib;
+#pragma unroll
+  for (int c = 0; c < LIN_G; ++c) {
+    GA[c] = Dual(ga[c]);
+    GB[c] = Dual(gb[c]);
+  }
+  const double w0 = L.w[3 * e], w1 = L.w[3 * e + 1], w2 = L.w[3 * e + 2];
+  const double *xa = L.xyz + 3 * (size_t)ia, *xb = L.xyz + 3 * (size_t)ib;
+#pragma unroll 1
+  for (int k = 0; k < 6; ++k) {
+    Dual XA[3] = {Dual(xa[0]), Dual(xa[1]), Dual(xa[2])};
+    Dual XB[3] = {Dual(xb[0], k == 3 ? 1.0 : 0.0), Dual(xb[1], k == 4 ? 1.0 : 0.0), Dual(xb[2], k == 5 ? 1.0 : 0.0)};
+    Dual o[6];
+    side_flux<Dual, double, 2, false, NSIG, Dual, Dual>(P, GA, GB, XA, XB, A, B, Dual(w0, k == 0 ? 1.0 : 0.0),
+                                                        Dual(w1, k == 1 ? 1.0 : 0.0), Dual(w2, k == 2 ? 1.0 : 0.0), o);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
+    if (k < 3) {
+      // the gradients: G_a holds (q_a + q_b) w / V_a, G_b holds -(q_a + q_b) w / V_b
+      const double *qa = L.rec0 + 8 * (size_t)ia, *qb = L.rec0 + 8 * (size_t)ib;
+      const double *sa = gbn + LIN_G * (size_t)ia + 5 * k, *sb = gbn + LIN_G * (size_t)ib + 5 * k;
+      const double iva = 1.0 / L.vol[ia], ivb = 1.0 / L.vol[ib];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double s = qa[c] + qb[c];
+        acc += sa[c] * s * iva - sb[c] * s * ivb;
+      }
+      gw[3 * (size_t)e + k] = acc;
+    } else {
+      gdx[3 * (size_t)e + k - 3] = acc;
+    }
+  }
+}
+
+// per boundary side: adds the gradients' share to gwb (the direct share of
+// the first-order boundary flux is k_bflux_metric_adjoint's): G_a holds
+// (3 q_a + q_b) w_b / V_a and G_b holds (3 q_b + q_a) w_b / V_b
+__global__ void k_bgrad_mesh_adjoint(LevelDev<double> L, const double *gbn, double *gwb) {
+  int s = gtid_x();
+  if (s >= L.nbs) return;
+  int ia = L.ba[s], ib = L.bb[s];
+  const double *qa = L.rec0 + 8 * (size_t)ia, *qb = L.rec0 + 8 * (size_t)ib;
+  const double iva = 1.0 / L.vol[ia], ivb = 1.0 / L.vol[ib];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double *sa = gbn + LIN_G * (size_t)ia + 5 * k, *sb = gbn + LIN_G * (size_t)ib + 5 * k;
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      acc += sa[c] * (3.0 * qa[c] + qb[c]) * iva + sb[c] * (3.0 * qb[c] + qa[c]) * ivb;
+    gwb[3 * (size_t)s + k] += acc;
+  }
+}
+
+// per node: d(lam^T R)/dV_i = -sum gbn_i . G_i / V_i (G is a sum over V)
+__global__ void k_vol_mesh_adjoint(LevelDev<double> L, const double *gbn, double *gvol) {
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const double *g = L.grec + GREC * (size_t)i, *s = gbn + LIN_G * (size_t)i;
+  double acc = 0;
+#pragma unroll
+  for (int c = 0; c < 15; ++c) acc += s[c] * g[c];
+  gvol[i] = -acc / L.vol[i];
 }
 
 // partial derivative of the pressure drag (calculateLiftAndDrag,

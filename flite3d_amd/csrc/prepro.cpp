@@ -343,6 +343,31 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
 // median-dual coefficients are sums of cross products of differences of
 // vertex, edge-midpoint, face-centroid and tet-centroid positions; the
 // transpose of G.(u x v) is (v x G) for u and (G x u) for v.
+// transpose of the dual volumes: each vertex of a positively oriented tet
+// holds a quarter of its volume v = e1.(e2 x e3)/6, e_k = x_k - x_0, whose
+// derivatives are (e2 x e3)/6, (e3 x e1)/6, (e1 x e2)/6 and minus their sum
+void volume_vjp(const fp_mesh &m, const double *gvol, double *gx) {
+  const double *X = m.lev[0].xyz.data();
+  for (const auto &q : m.tets) {
+    double e1[3], e2[3], e3[3], d1[3], d2[3], d3[3];
+    for (int c = 0; c < 3; ++c) {
+      e1[c] = X[3 * q[1] + c] - X[3 * q[0] + c];
+      e2[c] = X[3 * q[2] + c] - X[3 * q[0] + c];
+      e3[c] = X[3 * q[3] + c] - X[3 * q[0] + c];
+    }
+    cross(e2, e3, d1);
+    cross(e3, e1, d2);
+    cross(e1, e2, d3);
+    double gv = 0.25 * (gvol[q[0]] + gvol[q[1]] + gvol[q[2]] + gvol[q[3]]) / 6.0;
+    for (int c = 0; c < 3; ++c) {
+      gx[3 * size_t(q[1]) + c] += gv * d1[c];
+      gx[3 * size_t(q[2]) + c] += gv * d2[c];
+      gx[3 * size_t(q[3]) + c] += gv * d3[c];
+      gx[3 * size_t(q[0]) + c] -= gv * (d1[c] + d2[c] + d3[c]);
+    }
+  }
+}
+
 void metric_vjp(const fp_mesh &m, const double *gw, const double *gwb, double *gx) {
   const Level &L = m.lev[0];
   const double *X = L.xyz.data();
@@ -751,6 +776,29 @@ int fp_metric_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
     build_fine(m, n_nodes, xyz, n_tets, tets, n_btri, btri, btri_ind);
     std::vector<double> g(3 * size_t(n_nodes), 0.0);
     metric_vjp(m, gw, gwb, g.data());
+    for (int i = 0; i < n_nodes; ++i)
+      for (int c = 0; c < 3; ++c) gx[3 * size_t(m.old_index[i]) + c] = g[3 * size_t(i) + c];
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int fp_mesh_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
+                int n_btri, const int *btri, const int *btri_ind,
+                const double *gw, const double *gwb, const double *gvol, const double *gxn, double *gx) {
+  try {
+    if (n_nodes <= 0 || n_tets <= 0 || n_btri <= 0) throw std::runtime_error("bad sizes");
+    if (!gw || !gwb || !gx) throw std::runtime_error("null array");
+    fp_mesh m;
+    m.keep_cells = true;
+    build_fine(m, n_nodes, xyz, n_tets, tets, n_btri, btri, btri_ind);
+    std::vector<double> g(3 * size_t(n_nodes), 0.0);
+    metric_vjp(m, gw, gwb, g.data());
+    if (gvol) volume_vjp(m, gvol, g.data());
+    if (gxn)
+      for (size_t k = 0; k < g.size(); ++k) g[k] += gxn[k];
     for (int i = 0; i < n_nodes; ++i)
       for (int c = 0; c < 3; ++c) gx[3 * size_t(m.old_index[i]) + c] = g[3 * size_t(i) + c];
     return 0;

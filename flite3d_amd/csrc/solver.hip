@@ -2028,12 +2028,93 @@ int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw
   return 0;
 }
 
+int flite_mesh_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb,
+                       double *gvol, double *gxyz, int ld_x) {
+  TRY(check_linearisable(c));
+  Level &L = *c->lev[0];
+  if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs || !gvol || !gxyz || ld_x < L.n)
+    return fail(FLITE_E_ARG, "bad output arrays");
+  const int n = L.n, ne = L.ne, nbs = L.nbs;
+  if (!second_order(c, L)) {
+    // the first-order residual depends on the side weights only
+    TRY(flite_metric_adjoint(c, lam, ld_lam, gw, ld_w, gwb, ld_wb));
+    for (int i = 0; i < n; ++i) gvol[i] = 0.0;
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < n; ++i) gxyz[(size_t)k * ld_x + i] = 0.0;
+    return 0;
+  }
+  // records, gradients and bounds of the state; the state adjoint's side
+  // shares give d(lam^T R)/d(G, B) per node (gbn); then the side, boundary
+  // side and volume shares
+  TRY(op_prep(c, L, true));
+  TRY(op_grad(c, L));
+  TRY(alloc_lin(L));
+  if (!L.abuf2) TRY(L.pool.alloc(&L.abuf2, (size_t)LIN_SIDE * std::max(1, ne)));
+  if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, ne)));
+  TRY(put_vector5(c, lam, ld_lam));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  double *ub = L.lin, *gbn = L.lin + 5 * (size_t)n, *gv = L.lin + 30 * (size_t)n;
+  double *dgw = L.abuf, *dgdx = L.abuf + 3 * (size_t)ne;
+  if (ne > 0) {
+    if (P.nsig == 4)
+      hipLaunchKernelGGL(k_flux_adjoint2<4>, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf2);
+    else
+      hipLaunchKernelGGL(k_flux_adjoint2<2>, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf2);
+  }
+  if (nbs > 0) hipLaunchKernelGGL(k_bflux_adjoint, dim3(nblk(nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
+  hipLaunchKernelGGL(k_adjoint_gather2, dim3(nblk(n)), dim3(NTHR), 0, c->stream, V, (const double *)L.abuf2, ub, gbn);
+  if (ne > 0) {
+    if (P.nsig == 4)
+      hipLaunchKernelGGL(k_flux_mesh_adjoint2<4>, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, V.src,
+                         (const double *)gbn, dgw, dgdx);
+    else
+      hipLaunchKernelGGL(k_flux_mesh_adjoint2<2>, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, V.src,
+                         (const double *)gbn, dgw, dgdx);
+  }
+  if (nbs > 0) {
+    hipLaunchKernelGGL(k_bflux_metric_adjoint, dim3(nblk(nbs)), dim3(NTHR), 0, c->stream, V, P, V.src, V.bbuf);
+    hipLaunchKernelGGL(k_bgrad_mesh_adjoint, dim3(nblk(nbs)), dim3(NTHR), 0, c->stream, V, (const double *)gbn,
+                       V.bbuf);
+  }
+  hipLaunchKernelGGL(k_vol_mesh_adjoint, dim3(nblk(n)), dim3(NTHR), 0, c->stream, V, (const double *)gbn, gv);
+  TRY(check_launch());
+  std::vector<double> hw(3 * (size_t)ne), hdx(3 * (size_t)ne), hb(3 * (size_t)nbs), hv(n);
+  std::vector<int> ea(ne), eb(ne);
+  if (ne > 0) {
+    HIPTRY(hipMemcpyAsync(hw.data(), dgw, hw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(hdx.data(), dgdx, hdx.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(ea.data(), L.ea, ne * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(eb.data(), L.eb, ne * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (nbs > 0) HIPTRY(hipMemcpyAsync(hb.data(), V.bbuf, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipMemcpyAsync(hv.data(), gv, hv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ne; ++k)
+    for (int q = 0; q < 3; ++q) gw[(size_t)q * ld_w + L.side_host[k]] = hw[3 * (size_t)k + q];
+  for (int s = 0; s < nbs; ++s)
+    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] = hb[3 * (size_t)s + q];
+  // dx = X_b - X_a: its share goes to +b and -a (device numbering)
+  std::vector<double> gx(3 * (size_t)n, 0.0);
+  for (int k = 0; k < ne; ++k)
+    for (int q = 0; q < 3; ++q) {
+      gx[3 * (size_t)eb[k] + q] += hdx[3 * (size_t)k + q];
+      gx[3 * (size_t)ea[k] + q] -= hdx[3 * (size_t)k + q];
+    }
+  for (int hh = 0; hh < n; ++hh) {
+    const int d = L.h2d[hh];
+    gvol[hh] = hv[d];
+    for (int q = 0; q < 3; ++q) gxyz[(size_t)q * ld_x + hh] = gx[3 * (size_t)d + q];
+  }
+  return 0;
+}
+
 // pressure drag of calculateLiftAndDrag (DEF:14757-14803) as a function of
 // the boundary-side weights at a fixed state: the wall node areas
 // A_i = sum 4|w_b| and normals n_i = m_i/|m_i|, m_i = -sum w_b, over the
 // boundary sides at the node (as build_level forms warea and nrm)
 int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_wb) {
-  TRY(check_linearisable(c, false));
+  TRY(check_linearisable(c));  // the drag itself does not depend on the residual's order
   Level &L = *c->lev[0];
   const Staged &S = c->staged[0];
   if (!gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output array");

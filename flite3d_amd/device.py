@@ -53,6 +53,7 @@ def lib():
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
+        L.flite_mesh_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i, P(d), P(d), i]
         L.flite_set_coarse_precision.argtypes = [vp, i]
         L.flite_autotune.argtypes = [vp, i, P(i)]
         L.flite_level_lanes.argtypes = [vp, i, P(i)]
@@ -233,6 +234,23 @@ class Solver:
         self._check(self.L.flite_metric_adjoint(self.ctx, x.ctypes.data_as(D), n, gw.ctypes.data_as(D), ne,
                                                 gwb.ctypes.data_as(D), nbs))
         return gw, gwb
+
+    def mesh_adjoint(self, lam):
+        """d(lam^T R) with respect to side_w (3, E), bside_w (3, Eb), the
+        dual volumes (N,) and the coordinates directly (3, N), at the
+        current state (first- or second-order HLLC)."""
+        n = self.n(1)
+        ne, nbs = self._sizes1()
+        x = self._vec5(lam)
+        gw = np.zeros((3, ne))
+        gwb = np.zeros((3, nbs))
+        gv = np.zeros(n)
+        gx = np.zeros((3, n))
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_mesh_adjoint(self.ctx, x.ctypes.data_as(D), n, gw.ctypes.data_as(D), ne,
+                                              gwb.ctypes.data_as(D), nbs, gv.ctypes.data_as(D),
+                                              gx.ctypes.data_as(D), n))
+        return gw, gwb, gv, gx
 
     def drag_metric_gradient(self):
         """(pressure drag, dDrag/d bside_w (3, Eb)) at the current state."""

@@ -279,6 +279,35 @@ def metric_vjp(xyz, tets, btri, bind, gw, gwb):
     return gx
 
 
+def mesh_vjp(xyz, tets, btri, bind, gw, gwb, gvol=None, gxn=None):
+    """fp_mesh_vjp: as metric_vjp plus the dual volumes' and a direct
+    coordinate term's transposes.  gvol is (n_nodes,) and gxn (3, n_nodes)
+    in the level's node numbering (as the library reports them); returns an
+    (n_nodes, 3) array in the input node order."""
+    lib = _native.prepro()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    btri = np.ascontiguousarray(btri, dtype=np.int32)
+    bind = np.ascontiguousarray(bind, dtype=np.int32)
+    gw = np.ascontiguousarray(gw, dtype=np.float64)
+    gwb = np.ascontiguousarray(gwb, dtype=np.float64)
+    D = ctypes.POINTER(ctypes.c_double)
+    gv = np.ascontiguousarray(gvol, dtype=np.float64) if gvol is not None else None
+    gn = np.ascontiguousarray(np.asarray(gxn, dtype=np.float64).T) if gxn is not None else None
+    gx = np.zeros((xyz.shape[0], 3))
+    rc = lib.fp_mesh_vjp(xyz.shape[0], _ptr(xyz, ctypes.c_double),
+                         tets.shape[0], _ptr(tets, ctypes.c_int),
+                         btri.shape[0], _ptr(btri, ctypes.c_int),
+                         _ptr(bind, ctypes.c_int),
+                         _ptr(gw, ctypes.c_double), _ptr(gwb, ctypes.c_double),
+                         _ptr(gv, ctypes.c_double) if gv is not None else ctypes.cast(None, D),
+                         _ptr(gn, ctypes.c_double) if gn is not None else ctypes.cast(None, D),
+                         _ptr(gx, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("mesh_vjp failed: " + lib.fp_last_error().decode())
+    return gx
+
+
 def bump_levels(n, n_levels, jitter=0.0, seed=0, first_cell=None,
                 ratio=1.15, h=0.05, wall_indicator=1, progress=None):
     xyz, tets, btri, bind = bump_box(n, jitter=jitter, seed=seed,

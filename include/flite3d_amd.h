@@ -244,6 +244,19 @@ int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld
  * coordinates (fp_metric_vjp in flite3d_prepro.h). */
 int flite_metric_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb);
 int flite_drag_metric_gradient(flite_ctx *ctx, double *drag, double *gwb, int ld_wb);
+/* The same for first- and second-order HLLC (no counterpart in the
+ * reference; for shape-sensitivity callers): d(lam^T R) with respect to
+ * every mesh quantity the level-1 residual reads,
+ *   gw, gwb   side and boundary-side weights (layout as flite_metric_adjoint),
+ *   gvol[N]   dual volumes (host numbering),
+ *   gxyz      coordinates directly (N,3) column-major, ld_x >= N.
+ * Second order reads the volumes and the weights through the Green-Gauss
+ * gradients and the coordinates through the reconstruction's side vector
+ * (x_b - x_a) and clip length; first order gives gvol = gxyz = 0.  The
+ * coordinate gradient of J is then fp_mesh_vjp's chain of
+ * (-gw, gJwb - gwb, -gvol, -gxyz) (flite3d_prepro.h). */
+int flite_mesh_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb,
+                       double *gvol, double *gxyz, int ld_x);
 
 /* Precision of the coarse levels from the next call on: 1 = FP32, 0 = FP64
  * (flite_params.coarse_fp32 sets the initial one).  No counterpart in the

@@ -93,6 +93,17 @@ int fp_fine_old_index(const fp_mesh *m, int *old_index);
 int fp_metric_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
                   int n_btri, const int *btri, const int *btri_ind,
                   const double *gw, const double *gwb, double *gx);
+/* The same chain with the dual volumes and a direct coordinate term (no
+ * counterpart in the reference):
+ *   gx = (d side_w / d xyz)^T gw + (d bside_w / d xyz)^T gwb
+ *      + (d volume / d xyz)^T gvol + gxn
+ * with gvol [n_nodes] and gxn [n_nodes*3, row-major] in the build's node
+ * numbering (the level's, as uploaded); either may be NULL.  The median-dual
+ * volume of a node is a quarter of each positively oriented tet's volume
+ * at it. */
+int fp_mesh_vjp(int n_nodes, const double *xyz, int n_tets, const int *tets,
+                int n_btri, const int *btri, const int *btri_ind,
+                const double *gw, const double *gwb, const double *gvol, const double *gxn, double *gx);
 
 #ifdef __cplusplus
 }

@@ -66,3 +66,35 @@ def test_vjp_rejects_bad_input():
         assert "out of range" in str(e)
     else:
         raise AssertionError("accepted a tet with an out-of-range node")
+
+
+def volumes(xyz, tets, btri, bind):
+    lv, _ = mesh.build_levels(xyz, tets, btri, bind, 1)
+    return lv[0].volume
+
+
+def test_volume_vjp():
+    """fp_mesh_vjp's dual-volume part: the volumes are cubic in the
+    coordinates, so central differences with a small step, translation
+    invariance and homogeneity of degree three (sum_i x_i . g_i =
+    3 <g_V, V>) pin it; the direct coordinate term passes through in the
+    input node order."""
+    (xyz, tets, btri, bind), L0 = case()
+    rng = np.random.default_rng(4)
+    gv = rng.standard_normal(L0.n_nodes)
+    zw, zb = np.zeros_like(L0.side_w), np.zeros_like(L0.bside_w)
+    gx = mesh.mesh_vjp(xyz, tets, btri, bind, zw, zb, gvol=gv)
+    d = rng.standard_normal(xyz.shape)
+    h = 1e-5
+    fd = (gv @ volumes(xyz + h * d, tets, btri, bind) - gv @ volumes(xyz - h * d, tets, btri, bind)) / (2 * h)
+    an = float(np.sum(gx * d))
+    assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
+    assert np.abs(gx.sum(axis=0)).max() <= 1e-11 * np.abs(gx).max()
+    assert abs(float(np.sum(xyz * gx)) - 3 * float(gv @ L0.volume)) <= 1e-10 * abs(float(gv @ L0.volume))
+    # the direct term: level numbering in, input numbering out
+    _, old = mesh.build_levels(xyz, tets, btri, bind, 1)
+    gn = rng.standard_normal((3, L0.n_nodes))
+    gx2 = mesh.mesh_vjp(xyz, tets, btri, bind, zw, zb, gxn=gn)
+    ref = np.zeros_like(xyz)
+    ref[np.asarray(old)] = gn.T
+    assert np.array_equal(gx2, ref)
