@@ -224,7 +224,7 @@ def smooth_state(coords, u):
     return out
 
 
-def second_order_checks(lv, cycles, ndir, h, smooth=False):
+def second_order_checks(lv, cycles, ndir, h, smooth=False, raw=None, nxdir=0, hx=1e-6):
     """Second-order HLLC (MUSCL + LED) linearisation at the config-3
     conditions, about that scheme's own state after `cycles` cycles (or a
     smooth non-uniform state): the dot-product identity and the tangent
@@ -234,9 +234,14 @@ def second_order_checks(lv, cycles, ndir, h, smooth=False):
     max |J du|.  Where neighbouring values agree to far less than the step
     (the converged field away from the bump), the LED bounds
     1/(q_min - q_i) vary on that smaller scale and no difference quotient
-    with this step resolves the derivative."""
-    from flite3d_amd import device
-    s = device.Solver(euler_params(0.8, 1.25, order=2, scheme=3), lv)
+    with this step resolves the derivative.
+    With raw and nxdir > 0 also the coordinate gradient of drag - lam^T R
+    (flite_mesh_adjoint, the drag metric gradient, fp_mesh_vjp) against
+    central differences with the metrics rebuilt from X +- hx dX, moving
+    the interior and wall nodes."""
+    from flite3d_amd import device, mesh
+    P2 = euler_params(0.8, 1.25, order=2, scheme=3)
+    s = device.Solver(P2, lv)
     s.init_freestream()
     drop = None
     if smooth:
@@ -286,8 +291,41 @@ def second_order_checks(lv, cycles, ndir, h, smooth=False):
         central += int(c_ok.sum())
         one_sided += int((~c_ok & s_ok).sum())
         neither += int((~c_ok & ~s_ok).sum())
+    x_out = {}
+    if raw is not None and nxdir > 0:
+        lam = rng.standard_normal((5, n))
+        s.set(1, "U", u)
+        t0 = time.perf_counter()
+        gw, gwb, gv, gxd = s.mesh_adjoint(lam)
+        _, gjwb = s.drag_metric_gradient()
+        xyz, tets, btri, bind = raw
+        gx = mesh.mesh_vjp(xyz, tets, btri, bind, -gw, gjwb - gwb, gvol=-gv, gxn=-gxd)
+        t_shape = time.perf_counter() - t0
+        movable = np.ones(xyz.shape[0], bool)
+        movable[np.unique(btri[bind != 1])] = False
+
+        def objective(x):
+            lvx, _ = mesh.build_levels(x, tets, btri, bind, 2)
+            sx = device.Solver(P2, lvx)
+            sx.set(1, "U", u)
+            sx.apply(1, "PRESSURE")
+            sx.apply(1, "RHS")
+            r = sx.get(1, "RHS")[:5]
+            dr, _ = sx.drag_gradient()
+            sx.close()
+            return dr - float(np.sum(lam * r))
+
+        x_err = []
+        for _ in range(nxdir):
+            dx = rng.standard_normal(xyz.shape) * movable[:, None]
+            dx /= np.linalg.norm(dx)
+            fd = (objective(xyz + hx * dx) - objective(xyz - hx * dx)) / (2 * hx)
+            an = float(np.sum(gx * dx))
+            x_err.append(abs(fd - an) / max(abs(an), 1e-300))
+        x_out = {"coordinate_gradient_directions": nxdir, "coordinate_fd_step": hx,
+                 "coordinate_gradient_vs_fd_max_rel_err": max(x_err), "s_per_coordinate_gradient": round(t_shape, 3)}
     s.close()
-    return {"order": 2, "state": "smooth" if smooth else f"{cycles} cycles",
+    return {**x_out, "order": 2, "state": "smooth" if smooth else f"{cycles} cycles",
             "residual_drop_log10": drop, "directions": ndir,
             "dot_product_max_rel_err": max(dot_err), "fd_step": h,
             "entries_equal_to_central_fd": central, "entries_equal_to_one_sided_fd": one_sided,
@@ -408,7 +446,8 @@ def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6, ndir2=4):
             "note": "apply times include host<->device transfer of the vectors",
             "drag": drag0, "nodes": lv[0].n_nodes, "sides": lv[0].n_sides,
             "second_order": second_order_checks(lv, cycles, ndir2, h),
-            "second_order_smooth_state": second_order_checks(lv, cycles, ndir2, h, smooth=True)}
+            "second_order_smooth_state": second_order_checks(lv, cycles, ndir2, h, smooth=True, raw=raw,
+                                                             nxdir=nxdir, hx=hx)}
 
 
 def bench_config4(nbatch, steps, warmup):
