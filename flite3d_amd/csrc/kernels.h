@@ -463,8 +463,9 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 // computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459), gathered
 // ---------------------------------------------------------------------------
+// the record of node i into out[26]; false on the lanes that do not own it
 template <typename R, int SUB, typename Walk>
-__device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub, Walk walk) {
+__device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, int i, int sub, Walk walk, R *out) {
   const R *qi = L.rec0 + 8 * (size_t)i;
   R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
   R g[15];
@@ -505,10 +506,9 @@ __device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub,
       mn[c] = lmin<SUB>(mn[c]);
       mx[c] = lmin<SUB>(mx[c]);
     }
-    if (sub) return;
+    if (sub) return false;
   }
   R vt = R(1) / L.vol[i];
-  R *out = L.grec + GREC * (size_t)i;
 #pragma unroll
   for (int c = 0; c < 15; ++c) out[c] = g[c] * vt;
 #pragma unroll
@@ -519,6 +519,51 @@ __device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub,
     out[20 + c] = hi > R(0) ? R(1) / hi : R(0);
   }
   out[25] = 0;
+  return true;
+}
+
+template <typename R, int SUB, typename Walk>
+__device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub, Walk walk) {
+  R v[GREC];
+  if (!grad_minmax_vals<R, SUB>(L, i, sub, walk, v)) return;
+  R *out = L.grec + GREC * (size_t)i;
+#pragma unroll
+  for (int c = 0; c < GREC; ++c) out[c] = v[c];
+}
+
+// one node per thread, the block's records stored through LDS in two
+// halves of 13 words: a thread's record lands in LDS at a 13-word stride
+// (conflict-free for 64-bit words), then consecutive lanes store
+// consecutive words of the block's contiguous record range, instead of
+// every store instruction touching one cache line per lane
+template <typename R>
+__global__ void __launch_bounds__(NTHR) k_grad_minmax_st(LevelDev<R> L, DevParams P) {
+  __shared__ R st[NTHR * (GREC / 2)];
+  int bb_;
+  int i = btid(L.nb, bb_);
+  L.to_batch(bb_);
+  P.to_batch(bb_);
+  const int i0 = i - (int)threadIdx.x;
+  R v[GREC];
+  bool ok = i < L.n && grad_minmax_vals<R, 1>(L, i, 0, [&](auto f) { walk_global<1>(L, i, 0, f); }, v);
+  const int cnt = L.n - i0 < NTHR ? L.n - i0 : NTHR;
+  R *base = L.grec + GREC * (size_t)i0;
+  constexpr int HW = GREC / 2;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < HW; ++c) st[threadIdx.x * HW + c] = v[h * HW + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < HW; ++k) {
+      int idx = k * NTHR + (int)threadIdx.x;
+      int node = idx / HW, cc = idx - node * HW;
+      if (node < cnt) base[GREC * (size_t)node + h * HW + cc] = st[idx];
+    }
+    __syncthreads();
+  }
 }
 
 template <typename R, int SUB>

@@ -220,6 +220,7 @@ struct flite_ctx {
   int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
   bool flux_per_side = false;  // FLITE_FLUX_KERNEL=side: one lane per side with direct record loads
   bool gather_per_node = false;  // FLITE_GATHER_KERNEL=node: no LDS-staged node blocks
+  bool grad_direct = false;      // FLITE_GRAD_STORE=direct: per-thread record stores in the gradient
   // One cycle's launches form the same sequence with the same arguments
   // every cycle, so flite_cycle captures them once into a HIP graph and
   // replays it (FLITE_CYCLE_GRAPH=0: launch them one by one).  Anything that
@@ -941,6 +942,10 @@ int op_grad(flite_ctx *c, Level &L) {
   Timed tm(c, "grad", L.gnum);
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
+    if (L.lanes == 1 && !c->grad_direct) {
+      hipLaunchKernelGGL(k_grad_minmax_st<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
+      return check_launch();
+    }
     return with_lanes(L.lanes, [&](auto sc) {
       constexpr int SUB = decltype(sc)::value;
       hipLaunchKernelGGL((k_grad_minmax<R, SUB>), dim3(nblk((long long)L.n * SUB) * L.nbatch), dim3(NTHR), 0, c->stream, V, P);
@@ -1480,6 +1485,11 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
     std::string v(e);
     if (v != "0" && v != "1") return fail(FLITE_E_ARG, "FLITE_CYCLE_GRAPH must be 0 or 1");
     c->graph_off = v == "0";
+  }
+  if (const char *e = std::getenv("FLITE_GRAD_STORE")) {
+    std::string v(e);
+    if (v != "direct" && v != "lds") return fail(FLITE_E_ARG, "FLITE_GRAD_STORE must be direct or lds");
+    c->grad_direct = v == "direct";
   }
   if (const char *e = std::getenv("FLITE_GATHER_KERNEL")) {
     std::string v(e);
