@@ -69,3 +69,38 @@ def test_batch_rejects_differing_settings():
     ps = [euler(1).to_struct(), euler(2).to_struct()]
     with pytest.raises(DeviceError):
         Solver(ps, list(lv))
+
+
+@pytest.mark.parametrize("case", ["roe_jst", "unsteady"])
+def test_jst_and_unsteady_batch_equals_single_runs(case):
+    """The JST node records and the physical-time states (uold, uold2) are
+    per condition too: each condition of the batch equals its single run."""
+    print(f"[case] batch {case}", flush=True)
+    lv = levels(8, 3)
+    conds = [(0.5, 2.0), (0.7, -2.0)]
+    if case == "roe_jst":
+        kw = dict(fluxtype=3, usematrixdissipation=True, secondorderdissipationfactor=0.5,
+                  fourthorderdissipationfactor=1 / 64.0, cflnumber=1.5)
+    else:
+        kw = dict(flowtype=1, physicaltimestep=0.05)
+    ps = [euler(1, machnumber=m, alpha=a, multigridscheme=3, **kw).to_struct() for m, a in conds]
+
+    def go(params):
+        s = Solver(params, list(lv))
+        s.init_freestream()
+        nb = len(params) if isinstance(params, list) else 1
+        for c in range(1, 5):
+            s.cycle(c)
+            if case == "unsteady" and c % 2 == 0:
+                s.advance_physical_time()
+        us = []
+        for b in range(nb):
+            s.select(b)
+            us.append(s.get(1, "U"))
+        s.close()
+        return us
+
+    ub = go(ps)
+    for b, p in enumerate(ps):
+        assert np.array_equal(ub[b], go(p)[0]), b
+    assert not np.array_equal(ub[0], ub[1])
