@@ -15,8 +15,11 @@
  *   - HLLC consistency F(U,U,n) = F(U).n, supersonic upwinding, mirror
  *     symmetry; the farfield Roe flux against the textbook Roe solver;
  *   - Green-Gauss exactness for linear fields at interior nodes;
- *   - LocalMinMax, time steps, pressure and SA source by brute force on
- *     small meshes (independent numpy restatements);
+ *   - LocalMinMax, time steps, pressure and the SA time steps by brute force
+ *     on small meshes (independent numpy restatements);
+ *   - SA source: the log-layer balance the model's cw1 is defined by, the
+ *     production and trip term away from walls, zero for nu~ = 0; SA
+ *     diffusion exact for linear nu~ (tests/test_oracle_sa.py);
  *   - viscous residual zero at interior nodes for linear velocity and
  *     temperature with constant viscosity;
  *   - MG: restriction conserves sum V u, injection prolongation, a
