@@ -199,3 +199,40 @@ def test_growing_v_fixed_point():
     u0 = o.get(1, "U")
     o.cycle(401)
     assert np.abs(o.get(1, "U") - u0)[:5].max() < 1e-11
+
+
+def test_boundary_weighting_exact_for_linear_flux():
+    """The boundary loop's 3:1 weighting (boundaryTerm 1, DEF:2233-2243):
+    with the central interior flux (fluxType 1, no dissipation) and every
+    boundary side of wall type (physical flux), a state whose fluxes are
+    linear in position (density linear, velocity and pressure constant)
+    has the residual V div F exactly at every node, the boundary nodes
+    included; a 4:0 or 2:2 boundary weighting would not."""
+    lv = levels(6, 1, jitter=0.15)
+    L0 = lv[0]
+    bi = L0.bside_index.copy()
+    bi[2] = 1
+    L = type(L0)(**{**L0.__dict__, "bside_index": bi})
+    iv = euler(1, fluxtype=1, secondorderdissipationfactor=0.0, fourthorderdissipationfactor=0.0)
+    o = make(iv, [L])
+    o.init_freestream()
+    b = np.array([0.3, -0.2, 0.4])
+    vel = np.array([0.5, 0.1, -0.2])
+    p0 = 1.0
+    x = L.coords
+    rho = 1.0 + b @ x
+    u = np.zeros((6, L.n_nodes))
+    u[0] = rho
+    u[1:4] = rho * vel[:, None]
+    u[4] = p0 / 0.4 + 0.5 * rho * (vel @ vel)
+    o.set(1, "U", u)
+    o.apply(1, "PRESSURE")
+    o.apply(1, "RHS")
+    r = o.get(1, "RHS")[:5]
+    # div F: d(rho u_j)/dx_j = vel.b, momentum vel_k (vel.b), energy (|vel|^2 / 2)(vel.b)
+    vb = vel @ b
+    ref = np.vstack([np.full(L.n_nodes, vb), vel[:, None] * vb * np.ones(L.n_nodes),
+                     np.full(L.n_nodes, 0.5 * (vel @ vel) * vb)]) * L.volume
+    err = np.abs(r - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+    o.close()
