@@ -1866,6 +1866,374 @@ __global__ void k_vol_mesh_adjoint(LevelDev<double> L, const double *gbn, double
   gvol[i] = -acc / L.vol[i];
 }
 
+// ---------------------------------------------------------------------------
+// Tangent and adjoint of the laminar viscous term (makeViscosityTerm,
+// DEF:5198-5893; dissipation(:,2:5) = -A).  A node's record a = (u, v, w, T)
+// with T = gamma/(gamma-1) p/rho; its Green-Gauss sums g = sum (a_i + a_j)
+// w + sum_b (3 a_i + a_o) w_b; the stress and heat flux from g/V, the
+// Sutherland viscosity of max(T, 0.001) and the adiabatic-wall projection;
+// A = sum over sides of E(tau_i, tau_j, u_i, u_j; w) and over non-symmetry
+// boundary sides of B(tau_i, tau_o, u_i, u_o; w_b).  The functions below
+// follow ViscNodePart / ViscFluxPart operation for operation, on any
+// arithmetic type.
+// ---------------------------------------------------------------------------
+constexpr int VLIN = 56;  // scratch doubles per node of the viscous linearisation
+
+template <typename T>
+__device__ __forceinline__ void visc_tau(const DevParams &P, const T *g, double vol, T Traw, bool wall,
+                                         const double *nrm, T *t) {
+  T vt = T(1.0) / T(vol);
+  T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
+  T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
+  T dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
+  T Tn = Traw;
+  if (Tn <= T(0.001)) Tn = T(0.001);
+  T tc = T((P.gamma - 1.0) * P.mach * P.mach), Tin = T(P.inflow_temperature);
+  T tct = tc * Tn;
+  T mu = T(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + T(198.6)) / (tc * Tin * Tn + T(198.6))));
+  T k = T(1.0 / P.Pr) * mu;
+  T two3 = T(2.0 / 3.0), four3 = T(4.0 / 3.0);
+  t[0] = mu * (four3 * dudx - two3 * (dvdy + dwdz));
+  t[1] = mu * (dudy + dvdx);
+  t[2] = mu * (dudz + dwdx);
+  t[3] = mu * (four3 * dvdy - two3 * (dudx + dwdz));
+  t[4] = mu * (dvdz + dwdy);
+  t[5] = mu * (four3 * dwdz - two3 * (dudx + dvdy));
+  t[6] = k * dTdx;
+  t[7] = k * dTdy;
+  t[8] = k * dTdz;
+  if (wall) {
+    T nx = T(nrm[0]), ny = T(nrm[1]), nz = T(nrm[2]);
+    T nt = t[6] * nx + t[7] * ny + t[8] * nz;
+    t[6] -= nt * nx;
+    t[7] -= nt * ny;
+    t[8] -= nt * nz;
+  }
+}
+
+template <typename T, typename W>
+__device__ __forceinline__ void visc_edge(const T *t1, const T *tj, const T *q1, const T *q2, W wx_, W wy_, W wz_,
+                                          T *a) {
+  T wx = T(wx_), wy = T(wy_), wz = T(wz_);
+  T u1 = q1[0], v1 = q1[1], w1 = q1[2], u2 = q2[0], v2 = q2[1], w2 = q2[2];
+  T d[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) d[c] = t1[c] + tj[c];
+  T dut1 = u1 * t1[0] + u2 * tj[0] + v1 * t1[1] + v2 * tj[1] + w1 * t1[2] + w2 * tj[2];
+  T dut2 = u1 * t1[1] + u2 * tj[1] + v1 * t1[3] + v2 * tj[3] + w1 * t1[4] + w2 * tj[4];
+  T dut3 = u1 * t1[2] + u2 * tj[2] + v1 * t1[4] + v2 * tj[4] + w1 * t1[5] + w2 * tj[5];
+  a[0] += wx * d[0] + wy * d[1] + wz * d[2];
+  a[1] += wx * d[1] + wy * d[3] + wz * d[4];
+  a[2] += wx * d[2] + wy * d[4] + wz * d[5];
+  a[3] += wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3);
+}
+
+template <typename T, typename W>
+__device__ __forceinline__ void visc_bside(const T *t1, const T *to, const T *q1, const T *qo, W wx_, W wy_, W wz_,
+                                           T *a) {
+  T wx = T(wx_), wy = T(wy_), wz = T(wz_);
+  T u1 = q1[0], v1 = q1[1], w1 = q1[2], u2 = qo[0], v2 = qo[1], w2 = qo[2];
+  T d[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) d[c] = t1[c] + to[c];
+  T du11 = u1 * t1[0] + v1 * t1[1] + w1 * t1[2], du12 = u2 * to[0] + v2 * to[1] + w2 * to[2];
+  T du21 = u1 * t1[1] + v1 * t1[3] + w1 * t1[4], du22 = u2 * to[1] + v2 * to[3] + w2 * to[4];
+  T du31 = u1 * t1[2] + v1 * t1[4] + w1 * t1[5], du32 = u2 * to[2] + v2 * to[4] + w2 * to[5];
+  T dut1 = du11 + du12, dut2 = du21 + du22, dut3 = du31 + du32;
+  a[0] += (wx * d[0] + wy * d[1] + wz * d[2]) + T(2) * (wx * t1[0] + wy * t1[1] + wz * t1[2]);
+  a[1] += (wx * d[1] + wy * d[3] + wz * d[4]) + T(2) * (wx * t1[1] + wy * t1[3] + wz * t1[4]);
+  a[2] += (wx * d[2] + wy * d[4] + wz * d[5]) + T(2) * (wx * t1[2] + wy * t1[4] + wz * t1[5]);
+  a[3] += (wx * (d[6] + dut1) + wy * (d[7] + dut2) + wz * (d[8] + dut3)) +
+          T(2) * (wx * (t1[6] + du11) + wy * (t1[7] + du21) + wz * (t1[8] + du31));
+}
+
+// node record (u, v, w, T) as dual numbers from the gather record and its
+// tangent dq (null: no tangent)
+__device__ __forceinline__ void visc_rec(const LevelDev<double> &L, const DevParams &P, const double *dq, int i,
+                                         ad::Dual *a) {
+  using ad::Dual;
+  const double *q = L.rec0 + 8 * (size_t)i;
+  const double *t = dq ? dq + 6 * (size_t)i : nullptr;
+  Dual rho(q[0], t ? t[0] : 0.0), p(q[5], t ? t[5] : 0.0);
+  a[0] = Dual(q[1], t ? t[1] : 0.0);
+  a[1] = Dual(q[2], t ? t[2] : 0.0);
+  a[2] = Dual(q[3], t ? t[3] : 0.0);
+  a[3] = Dual(P.gamma / (P.gamma - 1.0)) * p / rho;
+}
+
+__device__ __forceinline__ bool visc_wall(const LevelDev<double> &L, int i) {
+  return L.gnum == 1 && (L.ncls[i] & NC_MASK) == NC_WALL;
+}
+
+// Green-Gauss sums g[12] of the node records (dual; tangents from dq)
+__device__ __forceinline__ void visc_grad_sums(const LevelDev<double> &L, const DevParams &P, const double *dq, int i,
+                                               const ad::Dual *ai, ad::Dual *g) {
+  using ad::Dual;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) g[c] = Dual(0.0);
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+    Dual aj[4];
+    visc_rec(L, P, dq, j, aj);
+    const double ww[3] = {wx, wy, wz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[4 * d + q] += (ai[q] + aj[q]) * Dual(ww[d]);
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    Dual ao[4];
+    visc_rec(L, P, dq, o, ao);
+    const double ww[3] = {L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[4 * d + q] += (Dual(2.0) * ai[q] + (ai[q] + ao[q])) * Dual(ww[d]);
+  }
+}
+
+// tangent pass 1: the stress and heat-flux record of each node on dual
+// numbers, taud[18 i + 2 c] = value, [18 i + 2 c + 1] = tangent (dq null:
+// values only, for the adjoint)
+__global__ void k_visc_tangent_node(LevelDev<double> L, DevParams P, const double *dq, ad::Dual *taud) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  Dual ai[4], g[12], t[9];
+  visc_rec(L, P, dq, i, ai);
+  visc_grad_sums(L, P, dq, i, ai, g);
+  visc_tau(P, g, L.vol[i], ai[3], visc_wall(L, i), L.nrm + 3 * (size_t)i, t);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) taud[9 * (size_t)i + c] = t[c];
+}
+
+// tangent pass 2: dvis(c, i) = -dA (c = 1..4), 0 for c = 0
+__global__ void k_visc_tangent_flux(LevelDev<double> L, DevParams P, const double *dq, const ad::Dual *taud,
+                                    double *dvis) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  Dual t1[9], q1[4], a[4] = {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0)};
+#pragma unroll
+  for (int c = 0; c < 9; ++c) t1[c] = taud[9 * (size_t)i + c];
+  visc_rec(L, P, dq, i, q1);
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+    Dual tj[9], qj[4];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) tj[c] = taud[9 * (size_t)j + c];
+    visc_rec(L, P, dq, j, qj);
+    visc_edge(t1, tj, q1, qj, wx, wy, wz, a);
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int ind = L.bind[s];
+    if (ind == 2 || ind == 9) continue;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    Dual to[9], qo[4];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) to[c] = taud[9 * (size_t)o + c];
+    visc_rec(L, P, dq, o, qo);
+    visc_bside(t1, to, q1, qo, L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2], a);
+  }
+  dvis[i] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) dvis[(c + 1) * n + i] = -a[c].d;
+}
+
+// adjoint pass 1: shares of lam^T(-A) on node i's stress record (taub 9)
+// and velocities through the energy flux (ub 3), from its own terms and its
+// neighbours' terms; tau values in taud (pass 1 without tangents)
+__global__ void k_visc_adjoint_tau(LevelDev<double> L, DevParams P, const double *lam, const ad::Dual *taud,
+                                   double *taub, double *ubv) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  double ti[9], qi[3], mi[4];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) ti[c] = taud[9 * (size_t)i + c].v;
+  {
+    const double *q = L.rec0 + 8 * (size_t)i;
+    qi[0] = q[1];
+    qi[1] = q[2];
+    qi[2] = q[3];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) mi[c] = -lam[(c + 1) * n + i];
+  double tb[9], ub[3];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) tb[c] = 0;
+  ub[0] = ub[1] = ub[2] = 0;
+  // one term's shares on node i's inputs: term = m . F(self first or second)
+  auto shares = [&](const double *to, const double *qo, double wx, double wy, double wz, const double *m,
+                    bool self_first, bool bside) {
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) {
+      Dual ts[9], tt[9], qs[3], qt[3], a[4] = {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0)};
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        ts[c] = Dual(ti[c], k == c ? 1.0 : 0.0);
+        tt[c] = Dual(to[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        qs[c] = Dual(qi[c], k == 9 + c ? 1.0 : 0.0);
+        qt[c] = Dual(qo[c]);
+      }
+      if (self_first) {
+        if (bside) visc_bside(ts, tt, qs, qt, wx, wy, wz, a);
+        else visc_edge(ts, tt, qs, qt, wx, wy, wz, a);
+      } else {
+        if (bside) visc_bside(tt, ts, qt, qs, wx, wy, wz, a);
+        else visc_edge(tt, ts, qt, qs, wx, wy, wz, a);
+      }
+      double acc = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc += m[c] * a[c].d;
+      if (k < 9) tb[k] += acc;
+      else ub[k - 9] += acc;
+    }
+  };
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+    double tj[9], qj[3], mj[4];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) tj[c] = taud[9 * (size_t)j + c].v;
+    const double *q = L.rec0 + 8 * (size_t)j;
+    qj[0] = q[1];
+    qj[1] = q[2];
+    qj[2] = q[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) mj[c] = -lam[(c + 1) * n + j];
+    shares(tj, qj, wx, wy, wz, mi, true, false);     // node i's term, weight w_ij
+    shares(tj, qj, -wx, -wy, -wz, mj, false, false);  // node j's term, weight w_ji = -w_ij
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int ind = L.bind[s];
+    if (ind == 2 || ind == 9) continue;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    double to[9], qo[3], mo[4];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) to[c] = taud[9 * (size_t)o + c].v;
+    const double *q = L.rec0 + 8 * (size_t)o;
+    qo[0] = q[1];
+    qo[1] = q[2];
+    qo[2] = q[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) mo[c] = -lam[(c + 1) * n + o];
+    const double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    shares(to, qo, wx, wy, wz, mi, true, true);   // node i's boundary term
+    shares(to, qo, wx, wy, wz, mo, false, true);  // the partner's boundary term
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) taub[9 * (size_t)i + c] = tb[c];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ubv[3 * (size_t)i + c] = ub[c];
+}
+
+// adjoint pass 2: the stress law transposed at each node, gbar[12] on the
+// Green-Gauss sums and tbar on T (through the viscosity)
+__global__ void k_visc_adjoint_node(LevelDev<double> L, DevParams P, const double *taub, double *gbar,
+                                    double *tbar) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  Dual ai[4], g[12];
+  visc_rec(L, P, nullptr, i, ai);
+  visc_grad_sums(L, P, nullptr, i, ai, g);
+  const double *tb = taub + 9 * (size_t)i;
+  const bool wall = visc_wall(L, i);
+#pragma unroll 1
+  for (int k = 0; k < 13; ++k) {
+    Dual gs[12], t[9];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) gs[c] = Dual(g[c].v, k == c ? 1.0 : 0.0);
+    Dual T(ai[3].v, k == 12 ? 1.0 : 0.0);
+    visc_tau(P, gs, L.vol[i], T, wall, L.nrm + 3 * (size_t)i, t);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc += tb[c] * t[c].d;
+    if (k < 12) gbar[12 * (size_t)i + k] = acc;
+    else tbar[i] = acc;
+  }
+}
+
+// adjoint pass 3: the Green-Gauss sums transposed, abar[4 j + q] on node j's
+// record (u, v, w, T), plus its direct shares (T through the viscosity, the
+// velocities through the energy flux)
+__global__ void k_visc_adjoint_grad(LevelDev<double> L, const double *gbar, const double *tbar, const double *ubv,
+                                    double *abar) {
+  int j = gtid_x();
+  if (j >= L.n) return;
+  const double *gj = gbar + 12 * (size_t)j;
+  double r[4] = {ubv[3 * (size_t)j], ubv[3 * (size_t)j + 1], ubv[3 * (size_t)j + 2], tbar[j]};
+  double sw[3] = {0, 0, 0};
+  for_edges_w<1>(L, j, 0, [&](int i, double wx, double wy, double wz) {
+    sw[0] += wx;
+    sw[1] += wy;
+    sw[2] += wz;
+    const double *gi = gbar + 12 * (size_t)i;
+    const double ww[3] = {wx, wy, wz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[q] -= gi[4 * d + q] * ww[d];
+  });
+  double sb[3] = {0, 0, 0};
+  for (int k = L.bs_ptr[j]; k < L.bs_ptr[j + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const double ww[3] = {L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2]};
+    const double *go = gbar + 12 * (size_t)o;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      sb[d] += ww[d];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[q] += go[4 * d + q] * ww[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double s = sw[d] + 3.0 * sb[d];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] += gj[4 * d + q] * s;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) abar[4 * (size_t)j + q] = r[q];
+}
+
+// adjoint pass 4: out(c, i) += (da/dU)^T abar (the record's dependence on
+// the conserved state)
+__global__ void k_visc_adjoint_record(LevelDev<double> L, DevParams P, const double *abar, double *out) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  double u[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) u[c] = L.u[c * n + i];
+  const double *ab = abar + 4 * (size_t)i;
+#pragma unroll 1
+  for (int k = 0; k < 5; ++k) {
+    Dual U[5], A[8];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) U[c] = Dual(u[c], k == c ? 1.0 : 0.0);
+    state_record(U, P.gamma, A);
+    Dual T = Dual(P.gamma / (P.gamma - 1.0)) * A[5] / A[0];
+    out[k * n + i] += ab[0] * A[1].d + ab[1] * A[2].d + ab[2] * A[3].d + ab[3] * T.d;
+  }
+}
+
+// tangent: out += dvis
+__global__ void k_add_vector5(double *out, const double *add, int n) {
+  int i = gtid_x();
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) out[(size_t)c * n + i] += add[(size_t)c * n + i];
+}
+
 // partial derivative of the pressure drag (calculateLiftAndDrag,
 // DEF:14757-14803: drag = (2/S) sum_wall -A_i (p_i - p_inf) n_i.t) with
 // respect to the conserved state of each node; zero off the wall

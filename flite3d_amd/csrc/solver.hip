@@ -184,6 +184,7 @@ struct Level {
   // shares, per-node scratch, extremal neighbours of the limiter bounds
   double *abuf2 = nullptr, *lin = nullptr;
   int *amin = nullptr;
+  double *vlin = nullptr;  // laminar viscous linearisation scratch (allocated on first use)
   std::vector<int> side_host;  // device side -> uploaded side index
   int *m_idx = nullptr;
 
@@ -1865,7 +1866,7 @@ int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], 
 
 // ---- tangent / adjoint of the level-1 residual ---------------------------
 namespace {
-int check_linearisable(flite_ctx *c, bool second_order_ok = true) {
+int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_ok = false) {
   TRY(check_ctx(c));
   const Level &L = *c->lev[0];
   const flite_params &hp = c->hp;
@@ -1874,9 +1875,54 @@ int check_linearisable(flite_ctx *c, bool second_order_ok = true) {
   if (hp.flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: HLLC (fluxType=4) only");
   if (!second_order_ok && L.has_coords && hp.hllc_flux_order > 1)
     return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
-  if (hp.reynolds_number > 0.0 || hp.turbulence_model != 0)
-    return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: inviscid flow (ReynoldsNumber=0) only");
+  if (hp.turbulence_model != 0)
+    return fail(FLITE_E_UNSUPPORTED, "linearisation: no turbulence model (the Spalart-Allmaras terms are not linearised)");
+  if (!laminar_ok && hp.reynolds_number > 0.0)
+    return fail(FLITE_E_UNSUPPORTED, "metric and drag derivatives: inviscid flow (ReynoldsNumber=0) only");
   return 0;
+}
+
+bool laminar(const flite_ctx *c) { return c->hp.reynolds_number > 0.0; }
+
+// laminar viscous linearisation scratch: VLIN doubles per node
+int alloc_vlin(Level &L) {
+  if (!L.vlin) TRY(L.pool.alloc(&L.vlin, (size_t)VLIN * std::max(1, L.n)));
+  return 0;
+}
+
+// tangent of the viscous term along the direction in V.src, added to V.rhs
+int visc_tangent(flite_ctx *c, Level &L) {
+  TRY(alloc_vlin(L));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  const size_t n = L.n;
+  double *dq = L.vlin, *dvis = L.vlin + 24 * n;
+  ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin + 6 * n);
+  hipLaunchKernelGGL(k_record_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, dq);
+  hipLaunchKernelGGL(k_visc_tangent_node, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq, taud);
+  hipLaunchKernelGGL(k_visc_tangent_flux, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
+                     (const ad::Dual *)taud, dvis);
+  hipLaunchKernelGGL(k_add_vector5, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V.rhs, (const double *)dvis, L.n);
+  return check_launch();
+}
+
+// adjoint of the viscous term against the multipliers in V.src, added to V.rhs
+int visc_adjoint(flite_ctx *c, Level &L) {
+  TRY(alloc_vlin(L));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  const size_t n = L.n;
+  ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin);
+  double *taub = L.vlin + 18 * n, *ubv = L.vlin + 27 * n, *gbar = L.vlin + 30 * n, *tbar = L.vlin + 42 * n,
+         *abar = L.vlin + 43 * n;
+  dim3 g(nblk(L.n)), b(NTHR);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, taud);
+  hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)taud, taub, ubv);
+  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, (const double *)taub, gbar, tbar);
+  hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)gbar, (const double *)tbar,
+                     (const double *)ubv, abar);
+  hipLaunchKernelGGL(k_visc_adjoint_record, g, b, 0, c->stream, V, P, (const double *)abar, V.rhs);
+  return check_launch();
 }
 
 // host (N, 5) column-major -> level-1 device array src (6 components)
@@ -1914,7 +1960,7 @@ int alloc_lin(Level &L) {
 }  // namespace
 
 int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
-  TRY(check_linearisable(c));
+  TRY(check_linearisable(c, true, true));
   Level &L = *c->lev[0];
   TRY(put_vector5(c, du, ld_du));
   DevParams P = c->P;
@@ -1945,11 +1991,15 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
   TRY(check_launch());
   StageArgs S{};
   TRY(op_stage(c, L, MODE_RHS, S));
+  if (laminar(c)) {
+    if (!second_order(c, L)) TRY(op_prep(c, L, true));
+    TRY(visc_tangent(c, L));
+  }
   return get_vector5(c, out, ld_out);
 }
 
 int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
-  TRY(check_linearisable(c));
+  TRY(check_linearisable(c, true, true));
   Level &L = *c->lev[0];
   if (second_order(c, L)) {
     // records, gradients and bounds of the current state, the bounds'
@@ -1976,6 +2026,7 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
     hipLaunchKernelGGL(k_record_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)ub,
                        (const double *)qb, V.rhs);
     TRY(check_launch());
+    if (laminar(c)) TRY(visc_adjoint(c, L));
     return get_vector5(c, out, ld_out);
   }
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
@@ -1986,6 +2037,10 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
   if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_adjoint, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
   hipLaunchKernelGGL(k_adjoint_gather, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, L.abuf, V.rhs);
   TRY(check_launch());
+  if (laminar(c)) {
+    TRY(op_prep(c, L, true));
+    TRY(visc_adjoint(c, L));
+  }
   return get_vector5(c, out, ld_out);
 }
 

@@ -207,8 +207,10 @@ int flite_apply(flite_ctx *ctx, int level, int op);
  * 1525-2512: interior HLLC sides and boundary sides, as FLITE_OP_RHS
  * computes it) differentiated exactly at the current state, for first- and
  * second-order HLLC (MUSCL reconstruction with the Green-Gauss gradients and
- * LED clipping) and inviscid flow (viscous or SA settings return
- * FLITE_E_UNSUPPORTED).  The second-order residual is piecewise smooth: at
+ * LED clipping), inviscid or laminar: with ReynoldsNumber > 0 the residual
+ * includes the viscous term of makeViscosityTerm (mgnsg3d_def_par.F90:
+ * 5198-5893, the dissipation the smoother adds).  Spalart-Allmaras settings
+ * return FLITE_E_UNSUPPORTED.  The second-order residual is piecewise smooth: at
  * a limiter switch these give the derivative of the branch the state
  * selects.  Vectors are (N, 5) column-major in host numbering with leading
  * dimension ld.  The reference has no differentiated code; these entry
@@ -229,7 +231,7 @@ int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld
 /* Derivatives with respect to the level-1 mesh metrics, for gradients with
  * respect to the node coordinates (no counterpart in the reference; for
  * shape-sensitivity callers).  Same residual and restrictions as
- * flite_adjoint_apply, first-order HLLC only.  Outputs use the uploaded side order and layout,
+ * flite_adjoint_apply, first-order HLLC and inviscid flow only.  Outputs use the uploaded side order and layout,
  * column-major (E,3) like flite_level_view.side_weights and (Eb,3) like
  * bside_weights, with leading dimensions ld_w >= n_sides, ld_wb >= n_bsides:
  *   flite_metric_adjoint:  gw  = d(lam^T R)/d side_weights,

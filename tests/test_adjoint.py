@@ -273,6 +273,84 @@ def test_first_apply_on_a_fresh_context_equals_the_second(order):
         s.close()
 
 
+# ---- laminar flow: the viscous term linearised too ----
+def make_laminar(n, order):
+    """The smooth state on the freestream before the no-slip condition (the
+    wall nodes' zero velocity would tie neighbouring values)."""
+    from cases import laminar
+    lv = levels(n, 2, jitter=0.15)
+    iv = laminar(hllcfluxorder=order, reynoldsnumber=200.0, wallsareisentropic=True)
+    s = Solver(iv.to_struct(), list(lv))
+    s.init_freestream()
+    inf = np.array(iv.inflowfield[0])
+    base = np.zeros((6, lv[0].n_nodes))
+    base[:5] = inf[:5, None]
+    base[1:4] *= inf[0]
+    base[5] = s.get(1, "U")[5]
+    u = smooth_state(lv, base)
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    return s, lv, u
+
+
+def residual_laminar(s, u):
+    """makeRHS plus the viscous dissipation (dissipation(:,2:5) = -viscous)."""
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    s.apply(1, "RHS")
+    s.apply(1, "VISCOUS")
+    return s.get(1, "RHS")[:5] + s.get(1, "DISSIPATION")[:5]
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_laminar_tangent_matches_finite_differences(order):
+    """Every entry equals the central difference, or, at second order, a
+    one-sided one where a limiter switch lies within H; the viscous part
+    (Green-Gauss gradients of u, v, w, T, Sutherland viscosity, adiabatic
+    wall projection, edge and boundary fluxes) is smooth."""
+    print(f"[case] laminar tangent FD order={order}", flush=True)
+    s, lv, u = make_laminar(8, order)
+    rng = np.random.default_rng(31)
+    one_sided = 0
+    for du in directions(rng, u, 3):
+        jt = s.tangent(du)
+        r0 = residual_laminar(s, u)
+        up, um = u.copy(), u.copy()
+        up[:5] += H * du
+        um[:5] -= H * du
+        rp, rm = residual_laminar(s, up), residual_laminar(s, um)
+        s.set(1, "U", u)
+        tol = 1e-6 * np.abs(jt).max()
+        central = np.abs((rp - rm) / (2 * H) - jt) <= tol
+        fwd = np.abs((rp - r0) / H - jt) <= tol
+        bwd = np.abs((r0 - rm) / H - jt) <= tol
+        ok = central | fwd | bwd if order == 2 else central
+        assert np.all(ok), (np.argwhere(~ok)[:5], np.abs((rp - rm) / (2 * H) - jt).max() / np.abs(jt).max())
+        one_sided += int((~central).sum())
+    assert one_sided <= 0.001 * jt.size * 3, one_sided
+    # the viscous part is really there: the inviscid Jacobian differs
+    from cases import euler as _euler
+    si = Solver(_euler(order).to_struct(), list(lv))
+    si.set(1, "U", u)
+    si.init_from_state()
+    du = directions(rng, u, 1)[0]
+    assert np.abs(s.tangent(du) - si.tangent(du)).max() > 1e-2 * np.abs(s.tangent(du)).max()
+    si.close()
+    s.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_laminar_adjoint_dot_product(order):
+    print(f"[case] laminar adjoint dot product order={order}", flush=True)
+    s, lv, u = make_laminar(8, order)
+    rng = np.random.default_rng(32)
+    for du, lam in zip(directions(rng, u, 4), directions(rng, u, 4)):
+        a = np.sum(lam * s.tangent(du))
+        b = np.sum(s.adjoint(lam) * du)
+        assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)) + 1e-300, (a, b)
+    s.close()
+
+
 def test_linearisation_rejects_unsupported_settings():
     from flite3d_amd.device import DeviceError
     from cases import rans
