@@ -115,6 +115,9 @@ struct LevelDev {
   R *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;  // tau, dnu: VREC-strided, one shared record
   R *jst;  // JST node record (8 per node): L(U) 5, pressure switch, side count, pad
   R *uold, *uold2;  // solutions at the last two physical times (6 per node, unsteady only)
+  // moving meshes (null on a fixed mesh): GCL weight per side and 2 per
+  // boundary side, previous volumes and movement (3 per node AoS) on level 1
+  const R *gcl, *bgcl, *volp, *volp2, *cmv;
   // batched conditions: nb copies of every state array, condition-major;
   // gstride = gradient-record elements per condition (0 if none)
   int nb;
@@ -650,7 +653,7 @@ __device__ __forceinline__ T sa_side_convection(const T *A, const T *B, T wx, T 
 
 template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R, typename TX = R>
 __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const TX *X1, const TX *X2,
-                                          const T *A, const T *B, T wx, T wy, T wz, T *o) {
+                                          const T *A, const T *B, T wx, T wy, T wz, T *o, R wg = R(0)) {
   T gam = T(P.gamma), gm1 = gam - T(1);
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
   T roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
@@ -733,6 +736,13 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   if (!lft && !rgt) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = fc[c];
+    if (wg != R(0)) {  // moving mesh: the left state's GCL share (DEF:2081-2085)
+      o[0] -= A[0] * wg;
+      o[1] -= ru1 * wg;
+      o[2] -= rv1 * wg;
+      o[3] -= rw1 * wg;
+      o[4] -= A[4] * wg;
+    }
     o[5] = fsa;
     return;
   }
@@ -748,6 +758,11 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   f[3] = rnx * (rwk * uk) + rny * (rwk * vk) + rnz * (rwk * wk + pk);
   f[4] = rnx * (rhk * uk) + rny * (rhk * vk) + rnz * (rhk * wk);
   T fk = sk - sm;
+  // the state the flux is taken from (DEF:2081-2085 subtract it times the
+  // side's GCL weight / dt on a moving mesh): the star state, or U_K with
+  // the node's own density and energy and the reconstructed momentum (the
+  // reference's r1, E1 are not reconstructed, its ru1..rw1 are)
+  T q0 = lft ? A[0] : B[0], q1 = ruk, q2 = rvk, q3 = rwk, q4 = lft ? A[4] : B[4];
   if (star && fk != T(0)) {
     fk = frcp(fk);
     T a = sk - nvk;
@@ -761,9 +776,21 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
     f[2] += sk * (s2 - rvk);
     f[3] += sk * (s3 - rwk);
     f[4] += sk * (s4 - ek);
+    q0 = s0;
+    q1 = s1;
+    q2 = s2;
+    q3 = s3;
+    q4 = s4;
   }
 #pragma unroll
   for (int c = 0; c < 5; ++c) o[c] = f[c] * rnorm1;
+  if (wg != R(0)) {
+    o[0] -= q0 * wg;
+    o[1] -= q1 * wg;
+    o[2] -= q2 * wg;
+    o[3] -= q3 * wg;
+    o[4] -= q4 * wg;
+  }
   o[5] = fsa;
 }
 
@@ -788,7 +815,8 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
     X1 = L.xyz + 3 * (size_t)ia;
     X2 = L.xyz + 3 * (size_t)ib;
   }
-  side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, L.fbuf + 6 * (size_t)e);
+  R wg = L.gcl ? L.gcl[e] / R(P.dtp) : R(0);
+  side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, L.fbuf + 6 * (size_t)e, wg);
 }
 
 // The same flux with the two ends' records staged in LDS per block of
@@ -861,7 +889,8 @@ __global__ __launch_bounds__(NTHR) void k_flux_blk(LevelDev<R> L, DevParams P) {
       X1 = na + 8 + GREC;
       X2 = nb + 8 + GREC;
     }
-    side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, fo);
+    R wg = L.gcl ? L.gcl[e] / R(P.dtp) : R(0);
+    side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, fo, wg);
   }
   __syncthreads();  // node records no longer needed: the side fluxes take their place
   R *sf = sn;
@@ -949,10 +978,31 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
   int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
   R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
   R *o = L.bbuf + 10 * (size_t)s;
+  // moving mesh: -(3 U_1 + U_2) g_1 / dt to the first end, -(U_1 + 3 U_2)
+  // g_2 / dt to the second (boundaryTerm 1), -4 U_k g_k / dt otherwise, on
+  // every boundary side (DEF:2124-2179)
+  R gc[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) gc[c] = 0;
+  if (L.bgcl) {
+    const size_t n = L.n;
+    R g1 = L.bgcl[2 * (size_t)s] / R(P.dtp), g2 = L.bgcl[2 * (size_t)s + 1] / R(P.dtp);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      R a = L.u[c * n + ia], b = L.u[c * n + ib];
+      if (P.bterm == 1) {
+        gc[c] = -(R(3) * a + b) * g1;
+        gc[5 + c] = -(a + R(3) * b) * g2;
+      } else {
+        gc[c] = -a * (R(4) * g1);
+        gc[5 + c] = -b * (R(4) * g2);
+      }
+    }
+  }
   R rnorm = sqrt(wx * wx + wy * wy + wz * wz);
   if (!(rnorm > R(1e-10)) || !(ind <= 4 || ind == 9)) {
 #pragma unroll
-    for (int c = 0; c < 10; ++c) o[c] = 0;
+    for (int c = 0; c < 10; ++c) o[c] = gc[c];
     return;
   }
   R irn = R(1) / rnorm;
@@ -973,11 +1023,11 @@ __global__ void k_bflux(LevelDev<R> L, DevParams P) {
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
     if (P.bterm == 1) {
-      o[c] = rnorm * (R(3) * f1[c] + f2[c]);
-      o[5 + c] = rnorm * (f1[c] + R(3) * f2[c]);
+      o[c] = gc[c] + rnorm * (R(3) * f1[c] + f2[c]);
+      o[5 + c] = gc[5 + c] + rnorm * (f1[c] + R(3) * f2[c]);
     } else {
-      o[c] = rnorm * R(4) * f1[c];
-      o[5 + c] = rnorm * R(4) * f2[c];
+      o[c] = gc[c] + rnorm * R(4) * f1[c];
+      o[5 + c] = gc[5 + c] + rnorm * R(4) * f2[c];
     }
   }
 }
@@ -1009,12 +1059,17 @@ __global__ void k_flux_simple(LevelDev<R> L, DevParams P) {
   R u1 = rinv1 * ru1, v1 = rinv1 * rv1, w1 = rinv1 * rw1, rh1 = E1 + pr1;
   R u2 = rinv2 * ru2, v2 = rinv2 * rv2, w2 = rinv2 * rw2, rh2 = E2 + pr2;
   R f[5];
+  const R wg = L.gcl ? L.gcl[e] / R(P.dtp) : R(0);
+  const R us[5] = {r1 + r2, ru1 + ru2, rv1 + rv2, rw1 + rw2, E1 + E2};
   if constexpr (FT == 1) {
     f[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
     f[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
     f[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
     f[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
     f[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
+    if (wg != R(0))
+#pragma unroll
+      for (int c = 0; c < 5; ++c) f[c] -= us[c] * wg;
   } else {
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c] = 0;
@@ -1059,6 +1114,9 @@ __global__ void k_flux_simple(LevelDev<R> L, DevParams P) {
       }
 #pragma unroll
       for (int c = 0; c < 5; ++c) f[c] = rnorm * ((F1[c] + F2[c]) + D[c]);
+      if (wg != R(0))
+#pragma unroll
+        for (int c = 0; c < 5; ++c) f[c] -= us[c] * wg;
     }
   }
   R *o = L.fbuf + 6 * (size_t)e;
@@ -2762,14 +2820,27 @@ __device__ __forceinline__ void bc_increment(const LevelDev<R> &L, const DevPara
   if (P.sa && cls != NC_SYM) r[5] = 0;
 }
 
-// setBCsOnSolutionField (DEF:4364-4976) for node i on q[0..5]
+// setBCsOnSolutionField (DEF:4364-4976) for node i on q[0..5]; rho_g is
+// the density of the level's own solution at the time of the call (grp%u,
+// which the wall velocity of an Euler wall multiplies whichever array q is).
+// Wall velocities come from the level-1 coordinate movement of a moving
+// mesh; coarse levels carry none (the reference reads an array it never
+// allocates there) and take zero.
 template <typename R>
-__device__ __forceinline__ void bc_solution(const LevelDev<R> &L, const DevParams &P, int i, R *q) {
+__device__ __forceinline__ void bc_solution(const LevelDev<R> &L, const DevParams &P, int i, R *q, R rho_g) {
   int cls = L.ncls[i] & NC_MASK;
   if (cls == NC_NONE || cls == NC_FAR1 || cls == NC_FAR2) return;
   R nx = L.nrm[3 * (size_t)i], ny = L.nrm[3 * (size_t)i + 1], nz = L.nrm[3 * (size_t)i + 2];
+  const bool moving = L.cmv && L.gnum == 1 && cls == NC_WALL;
   if (cls == NC_WALL && P.Re > 0.0) {
-    q[1] = q[2] = q[3] = 0;
+    if (moving) {  // u(2:4) = u(1) * coordinateMovement / dt (DEF:4408)
+      R s = q[0] / R(P.dtp);
+      q[1] = s * L.cmv[3 * (size_t)i];
+      q[2] = s * L.cmv[3 * (size_t)i + 1];
+      q[3] = s * L.cmv[3 * (size_t)i + 2];
+    } else {
+      q[1] = q[2] = q[3] = 0;
+    }
     if (!P.isentropic) q[4] = q[0] * R(P.inflow_temperature / P.gamma);
     if (P.sa) q[5] = 0;
     return;
@@ -2778,6 +2849,15 @@ __device__ __forceinline__ void bc_solution(const LevelDev<R> &L, const DevParam
   q[1] -= nv * nx;
   q[2] -= nv * ny;
   q[3] -= nv * nz;
+  if (moving && P.add_time) {  // the wall's normal velocity (DEF:4433-4442)
+    R dtp = R(P.dtp);
+    R wn = (L.cmv[3 * (size_t)i] / dtp) * nx + (L.cmv[3 * (size_t)i + 1] / dtp) * ny +
+           (L.cmv[3 * (size_t)i + 2] / dtp) * nz;
+    R m = rho_g * wn;
+    q[1] += m * nx;
+    q[2] += m * ny;
+    q[3] += m * nz;
+  }
   if (cls == NC_INVWALL && !P.isentropic)
     q[4] = q[0] * R(P.inflow_temperature / P.gamma) + R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0];
   if (P.sa && cls != NC_SYM) q[5] = 0;
@@ -2882,15 +2962,17 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
     return;
   }
   if (MODE == MODE_STAGE && P.unsteady && P.add_time) {
-    // addTimeDifferences (DEF:1184-1241) on a fixed mesh: BDF2 on grid 1,
-    // BDF1 elsewhere, over components 1:5 (1:6 with SA)
+    // addTimeDifferences (DEF:1184-1241): BDF2 on grid 1 with the volumes
+    // of the last two physical times (equal to V on a fixed mesh), BDF1
+    // elsewhere, over components 1:5 (1:6 with SA)
     const R V = L.vol[i], dtp = R(P.dtp);
+    const R VP = L.volp ? L.volp[i] : V, VP2 = L.volp2 ? L.volp2[i] : V;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       if (c == 5 && !SA) break;
       const size_t k = (size_t)c * n + i;
       if (L.gnum == 1 && P.bdf2)
-        r[c] = r[c] + (R(1.5) * V * L.u[k] - R(2.0) * V * L.uold[k] + R(0.5) * V * L.uold2[k]) / dtp;
+        r[c] = r[c] + (R(1.5) * V * L.u[k] - R(2.0) * VP * L.uold[k] + R(0.5) * VP2 * L.uold2[k]) / dtp;
       else
         r[c] = r[c] + V * (L.u[k] - L.uold[k]) / dtp;
     }
@@ -2932,7 +3014,9 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
   } else if (q[0] < R(P.min_rho)) {
     q[0] = R(P.min_rho);
   }
-  bc_solution(L, P, i, q);
+  // restrictGrid applies the state conditions to the updated state it
+  // holds in rhs (DEF:12333-12336) while grp%u keeps the grid's own state
+  bc_solution(L, P, i, q, MODE == MODE_FINE ? (ug0 < R(P.min_rho) ? R(P.min_rho) : ug0) : q[0]);
   R pi = L.p[i];
   if (MODE == MODE_STAGE) {
     if (S.fas) {
@@ -3015,7 +3099,7 @@ __global__ void k_bc_state(LevelDev<R> L, DevParams P, R *a, int outer) {
   int n = L.n;
   R q[6];
   for (int c = 0; c < 6; ++c) q[c] = a[(size_t)c * n + i];
-  bc_solution(L, P, i, q);
+  bc_solution(L, P, i, q, a == L.u ? q[0] : L.u[i]);
   if (outer) outer_bc(L, P, i, q, L.p[i]);
   for (int c = 0; c < 6; ++c) a[(size_t)c * n + i] = q[c];
 }
@@ -3069,7 +3153,7 @@ __global__ void k_prolong_fine(LevelDev<RF> F, DevParams P, const int *f2c, cons
   RF q[6];
   for (int c = 0; c < 6; ++c) q[c] = F.u[(size_t)c * n + i];
   for (int c = 0; c < ncomp_add; ++c) q[c] = q[c] + d[c];
-  bc_solution(F, P, i, q);
+  bc_solution(F, P, i, q, q[0]);
   outer_bc(F, P, i, q, F.p[i]);
   if (P.sa) q[5] = q[5] > RF(0) ? q[5] : RF(0);
   q[0] = q[0] > RF(P.min_rho) ? q[0] : RF(P.min_rho);

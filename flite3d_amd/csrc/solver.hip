@@ -136,6 +136,9 @@ struct Staged {
   int n = 0, ne = 0, nbs = 0, nb = 0, nfine = 0;
   std::vector<int> e1, e2, b1, b2, bi, nia, f2c;
   std::vector<double> w, wb, vol, xyz, wd, rw;  // w/wb/xyz column-major as in the view
+  // moving mesh (empty on a fixed mesh): GCL weights per side and per
+  // boundary side (Eb,2), previous volumes and movement (N,3) on level 1
+  std::vector<double> gcl, bgcl, volp, volp2, cmv;
   int reg[21];
 };
 
@@ -157,6 +160,7 @@ struct Level {
   void *rec0, *grec, *tau, *dnu, *fbuf, *bbuf, *ws;
   void *jst = nullptr;  // JST node records (fluxType 1-3 only)
   void *uold = nullptr, *uold2 = nullptr;  // unsteady only
+  void *gcl = nullptr, *bgcl = nullptr, *volp = nullptr, *volp2 = nullptr, *cmv = nullptr;  // moving mesh only
   int *ea, *eb, *ba, *bb, *bind, *ell_ptr, *ell_k, *bs_ptr, *bs_list, *ncls, *host_index, *wall_nodes;
   int *ell_e, *ell_nbr;
   void *ellw;
@@ -208,6 +212,8 @@ struct Level {
     v.jst = (R *)jst;
     v.uold = (R *)uold;
     v.uold2 = (R *)uold2;
+    v.gcl = (const R *)gcl; v.bgcl = (const R *)bgcl;
+    v.volp = (const R *)volp; v.volp2 = (const R *)volp2; v.cmv = (const R *)cmv;
     v.nb = nbatch;
     v.gstride = gstride;
     return v;
@@ -249,6 +255,10 @@ struct flite_ctx {
   // uploaded level views, built on the device at first use
   std::vector<Staged> staged;
   std::vector<char> staged_ok;
+  // solution arrays carried over a mesh replacement (flite_level_geometry):
+  // per level u, uold, uold2 and laminar viscosity in host numbering, each
+  // nbatch x ncomp blocks of n
+  std::vector<std::vector<std::vector<double>>> carry;
   bool built = false;
 };
 
@@ -466,6 +476,17 @@ int stage_level(flite_ctx *c, int level, const flite_level_view *v, Staged &S) {
       if (S.f2c[i] < 0 || S.f2c[i] >= n) return fail(FLITE_E_ARG, "prolongation index out of range");
     }
     S.rw.assign(v->restriction, v->restriction + S.nfine);
+  }
+  // moving-mesh arrays: used on unsteady runs only (the physical time step
+  // divides them); each may be absent
+  if (c->hp.flow_type != 0) {
+    if (v->side_gcl) S.gcl.assign(v->side_gcl, v->side_gcl + ne);
+    if (v->bside_gcl) S.bgcl.assign(v->bside_gcl, v->bside_gcl + 2 * (size_t)nbs);
+    if (level == 1) {
+      if (v->node_volume_p) S.volp.assign(v->node_volume_p, v->node_volume_p + n);
+      if (v->node_volume_p2) S.volp2.assign(v->node_volume_p2, v->node_volume_p2 + n);
+      if (v->coordinate_movement) S.cmv.assign(v->coordinate_movement, v->coordinate_movement + 3 * (size_t)n);
+    }
   }
   return 0;
 }
@@ -827,6 +848,30 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(up_real(&L.nrm, nrm));
   TRY(up_real(&L.warea, warea));
   if (L.has_coords) TRY(up_real(&L.xyz, xyz)); else L.xyz = nullptr;
+  // moving mesh: the same permutations as w, wb and vol
+  if (!S.gcl.empty()) {
+    std::vector<double> g(ne);
+    for (int k = 0; k < ne; ++k) g[k] = S.gcl[L.side_host[k]];
+    TRY(up_real(&L.gcl, g));
+  }
+  if (!S.bgcl.empty()) {
+    std::vector<double> g(2 * (size_t)nbs);
+    for (int s = 0; s < nbs; ++s) {
+      g[2 * (size_t)s] = S.bgcl[s];
+      g[2 * (size_t)s + 1] = S.bgcl[(size_t)nbs + s];
+    }
+    TRY(up_real(&L.bgcl, g));
+  }
+  auto up_nodes = [&](void **out, const std::vector<double> &hv, int nc) -> int {
+    if (hv.empty()) return 0;
+    std::vector<double> d((size_t)nc * n);
+    for (int h = 0; h < n; ++h)
+      for (int k = 0; k < nc; ++k) d[(size_t)nc * L.h2d[h] + k] = hv[(size_t)k * n + h];
+    return up_real(out, d);
+  };
+  TRY(up_nodes(&L.volp, S.volp, 1));
+  TRY(up_nodes(&L.volp2, S.volp2, 1));
+  TRY(up_nodes(&L.cmv, S.cmv, 3));
   TRY(al_real(&L.u, (size_t)(6 * (size_t)n) * NB));
   TRY(al_real(&L.uprev, (size_t)(6 * (size_t)n) * NB));
   TRY(al_real(&L.rhs, (size_t)(6 * (size_t)n) * NB));
@@ -907,6 +952,38 @@ int upload_level(flite_ctx *c, int level, const flite_level_view *v) {
   return 0;
 }
 
+// a level's device array of nbatch x nc blocks of n values <-> host numbering
+int carry_out(const Level &L, const void *d, int nc, std::vector<double> &h) {
+  const size_t nblk_ = (size_t)nc * L.nbatch, cnt = nblk_ * L.n;
+  std::vector<double> raw(cnt);
+  if (L.fp32) {
+    std::vector<float> t(cnt);
+    HIPTRY(hipMemcpy(t.data(), d, cnt * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < cnt; ++k) raw[k] = t[k];
+  } else {
+    HIPTRY(hipMemcpy(raw.data(), d, cnt * 8, hipMemcpyDeviceToHost));
+  }
+  h.resize(cnt);
+  for (size_t b = 0; b < nblk_; ++b)
+    for (int hh = 0; hh < L.n; ++hh) h[b * L.n + hh] = raw[b * L.n + L.h2d[hh]];
+  return 0;
+}
+
+int carry_in(const Level &L, void *d, int nc, const std::vector<double> &h) {
+  const size_t nblk_ = (size_t)nc * L.nbatch, cnt = nblk_ * L.n;
+  if (h.size() != cnt) return fail(FLITE_E_STATE, "carried solution does not fit the rebuilt level");
+  std::vector<double> raw(cnt);
+  for (size_t b = 0; b < nblk_; ++b)
+    for (int hh = 0; hh < L.n; ++hh) raw[b * L.n + L.h2d[hh]] = h[b * L.n + hh];
+  if (L.fp32) {
+    std::vector<float> t(raw.begin(), raw.end());
+    HIPTRY(hipMemcpy(d, t.data(), cnt * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPTRY(hipMemcpy(d, raw.data(), cnt * 8, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 // builds the device levels once every level is staged
 int ensure_built(flite_ctx *c) {
   if (c->built) return 0;
@@ -917,6 +994,17 @@ int ensure_built(flite_ctx *c) {
   for (int l = 0; l < c->nl; ++l)
     TRY(build_level(c, l + 1, c->staged[l], c->orders[l], c->hp.coarse_fp32 != 0, c->lev[l]));
   c->built = true;
+  if (!c->carry.empty()) {  // a moved mesh: the solution arrays carry over
+    auto carry = std::move(c->carry);
+    c->carry.clear();
+    for (int l = 0; l < c->nl; ++l) {
+      Level &L = *c->lev[l];
+      TRY(carry_in(L, L.u, 6, carry[l][0]));
+      TRY(carry_in(L, L.uold, 6, carry[l][1]));
+      TRY(carry_in(L, L.uold2, 6, carry[l][2]));
+      TRY(carry_in(L, L.mu, 1, carry[l][3]));
+    }
+  }
   return 0;
 }
 
@@ -1441,9 +1529,9 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   }
   if (p->flux_type < 1 || p->flux_type > 4) return fail(FLITE_E_UNSUPPORTED, "fluxType must be 1-4");
   if (p->rk_stages < 1 || p->rk_stages > 5) return fail(FLITE_E_ARG, "rk_stages must be 1..5");
-  if (p->flow_type != 0 && p->flow_type != 1)
-    return fail(FLITE_E_UNSUPPORTED, "only steady flow and unsteady flow on a fixed mesh (flowType 0, 1)");
-  if (p->flow_type == 1 && !(p->physical_time_step > 0.0)) return fail(FLITE_E_ARG, "physical_time_step must be > 0");
+  if (p->flow_type < 0 || p->flow_type > 2)
+    return fail(FLITE_E_UNSUPPORTED, "flowType must be 0 (steady), 1 (fixed mesh) or 2 (moving mesh)");
+  if (p->flow_type != 0 && !(p->physical_time_step > 0.0)) return fail(FLITE_E_ARG, "physical_time_step must be > 0");
   if (p->flux_type != 4 && (p->dissipation_scheme != 2 || p->coarse_grid_dissipation_scheme != 2))
     return fail(FLITE_E_UNSUPPORTED, "only dissipation scheme 2 (JST) is implemented");
   if (p->turbulence_model != 0 && p->turbulence_model != 1)
@@ -1527,6 +1615,30 @@ int flite_level_upload(flite_ctx *c, int level, const flite_level_view *v) {
   if (!c) return fail(FLITE_E_ARG, "null context");
   if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
   TRY(sync(c));
+  return upload_level(c, level, v);
+}
+
+int flite_level_geometry(flite_ctx *c, int level, const flite_level_view *v) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
+  if (c->hp.flow_type != 2) return fail(FLITE_E_STATE, "a moving mesh needs flow_type 2");
+  if (!c->staged_ok[level - 1]) return fail(FLITE_E_STATE, "level not uploaded");
+  const Staged &O = c->staged[level - 1];
+  if (!v) return fail(FLITE_E_ARG, "null level view");
+  if (v->n_nodes != O.n || v->n_sides != O.ne || v->n_bsides != O.nbs || v->n_bnodes != O.nb)
+    return fail(FLITE_E_ARG, "the moved mesh must keep the level's node, side and boundary counts");
+  if (c->built && c->carry.empty()) {
+    TRY(sync(c));
+    c->carry.assign(c->nl, std::vector<std::vector<double>>(4));
+    for (int l = 0; l < c->nl; ++l) {
+      const Level &L = *c->lev[l];
+      TRY(carry_out(L, L.u, 6, c->carry[l][0]));
+      TRY(carry_out(L, L.uold, 6, c->carry[l][1]));
+      TRY(carry_out(L, L.uold2, 6, c->carry[l][2]));
+      TRY(carry_out(L, L.mu, 1, c->carry[l][3]));
+    }
+  }
+  if (c->built) TRY(sync(c));
   return upload_level(c, level, v);
 }
 
@@ -1877,6 +1989,7 @@ int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_o
     return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
   if (hp.turbulence_model != 0)
     return fail(FLITE_E_UNSUPPORTED, "linearisation: no turbulence model (the Spalart-Allmaras terms are not linearised)");
+  if (hp.flow_type == 2) return fail(FLITE_E_UNSUPPORTED, "linearisation: fixed meshes only (no GCL terms)");
   if (!laminar_ok && hp.reynolds_number > 0.0)
     return fail(FLITE_E_UNSUPPORTED, "metric and drag derivatives: inviscid flow (ReynoldsNumber=0) only");
   return 0;

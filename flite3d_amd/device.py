@@ -31,6 +31,7 @@ def lib():
         L.flite_ctx_create.argtypes = [P(FliteParams), i, i, P(vp)]
         L.flite_batch_select.argtypes = [vp, i]
         L.flite_level_upload.argtypes = [vp, i, P(FliteLevelView)]
+        L.flite_level_geometry.argtypes = [vp, i, P(FliteLevelView)]
         L.flite_ctx_destroy.argtypes = [vp]
         L.flite_ctx_destroy.restype = None
         L.flite_last_error.restype = ctypes.c_char_p
@@ -84,6 +85,15 @@ class Solver:
     def _check(self, rc):
         if rc != 0:
             raise DeviceError(rc, self.L.flite_last_error().decode())
+
+    def replace_mesh(self, levels):
+        """The next physical step's mesh of a moving-mesh run (flowType 2,
+        mgnsg3d_mg_par.F90:525-548): new geometry for every level, the
+        solution arrays kept."""
+        for k, lv in enumerate(levels):
+            v = level_view(lv, k == 0)
+            self._check(self.L.flite_level_geometry(self.ctx, k + 1, ctypes.byref(v)))
+        self.levels = levels
 
     def close(self):
         if getattr(self, "ctx", None):

@@ -102,6 +102,12 @@ module flite_gpu
      integer(c_int) :: n_fine
      type(c_ptr) :: prolongation             ! pod%prolongationArray(n_fine)
      type(c_ptr) :: restriction              ! rod%restrictionArray(n_fine)
+     ! moving meshes (flowType 2); null on a fixed mesh
+     type(c_ptr) :: side_gcl = c_null_ptr            ! GCLWeightsArray(E)
+     type(c_ptr) :: bside_gcl = c_null_ptr           ! brp%GCLWeightsArray(Eb,2)
+     type(c_ptr) :: node_volume_p = c_null_ptr       ! nodeVolumeP(N), level 1
+     type(c_ptr) :: node_volume_p2 = c_null_ptr      ! nodeVolumeP2(N), level 1
+     type(c_ptr) :: coordinate_movement = c_null_ptr ! coordinateMovement(N,3), level 1
   end type flite_level_view
 
   interface

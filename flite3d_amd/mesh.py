@@ -170,6 +170,14 @@ class Level:
     seed: np.ndarray
     f2c: np.ndarray | None = None
     rweight: np.ndarray | None = None
+    # moving meshes (flowType 2), as the solver holds them after reading a
+    # step's .sol: GCL weight per side [E], per boundary side [2, Eb],
+    # previous volumes [N] and coordinate movement [3, N] (level 1)
+    gcl: np.ndarray | None = None
+    bgcl: np.ndarray | None = None
+    volume_p: np.ndarray | None = None
+    volume_p2: np.ndarray | None = None
+    coord_move: np.ndarray | None = None
 
     @property
     def n_sides(self):

@@ -170,8 +170,8 @@ class InputVariables:
     def check_supported(self):
         v = self.v
         bad = []
-        if v["flowtype"] not in (0, 1):
-            bad.append("flowType 2 (moving meshes)")
+        if v["flowtype"] not in (0, 1, 2):
+            bad.append("flowType other than 0-2")
         if v["flowtype"] != 0 and v["explicit"]:
             bad.append("explicit time stepping")
         if v["turbulencemodel"] not in (0, 1):
@@ -332,6 +332,11 @@ class FliteLevelView(ctypes.Structure):
         ("n_fine", ctypes.c_int),
         ("prolongation", ctypes.POINTER(ctypes.c_int)),
         ("restriction", ctypes.POINTER(ctypes.c_double)),
+        ("side_gcl", ctypes.POINTER(ctypes.c_double)),
+        ("bside_gcl", ctypes.POINTER(ctypes.c_double)),
+        ("node_volume_p", ctypes.POINTER(ctypes.c_double)),
+        ("node_volume_p2", ctypes.POINTER(ctypes.c_double)),
+        ("coordinate_movement", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -368,6 +373,12 @@ def level_view(level, is_fine):
         v.restriction = P(level.rweight.astype(np.float64), ctypes.c_double)
     else:
         v.n_fine = 0
+    # moving mesh (flowType 2): optional attributes of the level
+    for name, attr in (("side_gcl", "gcl"), ("bside_gcl", "bgcl"), ("node_volume_p", "volume_p"),
+                       ("node_volume_p2", "volume_p2"), ("coordinate_movement", "coord_move")):
+        a = getattr(level, attr, None)
+        if a is not None:
+            setattr(v, name, P(np.asarray(a, dtype=np.float64), ctypes.c_double))
     v._keep = keep
     return v
 

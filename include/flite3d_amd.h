@@ -111,6 +111,15 @@ typedef struct flite_level_view {
   int n_fine;                      /* nodes of the next finer level */
   const int *prolongation;         /* pod%prolongationArray(n_fine): fine -> coarse */
   const double *restriction;       /* rod%restrictionArray(n_fine): V_f / V_c */
+  /* moving meshes (flow_type 2), as readGridComputationData holds them after
+   * its read; NULL on a fixed mesh (zero GCL weights and movement, previous
+   * volumes equal to nodeVolume) */
+  const double *side_gcl;          /* GCLWeightsArray(E): 1.5 g - 0.5 g_prev on level 1
+                                      with timeScheme 2 (DEF:13855-13861), g on coarse levels */
+  const double *bside_gcl;         /* brp%GCLWeightsArray(Eb,2), the same rule (DEF:816-819) */
+  const double *node_volume_p;     /* nodeVolumeP(N), level 1 (DEF:13891) */
+  const double *node_volume_p2;    /* nodeVolumeP2(N), level 1 */
+  const double *coordinate_movement; /* coordinateMovement(N,3), level 1 (DEF:13896-13900) */
 } flite_level_view;
 
 typedef struct flite_ctx flite_ctx;
@@ -126,6 +135,13 @@ typedef struct flite_ctx flite_ctx;
 int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx **out);
 /* Replaces readGridComputationData's device side (mgnsg3d_def_par.F90:13812-14708). */
 int flite_level_upload(flite_ctx *ctx, int level, const flite_level_view *v);
+/* The next physical step's mesh of a moving-mesh run (flow_type 2): the
+ * readComputationData(COMPINFILE, 1) of multiGridCycles (mgnsg3d_mg_par.F90:
+ * 525-548), which replaces the geometry and keeps every solution array.
+ * Same node, side and boundary-side counts as the level's last upload; u,
+ * uold, uold2 and the laminar viscosity of every level and condition carry
+ * over.  Call it for each level whose geometry changes, finest first. */
+int flite_level_geometry(flite_ctx *ctx, int level, const flite_level_view *v);
 void flite_ctx_destroy(flite_ctx *ctx);
 const char *flite_last_error(void);
 

@@ -29,6 +29,7 @@ def lib():
         L.or_create.restype = vp
         L.or_create.argtypes = [P(FliteParams), i]
         L.or_level_upload.argtypes = [vp, i, P(FliteLevelView)]
+        L.or_level_geometry.argtypes = [vp, i, P(FliteLevelView)]
         L.or_destroy.argtypes = [vp]
         L.or_last_error.restype = ctypes.c_char_p
         L.or_field_get.argtypes = [vp, i, i, P(d), i]
@@ -75,6 +76,13 @@ class Oracle:
     def _check(self, rc):
         if rc != 0:
             raise OracleError(f"oracle error {rc}: " + self.L.or_last_error().decode())
+
+    def replace_mesh(self, levels):
+        """A moving mesh's next geometry (MG:525-548), the solution kept."""
+        for k, lv in enumerate(levels):
+            v = level_view(lv, k == 0)
+            self._check(self.L.or_level_geometry(self.ctx, k + 1, ctypes.byref(v)))
+        self.levels = levels
 
     def close(self):
         if self.ctx:

@@ -79,6 +79,10 @@ typedef struct {
   double *uold, *uold2; /* solutions at the last two physical times (unsteady) */
   double *p, *dt, *dtt, *mu, *vort, *divg, *tdiff;
   double *ws; /* wallStress(Nb,4): ws[c*nb+i] */
+  /* moving mesh (NULL on a fixed mesh): GCLWeightsArray(E), brp%GCLWeightsArray
+   * (Eb,2) as gcl_b[end*nbs+s], nodeVolumeP/P2(N) and coordinateMovement(N,3)
+   * on level 1 */
+  double *gcl, *gclb, *volp, *volp2, *cmv;
   int has_src;
   int uploaded;
 } orl;
@@ -123,7 +127,8 @@ static void free_level(orl *L) {
   void *ptrs[] = {L->e1, L->e2, L->w, L->b1, L->b2, L->bi, L->wb, L->vol,
                   L->xyz, L->wd, L->nia, L->nrm, L->warea, L->f2c, L->rw,
                   L->u, L->uprev, L->rhs, L->diss, L->src, L->p, L->dt,
-                  L->dtt, L->mu, L->vort, L->divg, L->tdiff, L->ws, L->uold, L->uold2};
+                  L->dtt, L->mu, L->vort, L->divg, L->tdiff, L->ws, L->uold, L->uold2,
+                  L->gcl, L->gclb, L->volp, L->volp2, L->cmv};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
   memset(L, 0, sizeof *L);
 }
@@ -236,8 +241,42 @@ int or_level_upload(or_ctx *c, int level, const flite_level_view *v) {
     c->nh = n;
     c->help = zalloc(25 * (size_t)n);
   }
+  if (c->P.flow_type != 0) { /* moving-mesh arrays: unsteady runs only */
+    if (v->side_gcl) { L->gcl = zalloc(ne); memcpy(L->gcl, v->side_gcl, sizeof(double) * ne); }
+    if (v->bside_gcl) { L->gclb = zalloc(2 * (size_t)nbs); memcpy(L->gclb, v->bside_gcl, sizeof(double) * 2 * nbs); }
+    if (level == 1 && v->node_volume_p) { L->volp = zalloc(n); memcpy(L->volp, v->node_volume_p, sizeof(double) * n); }
+    if (level == 1 && v->node_volume_p2) { L->volp2 = zalloc(n); memcpy(L->volp2, v->node_volume_p2, sizeof(double) * n); }
+    if (level == 1 && v->coordinate_movement) {
+      L->cmv = zalloc(3 * (size_t)n);
+      memcpy(L->cmv, v->coordinate_movement, sizeof(double) * 3 * n);
+    }
+  }
   L->uploaded = 1;
   return 0;
+}
+
+/* readComputationData(COMPINFILE, 1) of a moving-mesh step (MG:525-548):
+ * new geometry, the solution arrays kept */
+int or_level_geometry(or_ctx *c, int level, const flite_level_view *v) {
+  if (!c || !v || level < 1 || level > c->nl) return fail(FLITE_E_ARG, "bad level");
+  orl *L = &c->L[level - 1];
+  if (!L->uploaded || v->n_nodes != L->n || v->n_sides != L->ne || v->n_bsides != L->nbs || v->n_bnodes != L->nb)
+    return fail(FLITE_E_ARG, "the moved mesh must keep the level's counts");
+  size_t n6 = 6 * (size_t)L->n;
+  double *keep[4] = {zalloc(n6), zalloc(n6), zalloc(n6), zalloc(L->n)};
+  memcpy(keep[0], L->u, sizeof(double) * n6);
+  memcpy(keep[1], L->uold, sizeof(double) * n6);
+  memcpy(keep[2], L->uold2, sizeof(double) * n6);
+  memcpy(keep[3], L->mu, sizeof(double) * L->n);
+  int rc = or_level_upload(c, level, v);
+  if (!rc) {
+    memcpy(L->u, keep[0], sizeof(double) * n6);
+    memcpy(L->uold, keep[1], sizeof(double) * n6);
+    memcpy(L->uold2, keep[2], sizeof(double) * n6);
+    memcpy(L->mu, keep[3], sizeof(double) * L->n);
+  }
+  for (int k = 0; k < 4; ++k) free(keep[k]);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------
@@ -319,12 +358,16 @@ static int bc_solution(const or_ctx *C, orl *L, double *u) {
   if (REG(L, -12) <= REG(L, -11) || REG(L, -10) <= REG(L, -9) || REG(L, -8) <= REG(L, -7) ||
       REG(L, -6) <= REG(L, -5))
     return fail(FLITE_E_UNSUPPORTED, "engine boundary nodes are not supported");
+  /* coordinateMovement exists on level 1 only (DEF:13882); coarse levels
+   * take zero movement */
+  const double *cm = L->gnum == 1 ? L->cmv : NULL;
+  double dtp = P->physical_time_step;
   if (P->reynolds_number > 0.0) {
     FOR_RANGE(L, -20, ib, ip) {
-      /* u(2:4) = u(1) * coordinateMovement / dt with zero movement */
-      u[n + ip] = 0.0;
-      u[2 * (size_t)n + ip] = 0.0;
-      u[3 * (size_t)n + ip] = 0.0;
+      /* u(2:4) = u(1) * coordinateMovement / dt (DEF:4408) */
+      u[n + ip] = cm ? u[ip] * cm[ip] / dtp : 0.0;
+      u[2 * (size_t)n + ip] = cm ? u[ip] * cm[n + ip] / dtp : 0.0;
+      u[3 * (size_t)n + ip] = cm ? u[ip] * cm[2 * (size_t)n + ip] / dtp : 0.0;
       if (!P->walls_are_isentropic) u[4 * (size_t)n + ip] = u[ip] * P->inflow_temperature / P->gamma;
       if (P->turbulence_model == 1) u[5 * (size_t)n + ip] = 0.0;
     }
@@ -333,9 +376,15 @@ static int bc_solution(const or_ctx *C, orl *L, double *u) {
       double nx = L->nrm[ip], ny = L->nrm[n + ip], nz = L->nrm[2 * (size_t)n + ip];
       double m1 = u[n + ip], m2 = u[2 * (size_t)n + ip], m3 = u[3 * (size_t)n + ip];
       double nv = m1 * nx + m2 * ny + m3 * nz;
-      u[n + ip] = m1 - nv * nx;
-      u[2 * (size_t)n + ip] = m2 - nv * ny;
-      u[3 * (size_t)n + ip] = m3 - nv * nz;
+      /* the wall's normal velocity on level 1 of a time-accurate run, times
+       * the density of the level's own solution grp%u (DEF:4433-4442) */
+      double wn = 0.0;
+      if (cm && P->add_time_step)
+        wn = (cm[ip] / dtp) * nx + (cm[n + ip] / dtp) * ny + (cm[2 * (size_t)n + ip] / dtp) * nz;
+      double m = L->u[ip] * wn;
+      u[n + ip] = m1 - nv * nx + m * nx;
+      u[2 * (size_t)n + ip] = m2 - nv * ny + m * ny;
+      u[3 * (size_t)n + ip] = m3 - nv * nz + m * nz;
       if (P->turbulence_model > 0) u[5 * (size_t)n + ip] = 0.0;
     }
   }
@@ -848,7 +897,7 @@ static double led(double a, double bmin, double bmax, int nsig) {
 static void hllc_side(const flite_params *P, int order, int autoclip, int nsig,
                       const double *s1, double pr1, const double *s2, double pr2,
                       double wx, double wy, double wz, const double *dx3,
-                      const double *gr1, const double *gr2, double *f) {
+                      const double *gr1, const double *gr2, double *f, double *usel) {
   double gam = P->gamma;
   double r1 = s1[0], rinv1 = 1. / s1[0], ru1 = s1[1], rv1 = s1[2], rw1 = s1[3];
   double u1 = rinv1 * ru1, v1 = rinv1 * rv1, w1 = rinv1 * rw1, E1 = s1[4], rh1 = s1[4] + pr1;
@@ -861,7 +910,6 @@ static void hllc_side(const flite_params *P, int order, int autoclip, int nsig,
   f[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
   f[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
   double roe1 = r1, roe2 = r2, re1 = E1, re2 = E2;
-  (void)E1; (void)E2;
   wx = 2.0 * wx;
   wy = 2.0 * wy;
   wz = 2.0 * wz;
@@ -974,21 +1022,31 @@ static void hllc_side(const flite_params *P, int order, int autoclip, int nsig,
     rs[0] = roe2; rs[1] = ru2; rs[2] = rv2; rs[3] = rw2; rs[4] = re2;
   }
   double st1[5] = {roe1, ru1, rv1, rw1, re1}, st2[5] = {roe2, ru2, rv2, rw2, re2};
+  /* the state each branch leaves in r1, ru1, rv1, rw1, E1 for the GCL term
+   * (DEF:2033-2073): density and energy of the node, momentum as
+   * reconstructed, or the star state */
+  double g1[5] = {r1, ru1, rv1, rw1, E1}, g2[5] = {r2, ru2, rv2, rw2, E2};
+  const double *gs = g1;
   if (sl > 0.0) {
     for (int k = 0; k < 5; ++k) f[k] = fl[k] * rnorm1;
   } else if (sl <= 0.0 && sm > 0.0) {
     for (int k = 0; k < 5; ++k) f[k] = (fl[k] + sl * (ls[k] - st1[k])) * rnorm1;
+    gs = ls;
   } else if (sm <= 0.0 && sr >= 0.0) {
     for (int k = 0; k < 5; ++k) f[k] = (fr[k] + sr * (rs[k] - st2[k])) * rnorm1;
+    gs = rs;
   } else if (sr < 0.0) {
     for (int k = 0; k < 5; ++k) f[k] = fr[k] * rnorm1;
+    gs = g2;
   }
+  if (usel)
+    for (int k = 0; k < 5; ++k) usel[k] = gs[k];
 }
 
 int or_hllc_flux(const flite_params *p, const double ul[5], double pl,
                  const double ur[5], double pr, const double w[3], double f[5]) {
   double z[3] = {0, 0, 0};
-  hllc_side(p, 1, 0, 2, ul, pl, ur, pr, w[0], w[1], w[2], z, NULL, NULL, f);
+  hllc_side(p, 1, 0, 2, ul, pl, ur, pr, w[0], w[1], w[2], z, NULL, NULL, f, NULL);
   return 0;
 }
 
@@ -1134,13 +1192,21 @@ static int make_rhs(or_ctx *C, orl *L) {
     for (int c = 0; c < 3; ++c) dx[c] = L->xyz[(size_t)c * n + i2] - L->xyz[(size_t)c * n + i1];
     if (order > 1)
       for (int c = 0; c < 25; ++c) { gr1[c] = H(C, c, i1); gr2[c] = H(C, c, i2); }
-    double f[5];
-    if (P->flux_type == 4)
-      hllc_side(P, order, autoclip, nsig, s1, L->p[i1], s2, L->p[i2], L->w[e], L->w[L->ne + e],
-                L->w[2 * (size_t)L->ne + e], dx, gr1, gr2, f);
-    else
-      simple_side(P, P->flux_type, s1, L->p[i1], s2, L->p[i2], L->w[e], L->w[L->ne + e],
-                  L->w[2 * (size_t)L->ne + e], f);
+    double f[5], usel[5];
+    double wx = L->w[e], wy = L->w[L->ne + e], wz = L->w[2 * (size_t)L->ne + e];
+    /* wgcl = GCLWeightsArray / physicalTimestep (DEF:1662) */
+    double wgcl = L->gcl ? L->gcl[e] / P->physical_time_step : 0.0;
+    if (P->flux_type == 4) {
+      hllc_side(P, order, autoclip, nsig, s1, L->p[i1], s2, L->p[i2], wx, wy, wz, dx, gr1, gr2, f, usel);
+      if (wgcl != 0.0) /* the state the HLLC branch left behind (DEF:2081-2085) */
+        for (int c = 0; c < 5; ++c) f[c] -= usel[c] * wgcl;
+    } else {
+      simple_side(P, P->flux_type, s1, L->p[i1], s2, L->p[i2], wx, wy, wz, f);
+      /* -(U1 + U2) wgcl: always for the central flux, inside the |w| > 1e-10
+       * branch for LLF and Roe (DEF:1665-1669, 1718-1722, 1787-1791) */
+      if (wgcl != 0.0 && (P->flux_type == 1 || sqrt(wx * wx + wy * wy + wz * wz) > 1.0e-10))
+        for (int c = 0; c < 5; ++c) f[c] -= (s1[c] + s2[c]) * wgcl;
+    }
     for (int c = 0; c < 5; ++c) {
       RHS(L, c, i1) += f[c];
       RHS(L, c, i2) -= f[c];
@@ -1151,6 +1217,18 @@ static int make_rhs(or_ctx *C, orl *L) {
     double wx = L->wb[s], wy = L->wb[L->nbs + s], wz = L->wb[2 * (size_t)L->nbs + s];
     double s1[5], s2[5];
     for (int c = 0; c < 5; ++c) { s1[c] = U(L, c, i1); s2[c] = U(L, c, i2); }
+    if (L->gclb) { /* GCL terms of every boundary side (DEF:2124-2179) */
+      double g1 = L->gclb[s] / P->physical_time_step, g2 = L->gclb[L->nbs + s] / P->physical_time_step;
+      for (int c = 0; c < 5; ++c) {
+        if (P->boundary_term == 1) {
+          RHS(L, c, i1) -= (3. * s1[c] + s2[c]) * g1;
+          RHS(L, c, i2) -= (s1[c] + 3. * s2[c]) * g2;
+        } else {
+          RHS(L, c, i1) -= s1[c] * (4.0 * g1);
+          RHS(L, c, i2) -= s2[c] * (4.0 * g2);
+        }
+      }
+    }
     double f1[5], f2[5];
     double rnorm = sqrt(wx * wx + wy * wy + wz * wz);
     if (ind <= 2 || ind == 9) {
@@ -1400,10 +1478,13 @@ static void time_differences(or_ctx *C, orl *L) {
   double dtp = P->physical_time_step;
   for (int i = 0; i < n; ++i) {
     double V = L->vol[i];
+    /* nodeVolumeP, nodeVolumeP2: the volumes at the last two physical times
+     * (equal to V on a fixed mesh) */
+    double VP = L->volp ? L->volp[i] : V, VP2 = L->volp2 ? L->volp2[i] : V;
     for (int c = 0; c < nc; ++c) {
       size_t k = (size_t)c * n + i;
       if (L->gnum == 1 && P->time_scheme == 2)
-        L->rhs[k] = L->rhs[k] + (1.5 * V * L->u[k] - 2.0 * V * L->uold[k] + 0.5 * V * L->uold2[k]) / dtp;
+        L->rhs[k] = L->rhs[k] + (1.5 * V * L->u[k] - 2.0 * VP * L->uold[k] + 0.5 * VP2 * L->uold2[k]) / dtp;
       else
         L->rhs[k] = L->rhs[k] + V * (L->u[k] - L->uold[k]) / dtp;
     }

@@ -27,6 +27,7 @@ typedef struct or_ctx or_ctx;
 
 or_ctx *or_create(const flite_params *p, int n_levels);
 int or_level_upload(or_ctx *c, int level, const flite_level_view *v);
+int or_level_geometry(or_ctx *c, int level, const flite_level_view *v);
 void or_destroy(or_ctx *c);
 const char *or_last_error(void);
 

@@ -398,14 +398,28 @@ def test_unsteady_cycles(case, scheme):
     s.close(); o.close(); st.close()
 
 
-def test_unsteady_rejects_moving_meshes():
+def test_unsteady_argument_errors():
+    import dataclasses
     from flite3d_amd.device import DeviceError
     lv = levels(8, 2)
     P = euler(1).to_struct()
-    P.flow_type = 2
+    P.flow_type = 3
+    with pytest.raises(DeviceError):
+        Solver(P, list(lv))
+    P = euler(1, flowtype=2, physicaltimestep=0.05).to_struct()
+    P.physical_time_step = 0.0
     with pytest.raises(DeviceError):
         Solver(P, list(lv))
     s = Solver(euler(1).to_struct(), list(lv))
     with pytest.raises(DeviceError):
         s.advance_physical_time()  # steady run: no physical time
+    with pytest.raises(DeviceError):
+        s.replace_mesh(list(lv))  # a fixed mesh cannot be replaced
+    s.close()
+    s = Solver(euler(1, flowtype=2, physicaltimestep=0.05).to_struct(), list(lv))
+    s.init_freestream()
+    with pytest.raises(DeviceError):  # the moved mesh keeps the level's counts
+        s.replace_mesh([lv[1], lv[1]])
+    with pytest.raises(DeviceError):  # nor may the tangent run on a moving mesh
+        s.tangent(np.zeros((5, lv[0].n_nodes)))
     s.close()
