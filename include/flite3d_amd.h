@@ -81,9 +81,9 @@ typedef struct flite_params {
   int coarse_grid_dissipation_scheme;     /* 2 = JST (scheme 2 only) */
   int use_matrix_dissipation;             /* 1: Roe-matrix scaling instead of the scalar d0 */
   double hartens_correction_factor;       /* entropy fix of the matrix scaling */
-  /* dual time stepping on a fixed mesh (flowType 1; addTimeDifferences,
-   * mgnsg3d_def_par.F90:1184-1241; solveSystem, 1427-1436) */
-  int flow_type;                   /* 0 steady, 1 unsteady (fixed mesh) */
+  /* dual time stepping (addTimeDifferences, mgnsg3d_def_par.F90:1184-1241;
+   * solveSystem, 1427-1436) */
+  int flow_type;                   /* 0 steady, 1 fixed mesh, 2 moving mesh (flite_level_geometry) */
   int time_scheme;                 /* 2: BDF2 on level 1, otherwise BDF1 */
   int add_time_step;               /* 0 drops the time differences (ivd%addTimeStep) */
   double physical_time_step;
@@ -158,7 +158,7 @@ int flite_init_freestream(flite_ctx *ctx);
 int flite_init_from_state(flite_ctx *ctx);
 
 /* ---- stepping (mgnsg3d_mg_par.F90:939-1186 callers) --------------------- */
-/* End of a physical time step of an unsteady run (flow_type 1):
+/* End of a physical time step of an unsteady run (flow_type 1 or 2):
  * uold2 <- uold <- u on level 1, as multiGridCycles does after each
  * physical step (mgnsg3d_mg_par.F90:889-890).  Coarse levels keep the uold
  * restricted at the start (mgnsg3d_mg_par.F90:590-591), as in the reference. */
