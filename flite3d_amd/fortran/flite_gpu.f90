@@ -118,6 +118,13 @@ module flite_gpu
        type(c_ptr), intent(out) :: ctx
      end function flite_ctx_create
 
+     integer(c_int) function flite_level_geometry(ctx, level, v) bind(C, name="flite_level_geometry")
+       import :: c_int, c_ptr, flite_level_view
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: level
+       type(flite_level_view), intent(in) :: v
+     end function flite_level_geometry
+
      integer(c_int) function flite_level_upload(ctx, level, v) bind(C, name="flite_level_upload")
        import :: c_int, c_ptr, flite_level_view
        type(c_ptr), value :: ctx

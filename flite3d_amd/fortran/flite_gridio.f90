@@ -32,12 +32,19 @@ module flite_gridio
      integer(c_int) :: numberOfFineNodes = 0
      integer(c_int), allocatable :: prolongationArray(:)        ! pod%prolongationArray(Nfine)
      real(c_double), allocatable :: restrictionArray(:)         ! rod%restrictionArray(Nfine)
+     ! moving meshes, as readGridComputationData leaves them (DEF:816-819,
+     ! 13855-13861, 13891-13900)
+     real(c_double), allocatable :: GCLWeightsArray(:)          ! (E)
+     real(c_double), allocatable :: bGCLWeightsArray(:, :)      ! brp%GCLWeightsArray(Eb,2)
+     real(c_double), allocatable :: nodeVolumeP(:), nodeVolumeP2(:)  ! level 1
+     real(c_double), allocatable :: coordinateMovement(:, :)    ! (N,3), level 1
   end type GridLevelData
 
 contains
 
-  subroutine readComputationData(unit, numberOfGridsToUse, grp, numberOfGrids)
+  subroutine readComputationData(unit, numberOfGridsToUse, grp, numberOfGrids, timeScheme)
     integer, intent(in) :: unit, numberOfGridsToUse
+    integer, intent(in) :: timeScheme  ! ivd%timeScheme: 2 combines the GCL weights of two steps
     type(GridLevelData), allocatable, intent(out) :: grp(:)
     integer, intent(out) :: numberOfGrids
     integer :: i, domain, numberOfGridDomains, nInFile
@@ -49,22 +56,26 @@ contains
     allocate (grp(numberOfGrids))
     do i = 1, numberOfGrids
        if (i == 1) then
-          call readGridComputationData(unit, grp(i), .true., 0)
+          call readGridComputationData(unit, grp(i), .true., 0, timeScheme)
        else
-          call readGridComputationData(unit, grp(i), .false., grp(i - 1)%numberOfNodes)
+          call readGridComputationData(unit, grp(i), .false., grp(i - 1)%numberOfNodes, timeScheme)
        end if
     end do
   end subroutine readComputationData
 
-  subroutine readBoundarySolverData(unit, g)
-    integer, intent(in) :: unit
+  subroutine readBoundarySolverData(unit, g, isFine, timeScheme)
+    integer, intent(in) :: unit, timeScheme
     type(GridLevelData), intent(inout) :: g
+    logical, intent(in) :: isFine
     integer :: i, k, nEngine
     real(c_double) :: gcl(2), gclPrev(2)
     read (unit) g%numberOfBoundarySides
     allocate (g%bsideIndexArray(g%numberOfBoundarySides, 3), g%bsideWeightsArray(g%numberOfBoundarySides, 3))
+    allocate (g%bGCLWeightsArray(g%numberOfBoundarySides, 2))
     do i = 1, g%numberOfBoundarySides
        read (unit) g%bsideIndexArray(i, 1:3), g%bsideWeightsArray(i, 1:3), gcl, gclPrev
+       g%bGCLWeightsArray(i, :) = gcl
+       if (timeScheme == 2 .and. isFine) g%bGCLWeightsArray(i, :) = 1.5_c_double * gcl - 0.5_c_double * gclPrev
     end do
     read (unit) g%numberOfBoundaryNodes, g%numberOfNormals
     do k = -20, 0
@@ -79,27 +90,38 @@ contains
     if (nEngine /= 0) error stop "ERROR: engine inlet sides are not on the GPU path"
   end subroutine readBoundarySolverData
 
-  subroutine readGridComputationData(unit, g, isFine, numberOfFineNodes)
-    integer, intent(in) :: unit, numberOfFineNodes
+  subroutine readGridComputationData(unit, g, isFine, numberOfFineNodes, timeScheme)
+    integer, intent(in) :: unit, numberOfFineNodes, timeScheme
     type(GridLevelData), intent(inout) :: g
     logical, intent(in) :: isFine
     integer :: i, idx, wdn, seed, ntrip, ntripField, nComSides, nBComSides, nSend, nRecv
     integer :: nIntFine, nIntCoarse, nCom, f, c
     real(c_double) :: vol, volP, volP2, wd, gcl, gclPrev, length, x(3), xP(3), xP2(3), rw
 
-    call readBoundarySolverData(unit, g)
+    call readBoundarySolverData(unit, g, isFine, timeScheme)
     read (unit) g%numberOfSides
     allocate (g%sideIndexArray(g%numberOfSides, 2), g%sideWeightsArray(g%numberOfSides, 3))
+    allocate (g%GCLWeightsArray(g%numberOfSides))
     do i = 1, g%numberOfSides
        read (unit) g%sideIndexArray(i, 1:2), g%sideWeightsArray(i, 1:3), gcl, gclPrev, length
+       g%GCLWeightsArray(i) = gcl
+       if (timeScheme == 2 .and. isFine) g%GCLWeightsArray(i) = 1.5_c_double * gcl - 0.5_c_double * gclPrev
     end do
     read (unit) g%numberOfNodes
     allocate (g%nodeVolume(g%numberOfNodes), g%wallDistance(g%numberOfNodes), &
          g%wallDistanceBoundaryNodeArray(g%numberOfNodes), g%seedNodeRegister(g%numberOfNodes))
     if (isFine) then
-       allocate (g%coordinates(g%numberOfNodes, 3))
+       allocate (g%coordinates(g%numberOfNodes, 3), g%coordinateMovement(g%numberOfNodes, 3))
+       allocate (g%nodeVolumeP(g%numberOfNodes), g%nodeVolumeP2(g%numberOfNodes))
        do i = 1, g%numberOfNodes
           read (unit) idx, vol, volP, volP2, wd, wdn, x, xP, xP2
+          g%nodeVolumeP(idx) = volP
+          g%nodeVolumeP2(idx) = volP2
+          if (timeScheme == 1) then
+             g%coordinateMovement(idx, :) = x - xP
+          else
+             g%coordinateMovement(idx, :) = 1.5_c_double * x - 2.0_c_double * xP + 0.5_c_double * xP2
+          end if
           g%nodeVolume(idx) = vol
           g%wallDistance(idx) = wd
           g%wallDistanceBoundaryNodeArray(idx) = wdn
@@ -153,9 +175,10 @@ contains
   end subroutine readGridComputationData
 
   ! non-owning C view of one level (index_base 1: Fortran numbering)
-  function level_view(g, isFine) result(v)
+  function level_view(g, isFine, moving) result(v)
     type(GridLevelData), intent(in), target :: g
     logical, intent(in) :: isFine
+    logical, intent(in), optional :: moving  ! pass the moving-mesh arrays (flowType 2)
     type(flite_level_view) :: v
     v%index_base = 1
     v%n_nodes = g%numberOfNodes
@@ -180,6 +203,17 @@ contains
        v%n_fine = g%numberOfFineNodes
        v%prolongation = c_loc(g%prolongationArray)
        v%restriction = c_loc(g%restrictionArray)
+    end if
+    if (present(moving)) then
+       if (moving) then
+          v%side_gcl = c_loc(g%GCLWeightsArray)
+          v%bside_gcl = c_loc(g%bGCLWeightsArray)
+          if (isFine) then
+             v%node_volume_p = c_loc(g%nodeVolumeP)
+             v%node_volume_p2 = c_loc(g%nodeVolumeP2)
+             v%coordinate_movement = c_loc(g%coordinateMovement)
+          end if
+       end if
     end if
   end function level_view
 

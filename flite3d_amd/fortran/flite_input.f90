@@ -275,8 +275,8 @@ contains
   ! silently running something else
   subroutine check_supported(ivd)
     type(InputVariablesData), intent(in) :: ivd
-    if (ivd%flowType /= 0 .and. ivd%flowType /= 1) &
-         error stop "ERROR: this driver runs steady flow and unsteady flow on a fixed mesh (flowType 0, 1)"
+    if (ivd%flowType < 0 .or. ivd%flowType > 2) &
+         error stop "ERROR: flowType must be 0 (steady), 1 (fixed mesh) or 2 (moving mesh)"
     if (ivd%flowType /= 0 .and. ivd%explicit) error stop "ERROR: explicit time stepping is not on the GPU path"
     if (ivd%turbulenceModel /= 0 .and. ivd%turbulenceModel /= 1) &
          error stop "ERROR: only turbulenceModel 0 and 1 (Spalart-Allmaras) run on the GPU path"

@@ -14,7 +14,10 @@
 ! the result is merged into <case>.unk (makeplot, MG:1599-1707).  Unsteady
 ! runs on a fixed mesh (flowType 1) loop over the physical time steps
 ! restartNumber .. numberOfPhysicalTimesteps around multiGridCycles' loop
-! (MG:486-890); a moving mesh (flowType 2, a new .sol per step) is not run.  Only single-domain runs exist here: the MPI partitioning
+! (MG:486-890); a moving mesh (flowType 2) reads <case>_<k>.sol_1 with k the
+! step's place in numberOfStepsPerCycle, first at the start step (MG:164-193)
+! and then before every later step (MG:505-548), and hands each level's new
+! geometry to flite_level_geometry.  Only single-domain runs exist here: the MPI partitioning
 ! collapses to one domain on one GPU.
 !
 ! Usage: mgnsg3d_gpu [--coarse-fp32]
@@ -31,10 +34,10 @@ program mgnsg3d_gpu
   implicit none
 
   type(InputVariablesData) :: ivd
-  type(GridLevelData), allocatable, target :: grp(:)
+  type(GridLevelData), allocatable, target :: grp(:), grpNext(:)
   type(flite_params) :: p
   type(flite_level_view) :: v
-  character(len=256) :: fileName, arg
+  character(len=256) :: fileName, arg, solName
   integer :: numberOfGrids, i, l, nvar, nres, k, physicalTimestepNumber, firstStep, lastStep
   logical :: coarseFp32, switched, havePlotReg
   real(c_double), allocatable :: u(:, :)
@@ -68,9 +71,11 @@ program mgnsg3d_gpu
   if (ivd%turbulenceModel == 1) nvar = 6
   nres = nvar
 
-  write (*, '(A)') "reading "//trim(fileName)//".sol_1 ..."
-  open (20, file=trim(fileName)//".sol_1", form='unformatted', status='old', access='sequential')
-  call readComputationData(20, ivd%numberOfGridsToUse, grp, numberOfGrids)
+  solName = trim(fileName)//".sol_1"
+  if (ivd%flowType == 2) solName = step_sol_name(ivd%restartNumber)
+  write (*, '(A)') "reading "//trim(solName)//" ..."
+  open (20, file=trim(solName), form='unformatted', status='old', access='sequential')
+  call readComputationData(20, ivd%numberOfGridsToUse, grp, numberOfGrids, ivd%timeScheme)
   close (20)
   write (*, '(A,I3,A,I10,A,I11,A)') "Number of grids:", numberOfGrids, "  fine grid:", &
        grp(1)%numberOfNodes, " nodes,", grp(1)%numberOfSides, " sides"
@@ -78,7 +83,7 @@ program mgnsg3d_gpu
   p = to_flite_params(ivd, coarseFp32)
   call flite_check(flite_ctx_create(p, numberOfGrids, 1, gpu_ctx), "flite_ctx_create")
   do l = 1, numberOfGrids
-     v = level_view(grp(l), l == 1)
+     v = level_view(grp(l), l == 1, ivd%flowType == 2)
      call flite_check(flite_level_upload(gpu_ctx, l, v), "readGridComputationData")
   end do
 
@@ -107,6 +112,20 @@ program mgnsg3d_gpu
 
   do physicalTimestepNumber = firstStep, lastStep
   if (ivd%flowType /= 0) write (*, '(A,I7)') " >>>>>>>>>----> number of timestep = ", physicalTimestepNumber
+  if (ivd%flowType == 2 .and. physicalTimestepNumber > firstStep) then
+     ! the step's mesh (MG:505-548): new geometry, the solution kept
+     solName = step_sol_name(physicalTimestepNumber)
+     write (*, '(A)') " new data File "//trim(solName)//" opened ..."
+     open (20, file=trim(solName), form='unformatted', status='old', access='sequential')
+     call readComputationData(20, ivd%numberOfGridsToUse, grpNext, l, ivd%timeScheme)
+     close (20)
+     if (l /= numberOfGrids) error stop "ERROR: the step's grid file has another number of grids"
+     do l = 1, numberOfGrids
+        v = level_view(grpNext(l), l == 1, .true.)
+        call flite_check(flite_level_geometry(gpu_ctx, l, v), "readComputationData")
+     end do
+     call move_alloc(grpNext, grp)
+  end if
   currentResidual = 10000.0_c_double
   stopAtResidualReduction = -1.0_c_double
   if (ivd%numberOfMGIterations < 0) stopAtResidualReduction = abs(dble(ivd%numberOfMGIterations))
@@ -165,6 +184,19 @@ program mgnsg3d_gpu
   call flite_ctx_destroy(gpu_ctx)
 
 contains
+
+  ! <case>_<k>.sol_1 of a moving mesh, k = mod(step, numberOfStepsPerCycle)
+  ! with 0 read as numberOfStepsPerCycle (MG:164-193, 505-548)
+  function step_sol_name(step) result(name)
+    integer, intent(in) :: step
+    character(len=256) :: name
+    character(len=16) :: ext
+    integer :: k
+    k = mod(step, ivd%numberOfStepsPerCycle)
+    if (k == 0) k = ivd%numberOfStepsPerCycle
+    write (ext, '(I0)') k
+    name = trim(fileName)//"_"//trim(ext)//".sol_1"
+  end function step_sol_name
 
   ! writeResiduals (MG:1271-1516): screen line, NaN check, .rsd line
   subroutine writeResiduals(iterationNumber)

@@ -139,6 +139,10 @@ def write_sol(path, levels: list[Level]):
             b["i"][:, :2] = L.bside_index[:2].T + 1
             b["i"][:, 2] = L.bside_index[2]
             b["w"] = L.bside_w.T
+            if getattr(L, "bgcl", None) is not None:
+                # a moving mesh's weights as the solver holds them: GCLprev
+                # = GCL, so 1.5 GCL - 0.5 GCLprev (timeScheme 2) is GCL too
+                b["gcl"] = b["gclp"] = np.asarray(L.bgcl).T
             W.rec(nbs)
             W.table(_BSIDE, b)
             nia = np.asarray(L.node_ind_array, np.int64)
@@ -156,6 +160,8 @@ def write_sol(path, levels: list[Level]):
             X = fine.coords[:, np.asarray(L.seed, np.int64)]  # seeds are level-1 nodes
             d = X[:, L.side_index[1]] - X[:, L.side_index[0]]
             s["len"] = np.sqrt((d * d).sum(0))
+            if getattr(L, "gcl", None) is not None:
+                s["gcl"] = s["gclp"] = L.gcl
             W.rec(ne)
             W.table(_SIDE, s)
             n = L.n_nodes
@@ -168,6 +174,14 @@ def write_sol(path, levels: list[Level]):
                 t["wn"] = L.wall_node + 1
                 x = L.coords.T
                 t["x"] = np.concatenate([x, x, x], axis=1)
+                if getattr(L, "volume_p", None) is not None:
+                    t["v"][:, 1] = L.volume_p
+                    t["v"][:, 2] = L.volume_p2
+                if getattr(L, "coord_move", None) is not None:
+                    # previous coordinates x - dx and x - 2 dx: the movement
+                    # is dx for timeScheme 1 and 2 alike
+                    m = np.asarray(L.coord_move).T
+                    t["x"] = np.concatenate([x, x - m, x - 2 * m], axis=1)
                 W.table(_FNODE, t)
                 W.rec(0)  # trip nodes
                 W.rec(0)  # trip field nodes

@@ -159,7 +159,7 @@ def test_driver_reads_inputs(tmp_path):
 
 @pytest.mark.parametrize("extra,msg", [
     (",\n ivd%movingwall = .true.", "moving walls"),
-    (",\n ivd%flowType = 2", "fixed mesh"),
+    (",\n ivd%flowType = 3", "moving mesh"),
 ])
 def test_driver_rejects_options_outside_the_path(tmp_path, extra, msg):
     lv = levels(8, 2)
@@ -258,6 +258,56 @@ def test_driver_unsteady_matches_python_path(tmp_path):
     assert np.allclose(u_f, u_s, rtol=1e-14, atol=0), np.abs(u_f - u_s).max()
     lines = [x.split() for x in open(os.path.join(d, "liftdrag.res")).read().split("\n") if x.strip()]
     assert [float(x[0]) for x in lines] == pytest.approx([0.05, 0.10, 0.15])
+
+
+@pytest.mark.gpu
+def test_driver_moving_mesh_matches_python_path(tmp_path):
+    """flowType 2: the driver starts on <case>_1.sol_1 and reads
+    <case>_2.sol_1 and <case>_3.sol_1 before steps 2 and 3 (MG:164-193,
+    505-548); its .res_1 equals the Python path replacing the meshes.  The
+    files hold GCL weights, previous volumes and previous coordinates; the
+    driver forms the movement from coordinates (x - x_prev in rounding), so
+    the bar is 1e-11."""
+    print("[case] fortran driver moving mesh euler 8^3, 3 steps x 2 cycles", flush=True)
+    from flite3d_amd.device import Solver
+    from cases import euler
+    from test_moving_mesh import moving
+    lv = levels(8, 2)
+    d = str(tmp_path)
+    extra = (",\n ivd%flowType = 2,\n ivd%physicalTimeStep = 0.05,\n ivd%numberOfPhysicalTimesteps = 3"
+             ",\n ivd%numberOfStepsPerCycle = 3,\n ivd%timeScheme = 2")
+    write_case(d, lv, iters=2, re=0.0, turb=0, order=1, mach=0.5, alpha=2.0, cfl=2.0, restart=1, extra=extra)
+    rng = np.random.default_rng(21)
+    dx = np.array([0.002, 0.0, -0.001])
+    meshes = [moving(lv, rng, 2 + k, dx, consistent=True) for k in range(3)]
+    for k, m in enumerate(meshes):
+        solfile.write_sol(os.path.join(d, f"bump_{k + 1}.sol_1"), m)
+    u0 = np.zeros((5, lv[0].n_nodes))
+    inf = euler(1, machnumber=0.5, alpha=2.0).inflowfield[0]
+    for c in range(5):
+        u0[c] = inf[c]
+    solfile.write_results(os.path.join(d, "bump.rst_1"), u0, 0)
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bump_3.sol_1" in r.stdout
+    u_f = solfile.read_results(os.path.join(d, "bump.res_1"), 0)
+    iv = euler(1, machnumber=0.5, alpha=2.0, cflnumber=2.0, flowtype=2, physicaltimestep=0.05,
+               timescheme=2, wallsareisentropic=True)
+    s = Solver(iv.to_struct(), meshes[0])
+    full = np.zeros((6, lv[0].n_nodes))
+    full[:5] = solfile.read_results(os.path.join(d, "bump.rst_1"), 0)
+    s.set(1, "U", full)
+    s.init_from_state()
+    for step in range(3):
+        if step:
+            s.replace_mesh(meshes[step])
+        for c in (1, 2):
+            s.cycle(c)
+        s.advance_physical_time()
+    u_s = s.get(1, "U")[:5]
+    s.close()
+    err = np.abs(u_f - u_s).max(axis=1) / np.abs(u_s).max(axis=1)
+    assert err.max() < 1e-11, err
 
 
 @pytest.mark.gpu
