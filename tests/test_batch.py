@@ -71,7 +71,7 @@ def test_batch_rejects_differing_settings():
         Solver(ps, list(lv))
 
 
-@pytest.mark.parametrize("case", ["roe_jst", "unsteady"])
+@pytest.mark.parametrize("case", ["roe_jst", "unsteady", "moving"])
 def test_jst_and_unsteady_batch_equals_single_runs(case):
     """The JST node records and the physical-time states (uold, uold2) are
     per condition too: each condition of the batch equals its single run."""
@@ -81,18 +81,28 @@ def test_jst_and_unsteady_batch_equals_single_runs(case):
     if case == "roe_jst":
         kw = dict(fluxtype=3, usematrixdissipation=True, secondorderdissipationfactor=0.5,
                   fourthorderdissipationfactor=1 / 64.0, cflnumber=1.5)
+    elif case == "moving":
+        kw = dict(flowtype=2, physicaltimestep=0.05)
     else:
         kw = dict(flowtype=1, physicaltimestep=0.05)
     ps = [euler(1, machnumber=m, alpha=a, multigridscheme=3, **kw).to_struct() for m, a in conds]
+    meshes = [list(lv)]
+    if case == "moving":
+        from test_moving_mesh import moving
+        rng = np.random.default_rng(31)
+        dx = np.array([0.002, 0.0, -0.001])
+        meshes = [moving(lv, rng, 2 + k, dx, consistent=True) for k in range(2)]
 
     def go(params):
-        s = Solver(params, list(lv))
+        s = Solver(params, meshes[0])
         s.init_freestream()
         nb = len(params) if isinstance(params, list) else 1
         for c in range(1, 5):
             s.cycle(c)
-            if case == "unsteady" and c % 2 == 0:
+            if case != "roe_jst" and c % 2 == 0:
                 s.advance_physical_time()
+                if case == "moving" and c == 2:
+                    s.replace_mesh(meshes[1])  # carries every condition's u, uold, uold2
         us = []
         for b in range(nb):
             s.select(b)
