@@ -763,7 +763,10 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   // the node's own density and energy and the reconstructed momentum (the
   // reference's r1, E1 are not reconstructed, its ru1..rw1 are)
   T q0 = lft ? A[0] : B[0], q1 = ruk, q2 = rvk, q3 = rwk, q4 = lft ? A[4] : B[4];
-  if (star && fk != T(0)) {
+  if (star && fk == T(0)) {  // degenerate star state: U_K as reconstructed (DEF:1997-2003, 2012-2018)
+    q0 = rk;
+    q4 = ek;
+  } else if (star) {
     fk = frcp(fk);
     T a = sk - nvk;
     T pks = rk * (nvk - sk) * (nvk - sm) + pk;
@@ -2961,10 +2964,11 @@ __global__ void k_stage(LevelDev<R> L, DevParams P, StageArgs S) {
     for (int c = 0; c < 6; ++c) L.rhs[(size_t)c * n + i] = r[c];
     return;
   }
-  if (MODE == MODE_STAGE && P.unsteady && P.add_time) {
+  if (MODE != MODE_RHS && P.unsteady && P.add_time) {
     // addTimeDifferences (DEF:1184-1241): BDF2 on grid 1 with the volumes
     // of the last two physical times (equal to V on a fixed mesh), BDF1
-    // elsewhere, over components 1:5 (1:6 with SA)
+    // elsewhere, over components 1:5 (1:6 with SA).  restrictGrid adds it
+    // on both grids (DEF:12208 on the restricted state, DEF:12285)
     const R V = L.vol[i], dtp = R(P.dtp);
     const R VP = L.volp ? L.volp[i] : V, VP2 = L.volp2 ? L.volp2[i] : V;
 #pragma unroll

@@ -260,6 +260,7 @@ struct flite_ctx {
   // nbatch x ncomp blocks of n
   std::vector<std::vector<std::vector<double>>> carry;
   bool built = false;
+  bool preamble_due = false;  // a replaced mesh still owes the step's preamble
 };
 
 namespace {
@@ -1004,6 +1005,9 @@ int ensure_built(flite_ctx *c) {
       TRY(carry_in(L, L.uold2, 6, carry[l][2]));
       TRY(carry_in(L, L.mu, 1, carry[l][3]));
     }
+    // the step's preamble (MG:586-601) follows its readComputationData
+    // (MG:545): coarse u and uold restricted with the new transfer weights
+    c->preamble_due = true;
   }
   return 0;
 }
@@ -1442,17 +1446,48 @@ int do_cycle(flite_ctx *c, int cycle) {
   return 0;
 }
 
+// multiGridCycles preamble (MG:586-602): laminar viscosity, u and (for
+// unsteady runs) uold restricted to every coarse level
+int preamble(flite_ctx *c) {
+  double mu0 = c->hp.reynolds_number > 0.0 ? 1.0 / c->hp.reynolds_number : 0.0;
+  for (int l = 0; l < c->nl; ++l) {
+    Level &L = *c->lev[l];
+    TRY(with_level(L, [&](auto V) {
+      using R = std::remove_pointer_t<decltype(V.u)>;
+      size_t cnt = (size_t)L.n * L.nbatch;
+      hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), cnt);
+      return check_launch();
+    }));
+    if (l > 0) {
+      TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
+      if (c->hp.flow_type != 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->uold, L.uold, 6));
+    }
+  }
+  return sync(c);
+}
+
+// uold2 <- src_old, uold <- src on level 1 (unsteady runs)
+int set_old_states(flite_ctx *c, const void *old_src, const void *src) {
+  if (c->hp.flow_type == 0) return 0;
+  Level &L = *c->lev[0];
+  TRY(op_copy(c, L, L.uold2, old_src, 6));
+  return op_copy(c, L, L.uold, src, 6);
+}
+
 int check_ctx(flite_ctx *c) {
   if (!c) return fail(FLITE_E_ARG, "null context");
   TRY(ensure_built(c));
+  if (c->preamble_due) {
+    c->preamble_due = false;
+    TRY(preamble(c));
+  }
   return 0;
 }
 
 int check_level(flite_ctx *c, int level) {
   if (!c) return fail(FLITE_E_ARG, "null context");
   if (level < 1 || level > c->nl) return fail(FLITE_E_ARG, "level out of range");
-  TRY(ensure_built(c));
-  return 0;
+  return check_ctx(c);
 }
 
 void *field_ptr(Level &L, int field, int *ncomp, size_t *stride_nodes) {
@@ -1470,6 +1505,8 @@ void *field_ptr(Level &L, int field, int *ncomp, size_t *stride_nodes) {
     case FLITE_FIELD_DIVERGENCE: *ncomp = 1; return L.divg;
     case FLITE_FIELD_SOURCE: *ncomp = 6; return L.src;
     case FLITE_FIELD_TURB_DIFFUSION: *ncomp = 1; return L.tdiff;
+    case FLITE_FIELD_UOLD: *ncomp = 6; return L.uold;
+    case FLITE_FIELD_UOLD2: *ncomp = 6; return L.uold2;
     default: return nullptr;
   }
 }
@@ -1484,7 +1521,7 @@ extern "C" {
 const char *flite_last_error(void) { return g_err.c_str(); }
 
 int flite_field_ncomp(int field) {
-  static const int nc[FLITE_FIELD_COUNT] = {6, 6, 6, 5, 1, 1, 1, 1, 1, 1, 6, 1, 25, 3};
+  static const int nc[FLITE_FIELD_COUNT] = {6, 6, 6, 5, 1, 1, 1, 1, 1, 1, 6, 1, 25, 3, 6, 6};
   return field >= 0 && field < FLITE_FIELD_COUNT ? nc[field] : -1;
 }
 
@@ -1682,6 +1719,8 @@ int flite_field_get(flite_ctx *c, int level, int field, double *out, int ld) {
   }
   int nc;
   size_t stride;
+  if ((field == FLITE_FIELD_UOLD || field == FLITE_FIELD_UOLD2) && c->hp.flow_type == 0)
+    return fail(FLITE_E_STATE, "no solutions of earlier physical times in a steady run");
   void *p = field_ptr(L, field, &nc, &stride);
   if (!p) return fail(FLITE_E_ARG, "bad field");
   p = (char *)p + sb * nc * stride * es;
@@ -1697,6 +1736,8 @@ int flite_field_set(flite_ctx *c, int level, int field, const double *in, int ld
   if (!in || ld < L.n) return fail(FLITE_E_ARG, "bad input buffer");
   int nc;
   size_t stride;
+  if ((field == FLITE_FIELD_UOLD || field == FLITE_FIELD_UOLD2) && c->hp.flow_type == 0)
+    return fail(FLITE_E_STATE, "no solutions of earlier physical times in a steady run");
   void *p = field_ptr(L, field, &nc, &stride);
   if (!p) return fail(FLITE_E_ARG, "field cannot be set");
   p = (char *)p + (size_t)c->sel * nc * stride * (L.fp32 ? 4 : 8);
@@ -1748,34 +1789,6 @@ int flite_state_get(flite_ctx *c, int level, double *u, int ld, int ncomp, int b
   return 0;
 }
 
-// multiGridCycles preamble (MG:586-602): laminar viscosity, u and (for
-// unsteady runs) uold restricted to every coarse level
-int preamble(flite_ctx *c) {
-  double mu0 = c->hp.reynolds_number > 0.0 ? 1.0 / c->hp.reynolds_number : 0.0;
-  for (int l = 0; l < c->nl; ++l) {
-    Level &L = *c->lev[l];
-    TRY(with_level(L, [&](auto V) {
-      using R = std::remove_pointer_t<decltype(V.u)>;
-      size_t cnt = (size_t)L.n * L.nbatch;
-      hipLaunchKernelGGL(k_fill<R>, dim3(nblk((long long)cnt)), dim3(NTHR), 0, c->stream, V.mu, R(mu0), cnt);
-      return check_launch();
-    }));
-    if (l > 0) {
-      TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->u, L.u, 6));
-      if (c->hp.flow_type != 0) TRY(op_restrict_array(c, *c->lev[l - 1], L, c->lev[l - 1]->uold, L.uold, 6));
-    }
-  }
-  return sync(c);
-}
-
-// uold2 <- src_old, uold <- src on level 1 (unsteady runs)
-int set_old_states(flite_ctx *c, const void *old_src, const void *src) {
-  if (c->hp.flow_type == 0) return 0;
-  Level &L = *c->lev[0];
-  TRY(op_copy(c, L, L.uold2, old_src, 6));
-  return op_copy(c, L, L.uold, src, 6);
-}
-
 int flite_init_from_state(flite_ctx *c) {
   TRY(check_ctx(c));
   // a restart (DEF:3866-3873): uold and uold2 take the given state
@@ -1788,7 +1801,9 @@ int flite_advance_physical_time(flite_ctx *c) {
   if (c->hp.flow_type == 0) return fail(FLITE_E_STATE, "steady run (flowType 0): no physical time");
   // a captured cycle reads uold and uold2 where they are, so it stays valid
   TRY(set_old_states(c, c->lev[0]->uold, c->lev[0]->u));
-  return sync(c);
+  // the next step's preamble (MG:586-601 sit inside the physical-step loop):
+  // laminar viscosity reset, u and uold restricted to every coarse level
+  return preamble(c);
 }
 
 int flite_init_freestream(flite_ctx *c) {

@@ -29,7 +29,8 @@ module flite_gpu
        FLITE_FIELD_DISSIPATION = 3, FLITE_FIELD_PRESSURE = 4, FLITE_FIELD_DT = 5, &
        FLITE_FIELD_DT_TURB = 6, FLITE_FIELD_LAMINAR_VISCOSITY = 7, FLITE_FIELD_VORTICITY = 8, &
        FLITE_FIELD_DIVERGENCE = 9, FLITE_FIELD_SOURCE = 10, FLITE_FIELD_TURB_DIFFUSION = 11, &
-       FLITE_FIELD_GRADIENT = 12, FLITE_FIELD_WALL_STRESS = 13
+       FLITE_FIELD_GRADIENT = 12, FLITE_FIELD_WALL_STRESS = 13, FLITE_FIELD_UOLD = 14, &
+       FLITE_FIELD_UOLD2 = 15
 
   ! flite_params: the InputVariablesData scalars the hot path uses, after
   ! readInputVariables' post-processing (DEF:386-415)

@@ -384,9 +384,10 @@ def level_view(level, is_fine):
 
 
 FIELD_NCOMP = {0: 6, 1: 6, 2: 6, 3: 5, 4: 1, 5: 1, 6: 1, 7: 1, 8: 1, 9: 1,
-               10: 6, 11: 1, 12: 25, 13: 3}
+               10: 6, 11: 1, 12: 25, 13: 3, 14: 6, 15: 6}
 FIELDS = dict(U=0, UPREV=1, RHS=2, DISSIPATION=3, PRESSURE=4, DT=5,
               DT_TURB=6, LAMINAR_VISCOSITY=7, VORTICITY=8, DIVERGENCE=9,
-              SOURCE=10, TURB_DIFFUSION=11, GRADIENT=12, WALL_STRESS=13)
+              SOURCE=10, TURB_DIFFUSION=11, GRADIENT=12, WALL_STRESS=13,
+              UOLD=14, UOLD2=15)
 OPS = dict(PRESSURE=0, PRESSURE_ABS=1, TIMESTEPS=2, DISSIPATION=3, VISCOUS=4,
            RHS=5, SA_TIMESTEPS=6, SA_DIFFUSION=7)

@@ -140,7 +140,10 @@ int flite_level_upload(flite_ctx *ctx, int level, const flite_level_view *v);
  * 525-548), which replaces the geometry and keeps every solution array.
  * Same node, side and boundary-side counts as the level's last upload; u,
  * uold, uold2 and the laminar viscosity of every level and condition carry
- * over.  Call it for each level whose geometry changes, finest first. */
+ * over.  Call it for each level whose geometry changes, finest first.  The
+ * next operation then runs the step's preamble (mgnsg3d_mg_par.F90:586-601)
+ * on the new geometry: laminar viscosity reset, u and uold restricted to
+ * every coarse level. */
 int flite_level_geometry(flite_ctx *ctx, int level, const flite_level_view *v);
 void flite_ctx_destroy(flite_ctx *ctx);
 const char *flite_last_error(void);
@@ -160,8 +163,10 @@ int flite_init_from_state(flite_ctx *ctx);
 /* ---- stepping (mgnsg3d_mg_par.F90:939-1186 callers) --------------------- */
 /* End of a physical time step of an unsteady run (flow_type 1 or 2):
  * uold2 <- uold <- u on level 1, as multiGridCycles does after each
- * physical step (mgnsg3d_mg_par.F90:889-890).  Coarse levels keep the uold
- * restricted at the start (mgnsg3d_mg_par.F90:590-591), as in the reference. */
+ * physical step (mgnsg3d_mg_par.F90:889-890), then the next step's preamble,
+ * which the reference runs inside its physical-step loop
+ * (mgnsg3d_mg_par.F90:586-601): laminar viscosity reset, u and uold
+ * restricted to every coarse level. */
 int flite_advance_physical_time(flite_ctx *ctx);
 /* doTimeIterations (mgnsg3d_def_par.F90:966-1000) */
 int flite_do_time_iterations(flite_ctx *ctx, int level, int relaxation_steps,
@@ -189,13 +194,15 @@ int flite_monitor(flite_ctx *ctx, double res[7], double maxres[7], int maxidx[7]
  *   FLITE_FIELD_U (6), UPREV (6), RHS (6), DISSIPATION (5), PRESSURE, DT,
  *   DT_TURB, LAMINAR_VISCOSITY, VORTICITY, DIVERGENCE, SOURCE (6),
  *   TURB_DIFFUSION, GRADIENT (25: nodeHelpArray 1-15 gradients, 16-25 limits),
- *   WALL_STRESS (3, boundary nodes only, zeros elsewhere) */
+ *   WALL_STRESS (3, boundary nodes only, zeros elsewhere),
+ *   UOLD (6), UOLD2 (6): solutions at the last two physical times (grp%uold,
+ *   grp%uold2; unsteady runs only) */
 enum {
   FLITE_FIELD_U = 0, FLITE_FIELD_UPREV, FLITE_FIELD_RHS, FLITE_FIELD_DISSIPATION,
   FLITE_FIELD_PRESSURE, FLITE_FIELD_DT, FLITE_FIELD_DT_TURB,
   FLITE_FIELD_LAMINAR_VISCOSITY, FLITE_FIELD_VORTICITY, FLITE_FIELD_DIVERGENCE,
   FLITE_FIELD_SOURCE, FLITE_FIELD_TURB_DIFFUSION, FLITE_FIELD_GRADIENT,
-  FLITE_FIELD_WALL_STRESS, FLITE_FIELD_COUNT
+  FLITE_FIELD_WALL_STRESS, FLITE_FIELD_UOLD, FLITE_FIELD_UOLD2, FLITE_FIELD_COUNT
 };
 int flite_field_get(flite_ctx *ctx, int level, int field, double *out, int ld);
 int flite_field_set(flite_ctx *ctx, int level, int field, const double *in, int ld);

@@ -52,7 +52,7 @@ def lib():
 
 
 FIELD_NCOMP = {0: 6, 1: 6, 2: 6, 3: 5, 4: 1, 5: 1, 6: 1, 7: 1, 8: 1, 9: 1,
-               10: 6, 11: 1, 12: 25, 13: 3}
+               10: 6, 11: 1, 12: 25, 13: 3, 14: 6, 15: 6}
 
 
 class OracleError(RuntimeError):

@@ -93,7 +93,10 @@ struct or_ctx {
   orl L[MAXLEV];
   double *help; /* nodeHelpArray(Nfine,25), shared by all levels (DEF:14214-14231) */
   int nh;
+  int pending; /* a level's geometry was replaced: the step's preamble is still due */
 };
+
+static void settle(or_ctx *C);
 
 #define U(L, c, i) ((L)->u[(size_t)(c) * (L)->n + (i)])
 #define UP(L, c, i) ((L)->uprev[(size_t)(c) * (L)->n + (i)])
@@ -276,6 +279,9 @@ int or_level_geometry(or_ctx *c, int level, const flite_level_view *v) {
     memcpy(L->mu, keep[3], sizeof(double) * L->n);
   }
   for (int k = 0; k < 4; ++k) free(keep[k]);
+  /* the step's preamble (MG:586-601) follows readComputationData (MG:545):
+   * it runs once every level has its new geometry, at the next operation */
+  if (!rc) c->pending = 1;
   return rc;
 }
 
@@ -1588,6 +1594,7 @@ static int single_iteration(or_ctx *C, orl *L, int calc_diss, int calc_turb, int
 }
 
 int or_do_time_iterations(or_ctx *c, int level, int steps, int calc_diss, int calc_turb, int iteration) {
+  settle(c);
   if (!c || level < 1 || level > c->nl || !c->L[level - 1].uploaded) return fail(FLITE_E_ARG, "bad level");
   orl *L = &c->L[level - 1];
   int n = L->n;
@@ -1624,6 +1631,7 @@ static void map_c2f(const orl *Cl, const orl *F, const double *cu, double *fu, i
 
 /* restrictGrid (DEF:12141-12430) */
 int or_restrict(or_ctx *C, int fine_level, int calc_turb) {
+  settle(C);
   if (!C || fine_level < 1 || fine_level >= C->nl) return fail(FLITE_E_ARG, "bad level");
   orl *F = &C->L[fine_level - 1], *K = &C->L[fine_level];
   const flite_params *P = &C->P;
@@ -1645,6 +1653,7 @@ int or_restrict(or_ctx *C, int fine_level, int calc_turb) {
   if (P->reynolds_number > 0.0) viscous(C, K, K->diss);
   memset(K->rhs, 0, sizeof(double) * 6 * (size_t)nc);
   if ((rc = make_rhs(C, K))) return rc;
+  time_differences(C, K); /* DEF:12208, on the restricted state */
   if (sac) {
     sa_timesteps(C, K);
     sa_diffusion(C, K);
@@ -1669,6 +1678,7 @@ int or_restrict(or_ctx *C, int fine_level, int calc_turb) {
   if (P->reynolds_number > 0.0) viscous(C, F, F->diss);
   memset(F->rhs, 0, sizeof(double) * 6 * (size_t)nf);
   if ((rc = make_rhs(C, F))) return rc;
+  time_differences(C, F); /* DEF:12285 */
   if (sac) {
     sa_diffusion(C, F);
     sa_rhs(C, F);
@@ -1709,6 +1719,7 @@ int or_restrict(or_ctx *C, int fine_level, int calc_turb) {
 
 /* prolongateGrid (DEF:12432-12570) */
 int or_prolongate(or_ctx *C, int coarse_level) {
+  settle(C);
   if (!C || coarse_level < 2 || coarse_level > C->nl) return fail(FLITE_E_ARG, "bad level");
   orl *K = &C->L[coarse_level - 1], *F = &C->L[coarse_level - 2];
   const flite_params *P = &C->P;
@@ -1742,6 +1753,7 @@ int or_prolongate(or_ctx *C, int coarse_level) {
 }
 
 int or_map_u_fine_to_coarse(or_ctx *C, int fine_level) {
+  settle(C);
   if (!C || fine_level < 1 || fine_level >= C->nl) return fail(FLITE_E_ARG, "bad level");
   map_f2c(&C->L[fine_level - 1], &C->L[fine_level], C->L[fine_level - 1].u, C->L[fine_level].u, 6);
   return 0;
@@ -1749,6 +1761,7 @@ int or_map_u_fine_to_coarse(or_ctx *C, int fine_level) {
 
 /* oneCycle (MG:907-1060), schemes <= 0, 1 and 3 */
 int or_cycle(or_ctx *C, int cycle) {
+  settle(C);
   const flite_params *P = &C->P;
   int ng = C->nl, rc;
   int ct = cycle <= P->number_of_turbulence_steps;
@@ -1809,9 +1822,18 @@ static void preamble(or_ctx *C) {
   }
 }
 
+/* the preamble a mesh replacement left due (or_level_geometry) */
+static void settle(or_ctx *C) {
+  if (C && C->pending) {
+    C->pending = 0;
+    preamble(C);
+  }
+}
+
 /* setUpInitialField2 freestream start (DEF:3940-3996): uold and uold2 take
  * the freestream before the boundary conditions act on u (DEF:3980-3994) */
 int or_init_freestream(or_ctx *C) {
+  C->pending = 0;
   orl *L = &C->L[0];
   const flite_params *P = &C->P;
   int n = L->n, rc;
@@ -1831,6 +1853,7 @@ int or_init_freestream(or_ctx *C) {
 /* a start from a given level-1 state (restart, DEF:3866-3873: uold and
  * uold2 take it too) followed by the preamble */
 int or_init_from_state(or_ctx *C) {
+  C->pending = 0;
   orl *L = &C->L[0];
   memcpy(L->uold, L->u, sizeof(double) * 6 * (size_t)L->n);
   memcpy(L->uold2, L->u, sizeof(double) * 6 * (size_t)L->n);
@@ -1838,18 +1861,24 @@ int or_init_from_state(or_ctx *C) {
   return 0;
 }
 
-/* end of a physical time step (MG:889-890): grid 1 only */
+/* end of a physical time step (MG:889-890): grid 1's uold2 <- uold <- u;
+ * then the preamble of the next step (MG:586-601, inside the physical-step
+ * loop): laminar viscosity reset, u and uold restricted to every coarse
+ * level */
 int or_advance_physical_time(or_ctx *C) {
+  settle(C);
   if (!C || !C->L[0].uploaded) return fail(FLITE_E_STATE, "level 1 not uploaded");
   orl *L = &C->L[0];
   memcpy(L->uold2, L->uold, sizeof(double) * 6 * (size_t)L->n);
   memcpy(L->uold, L->u, sizeof(double) * 6 * (size_t)L->n);
+  preamble(C);
   return 0;
 }
 
 /* writeResiduals reductions (MG:1271-1445) + calculateLiftAndDrag
  * (DEF:14710-14806) on grid 1 */
 int or_monitor(or_ctx *C, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
+  settle(C);
   orl *L = &C->L[0];
   const flite_params *P = &C->P;
   int n = L->n;
@@ -1941,11 +1970,16 @@ static double *field_ptr(orl *L, int field, int *ncomp) {
     case FLITE_FIELD_DIVERGENCE: *ncomp = 1; return L->divg;
     case FLITE_FIELD_SOURCE: *ncomp = 6; return L->src;
     case FLITE_FIELD_TURB_DIFFUSION: *ncomp = 1; return L->tdiff;
+    case FLITE_FIELD_UOLD: *ncomp = 6; return L->uold;
+    case FLITE_FIELD_UOLD2: *ncomp = 6; return L->uold2;
     default: return NULL;
   }
 }
 
 int or_field_get(or_ctx *C, int level, int field, double *out, int ld) {
+  settle(C);
+  if (C && C->P.flow_type == 0 && (field == FLITE_FIELD_UOLD || field == FLITE_FIELD_UOLD2))
+    return fail(FLITE_E_STATE, "no solutions of earlier physical times in a steady run");
   if (!C || level < 1 || level > C->nl) return fail(FLITE_E_ARG, "bad level");
   orl *L = &C->L[level - 1];
   if (ld < L->n) return fail(FLITE_E_ARG, "ld < n");
@@ -1967,6 +2001,9 @@ int or_field_get(or_ctx *C, int level, int field, double *out, int ld) {
 }
 
 int or_field_set(or_ctx *C, int level, int field, const double *in, int ld) {
+  settle(C);
+  if (C && C->P.flow_type == 0 && (field == FLITE_FIELD_UOLD || field == FLITE_FIELD_UOLD2))
+    return fail(FLITE_E_STATE, "no solutions of earlier physical times in a steady run");
   if (!C || level < 1 || level > C->nl) return fail(FLITE_E_ARG, "bad level");
   orl *L = &C->L[level - 1];
   if (ld < L->n) return fail(FLITE_E_ARG, "ld < n");
@@ -1978,6 +2015,7 @@ int or_field_set(or_ctx *C, int level, int field, const double *in, int ld) {
 }
 
 int or_apply(or_ctx *C, int level, int op) {
+  settle(C);
   if (!C || level < 1 || level > C->nl) return fail(FLITE_E_ARG, "bad level");
   orl *L = &C->L[level - 1];
   switch (op) {

@@ -363,9 +363,10 @@ def test_unsteady_time_iterations(case):
 
 @pytest.mark.parametrize("case,scheme", [("euler", 3), ("rans", 1)])
 def test_unsteady_cycles(case, scheme):
-    """Two physical steps of three V-cycles each: BDF2 on level 1, BDF1 with
-    the restricted initial uold on the coarse levels, the modified local
-    time step in the smoother and in the restriction."""
+    """Two physical steps of three V-cycles each: BDF2 on level 1, BDF1 on
+    the coarse levels with the uold each step's preamble restricts, the time
+    term and the modified local time step in the smoother and in both halves
+    of the restriction."""
     print(f"[case] unsteady cycles {case} scheme {scheme}", flush=True)
     if case == "rans":
         lv = levels(8, 3, first_cell=0.005)
@@ -386,6 +387,10 @@ def test_unsteady_cycles(case, scheme):
             assert np.allclose(ms["res"], mo["res"], rtol=1e-8, atol=1e-16), (c, ms["res"], mo["res"])
             assert np.allclose(ms["forces"], mo["forces"], rtol=1e-8, atol=1e-14), (c, ms["forces"], mo["forces"])
         s.advance_physical_time(); o.advance_physical_time()
+        for l in (2, 3):  # the next step's preamble: coarse u and uold restricted
+            for f in ("U", "UOLD"):
+                e = rel_err(s.get(l, f)[:nc], o.get(l, f)[:nc])
+                assert e.max() < 1e-9, (step, l, f, e)
     e = rel_err(s.get(1, "U")[:nc], o.get(1, "U")[:nc])
     assert e.max() < 1e-9, e
     # the time term acts: the same cycles of a steady run end elsewhere
