@@ -46,6 +46,9 @@ def lib():
         L.or_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
         L.or_hllc_flux.argtypes = [P(FliteParams), P(d), d, P(d), d, P(d), P(d)]
         L.or_farfield_flux.argtypes = [P(FliteParams), i, P(d), d, P(d), P(d)]
+        L.or_hllc_side.argtypes = [P(FliteParams), i, i, i, P(d), d, P(d), d, P(d), P(d), P(d), P(d), P(d)]
+        L.or_led.restype = d
+        L.or_led.argtypes = [d, d, d, i]
         L.or_simple_flux.argtypes = [P(FliteParams), i, P(d), d, P(d), d, P(d), P(d)]
         _lib = L
     return _lib
@@ -160,6 +163,21 @@ def hllc_flux(params, ul, pl, ur, pr, w):
     ww = (ctypes.c_double * 3)(*w)
     lib().or_hllc_flux(ctypes.byref(params), a, pl, b, pr, ww, f)
     return np.array(f[:])
+
+
+def hllc_side(params, order, autoclip, nsig, ul, pl, ur, pr, w, dx, g1, g2):
+    """One interior HLLC side with explicit end gradients and bounds."""
+    f = (ctypes.c_double * 5)()
+    arr = lambda v, k: (ctypes.c_double * k)(*[float(x) for x in v])
+    rc = lib().or_hllc_side(ctypes.byref(params), order, autoclip, nsig, arr(ul, 5), pl, arr(ur, 5), pr,
+                            arr(w, 3), arr(dx, 3), arr(g1, 25), arr(g2, 25), f)
+    if rc != 0:
+        raise OracleError(lib().or_last_error().decode())
+    return np.array(f[:])
+
+
+def led(a, bmin, bmax, nsig):
+    return lib().or_led(a, bmin, bmax, nsig)
 
 
 def simple_flux(params, flux_type, ul, pl, ur, pr, w):

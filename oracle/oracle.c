@@ -1056,6 +1056,17 @@ int or_hllc_flux(const flite_params *p, const double ul[5], double pl,
   return 0;
 }
 
+int or_hllc_side(const flite_params *p, int order, int autoclip, int nsig,
+                 const double ul[5], double pl, const double ur[5], double pr,
+                 const double w[3], const double dx[3], const double g1[25],
+                 const double g2[25], double f[5]) {
+  if (order < 1 || order > 2 || (nsig != 2 && nsig != 4)) return fail(FLITE_E_ARG, "bad order or nsig");
+  hllc_side(p, order, autoclip, nsig, ul, pl, ur, pr, w[0], w[1], w[2], dx, g1, g2, f, NULL);
+  return 0;
+}
+
+double or_led(double a, double bmin, double bmax, int nsig) { return led(a, bmin, bmax, nsig); }
+
 /* farfield Roe flux of one side end against the inflow state
  * (DEF:2258-2446) */
 static void farfield_roe(const flite_params *P, const double *inf, const double *s,

@@ -53,6 +53,17 @@ int or_monitor(or_ctx *c, double res[7], double maxres[7], int maxidx[7],
 int or_hllc_flux(const flite_params *p, const double ul[5], double pl,
                  const double ur[5], double pr, const double w[3], double f[5]);
 
+/* one interior side of makeRHS's HLLC loop (DEF:1794-2100) at either order,
+ * with the ends' nodeHelpArray rows given: g1, g2 = 15 gradient components
+ * (x of rho,u,v,w,E, then y, then z) and the LocalMinMax bounds (5 minima,
+ * 5 maxima, relative to the node); dx = x2 - x1; f[5] */
+int or_hllc_side(const flite_params *p, int order, int autoclip, int nsig,
+                 const double ul[5], double pl, const double ur[5], double pr,
+                 const double w[3], const double dx[3], const double g1[25],
+                 const double g2[25], double f[5]);
+/* the LED limiter of the reconstruction (DEF:1852-1889) */
+double or_led(double a, double bmin, double bmax, int nsig);
+
 /* interior flux of fluxType 1-3 (DEF:1664-1792, LLF and Roe patched as
  * documented in oracle.c) between two given states, side weights w[3] */
 int or_simple_flux(const flite_params *p, int flux_type, const double ul[5], double pl,
