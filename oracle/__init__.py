@@ -35,6 +35,7 @@ def lib():
         L.or_field_get.argtypes = [vp, i, i, P(d), i]
         L.or_field_set.argtypes = [vp, i, i, P(d), i]
         L.or_apply.argtypes = [vp, i, i]
+        L.or_magnitudes.argtypes = [vp, i, i, P(d), i]
         L.or_init_freestream.argtypes = [vp]
         L.or_init_from_state.argtypes = [vp]
         L.or_advance_physical_time.argtypes = [vp]
@@ -118,6 +119,14 @@ class Oracle:
 
     def apply(self, level, op):
         self._check(self.L.or_apply(self.ctx, level, OPS[op] if isinstance(op, str) else op))
+
+    def magnitudes(self, level, op):
+        """Per node and component, sum of |term| over what the operation adds
+        up (the per-node normaliser of the parity tests); runs the op."""
+        out = np.zeros((6, self.n(level)))
+        self._check(self.L.or_magnitudes(self.ctx, level, OPS[op] if isinstance(op, str) else op,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.n(level)))
+        return out
 
     def init_freestream(self):
         self._check(self.L.or_init_freestream(self.ctx))

@@ -8,27 +8,44 @@
  *
  * The reference is Fortran built against MPI with a patch set it needs to
  * compile and run here, so it is not built in this repository; this file is
- * pinned instead by the checks in tests/test_oracle_*.py:
+ * pinned instead by the checks in tests/test_oracle_*.py, each a fact the
+ * reference's algorithm fixes independently of how this file writes it:
  *   - freestream preservation of the full residual (closure of the side
  *     coefficients, DEF:14430-14568) for HLLC-1/2 and the viscous terms;
  *   - conservation: sum over nodes of the interior scatter is zero;
  *   - HLLC consistency F(U,U,n) = F(U).n, supersonic upwinding, mirror
  *     symmetry; the farfield Roe flux against the textbook Roe solver;
- *   - Green-Gauss exactness for linear fields at interior nodes;
+ *   - MUSCL: a linear field reconstructs both faces to the side midpoint
+ *     (side flux = physical flux there); minmod and the clip to exactly
+ *     half the clip factor times the side length; LED: slope 1 at zero,
+ *     the limit bmax/2, c <= min(a, bmax/2), c(bmax/2) = 2^(-1/nsig) bmax/2;
+ *   - Green-Gauss exactness for linear fields at every node;
  *   - LocalMinMax, time steps, pressure and the SA time steps by brute force
  *     on small meshes (independent numpy restatements);
+ *   - the update: density relaxation by exactly maxIncrementFactor, and one
+ *     smoothing step applying the Runge-Kutta stage polynomial
+ *     1 - a3 z + a2 a3 z^2 - a1 a2 a3 z^3 to a small perturbation of a
+ *     uniform flow (3 and 5 stages);
+ *   - forces: lift and drag of a uniform overpressure and of a uniform
+ *     shear on a flat wall patch, from the patch's area alone;
+ *   - viscosity: Sutherland's law, the SA eddy viscosity (fv1 = 1/2 at
+ *     chi = cv1, mu_t -> rho nu~/Re), zero momentum and the closed-form
+ *     viscous work of a uniform stress at every node, the turbulent heat
+ *     conduction's Prandtl numbers by the ratio of SA to laminar terms;
  *   - SA source: the log-layer balance the model's cw1 is defined by, the
- *     production and trip term away from walls, zero for nu~ = 0; SA
- *     diffusion exact for linear nu~ (tests/test_oracle_sa.py);
- *   - viscous residual zero at interior nodes for linear velocity and
- *     temperature with constant viscosity;
- *   - MG: restriction conserves sum V u, injection prolongation, a
- *     converged fine solution gives a zero coarse correction;
- *   - RK: the 3-stage amplification polynomial on a linear problem.
+ *     destruction fw(r) of the published model at r != 1 (cw2, cw3, fv2,
+ *     the clip at 100), the production and trip term away from walls, zero
+ *     for nu~ = 0; SA diffusion exact for linear nu~; SA convection upwind
+ *     (no upstream influence);
+ *   - MG: restriction conserves sum V u; prolongation injects u_H - R u_h
+ *     scaled exactly by its two relaxation factors; a converged fine
+ *     solution gives a zero coarse correction; a converged physical step of
+ *     multigrid cycles solves the fine grid's BDF equation;
  *   - central / LLF / Roe interior fluxes: consistency F(U,U) = F(U).n,
  *     the Roe flux against the textbook Roe solver, upwinding of a contact;
  *   - JST dissipation: zero for uniform states, conservative (sums to zero
- *     over the nodes), the scalar and matrix forms on hand-computed sides.
+ *     over the nodes), the scalar and matrix forms on hand-computed sides,
+ *     the fourth-difference coefficient dim(d4, d2).
  * Branches not reached by those checks (engine faces, inviscid-wall
  * indicator 9, dissipation schemes 1 and 3, k-omega/SST) are not restated:
  * the calls that would need them return FLITE_E_UNSUPPORTED.
@@ -94,9 +111,17 @@ struct or_ctx {
   double *help; /* nodeHelpArray(Nfine,25), shared by all levels (DEF:14214-14231) */
   int nh;
   int pending; /* a level's geometry was replaced: the step's preamble is still due */
+  /* or_magnitudes: when set, the side loops of the operation being run add
+   * |contribution| per node and component here ([c*n+i], 6 components) */
+  double *mag;
 };
 
 static void settle(or_ctx *C);
+
+/* |v| into the magnitude accumulator of or_magnitudes, when active */
+static inline void MAG(or_ctx *C, const orl *L, int c, int i, double v) {
+  if (C->mag) C->mag[(size_t)c * L->n + i] += fabs(v);
+}
 
 #define U(L, c, i) ((L)->u[(size_t)(c) * (L)->n + (i)])
 #define UP(L, c, i) ((L)->uprev[(size_t)(c) * (L)->n + (i)])
@@ -775,6 +800,8 @@ static void viscous(or_ctx *C, orl *L, double *rhs) {
     double f5 = wx * (dt7 + dut1) + wy * (dt8 + dut2) + wz * (dt9 + dut3);
     H(C, 9, i1) += f2; H(C, 10, i1) += f3; H(C, 11, i1) += f4; H(C, 12, i1) += f5;
     H(C, 9, i2) -= f2; H(C, 10, i2) -= f3; H(C, 11, i2) -= f4; H(C, 12, i2) -= f5;
+    MAG(C, L, 1, i1, f2); MAG(C, L, 2, i1, f3); MAG(C, L, 3, i1, f4); MAG(C, L, 4, i1, f5);
+    MAG(C, L, 1, i2, f2); MAG(C, L, 2, i2, f3); MAG(C, L, 3, i2, f4); MAG(C, L, 4, i2, f5);
   }
   for (int sd = 0; sd < L->nbs; ++sd) {
     int ind = L->bi[sd];
@@ -805,6 +832,18 @@ static void viscous(or_ctx *C, orl *L, double *rhs) {
     H(C, 10, i2) = H(C, 10, i2) + f3 + 2.0 * (wx * t22 + wy * t42 + wz * t52);
     H(C, 11, i2) = H(C, 11, i2) + f4 + 2.0 * (wx * t32 + wy * t52 + wz * t62);
     H(C, 12, i2) = H(C, 12, i2) + f5 + 2.0 * (wx * (t72 + dut12) + wy * (t82 + dut22) + wz * (t92 + dut32));
+    if (C->mag) {
+      double m1[4] = {f2 + 2.0 * (wx * t11 + wy * t21 + wz * t31), f3 + 2.0 * (wx * t21 + wy * t41 + wz * t51),
+                      f4 + 2.0 * (wx * t31 + wy * t51 + wz * t61),
+                      f5 + 2.0 * (wx * (t71 + dut11) + wy * (t81 + dut21) + wz * (t91 + dut31))};
+      double m2[4] = {f2 + 2.0 * (wx * t12 + wy * t22 + wz * t32), f3 + 2.0 * (wx * t22 + wy * t42 + wz * t52),
+                      f4 + 2.0 * (wx * t32 + wy * t52 + wz * t62),
+                      f5 + 2.0 * (wx * (t72 + dut12) + wy * (t82 + dut22) + wz * (t92 + dut32))};
+      for (int q = 0; q < 4; ++q) {
+        MAG(C, L, 1 + q, i1, m1[q]);
+        MAG(C, L, 1 + q, i2, m2[q]);
+      }
+    }
   }
   for (int c = 1; c < 5; ++c)
     for (int i = 0; i < n; ++i) rhs[(size_t)c * n + i] -= H(C, 8 + c, i);
@@ -1227,6 +1266,8 @@ static int make_rhs(or_ctx *C, orl *L) {
     for (int c = 0; c < 5; ++c) {
       RHS(L, c, i1) += f[c];
       RHS(L, c, i2) -= f[c];
+      MAG(C, L, c, i1, f[c]);
+      MAG(C, L, c, i2, f[c]);
     }
   }
   for (int s = 0; s < L->nbs; ++s) {
@@ -1240,6 +1281,8 @@ static int make_rhs(or_ctx *C, orl *L) {
         if (P->boundary_term == 1) {
           RHS(L, c, i1) -= (3. * s1[c] + s2[c]) * g1;
           RHS(L, c, i2) -= (s1[c] + 3. * s2[c]) * g2;
+          MAG(C, L, c, i1, (3. * s1[c] + s2[c]) * g1);
+          MAG(C, L, c, i2, (s1[c] + 3. * s2[c]) * g2);
         } else {
           RHS(L, c, i1) -= s1[c] * (4.0 * g1);
           RHS(L, c, i2) -= s2[c] * (4.0 * g2);
@@ -1279,6 +1322,10 @@ static int make_rhs(or_ctx *C, orl *L) {
       if (P->boundary_term == 1) {
         RHS(L, c, i1) += rnorm * (3. * f1[c] + f2[c]);
         RHS(L, c, i2) += rnorm * (f1[c] + 3. * f2[c]);
+        MAG(C, L, c, i1, rnorm * 3. * f1[c]);
+        MAG(C, L, c, i1, rnorm * f2[c]);
+        MAG(C, L, c, i2, rnorm * f1[c]);
+        MAG(C, L, c, i2, rnorm * 3. * f2[c]);
       } else {
         RHS(L, c, i1) += rnorm * 4. * f1[c];
         RHS(L, c, i2) += rnorm * 4. * f2[c];
@@ -1290,7 +1337,6 @@ static int make_rhs(or_ctx *C, orl *L) {
 
 /* makeSARHS (DEF:6809-6950): upwinded convection of nu~ into rhs(:,6) */
 static void sa_rhs(or_ctx *C, orl *L) {
-  (void)C;
   for (int e = 0; e < L->ne; ++e) {
     int i1 = L->e1[e], i2 = L->e2[e];
     double rinv1 = 1. / U(L, 0, i1), t1 = U(L, 5, i1);
@@ -1307,6 +1353,8 @@ static void sa_rhs(or_ctx *C, orl *L) {
     }
     RHS(L, 5, i1) += fc;
     RHS(L, 5, i2) -= fc;
+    MAG(C, L, 5, i1, fc);
+    MAG(C, L, 5, i2, fc);
   }
 }
 
@@ -1352,6 +1400,9 @@ static void sa_source(or_ctx *C, orl *L) {
     RHS(L, 5, i) = RHS(L, 5, i) - vt * cb1 * (1.0 - ft2) * Sc * mc;
     RHS(L, 5, i) = RHS(L, 5, i) + oneOverRe * vt * (cw1 * fw - cb1 * ft2 / k2) * muOverd * muOverd;
     RHS(L, 5, i) = RHS(L, 5, i) - vt * mc * L->divg[i];
+    MAG(C, L, 5, i, vt * cb1 * (1.0 - ft2) * Sc * mc);
+    MAG(C, L, 5, i, oneOverRe * vt * (cw1 * fw - cb1 * ft2 / k2) * muOverd * muOverd);
+    MAG(C, L, 5, i, vt * mc * L->divg[i]);
   }
 }
 
@@ -1406,6 +1457,8 @@ static void sa_diffusion(or_ctx *C, orl *L) {
     double fd2 = v2 * (wx * H(C, 11, i2) + wy * H(C, 12, i2) + wz * H(C, 13, i2));
     L->tdiff[i1] = L->tdiff[i1] - fact * (fd1 + fd2);
     L->tdiff[i2] = L->tdiff[i2] + fact * (fd1 + fd2);
+    MAG(C, L, 5, i1, fact * (fd1 + fd2));
+    MAG(C, L, 5, i2, fact * (fd1 + fd2));
   }
   for (int sd = 0; sd < L->nbs; ++sd) {
     int i1 = L->b1[sd], i2 = L->b2[sd];
@@ -1417,12 +1470,17 @@ static void sa_diffusion(or_ctx *C, orl *L) {
     if (P->boundary_term == 1) {
       L->tdiff[i1] = L->tdiff[i1] - fact * (3. * fd1 + fd2);
       L->tdiff[i2] = L->tdiff[i2] - fact * (fd1 + 3. * fd2);
+      MAG(C, L, 5, i1, fact * (3. * fd1 + fd2));
+      MAG(C, L, 5, i2, fact * (fd1 + 3. * fd2));
     } else {
       L->tdiff[i1] = L->tdiff[i1] - fact * 4. * fd1;
       L->tdiff[i2] = L->tdiff[i2] - fact * 4. * fd2;
     }
   }
-  for (int i = 0; i < n; ++i) L->tdiff[i] = L->tdiff[i] + H(C, 0, i);
+  for (int i = 0; i < n; ++i) {
+    L->tdiff[i] = L->tdiff[i] + H(C, 0, i);
+    MAG(C, L, 5, i, H(C, 0, i));
+  }
 }
 
 /* makeTurbulentTimeSteps (DEF:12878-13099) */
@@ -2023,6 +2081,24 @@ int or_field_set(or_ctx *C, int level, int field, const double *in, int ld) {
   if (!p) return fail(FLITE_E_ARG, "bad field");
   for (int c = 0; c < nc; ++c) memcpy(p + (size_t)c * L->n, in + (size_t)c * ld, sizeof(double) * L->n);
   return 0;
+}
+
+/* per node and component, the sum of the magnitudes of the terms one
+ * operation (FLITE_OP_RHS, VISCOUS or SA_DIFFUSION) adds up at the current
+ * state: the parity tests' per-node normaliser sum_edges |f_edge| */
+int or_magnitudes(or_ctx *C, int level, int op, double *out, int ld) {
+  settle(C);
+  if (!C || level < 1 || level > C->nl) return fail(FLITE_E_ARG, "bad level");
+  if (op != FLITE_OP_RHS && op != FLITE_OP_VISCOUS && op != FLITE_OP_SA_DIFFUSION)
+    return fail(FLITE_E_ARG, "magnitudes of RHS, VISCOUS or SA_DIFFUSION only");
+  orl *L = &C->L[level - 1];
+  if (ld < L->n) return fail(FLITE_E_ARG, "ld < n");
+  C->mag = zalloc(6 * (size_t)L->n);
+  int rc = or_apply(C, level, op);
+  for (int c = 0; c < 6 && !rc; ++c) memcpy(out + (size_t)c * ld, C->mag + (size_t)c * L->n, sizeof(double) * L->n);
+  free(C->mag);
+  C->mag = NULL;
+  return rc;
 }
 
 int or_apply(or_ctx *C, int level, int op) {

@@ -35,6 +35,9 @@ int or_field_get(or_ctx *c, int level, int field, double *out, int ld);
 int or_field_set(or_ctx *c, int level, int field, const double *in, int ld);
 
 int or_apply(or_ctx *c, int level, int op);
+/* sum over the terms an operation adds (RHS, VISCOUS, SA_DIFFUSION) of their
+ * magnitudes, per node and component: out(N, 6) */
+int or_magnitudes(or_ctx *c, int level, int op, double *out, int ld);
 int or_init_freestream(or_ctx *c);
 int or_init_from_state(or_ctx *c);
 int or_do_time_iterations(or_ctx *c, int level, int steps, int calc_diss,

@@ -586,8 +586,9 @@ def test_sa_destruction_follows_fw_of_r():
 def test_sa_convection_is_upwind():
     """makeSARHS (DEF:6873-6880) adds |w.a| (nu1 - nu2) to the central
     flux, a the Roe speed of u nu~: the first-order upwind flux.  A single
-    spike of nu~ in a uniform velocity then reaches only the neighbours it
-    flows to: the SA residual of every upstream neighbour is exactly zero."""
+    spike of nu~ = 1 in a uniform velocity u0 then reaches only the
+    neighbours it flows to: the SA residual of every upstream neighbour is
+    exactly zero, that of a downstream one -2 w(j->k).u0."""
     lv = levels(6, 1)
     L = lv[0]
     iv = rans()
@@ -616,7 +617,8 @@ def test_sa_convection_is_upwind():
             assert r6[j] == 0.0, (j, r6[j])
             up += 1
         elif wjk @ u0 < -1e-12:
-            assert abs(r6[j]) > 1e-8
+            # the spike's upwind flux 2 (w.u0) nu~_k leaves k and enters j
+            assert abs(r6[j] - 2.0 * (wjk @ u0)) <= 1e-14, (j, r6[j], 2.0 * (wjk @ u0))
             down += 1
     assert up > 2 and down > 2
     o.close()
@@ -645,3 +647,346 @@ def test_jst_fourth_difference_switches_off_where_the_second_is_larger():
         o.close()
     assert np.abs(out[1]).max() > 1e-6
     assert np.abs(out[0] - out[1]).max() <= 1e-15 * np.abs(out[1]).max()
+
+
+# ---- boundary conditions -------------------------------------------------------
+def face_nodes(L, tol=1e-12):
+    """Farfield nodes strictly inside one box face, with the face's inward
+    normal (the node normal there, DEF:15394-15460)."""
+    x, y, z = L.coords
+    inner = lambda a: (a > tol) & (a < 1 - tol)
+    faces = {
+        (1.0, 0.0, 0.0): (np.abs(x) < tol) & inner(y) & inner(z),
+        (-1.0, 0.0, 0.0): (np.abs(x - 1) < tol) & inner(y) & inner(z),
+        (0.0, 1.0, 0.0): (np.abs(y) < tol) & inner(x) & inner(z),
+        (0.0, -1.0, 0.0): (np.abs(y - 1) < tol) & inner(x) & inner(z),
+        (0.0, 0.0, -1.0): (np.abs(z - 1) < tol) & inner(x) & inner(y),
+    }
+    return {n: np.flatnonzero(m) for n, m in faces.items()}
+
+
+def test_farfield_conditions_by_the_sign_of_the_inflow_normal_velocity():
+    """setOuterBoundaryConditions (DEF:4018-4251), subsonic: where the
+    freestream enters (u_inf . n > 0, n the inward normal) the node takes
+    the freestream density and momentum and the energy of the node's own
+    pressure; elsewhere it keeps density and momentum and takes the energy
+    of the freestream pressure; nu~ is 0.1 on every farfield node."""
+    lv = levels(6, 1)
+    L = lv[0]
+    iv = rans(alpha=3.0, beta=4.0, machnumber=0.6)  # every face definite: no u_inf . n = 0
+    P = iv.to_struct(rk=(1.0,))  # one stage: PRESSURE keeps the perturbed start's pressure
+    o = oracle.Oracle(P, [L])
+    o.init_freestream()
+    rng = np.random.default_rng(11)
+    u = o.get(1, "U")
+    u[:5] *= 1 + 0.02 * rng.uniform(-1, 1, size=u[:5].shape)
+    u[5] = rng.uniform(0.0, 3.0, L.n_nodes)
+    o.set(1, "U", u)
+    o.do_time_iterations(1, 1, True)
+    un, p = o.get(1, "U"), o.get(1, "PRESSURE")[0]
+    inf = np.array(P.inflow_field[0])
+    rho0, vel0, pinf = inf[0], inf[1:4], 1.0 / (G * P.mach_number ** 2)
+    seen = {True: 0, False: 0}
+    for n, nodes in face_nodes(L).items():
+        inflow = vel0 @ np.array(n) > 0.0
+        seen[inflow] += nodes.size
+        for i in nodes:
+            if inflow:
+                assert un[0, i] == rho0 and np.all(un[1:4, i] == rho0 * vel0)
+                assert abs(un[4, i] - (p[i] / (G - 1) + 0.5 * rho0 * vel0 @ vel0)) <= 1e-15 * un[4, i]
+            else:
+                m = un[1:4, i]
+                assert abs(un[4, i] - (pinf / (G - 1) + 0.5 * m @ m / un[0, i])) <= 1e-15 * un[4, i]
+            assert un[5, i] == 0.1
+    assert seen[True] > 10 and seen[False] > 10
+
+
+def test_euler_wall_projects_the_increment_and_the_state():
+    """Euler wall (DEF:4431-4450, 5092-5110): the increment loses its
+    normal part, then the updated momentum does.  With m_prev already
+    tangential, m_new = P(m_prev - r) and d = m_prev - m_new - r equals
+    (r.n) n, which vanishes only if the increment r (left in RHS) was
+    projected; checked on every wall node of the bump, whose normals vary."""
+    lv = levels(6, 1)
+    L = lv[0]
+    o = oracle.Oracle(euler(1).to_struct(), [L])
+    o.init_freestream()
+    o.do_time_iterations(1, 1, True)  # the wall momentum is tangential now
+    o.do_time_iterations(1, 1, True)
+    up, un, r = o.get(1, "UPREV"), o.get(1, "U"), o.get(1, "RHS")
+    wall = L.node_ind_array[L.reg[0] - 1:L.reg[1]]
+    d = up[1:4, wall] - un[1:4, wall] - r[1:4, wall]
+    scale = np.abs(r[1:4, wall]).max()
+    assert scale > 1e-6
+    assert np.abs(d).max() <= 1e-13 * np.abs(up[1:4, wall]).max()
+
+
+def test_no_slip_and_isothermal_walls():
+    """Viscous walls (DEF:4400-4412, 4990-5000): zero momentum and nu~ on a
+    fixed mesh; with wallsAreIsentropic false the energy is
+    rho inflowTemperature / gamma (the reference's dimensional quirk), with it
+    true the energy is left to the scheme."""
+    lv = levels(6, 1, first_cell=0.02)
+    L = lv[0]
+    wall = L.node_ind_array[L.reg[0] - 1:L.reg[1]]
+    for iso in (False, True):
+        iv = rans(wallsareisentropic=iso, inflowtemperature=1.0 / (0.4 * 0.7 ** 2))
+        P = iv.to_struct()
+        o = oracle.Oracle(P, [L])
+        o.init_freestream()
+        o.do_time_iterations(1, 2, True)
+        un, r = o.get(1, "U"), o.get(1, "RHS")
+        assert np.all(un[1:4, wall] == 0.0) and np.all(un[5, wall] == 0.0)
+        assert np.all(r[1:4, wall] == 0.0)
+        if not iso:
+            assert np.abs(un[4, wall] - un[0, wall] * P.inflow_temperature / P.gamma).max() <= 1e-15 * un[4].max()
+        else:
+            assert np.abs(un[4, wall] - un[0, wall] * P.inflow_temperature / P.gamma).max() > 1e-6
+        o.close()
+
+
+def test_turbulence_limited_to_its_maximum():
+    """limitTurbulence (DEF:1259-1290): nu~ >= 0 and <= maxTurbulenceValue
+    after every stage."""
+    lv = levels(6, 1, first_cell=0.02)
+    L = lv[0]
+    iv = rans(maxturbulencevalue=50.0)
+    o = oracle.Oracle(iv.to_struct(), [L])
+    o.init_freestream()
+    u = o.get(1, "U")
+    u[5] = np.random.default_rng(12).uniform(-20.0, 400.0, L.n_nodes)
+    o.set(1, "U", u)
+    o.do_time_iterations(1, 1, True)
+    nu = o.get(1, "U")[5]
+    assert nu.min() >= 0.0 and nu.max() <= 50.0
+    assert np.count_nonzero(nu == 50.0) > 10
+
+
+# ---- dual time stepping: the modified local time step -------------------------
+def test_unsteady_local_time_step_is_dt_over_one_plus_three_halves_dt_over_dtp():
+    """solveSystem (DEF:1427-1436) scales the mean-flow time step to
+    dt / (1 + 1.5 dt / dt_p).  At the start of a physical step (u = uold =
+    uold2) the time term vanishes, so a one-stage step's increment at the
+    interior nodes is CFL dt' R / V with R the residual operation's output."""
+    lv = levels(6, 1)
+    L = lv[0]
+    for dtp in (0.01, 0.3):
+        iv = euler(1, flowtype=1, physicaltimestep=dtp, timescheme=2, cflnumber=1.3)
+        P = iv.to_struct(rk=(1.0,))
+        o = oracle.Oracle(P, [L])
+        o.init_freestream()
+        u = o.get(1, "U")
+        u[:5] *= 1 + 0.02 * np.random.default_rng(13).uniform(-1, 1, size=u[:5].shape)
+        o.set(1, "U", u)
+        o.init_from_state()  # uold = uold2 = u
+        o.apply(1, "PRESSURE")
+        o.apply(1, "RHS")
+        R = o.get(1, "RHS")[:5]
+        o.do_time_iterations(1, 1, True)
+        inc, dt = o.get(1, "RHS")[:5], o.get(1, "DT")[0]
+        m = interior_nodes(L)
+        ref = 1.3 * dt / (1 + 1.5 * dt / dtp) * R / L.volume
+        assert np.abs(inc[:, m] - ref[:, m]).max() <= 1e-13 * np.abs(ref[:, m]).max()
+        o.close()
+
+
+# ---- monitor ----------------------------------------------------------------
+def test_monitor_residuals_are_l2_and_max_norms_of_the_last_step():
+    """writeResiduals (MG:1302-1348): res_j = sqrt(sum_i (u_j - uprev_j)^2)
+    and the largest |u_j - uprev_j| with its node, over nres = 5 (6 with SA)
+    components."""
+    lv = levels(5, 1)
+    L = lv[0]
+    o = oracle.Oracle(rans().to_struct(), [L])
+    o.init_freestream()
+    rng = np.random.default_rng(14)
+    u = o.get(1, "U")
+    up = u * (1 + 0.01 * rng.normal(size=u.shape))
+    o.set(1, "U", u)
+    o.set(1, "UPREV", up)
+    m = o.monitor()
+    d = u - up
+    assert np.allclose(m["res"][:6], np.sqrt((d[:6] ** 2).sum(axis=1)), rtol=1e-14, atol=0)
+    assert np.allclose(m["maxres"][:6], np.abs(d[:6]).max(axis=1), rtol=0, atol=0)
+    assert np.all(m["maxidx"][:6] == np.abs(d[:6]).argmax(axis=1))
+    o.close()
+
+
+def test_adiabatic_wall_removes_the_normal_heat_flux():
+    """setTemperatureForAdiabaticWall (DEF:5170-5196): on wall nodes the heat
+    flux k grad T loses its normal part.  Still fluid with T = Tinf (1 +
+    eps z) on the flat wall patch: the flux is k g e_z off the wall and zero
+    on it, the interior sides cancel, and the energy terms of all nodes sum to
+    the boundary sides' 3:1-weighted fluxes, 4 k g sum_b w_b,z (off(i1) +
+    off(i2)) (DEF:5740-5814); k = mu/Pr up to O(eps)."""
+    L, wall, _ = flat_patch_levels()
+    iv = laminar(reynoldsnumber=100.0)
+    P = iv.to_struct()
+    o = oracle.Oracle(P, [L])
+    o.init_freestream()
+    eps = 1e-6
+    pinf = 1.0 / (G * P.mach_number ** 2)
+    u = np.zeros((6, L.n_nodes))
+    u[0] = 1.0
+    u[4] = pinf * (1 + eps * L.coords[2]) / (G - 1)
+    o.set(1, "U", u)
+    e = viscous_diss(o)[4]
+    g = G / (G - 1) * pinf * eps  # dT/dz with T = gamma/(gamma-1) p/rho
+    k = (1.0 / P.reynolds_number) / P.prandtl_number
+    off = np.ones(L.n_nodes)
+    off[wall] = 0.0
+    b1, b2 = L.bside_index[0], L.bside_index[1]
+    ref = -4.0 * k * g * (L.bside_w[2] * (off[b1] + off[b2])).sum()
+    assert abs(e.sum() - ref) < 1e-4 * abs(ref), (e.sum(), ref)
+    o.close()
+
+
+# ---- matrix JST: the Roe matrix with Harten's entropy fix --------------------------
+def roe_abs_matrix(ul, ur, n, hcf):
+    """|A| = R phi(|Lambda|) R^-1 for the normal Jacobian of the Euler flux
+    at the Roe average (velocity and total enthalpy; Roe 1981), from its
+    textbook right eigenvectors (acoustic u -/+ c n, entropy, two shear
+    waves along tangents t); phi is Harten's fix l -> (l^2/hcf + hcf)/2
+    below hcf."""
+    def prim(u):
+        rho = u[0]
+        vel = u[1:4] / rho
+        p = (G - 1.0) * (u[4] - 0.5 * rho * vel @ vel)
+        return rho, vel, (u[4] + p) / rho
+    rl, vl, Hl = prim(ul)
+    rr, vr, Hr = prim(ur)
+    sl, sr = np.sqrt(rl), np.sqrt(rr)
+    v = (sl * vl + sr * vr) / (sl + sr)
+    H = (sl * Hl + sr * Hr) / (sl + sr)
+    vn, q2 = v @ n, v @ v
+    c = np.sqrt((G - 1.0) * (H - 0.5 * q2))
+    t1 = np.cross(n, [0.3, 0.5, 0.7])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    R = np.stack([np.concatenate([[1.0], v - c * n, [H - c * vn]]),
+                  np.concatenate([[1.0], v, [0.5 * q2]]),
+                  np.concatenate([[0.0], t1, [v @ t1]]),
+                  np.concatenate([[0.0], t2, [v @ t2]]),
+                  np.concatenate([[1.0], v + c * n, [H + c * vn]])], axis=1)
+    lam = np.abs(np.array([vn - c, vn, vn, vn, vn + c]))
+    fix = np.array([0.5 * (l * l / hcf + hcf) if l < hcf else l for l in lam])
+    return R @ np.diag(fix) @ np.linalg.inv(R)
+
+
+@pytest.mark.parametrize("hcf", [0.0, 0.6])
+def test_matrix_jst_is_the_roe_matrix_on_a_general_jump(hcf):
+    """makeArtificialDissipation2 with useMatrixDissipation (DEF:11815-11950)
+    on one side of two nodes (unit weight along x, no second difference):
+    the fourth-difference term is -k4 |A| (2 jump) at the first node, |A|
+    the Roe matrix of the two states (with Harten's fix of width hcf) and
+    jump the undivided Laplacian's (rho, m, E + p) difference."""
+    from test_oracle_jst import cases, chain, jst
+    L = chain(2)
+    n = L.n_nodes
+    k4 = 0.3
+    iv = cases(3, usematrixdissipation=True, fourthorderdissipationfactor=k4,
+               secondorderdissipationfactor=0.0, hartenscorrectionfactor=hcf)
+    rng = np.random.default_rng(15)
+    for _ in range(20):
+        u = np.zeros((6, n))
+        u[0], u[4] = 1.0, 2.5
+        p = np.full(n, 1.0)
+        for k in (0, 1):
+            q = random_prim(rng)
+            q[1] = rng.uniform(-0.3, 0.3) if hcf else q[1]  # slow normal velocity: the fix acts
+            u[:5, k], p[k] = conserved(q)
+        d = jst(iv, L, u, p, L.volume.copy())
+        # the undivided Laplacian carries E + p in its energy slot (DEF:11700-11730)
+        jump = u[:5, 1] - u[:5, 0]
+        jump[4] += p[1] - p[0]
+        ref = -k4 * roe_abs_matrix(u[:5, 0], u[:5, 1], np.array([1.0, 0, 0]), hcf) @ (2 * jump)
+        assert np.allclose(d[:, 0], ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max()), (d[:, 0], ref)
+        assert np.allclose(d[:, 1], -ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
+# ---- HLLC against Toro's textbook form with the reference's wave speeds ------------
+BETA0 = np.sqrt((G - 1.0) / (2.0 * G))
+
+
+def hllc_toro(ul, ur, w):
+    """HLLC (Toro, Riemann Solvers, 3rd ed., eqs. 10.37-10.39) through the
+    side vector 2w: contact speed S* from the two outer speeds, star states
+    rho_K (S_K - u_K)/(S_K - S*) [1, S*, v_t, E/rho + (S* - u_K)(S* + p_K /
+    (rho_K (S_K - u_K)))].  Outer speeds: the smaller/larger of the nodes'
+    u_n -/+ beta0 c and the Roe-averaged u_n -/+ c (beta0 = sqrt((g-1)/2g))."""
+    m = 2.0 * np.asarray(w)
+    area = np.linalg.norm(m)
+    n = m / area
+    def prim(u):
+        rho = u[0]
+        vel = u[1:4] / rho
+        p = (G - 1.0) * (u[4] - 0.5 * rho * vel @ vel)
+        return rho, vel, p, (u[4] + p) / rho, np.sqrt(G * p / rho)
+    r1, v1, p1, H1, c1 = prim(ul)
+    r2, v2, p2, H2, c2 = prim(ur)
+    s1, s2 = np.sqrt(r1), np.sqrt(r2)
+    vt = (s1 * v1 + s2 * v2) / (s1 + s2)
+    Ht = (s1 * H1 + s2 * H2) / (s1 + s2)
+    ct = np.sqrt(abs((G - 1.0) * (Ht - 0.5 * vt @ vt)))
+    u1, u2, qt = v1 @ n, v2 @ n, vt @ n
+    SL = min(u1 - BETA0 * c1, qt - ct, u2 - BETA0 * c2)
+    SR = max(u2 + BETA0 * c2, qt + ct, u1 + BETA0 * c1)
+    Ss = (p2 - p1 + r1 * u1 * (SL - u1) - r2 * u2 * (SR - u2)) / (r1 * (SL - u1) - r2 * (SR - u2))
+    def F(u, r, v, p, un):
+        return np.concatenate([[r * un], r * v * un + p * n, [(u[4] + p) * un]])
+    def star(u, r, v, p, un, S):
+        f = r * (S - un) / (S - Ss)
+        return f * np.concatenate([[1.0], v + (Ss - un) * n, [u[4] / r + (Ss - un) * (Ss + p / (r * (S - un)))]])
+    if SL > 0:
+        f = F(ul, r1, v1, p1, u1)
+    elif Ss > 0:
+        f = F(ul, r1, v1, p1, u1) + SL * (star(ul, r1, v1, p1, u1, SL) - ul)
+    elif SR >= 0:
+        f = F(ur, r2, v2, p2, u2) + SR * (star(ur, r2, v2, p2, u2, SR) - ur)
+    else:
+        f = F(ur, r2, v2, p2, u2)
+    return f * area
+
+
+def test_hllc_is_toro_hllc_with_roe_average_wave_speeds():
+    from test_oracle_flux import random_state
+    P = euler(1).to_struct()
+    rng = np.random.default_rng(16)
+    for _ in range(300):
+        ul, pl = random_state(rng, mach=rng.uniform(0.0, 2.0))
+        ur, pr = random_state(rng, mach=rng.uniform(0.0, 2.0))
+        w = rng.normal(size=3) * rng.uniform(0.1, 2.0)
+        f = oracle.hllc_flux(P, ul, pl, ur, pr, w)
+        ref = hllc_toro(ul, ur, w)
+        assert np.allclose(f, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max()), (f, ref)
+
+
+def test_reconstruction_positivity_guards():
+    """DEF:1920-1944: a face density not above 0.1 or energy not above 1e-4
+    keeps the node's value, and a face pressure below 0.1 is raised to 0.1;
+    just inside and just outside each threshold."""
+    P = euler(2, automaticlipping=0, hllcclipping=1.0e6).to_struct()
+    w = np.array([0.3, -0.2, 0.5])
+    dx = np.array([0.1, 0.0, 0.0])
+    q1 = np.array([1.0, 0.2, 0.1, -0.1, 3.0])
+    q2 = np.array([1.1, 0.25, 0.05, -0.05, 3.2])
+    for comp, target in ((0, 0.1), (4, 1.0e-4)):
+        for delta in (-5e-3 * target, 5e-3 * target):
+            c = np.zeros(5)
+            c[comp] = target + delta - q1[comp]       # slope putting face 1 at target + delta
+            g = np.zeros((3, 5))
+            g[0] = 2.0 * c / dx[0]                    # 0.5 g.dx = c at both ends: minmod keeps c
+            u1, p1 = conserved(q1)
+            u2, p2 = conserved(q2)
+            f = oracle.hllc_side(P, 2, 0, 2, u1, p1, u2, p2, w, dx, grad_record(g), grad_record(g))
+            fa = q1 + c
+            fb = q2 - c
+            if not (fa[comp] > target):
+                fa[comp] = q1[comp]
+            ul, pl = conserved(fa)
+            ur, pr = conserved(fb)
+            pl, pr = max(pl, 0.1), max(pr, 0.1)
+            ul[4] = fa[4]
+            ur[4] = fb[4]
+            ref = oracle.hllc_flux(P, ul, pl, ur, pr, w)
+            assert np.allclose(f, ref, rtol=1e-12, atol=1e-13), (comp, delta, f, ref)
