@@ -3203,78 +3203,50 @@ __global__ void k_restrict_finish(LevelDev<R> K, DevParams P) {
 
 // ---------------------------------------------------------------------------
 // monitor reductions (writeResiduals MG:1302-1445, calculateLiftAndDrag
-// DEF:14710-14806): per-block partials, reduced in a fixed order
+// DEF:14710-14806), all on the device: a fixed grid of MON_BLOCKS blocks
+// strides over the nodes and the wall nodes, each thread accumulating its
+// share in stride order; wavefront butterflies and one LDS step reduce the
+// block to a record of MON_REC doubles; k_monitor_final reduces the records
+// in block order.  The order of every sum is fixed, so the results are
+// reproducible bit for bit.  Record: sum (u - uprev)^2 [7], max |u - uprev|
+// [7], host index of that maximum [7] (smallest index on ties), force
+// integrals [8] (lift, drag, latF, frictionDrag, latM, longM, pressureLift,
+// pressureDrag, before the 2/referenceArea factor).
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void k_monitor_partial(LevelDev<R> L, int nres, const int *host_index, double *part_sum, double *part_max,
-                                  int *part_idx, int bsel) {
-  L.to_batch(bsel);
-  __shared__ double ssum[NTHR];
-  __shared__ double smax[NTHR];
-  __shared__ int sidx[NTHR];
-  int t = threadIdx.x;
-  int i = gtid();
-  int n = L.n;
-  for (int j = 0; j < nres; ++j) {
-    double d = 0;
-    int hi = 0x7fffffff;
-    if (i < n) {
-      d = double(L.u[(size_t)j * n + i]) - double(L.uprev[(size_t)j * n + i]);
-      hi = host_index[i];
-    }
-    ssum[t] = d * d;
-    smax[t] = fabs(d);
-    sidx[t] = i < n ? hi : 0x7fffffff;
-    __syncthreads();
-    for (int s = NTHR / 2; s > 0; s >>= 1) {
-      if (t < s) {
-        ssum[t] += ssum[t + s];
-        double a = smax[t], b = smax[t + s];
-        if (b > a || (b == a && sidx[t + s] < sidx[t])) {
-          smax[t] = b;
-          sidx[t] = sidx[t + s];
-        }
-      }
-      __syncthreads();
-    }
-    if (t == 0) {
-      part_sum[(size_t)j * gridDim.x + blockIdx.x] = ssum[0];
-      part_max[(size_t)j * gridDim.x + blockIdx.x] = smax[0];
-      part_idx[(size_t)j * gridDim.x + blockIdx.x] = sidx[0];
-    }
-    __syncthreads();
-  }
-}
+constexpr int MON_BLOCKS = 512;
+constexpr int MON_REC = 29;
 
-// force integrand per wall node, reduced on the host in node order
+struct ForceFrame {
+  double ft[3], fn1[3], fn2[3];  // flow tangent, lift and lateral normals
+  double groundplane, datum[3];
+};
+
+// the integrand of calculateLiftAndDrag at wall node ip (DEF:14757-14803)
 template <typename R>
-__global__ void k_force_terms(LevelDev<R> L, DevParams P, const int *wall_nodes, int nw, const R *warea, double *out,
-                              double f0, double f1, double f2, double n10, double n11, double n12, double n20,
-                              double n21, double n22, double groundplane, double d0, double d1, double d2, int bsel) {
-  L.to_batch(bsel);
-  P.to_batch(bsel);
-  int k = gtid();
-  if (k >= nw) return;
-  int ip = wall_nodes[k];
-  double *o = out + 8 * (size_t)k;
-  double z = L.xyz ? double(L.xyz[3 * (size_t)ip + 2]) : 0.0;
+__device__ __forceinline__ void force_terms(const LevelDev<R> &L, const DevParams &P, const ForceFrame &F, int ip,
+                                            const R *warea, double *o) {
   for (int c = 0; c < 8; ++c) o[c] = 0;
-  if (!(z > groundplane + 0.01)) return;
+  double z = L.xyz ? double(L.xyz[3 * (size_t)ip + 2]) : 0.0;
+  if (!(z > F.groundplane + 0.01)) return;
   double x = L.xyz ? double(L.xyz[3 * (size_t)ip]) : 0.0;
   double y = L.xyz ? double(L.xyz[3 * (size_t)ip + 1]) : 0.0;
-  double xc = x - d0, yc = y - d1, zc = z - d2;
+  double xc = x - F.datum[0], yc = y - F.datum[1], zc = z - F.datum[2];
   double nx = L.nrm[3 * (size_t)ip], ny = L.nrm[3 * (size_t)ip + 1], nz = L.nrm[3 * (size_t)ip + 2];
   double a = warea[ip];
   double pinf = 1.0 / (P.gamma * P.mach * P.mach);
   double dp = double(L.p[ip]) - pinf;
-  double n1 = nx * n10 + ny * n11 + nz * n12, n2 = nx * n20 + ny * n21 + nz * n22, nt = nx * f0 + ny * f1 + nz * f2;
+  double n1 = nx * F.fn1[0] + ny * F.fn1[1] + nz * F.fn1[2];
+  double n2 = nx * F.fn2[0] + ny * F.fn2[1] + nz * F.fn2[2];
+  double nt = nx * F.ft[0] + ny * F.ft[1] + nz * F.ft[2];
   double lift = -a * dp * n1, latF = -a * dp * n2, drag = -a * dp * nt;
   double latM = xc * a * dp * n2 - yc * a * dp * nt;
   double longM = xc * a * dp * n1 - zc * a * dp * nt;
   double fd = 0;
   if (P.Re > 0.0) {
     double w1 = L.ws[3 * (size_t)ip], w2 = L.ws[3 * (size_t)ip + 1], w3 = L.ws[3 * (size_t)ip + 2];
-    double s1 = w1 * n10 + w2 * n11 + w3 * n12, s2 = w1 * n20 + w2 * n21 + w3 * n22, st = w1 * f0 + w2 * f1 + w3 * f2;
+    double s1 = w1 * F.fn1[0] + w2 * F.fn1[1] + w3 * F.fn1[2];
+    double s2 = w1 * F.fn2[0] + w2 * F.fn2[1] + w3 * F.fn2[2];
+    double st = w1 * F.ft[0] + w2 * F.ft[1] + w3 * F.ft[2];
     lift += a * s1;
     latF += a * s2;
     drag += a * st;
@@ -3290,6 +3262,124 @@ __global__ void k_force_terms(LevelDev<R> L, DevParams P, const int *wall_nodes,
   o[5] = longM;
   o[6] = -double(L.p[ip]) * (a - pinf) * n1;
   o[7] = -double(L.p[ip]) * (a - pinf) * nt;
+}
+
+// max |d| with the smallest index among equal values
+__device__ __forceinline__ void maxloc(double &m, int &k, double m2, int k2) {
+  if (m2 > m || (m2 == m && k2 < k)) {
+    m = m2;
+    k = k2;
+  }
+}
+
+// block-wide reduction of one sum and of one (max, index) pair: butterflies
+// inside each wavefront, then the NTHR/64 wavefront results in order
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0;
+  for (int q = 0; q < NTHR / 64; ++q) t += lds[q];
+  return t;
+}
+
+__device__ __forceinline__ void block_maxloc(double &m, int &k, double *lds, int *ildx) {
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) maxloc(m, k, __shfl_xor(m, s, 64), __shfl_xor(k, s, 64));
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    lds[w] = m;
+    ildx[w] = k;
+  }
+  __syncthreads();
+  m = lds[0];
+  k = ildx[0];
+  for (int q = 1; q < NTHR / 64; ++q) maxloc(m, k, lds[q], ildx[q]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(NTHR) void k_monitor_reduce(LevelDev<R> L, DevParams P, ForceFrame F, int nres,
+                                                         const int *host_index, const int *wall_nodes, int nw,
+                                                         const R *warea, double *part, int bsel) {
+  L.to_batch(bsel);
+  P.to_batch(bsel);
+  __shared__ double lds[NTHR / 64];
+  __shared__ int ildx[NTHR / 64];
+  const int n = L.n, stride = gridDim.x * NTHR;
+  double sum[7], mx[7];
+  int mi[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    sum[j] = 0;
+    mx[j] = 0;
+    mi[j] = 0x7fffffff;
+  }
+  for (int i = blockIdx.x * NTHR + threadIdx.x; i < n; i += stride) {
+    const int hi = host_index[i];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      if (j >= nres) break;
+      double d = double(L.u[(size_t)j * n + i]) - double(L.uprev[(size_t)j * n + i]);
+      sum[j] += d * d;
+      maxloc(mx[j], mi[j], fabs(d), hi);
+    }
+  }
+  double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = blockIdx.x * NTHR + threadIdx.x; k < nw; k += stride) {
+    double o[8];
+    force_terms(L, P, F, wall_nodes[k], warea, o);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) f[c] += o[c];
+  }
+  double *rec = part + (size_t)MON_REC * blockIdx.x;
+  for (int j = 0; j < 7; ++j) {
+    double t = block_sum(sum[j], lds);
+    double m = mx[j];
+    int k = mi[j];
+    block_maxloc(m, k, lds, ildx);
+    if (threadIdx.x == 0) {
+      rec[j] = t;
+      rec[7 + j] = m;
+      rec[14 + j] = double(k);
+    }
+  }
+  for (int c = 0; c < 8; ++c) {
+    double t = block_sum(f[c], lds);
+    if (threadIdx.x == 0) rec[21 + c] = t;
+  }
+}
+
+// the MON_BLOCKS records to one, in block order (one block of NTHR threads)
+__global__ __launch_bounds__(NTHR) void k_monitor_final(const double *part, int nrec, double *out) {
+  __shared__ double lds[NTHR / 64];
+  __shared__ int ildx[NTHR / 64];
+  for (int q = 0; q < MON_REC; ++q) {
+    const bool is_max = q >= 7 && q < 14, is_idx = q >= 14 && q < 21;
+    if (is_idx) continue;
+    double v = 0;
+    int k = 0x7fffffff;
+    for (int b = threadIdx.x; b < nrec; b += NTHR) {
+      const double *r = part + (size_t)MON_REC * b;
+      if (is_max)
+        maxloc(v, k, r[q], (int)r[q + 7]);
+      else
+        v += r[q];
+    }
+    if (is_max) {
+      block_maxloc(v, k, lds, ildx);
+      if (threadIdx.x == 0) {
+        out[q] = v;
+        out[q + 7] = double(k);
+      }
+    } else {
+      double t = block_sum(v, lds);
+      if (threadIdx.x == 0) out[q] = t;
+    }
+  }
 }
 
 }  // namespace fk

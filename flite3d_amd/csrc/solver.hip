@@ -181,8 +181,8 @@ struct Level {
   // transfer from the next finer level (coarse levels)
   int *f2c = nullptr, *cptr = nullptr, *clist = nullptr;
   double *rw = nullptr;
-  // monitor scratch
-  double *m_sum = nullptr, *m_max = nullptr, *m_force = nullptr;
+  // monitor records: MON_BLOCKS block records and the total (MON_REC doubles each)
+  double *m_part = nullptr;
   double *abuf = nullptr;  // adjoint side shares, 10 per side (allocated on first use)
   // second-order linearisation (allocated on first use): per-side adjoint
   // shares, per-node scratch, extremal neighbours of the limiter bounds
@@ -190,7 +190,6 @@ struct Level {
   int *amin = nullptr;
   double *vlin = nullptr;  // laminar viscous linearisation scratch (allocated on first use)
   std::vector<int> side_host;  // device side -> uploaded side index
-  int *m_idx = nullptr;
 
   template <typename R>
   LevelDev<R> view() const {
@@ -898,11 +897,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(al_real(&L.jst, c->hp.flux_type != 4 ? (size_t)(8 * (size_t)n) * NB : 1));
   TRY(al_real(&L.uold, c->hp.flow_type != 0 ? (size_t)(6 * (size_t)n) * NB : 1));
   TRY(al_real(&L.uold2, c->hp.flow_type != 0 ? (size_t)(6 * (size_t)n) * NB : 1));
-  int nbm = nblk(n);
-  TRY(P.alloc(&L.m_sum, 7 * (size_t)nbm));
-  TRY(P.alloc(&L.m_max, 7 * (size_t)nbm));
-  TRY(P.alloc(&L.m_idx, 7 * (size_t)nbm));
-  TRY(P.alloc(&L.m_force, 8 * (size_t)std::max<size_t>(1, L.wall_dev.size())));
+  TRY(P.alloc(&L.m_part, (size_t)MON_REC * (MON_BLOCKS + 1)));
   // transfer operators from the finer level
   if (level > 1) {
     Level &F = *c->lev[level - 2];
@@ -1927,67 +1922,72 @@ int flite_apply(flite_ctx *c, int level, int op) {
   return sync(c);
 }
 
-int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
-  TRY(check_ctx(c));
+}  // extern "C"
+
+namespace {
+// the monitor's reductions on the device into part (MON_BLOCKS + 1 records
+// of MON_REC doubles: the blocks' records, then the total), the total
+// copied into tot; res, maxres, maxidx and forces formed from it
+int monitor_into(flite_ctx *c, double *part, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
   Level &L = *c->lev[0];
   int nres = c->P.sa ? 6 : 5;
-  int nb = nblk(L.n);
   DevParams P = c->P;
   const int bsel = c->sel;  // selected condition
   const flite_params &hp = c->hp_b[bsel];
   double PI = 4.0 * std::atan(1.0);
   double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
-  double ft[3] = {std::cos(ar) * std::cos(br), std::sin(br), std::sin(ar)};
-  double fn1[3] = {-ft[2], 0.0, ft[0]};
-  double fn2[3] = {-ft[1], ft[0], 0.0};
+  ForceFrame F;
+  F.ft[0] = std::cos(ar) * std::cos(br);
+  F.ft[1] = std::sin(br);
+  F.ft[2] = std::sin(ar);
+  F.fn1[0] = -F.ft[2];
+  F.fn1[1] = 0.0;
+  F.fn1[2] = F.ft[0];
+  F.fn2[0] = -F.ft[1];
+  F.fn2[1] = F.ft[0];
+  F.fn2[2] = 0.0;
+  F.groundplane = hp.groundplane;
+  for (int k = 0; k < 3; ++k) F.datum[k] = hp.datum[k];
   int nw = (int)L.wall_dev.size();
+  double *total = part + (size_t)MON_REC * MON_BLOCKS;
   TRY(with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
-    hipLaunchKernelGGL(k_monitor_partial<R>, dim3(nb), dim3(NTHR), 0, c->stream, V, nres, L.host_index, L.m_sum,
-                       L.m_max, L.m_idx, bsel);
-    if (nw > 0)
-      hipLaunchKernelGGL(k_force_terms<R>, dim3(nblk(nw)), dim3(NTHR), 0, c->stream, V, P, L.wall_nodes, nw,
-                         (const R *)L.warea, L.m_force, ft[0], ft[1], ft[2], fn1[0], fn1[1], fn1[2], fn2[0], fn2[1],
-                         fn2[2], hp.groundplane, hp.datum[0], hp.datum[1], hp.datum[2], bsel);
+    hipLaunchKernelGGL(k_monitor_reduce<R>, dim3(MON_BLOCKS), dim3(NTHR), 0, c->stream, V, P, F, nres, L.host_index,
+                       L.wall_nodes, nw, (const R *)L.warea, part, bsel);
+    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(NTHR), 0, c->stream, (const double *)part, MON_BLOCKS, total);
     return check_launch();
   }));
-  std::vector<double> s(7 * (size_t)nb), m(7 * (size_t)nb), f(8 * (size_t)std::max(1, nw));
-  std::vector<int> ix(7 * (size_t)nb);
-  HIPTRY(hipMemcpyAsync(s.data(), L.m_sum, sizeof(double) * nres * nb, hipMemcpyDeviceToHost, c->stream));
-  HIPTRY(hipMemcpyAsync(m.data(), L.m_max, sizeof(double) * nres * nb, hipMemcpyDeviceToHost, c->stream));
-  HIPTRY(hipMemcpyAsync(ix.data(), L.m_idx, sizeof(int) * nres * nb, hipMemcpyDeviceToHost, c->stream));
-  if (nw > 0) HIPTRY(hipMemcpyAsync(f.data(), L.m_force, sizeof(double) * 8 * nw, hipMemcpyDeviceToHost, c->stream));
+  double tot[MON_REC];
+  HIPTRY(hipMemcpyAsync(tot, total, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
   TRY(sync(c));
   for (int j = 0; j < 7; ++j) { res[j] = 0; maxres[j] = 0; maxidx[j] = 0; }
   for (int j = 0; j < nres; ++j) {
-    double acc = 0, mx = 0;
-    int mi = 0x7fffffff;
-    for (int b = 0; b < nb; ++b) {
-      acc += s[(size_t)j * nb + b];
-      double v = m[(size_t)j * nb + b];
-      int vi = ix[(size_t)j * nb + b];
-      if (v > mx || (v == mx && v > 0 && vi < mi)) { mx = v; mi = vi; }
-    }
-    res[j] = std::sqrt(acc);
-    maxres[j] = mx;
-    maxidx[j] = (mx > 0 ? mi : 0) + c->staged[0].base;  // reference starts at node 1
+    res[j] = std::sqrt(tot[j]);
+    maxres[j] = tot[7 + j];
+    maxidx[j] = (tot[7 + j] > 0 ? (int)tot[14 + j] : 0) + c->staged[0].base;  // reference starts at node 1
   }
-  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k = 0; k < nw; ++k)
-    for (int q = 0; q < 8; ++q) tot[q] += f[8 * (size_t)k + q];
+  const double *f = tot + 21;
   double fct = 2.0 / hp.reference_area;
-  forces[0] = fct * tot[0];
-  forces[1] = fct * tot[1];
-  forces[2] = fct * tot[2];
-  forces[3] = fct * tot[3];
-  forces[4] = tot[4];
-  forces[5] = tot[5];
-  forces[6] = fct * tot[6];
-  forces[7] = fct * tot[7];
+  forces[0] = fct * f[0];
+  forces[1] = fct * f[1];
+  forces[2] = fct * f[2];
+  forces[3] = fct * f[3];
+  forces[4] = f[4];
+  forces[5] = f[5];
+  forces[6] = fct * f[6];
+  forces[7] = fct * f[7];
   forces[8] = 0.0;
   for (int j = 0; j < nres; ++j)
     if (std::isnan(res[j])) return fail(FLITE_E_NAN, "NaN in residual");
   return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
+  TRY(check_ctx(c));
+  return monitor_into(c, c->lev[0]->m_part, res, maxres, maxidx, forces);
 }
 
 
