@@ -35,6 +35,7 @@ def lib():
         L.or_field_get.argtypes = [vp, i, i, P(d), i]
         L.or_field_set.argtypes = [vp, i, i, P(d), i]
         L.or_apply.argtypes = [vp, i, i]
+        L.or_set_threads.argtypes = [vp, i]
         L.or_magnitudes.argtypes = [vp, i, i, P(d), i]
         L.or_init_freestream.argtypes = [vp]
         L.or_init_from_state.argtypes = [vp]
@@ -119,6 +120,10 @@ class Oracle:
 
     def apply(self, level, op):
         self._check(self.L.or_apply(self.ctx, level, OPS[op] if isinstance(op, str) else op))
+
+    def set_threads(self, threads):
+        """OpenMP threads of the side-flux loops (results do not depend on it)."""
+        self._check(self.L.or_set_threads(self.ctx, int(threads)))
 
     def magnitudes(self, level, op):
         """Per node and component, sum of |term| over what the operation adds

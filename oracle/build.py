@@ -16,7 +16,7 @@ LIB = os.path.join(OUT, "liboracle.so")
 def build():
     os.makedirs(OUT, exist_ok=True)
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-fno-fast-math",
-           "-ffp-contract=off", "-Wall", os.path.join(HERE, "oracle.c"),
+           "-ffp-contract=off", "-Wall", "-fopenmp", os.path.join(HERE, "oracle.c"),
            "-lm", "-o", LIB]
     print("[build]", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -114,6 +114,7 @@ struct or_ctx {
   /* or_magnitudes: when set, the side loops of the operation being run add
    * |contribution| per node and component here ([c*n+i], 6 components) */
   double *mag;
+  int threads; /* OpenMP threads of the side-flux and node loops (or_set_threads; 1 by default) */
 };
 
 static void settle(or_ctx *C);
@@ -148,6 +149,7 @@ or_ctx *or_create(const flite_params *p, int n_levels) {
   or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
   c->P = *p;
   c->nl = n_levels;
+  c->threads = 1;
   return c;
 }
 
@@ -728,6 +730,7 @@ static void viscous(or_ctx *C, orl *L, double *rhs) {
   double oneOverRe = 1. / P->reynolds_number, oneOverPr = 1. / P->prandtl_number;
   double oneOverPrt = 1. / P->turbulent_prandtl_number;
   double twoThirds = 2. / 3., fourThirds = 4. / 3.;
+#pragma omp parallel for schedule(static) num_threads(C->threads)
   for (int i = 0; i < n; ++i) {
     double vt = 1. / L->vol[i];
     double dudx = H(C, 0, i) * vt, dvdx = H(C, 1, i) * vt, dwdx = H(C, 2, i) * vt, dTdx = H(C, 3, i) * vt;
@@ -1239,9 +1242,15 @@ static int make_rhs(or_ctx *C, orl *L) {
     gradient(C, L);
     if (autoclip == 1) local_minmax(C, L);
   }
-  double gr1[25], gr2[25];
+  /* the side fluxes first (independent: threads when or_set_threads asked
+   * for them), then their scatter in side order, so every thread count
+   * gives the same bits */
+  double *fe = (double *)malloc(sizeof(double) * 5 * (size_t)(L->ne > 0 ? L->ne : 1));
+  if (!fe) return fail(FLITE_E_ARG, "out of host memory");
+#pragma omp parallel for schedule(static) num_threads(C->threads)
   for (int e = 0; e < L->ne; ++e) {
     int i1 = L->e1[e], i2 = L->e2[e];
+    double gr1[25], gr2[25];
     double s1[5], s2[5];
     for (int c = 0; c < 5; ++c) { s1[c] = U(L, c, i1); s2[c] = U(L, c, i2); }
     double dx[3];
@@ -1263,6 +1272,11 @@ static int make_rhs(or_ctx *C, orl *L) {
       if (wgcl != 0.0 && (P->flux_type == 1 || sqrt(wx * wx + wy * wy + wz * wz) > 1.0e-10))
         for (int c = 0; c < 5; ++c) f[c] -= (s1[c] + s2[c]) * wgcl;
     }
+    for (int c = 0; c < 5; ++c) fe[5 * (size_t)e + c] = f[c];
+  }
+  for (int e = 0; e < L->ne; ++e) {
+    int i1 = L->e1[e], i2 = L->e2[e];
+    const double *f = fe + 5 * (size_t)e;
     for (int c = 0; c < 5; ++c) {
       RHS(L, c, i1) += f[c];
       RHS(L, c, i2) -= f[c];
@@ -1270,6 +1284,7 @@ static int make_rhs(or_ctx *C, orl *L) {
       MAG(C, L, c, i2, f[c]);
     }
   }
+  free(fe);
   for (int s = 0; s < L->nbs; ++s) {
     int i1 = L->b1[s], i2 = L->b2[s], ind = L->bi[s];
     double wx = L->wb[s], wy = L->wb[L->nbs + s], wz = L->wb[2 * (size_t)L->nbs + s];
@@ -2099,6 +2114,12 @@ int or_magnitudes(or_ctx *C, int level, int op, double *out, int ld) {
   free(C->mag);
   C->mag = NULL;
   return rc;
+}
+
+int or_set_threads(or_ctx *C, int threads) {
+  if (!C || threads < 1) return fail(FLITE_E_ARG, "threads >= 1");
+  C->threads = threads;
+  return 0;
 }
 
 int or_apply(or_ctx *C, int level, int op) {

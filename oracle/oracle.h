@@ -35,6 +35,9 @@ int or_field_get(or_ctx *c, int level, int field, double *out, int ld);
 int or_field_set(or_ctx *c, int level, int field, const double *in, int ld);
 
 int or_apply(or_ctx *c, int level, int op);
+/* OpenMP threads of the side-flux loops (the timed CPU comparator); the
+ * results do not depend on the count.  1 by default. */
+int or_set_threads(or_ctx *c, int threads);
 /* sum over the terms an operation adds (RHS, VISCOUS, SA_DIFFUSION) of their
  * magnitudes, per node and component: out(N, 6) */
 int or_magnitudes(or_ctx *c, int level, int op, double *out, int ld);

@@ -236,3 +236,19 @@ def test_boundary_weighting_exact_for_linear_flux():
     err = np.abs(r - ref).max() / np.abs(ref).max()
     assert err < 1e-12, err
     o.close()
+
+
+def test_threads_give_the_same_bits():
+    """The oracle's OpenMP side-flux loops (the timed CPU comparator) leave
+    every result independent of the thread count."""
+    lv = levels(6, 2, first_cell=0.01)
+    out = []
+    for threads in (1, 3):
+        o = make(rans(multigridscheme=3), lv)
+        o.set_threads(threads)
+        o.init_freestream()
+        o.cycle(1)
+        o.cycle(2)
+        out.append(o.get(1, "U"))
+        o.close()
+    assert np.array_equal(out[0], out[1])
