@@ -8,9 +8,13 @@ C-ABI (libflite3d_amd.so) and through the oracle on the same inputs:
    1e-9, drag to 1e-10, monitor residuals every cycle to 1e-8.
 2. RANS-SA, 128^3 graded from 1e-6 (2.1M nodes): one fine residual (HLLC-2 +
    SA convection and source), viscous term, time steps, SA time steps and SA
-   diffusion, each to 1e-11 per node and component, normalised by
-   max(|oracle|, sum over the node's terms of |term|); freestream preserved
-   with the wall made farfield.
+   diffusion, per node and component, normalised by max(|oracle|, sum over
+   the node's terms of |term|), to 1e-11 or to ten times the oracle's own
+   change when its sides are summed in another order, whichever is larger:
+   on this grading the Green-Gauss gradients cancel to 1e-6 of their terms,
+   and reordering the sums alone moves the oracle's viscous term by 3e-7 of
+   the node scale (its residual by 5e-11); freestream preserved with the wall
+   made farfield.
 3. Discrete adjoint, Euler 64^3: 16 unit directions (seed 1), the
    tangent/adjoint dot product to 1e-12 and the tangent against central
    differences (h = 1e-7) of the oracle's residual.
@@ -18,7 +22,7 @@ C-ABI (libflite3d_amd.so) and through the oracle on the same inputs:
    coarse levels: four conditions equal their single runs bit for bit; one
    condition with FP64 coarse levels equals the oracle to 1e-9.
 5. RANS-SA, 200^3 graded from 1e-6 (8.1M nodes), 5 levels: one fine
-   residual to 1e-11 per node as in 2, and freestream preservation.
+   residual per node as in 2, and freestream preservation.
 """
 import dataclasses
 
@@ -102,12 +106,12 @@ def rans_state(L, rng_seed):
     return u
 
 
-def fine_operations(lv_fine, iv, what):
+def fine_operations(lv_fine, iv, what, gpu=Solver):
     """GPU and oracle on one fine level at the same state, in a smoothing
     step's order: PRESSURE, TIMESTEPS, VISCOUS, SA_TIMESTEPS, SA_DIFFUSION,
     RHS; returns per quantity (gpu, oracle, magnitudes)."""
     P = iv.to_struct()
-    s, o = Solver(P, [lv_fine]), Oracle(P, [lv_fine])
+    s, o = gpu(P, [lv_fine]), Oracle(P, [lv_fine])
     u = rans_state(lv_fine, 0)
     z = np.zeros((1, lv_fine.n_nodes))
     out = {}
@@ -141,10 +145,25 @@ def fine_operations(lv_fine, iv, what):
     return out
 
 
-def check_fine_operations(out):
+def reordered(L, seed=0):
+    """The level with its sides and boundary sides in another order: the
+    same mesh, every sum over sides taken in a different order."""
+    rng = np.random.default_rng(seed)
+    ps, pb = rng.permutation(L.n_sides), rng.permutation(L.n_bsides)
+    return dataclasses.replace(L, side_index=L.side_index[:, ps].copy(), side_w=L.side_w[:, ps].copy(),
+                               bside_index=L.bside_index[:, pb].copy(), bside_w=L.bside_w[:, pb].copy())
+
+
+def check_fine_operations(out, sens):
+    """GPU against the oracle, per node and component: within 1e-11 of the
+    node scale, or within ten times what reordering the oracle's own sums
+    changes (sens: the oracle on the reordered level)."""
     for k, (a, b, mag) in out.items():
         e = node_err(a, b, mag)
-        assert e.max() <= 1e-11, (k, e)
+        s = node_err(sens[k][1], b, mag)
+        tol = np.maximum(1e-11, 10.0 * s)
+        print(f"config check {k}: gpu {e.max():.2e}, oracle reordered {s.max():.2e}", flush=True)
+        assert np.all(e <= tol), (k, e, s)
         # the magnitudes bound the sums they normalise (triangle inequality)
         if mag.any():
             assert np.all(np.abs(b) <= mag * (1 + 1e-12) + 1e-300), k
@@ -153,7 +172,8 @@ def check_fine_operations(out):
 def freestream_residual(lv_fine, iv, what):
     """Residual of the freestream with every boundary side farfield: zero up
     to rounding, relative to the node's sum of |terms| (from the oracle at
-    the same state)."""
+    the same state).  The oracle's own value on this grading grows with the
+    mesh: 3e-14 at 32^3, 7e-14 at 64^3."""
     say(f"{what}: freestream preservation, wall sides made farfield")
     bi = lv_fine.bside_index.copy()
     bi[2][bi[2] == 1] = 3
@@ -184,8 +204,11 @@ def test_config2_rans_128_fine_operations(box128):
     lv = box128
     assert lv[0].n_nodes == 2146689 and lv[0].n_sides == 14827904
     iv = rans(reynoldsnumber=6.0e6, machnumber=0.7, alpha=1.0, multigridscheme=3)
-    check_fine_operations(fine_operations(lv[0], iv, "config 2"))
-    assert freestream_residual(lv[0], iv, "config 2") <= 1e-13
+    out = fine_operations(lv[0], iv, "config 2")
+    say("config 2: the oracle on the reordered level")
+    sens = fine_operations(reordered(lv[0]), iv, "config 2 reordered", gpu=Oracle)
+    check_fine_operations(out, sens)
+    assert freestream_residual(lv[0], iv, "config 2") <= 1e-12
 
 
 def test_config2_rans_128_multigrid_cycle(box128):
@@ -318,8 +341,18 @@ def test_config5_rans_200_fine_residual():
         x.apply(1, "VISCOUS")
     s.apply(1, "RHS")
     mag = o.magnitudes(1, "RHS")
-    e = node_err(s.get(1, "RHS")[:6], o.get(1, "RHS")[:6], mag)
-    assert e.max() <= 1e-11, e
+    out = {"rhs": (s.get(1, "RHS")[:6], o.get(1, "RHS")[:6], mag)}
     s.close()
     o.close()
-    assert freestream_residual(lv[0], iv, "config 5") <= 1e-13
+    say("config 5: the oracle on the reordered level")
+    r = Oracle(P, [reordered(lv[0])])
+    r.init_freestream()
+    r.set(1, "U", u)
+    r.apply(1, "PRESSURE")
+    r.apply(1, "DISSIPATION")
+    r.apply(1, "VISCOUS")
+    r.apply(1, "RHS")
+    sens = {"rhs": (None, r.get(1, "RHS")[:6], None)}
+    r.close()
+    check_fine_operations(out, sens)
+    assert freestream_residual(lv[0], iv, "config 5") <= 1e-12
