@@ -87,38 +87,69 @@ struct EdgeTable {
 };
 
 // kd-tree nearest neighbour over a point set (wall nodes).
+// Nearest point by a k-d tree over an implicit median layout: the subtree
+// over idx[lo, hi) keeps its point at mid = (lo + hi) / 2, splits on the
+// axis of its largest extent, and carries the bounding box of its points,
+// so a query far from a nearly flat point set (a wall seen from the far
+// field) prunes by the full 3-D distance to each box.  Exact: among points
+// at the smallest distance the smallest index wins, whatever the visiting
+// order.
 struct KdTree {
   std::vector<int> idx;
+  std::vector<unsigned char> axis;  // split axis per subtree (at its mid)
+  std::vector<double> box;          // 6 per subtree (at its mid): min xyz, max xyz
   const double *p = nullptr;
   void build(const double *pts, std::vector<int> ids) {
     p = pts;
     idx = std::move(ids);
-    build_rec(0, int(idx.size()), 0);
+    axis.assign(idx.size(), 0);
+    box.assign(6 * idx.size(), 0.0);
+    build_rec(0, int(idx.size()));
   }
-  void build_rec(int lo, int hi, int depth) {
-    if (hi - lo <= 1) return;
-    int mid = (lo + hi) / 2, ax = depth % 3;
-    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
-                     [&](int a, int b) { return p[3 * a + ax] < p[3 * b + ax]; });
-    build_rec(lo, mid, depth + 1);
-    build_rec(mid + 1, hi, depth + 1);
-  }
-  void query(const double *q, int lo, int hi, int depth, double &best,
-             int &besti) const {
+  void build_rec(int lo, int hi) {
     if (hi <= lo) return;
-    int mid = (lo + hi) / 2, ax = depth % 3;
-    const double *c = p + 3 * idx[mid];
+    int mid = (lo + hi) / 2;
+    double b[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int k = lo; k < hi; ++k)
+      for (int c = 0; c < 3; ++c) {
+        double v = p[3 * size_t(idx[k]) + c];
+        b[c] = std::min(b[c], v);
+        b[3 + c] = std::max(b[3 + c], v);
+      }
+    int ax = 0;
+    for (int c = 1; c < 3; ++c)
+      if (b[3 + c] - b[c] > b[3 + ax] - b[ax]) ax = c;
+    std::copy(b, b + 6, box.begin() + 6 * size_t(mid));
+    axis[mid] = (unsigned char)ax;
+    if (hi - lo <= 1) return;
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                     [&](int a, int b2) { return p[3 * size_t(a) + ax] < p[3 * size_t(b2) + ax]; });
+    build_rec(lo, mid);
+    build_rec(mid + 1, hi);
+  }
+  void query(const double *q, int lo, int hi, double &best, int &besti) const {
+    if (hi <= lo) return;
+    int mid = (lo + hi) / 2;
+    const double *b = box.data() + 6 * size_t(mid);
+    double lb = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double d = q[c] < b[c] ? b[c] - q[c] : (q[c] > b[3 + c] ? q[c] - b[3 + c] : 0.0);
+      lb += d * d;
+    }
+    if (lb > best) return;
+    const double *c = p + 3 * size_t(idx[mid]);
     double d2 = (c[0] - q[0]) * (c[0] - q[0]) + (c[1] - q[1]) * (c[1] - q[1]) +
                 (c[2] - q[2]) * (c[2] - q[2]);
     if (d2 < best || (d2 == best && idx[mid] < besti)) {
       best = d2;
       besti = idx[mid];
     }
+    int ax = axis[mid];
     double diff = q[ax] - c[ax];
     int nlo = diff < 0 ? lo : mid + 1, nhi = diff < 0 ? mid : hi;
     int flo = diff < 0 ? mid + 1 : lo, fhi = diff < 0 ? hi : mid;
-    query(q, nlo, nhi, depth + 1, best, besti);
-    if (diff * diff <= best) query(q, flo, fhi, depth + 1, best, besti);
+    query(q, nlo, nhi, best, besti);
+    query(q, flo, fhi, best, besti);
   }
 };
 
@@ -322,7 +353,7 @@ void build_fine(fp_mesh &m, int n, const double *xyz_in, int nt,
         if (iswall[i]) { L.wdist[i] = 0.0; L.wnode[i] = i; continue; }
         double best = 1e300;
         int bi = -1;
-        kd.query(X + 3 * size_t(i), 0, int(kd.idx.size()), 0, best, bi);
+        kd.query(X + 3 * size_t(i), 0, int(kd.idx.size()), best, bi);
         L.wdist[i] = std::sqrt(best);
         L.wnode[i] = bi;
       }

@@ -197,7 +197,7 @@ def freestream_residual(lv_fine, iv, what):
 @pytest.fixture(scope="module")
 def box128():
     say("config 2 mesh: 128^3 graded from 1e-6, 4 levels")
-    return mesh.bump_levels(128, 4, first_cell=1e-6, ratio=1.15)
+    return mesh.bump_levels(128, 4, first_cell=1e-6, ratio=1.15, progress=say)
 
 
 def test_config2_rans_128_fine_operations(box128):
@@ -326,7 +326,7 @@ def test_config4_batch_64_conditions():
 # ---- configuration 5 ------------------------------------------------------------
 def test_config5_rans_200_fine_residual():
     say("config 5 mesh: 200^3 graded from 1e-6, 5 levels")
-    lv = mesh.bump_levels(200, 5, first_cell=1e-6, ratio=1.15)
+    lv = mesh.bump_levels(200, 5, first_cell=1e-6, ratio=1.15, progress=say)
     assert lv[0].n_nodes == 8120601 and lv[0].n_sides == 56360600
     iv = rans(reynoldsnumber=1.0e7, machnumber=0.75, alpha=2.0, multigridscheme=3)
     P = iv.to_struct()

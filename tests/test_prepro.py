@@ -100,3 +100,23 @@ def test_agglomeration(box):
         assert C.n_nodes < F.n_nodes / 3
         assert np.all(C.side_index[0] < C.side_index[1])
         assert np.all(C.bside_index[0] <= C.bside_index[1])
+
+
+def test_wall_distance_is_the_nearest_wall_node():
+    """The wall distance and wall node of every fine node (the k-d tree
+    search) equal a brute-force scan: the nearest node of a wall face, the
+    smallest node number among equally near ones."""
+    xyz, tets, btri, bind = mesh.bump_box(10, jitter=0.1, seed=2, first_cell=1e-3)
+    lv, old = mesh.build_levels(xyz, tets, btri, bind, 1)
+    L = lv[0]
+    X = L.coords.T
+    wall = np.zeros(L.n_nodes, bool)
+    new = np.empty_like(old)
+    new[old] = np.arange(old.size)
+    wall[new[np.unique(btri[bind == 1])]] = True
+    w = np.flatnonzero(wall)
+    d2 = ((X[:, None, :] - X[None, w, :]) ** 2).sum(axis=2)
+    best = d2.min(axis=1)
+    node = np.array([w[np.flatnonzero(d2[i] == best[i]).min()] for i in range(L.n_nodes)])
+    assert np.array_equal(L.wall_node, node)
+    assert np.array_equal(L.wall_dist, np.sqrt(best))
