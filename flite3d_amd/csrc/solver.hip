@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <string>
@@ -260,6 +261,7 @@ struct flite_ctx {
   std::vector<std::vector<std::vector<double>>> carry;
   bool built = false;
   bool preamble_due = false;  // a replaced mesh still owes the step's preamble
+  int device = 0;             // the HIP device current when the context was created
 };
 
 namespace {
@@ -283,6 +285,30 @@ int with_lanes(int lanes, F f) {
 int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(FLITE_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Device scratch of flite_monitor_noalloc: the MON_BLOCKS block records and
+// the total, MON_REC doubles each (119 KB), held on every GPU the library
+// runs on until the process ends.  In host code `scratch` names a host-side
+// stand-in, not a device address: each GPU's copy is found once per device
+// with hipGetSymbolAddress (scratch_setup, called by flite_ctx_create with
+// that device current) and only that address is used.
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[SCRATCH_DEVICES] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[MON_REC * (MON_BLOCKS + 1)];
+namespace {
+
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(FLITE_E_HIP, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev]) return 0;
+  void *ptr = nullptr;
+  HIPTRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(FLITE_E_HIP, "no device address for the monitor scratch");
+  g_scratch_dev[dev] = static_cast<double *>(ptr);
   return 0;
 }
 
@@ -1575,7 +1601,11 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
   int ndev = 0;
   HIPTRY(hipGetDeviceCount(&ndev));
   if (ndev < 1) return fail(FLITE_E_HIP, "no HIP device");
+  int dev = 0;
+  HIPTRY(hipGetDevice(&dev));
+  TRY(scratch_setup(dev));  // this device's copy of the monitor scratch, once per device
   auto c = std::make_unique<flite_ctx>();
+  c->device = dev;
   c->hp = *p;
   c->P = make_dev_params(*p);
   c->P.cond = nullptr;
@@ -1988,6 +2018,21 @@ extern "C" {
 int flite_monitor(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
   TRY(check_ctx(c));
   return monitor_into(c, c->lev[0]->m_part, res, maxres, maxidx, forces);
+}
+
+int flite_monitor_noalloc(flite_ctx *c, double res[7], double maxres[7], int maxidx[7], double forces[9]) {
+  if (!c) return fail(FLITE_E_ARG, "null context");
+  if (!res || !maxres || !maxidx || !forces) return fail(FLITE_E_ARG, "null output array");
+  // building the levels or running a due preamble would allocate or launch
+  // more than the monitor: the context must be ready
+  if (!c->built || c->preamble_due)
+    return fail(FLITE_E_STATE, "flite_monitor_noalloc needs a built context (any stepping, field or monitor call "
+                               "builds it)");
+  if (c->device < 0 || c->device >= SCRATCH_DEVICES || !g_scratch_dev[c->device])
+    return fail(FLITE_E_STATE, "no monitor scratch on this context's device");
+  // the scratch holds MON_BLOCKS + 1 records whatever the level's size: the
+  // reduction's blocks stride over the nodes, so no size exceeds it
+  return monitor_into(c, g_scratch_dev[c->device], res, maxres, maxidx, forces);
 }
 
 

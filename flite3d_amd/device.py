@@ -49,6 +49,7 @@ def lib():
         L.flite_map_u_fine_to_coarse.argtypes = [vp, i]
         L.flite_cycle.argtypes = [vp, i]
         L.flite_monitor.argtypes = [vp, P(d), P(d), P(i), P(d)]
+        L.flite_monitor_noalloc.argtypes = [vp, P(d), P(d), P(i), P(d)]
         L.flite_sync.argtypes = [vp]
         L.flite_tangent_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]

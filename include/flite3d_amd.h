@@ -188,6 +188,17 @@ int flite_cycle(flite_ctx *ctx, int cycle_number);
  *   maxres[7]  max |u_j - uprev_j|, maxidx[7] its node (host numbering, index_base)
  *   forces[9]  lift, drag, latF, frictionDrag, latM, longM, pressureLift, pressureDrag, 0 */
 int flite_monitor(flite_ctx *ctx, double res[7], double maxres[7], int maxidx[7], double forces[9]);
+/* flite_monitor without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as flite_monitor, with the block records in a
+ * file-scope device array of the library (119 KB per GPU, held until the
+ * process ends) instead of the context's buffer; allocates nothing, host or
+ * device.  The context must be built (any earlier stepping, field or monitor
+ * call builds it; FLITE_E_STATE otherwise).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two host threads, two
+ * contexts' streams) race. */
+int flite_monitor_noalloc(flite_ctx *ctx, double res[7], double maxres[7], int maxidx[7], double forces[9]);
 
 /* ---- parity / inspection ------------------------------------------------ */
 /* Named node fields of a level, host numbering, column-major (N, ncomp):
