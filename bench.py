@@ -63,13 +63,7 @@ def build_case(n, n_levels, first_cell=1e-6):
 
 
 def params(fp32, re=6.0e6, mach=0.7, alpha=1.0):
-    from flite3d_amd.params import InputVariables
-    iv = InputVariables(reynoldsnumber=re, machnumber=mach, alpha=alpha, fluxtype=4,
-                        hllcfluxorder=2, automaticlipping=1, hllcclipping=1.0,
-                        gradientmethod=1, turbulencemodel=1, wallsareisentropic=True,
-                        cflnumber=1.0, turbulentcflnumber=1.0, multigridscheme=3,
-                        restartnumber=0)
-    return iv.to_struct(coarse_fp32=fp32)
+    return rans_params(fp32, reynoldsnumber=re, machnumber=mach, alpha=alpha)
 
 
 def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autotune=False):
@@ -104,43 +98,92 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autot
                 res0=r0, res=res[0], forces=m["forces"][:4].tolist(), lanes=lanes)
 
 
-def run_converge(levels, fp32, drop, cap_s, every=250, switch_drop=4.0):
+def run_converge(levels, fp32, drop, cap_s, every=250, switch_drop=4.0, prm=None, label=None):
     """Cycles the solver from freestream until the monitor residual (L2 of
     the density change over the last fine smoothing step, relative to cycle
-    1) falls by `drop` decades or `cap_s` seconds pass; returns the history.
-    With FP32 coarse levels (the mixed mode) the coarse levels switch to FP64
-    once the drop reaches `switch_drop` decades: FP32 coarse corrections
-    stall near a 1e-5 drop (their FAS forcing is rounded to FP32), and the
-    FP64 finish converges to the FP64 solution."""
+    1) falls by `drop` decades or `cap_s` seconds pass, or turns NaN; returns
+    the history and the deepest drop.  With FP32 coarse levels (the mixed
+    mode) the coarse levels switch to FP64 once the drop reaches
+    `switch_drop` decades (None: never): FP32 coarse corrections stall
+    short of a 1e-8 drop, and the FP64 finish converges to the FP64
+    solution."""
     from flite3d_amd import device
-    s = device.Solver(params(fp32), levels)
+    s = device.Solver(prm(fp32) if prm else params(fp32), levels)
     switched = None
     s.init_freestream()
     hist = []
     r0 = None
+    best = 0.0
     c = 1
     s.sync()
     t0 = time.perf_counter()
     while True:
         s.cycle(c)
         m = s.monitor()
-        r0 = r0 or m["res"][0]
-        d = float(np.log10(m["res"][0] / r0)) if m["res"][0] > 0 else -99.0
+        res = float(m["res"][0])
+        r0 = r0 or res
+        # a NaN residual or NaN forces end the run as NaN (a state gone NaN
+        # can stop changing, which reads as a zero residual); an exact zero
+        # with finite forces as a 99-decade drop
+        bad = not np.isfinite(res) or not np.all(np.isfinite(m["forces"][:2]))
+        d = float("nan") if bad else float(np.log10(res / r0)) if res > 0 else -99.0
+        if np.isfinite(d):
+            best = min(best, d)
         el = time.perf_counter() - t0
-        if fp32 and switched is None and d <= -switch_drop:
+        if fp32 and switch_drop is not None and switched is None and d <= -switch_drop:
             s.set_coarse_precision(False)
             switched = (c, round(el, 3))
             log(f"[converge] coarse levels FP32 -> FP64 at cycle {c} ({el:.1f}s)")
         if c % every == 0 or d <= -drop or el > cap_s or not np.isfinite(d):
             hist.append((c, round(el, 3), round(d, 4), float(m["forces"][1]), float(m["forces"][0])))
-            log(f"[converge] {'fp32-coarse' if fp32 else 'fp64'} cycle {c} t={el:.1f}s "
-                f"drop={d:.3f} drag={m['forces'][1]:.12e}")
+            log(f"[converge] {label or ('fp32-coarse' if fp32 else 'fp64')} cycle {c} t={el:.1f}s "
+                f"drop={d:.3f} best={best:.3f} drag={m['forces'][1]:.12e}")
         if d <= -drop or el > cap_s or not np.isfinite(d):
             break
         c += 1
     s.close()
-    return dict(cycles=c, time=el, drop=d, reached=d <= -drop, drag=hist[-1][3],
-                lift=hist[-1][4], coarse_fp64_from=switched, history=hist)
+    return dict(cycles=c, time=el, drop=d, best_drop=best, reached=bool(d <= -drop),
+                nan=not np.isfinite(d), drag=hist[-1][3], lift=hist[-1][4], coarse_fp64_from=switched,
+                history=hist)
+
+
+# the reference's convergence controls (run.inp / &inputVariables), each
+# changed alone from the benchmark's settings
+CONTROLS = [("CFL 1", {}), ("CFL 0.5, turbulent CFL 0.5", dict(cflnumber=0.5, turbulentcflnumber=0.5)),
+            ("CFL 2", dict(cflnumber=2.0)), ("turbulent CFL 0.3", dict(turbulentcflnumber=0.3)),
+            ("maxIncrementFactor 0.1", dict(maxincrementfactor=0.1)),
+            ("numberOfTurbulenceGrids 4", dict(numberofturbulencegrids=4)),
+            ("multigridScheme 1 (V-cycle)", dict(multigridscheme=1))]
+
+
+def euler2_params(fp32):
+    """Second-order HLLC Euler, M 0.5, alpha 2, CFL 2, growing V."""
+    from flite3d_amd.params import InputVariables
+    return InputVariables(reynoldsnumber=0.0, machnumber=0.5, alpha=2.0, fluxtype=4, hllcfluxorder=2,
+                          automaticlipping=1, cflnumber=2.0, multigridscheme=3,
+                          restartnumber=0).to_struct(coarse_fp32=fp32)
+
+
+def rans_params(fp32, **kw):
+    from flite3d_amd.params import InputVariables
+    base = dict(reynoldsnumber=6.0e6, machnumber=0.7, alpha=1.0, fluxtype=4, hllcfluxorder=2,
+                automaticlipping=1, hllcclipping=1.0, gradientmethod=1, turbulencemodel=1,
+                wallsareisentropic=True, cflnumber=1.0, turbulentcflnumber=1.0, multigridscheme=3,
+                restartnumber=0)
+    base.update(kw)
+    return InputVariables(**base).to_struct(coarse_fp32=fp32)
+
+
+def sweep_controls(levels, cap_s):
+    """FP64 runs from freestream with each control of CONTROLS, and with one
+    multigrid level fewer, until a 1e-8 drop, NaN or cap_s seconds."""
+    out = []
+    cases = [(lab, levels, kw) for lab, kw in CONTROLS] + [(f"{len(levels) - 1} levels", levels[:-1], {})]
+    for lab, lv, kw in cases:
+        r = run_converge(lv, False, 8.0, cap_s, every=1000, prm=lambda f, kw=kw: rans_params(f, **kw), label=lab)
+        r.pop("history")
+        out.append(dict(control=lab, **r))
+    return out
 
 
 def euler_params(mach, alpha, order=1, scheme=1, cfl=2.0):
@@ -539,6 +582,14 @@ def main():
     ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the timed steps: run each precision mode to a 1e-8 "
                          "monitor drop or this many seconds, and report the histories")
+    ap.add_argument("--controls", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the timed steps: FP64 runs with each of the reference's convergence "
+                         "controls changed alone, each to a 1e-8 drop, NaN or this many seconds")
+    ap.add_argument("--fp32-alone", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the timed steps: FP32 coarse levels with no switch to FP64, then "
+                         "FP64, each to a 1e-8 drop (1e-10 with --euler) or this many seconds")
+    ap.add_argument("--euler", action="store_true",
+                    help="with --fp32-alone: second-order Euler (M 0.5, alpha 2, CFL 2) instead of RANS-SA")
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
                     help="1: Euler 20^3 GPU vs CPU; 2: the RANS headline (default); "
                          "3: adjoint checks at 64^3; 4: 64 batched conditions at 64^3; "
@@ -562,8 +613,23 @@ def main():
     if a.config == 5:
         a.n, a.levels = 200, 5
         prm = lambda fp32: params(fp32, re=1.0e7, mach=0.75, alpha=2.0)  # noqa: E731
-    levels = build_case(a.n, a.levels, a.first_cell)
+    levels = build_case(a.n, a.levels, a.first_cell if a.first_cell > 0 else None)
     sides = [l.n_sides for l in levels]
+    workload = (f"{'Euler' if a.euler else 'RANS-SA'} bump box {a.n}^3 hexes, "
+                f"{f'first cell {a.first_cell:g}' if a.first_cell > 0 else 'uniform spacing'}, {a.levels} MG levels")
+    if a.controls > 0:
+        print(json.dumps(dict(workload=workload, cap_s=a.controls, runs=sweep_controls(levels, a.controls))),
+              flush=True)
+        return
+    if a.fp32_alone > 0:
+        prm = euler2_params if a.euler else None
+        runs = {tag: run_converge(levels, fp32, 10.0 if a.euler else 8.0, a.fp32_alone, every=1000,
+                                  switch_drop=None, prm=prm, label=tag)
+                for tag, fp32 in (("fp32-coarse", True), ("fp64", False))}
+        for r in runs.values():
+            r.pop("history")
+        print(json.dumps(dict(workload=workload, cap_s=a.fp32_alone, runs=runs)), flush=True)
+        return
     if a.converge > 0:
         out = {}
         for tag, fp32 in (("mixed", True), ("fp64", False)):
@@ -583,7 +649,10 @@ def main():
         log(f"[bench] {tag}: {r['ms_per_cycle']:.2f} ms/cycle, {r['gedges']:.2f} Gedges/s, "
             f"evals/level {r['evals']}, res drop {np.log10(r['res'] / r['res0']):.2f}")
         results[tag] = r
-    head = results["fp64" if a.fp64_only else "mixed"]
+    # the headline is the FP64 run: FP32 coarse levels do not converge alone
+    # (their rounding noise does not shrink with the correction), so the
+    # mixed mode is a switch used above a 1e-4 drop, reported beside it
+    head = results["fp64"]
     k_avg_ms = head["kernel_ms"] / max(1, head["kernel_launches"])
     algo = ALGO_BYTES_PER_EDGE["flux"] * sides[0]
     achieved = algo / (k_avg_ms * 1e-3) / 1e9
@@ -599,14 +668,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 fine / f32 coarse" if not a.fp64_only else "f64",
+        "dtype": "f64",
         "data": "synthetic bump-box mesh, freestream start",
         "config": {"workload": f"RANS-SA bump box {a.n}^3 hexes ({levels[0].n_nodes} nodes, "
                                f"{sides[0]} sides), first cell {a.first_cell:g}, {a.levels} MG levels, growing-V",
                    "evaluations_per_level": head["evals"],
                    "lanes_per_level": head["lanes"], "autotuned": a.autotune,
-                   "fp64_gedges": round(results["fp64"]["gedges"], 4),
-                   "fp64_ms_per_step": round(results["fp64"]["ms_per_cycle"], 3),
+                   "mixed_gedges": round(results["mixed"]["gedges"], 4) if "mixed" in results else None,
+                   "mixed_ms_per_step": round(results["mixed"]["ms_per_cycle"], 3) if "mixed" in results else None,
+                   "mixed_note": "FP32 coarse levels, FP64 fine level; usable above a 1e-4 drop, then "
+                                 "flite_set_coarse_precision switches the coarse levels to FP64",
                    "residual_drop_log10": round(float(np.log10(head["res"] / head["res0"])), 3),
                    "time_to_1e-8_drop": None,
                    "time_to_1e-8_drop_note": "not reached on this grading: from freestream both modes stall "

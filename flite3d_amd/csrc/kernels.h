@@ -88,16 +88,19 @@ struct LevelDev {
   const R *ellw;       // signed side weights per ELL slot, SoA: [c * nslot + slot]
   long long nslot;
   const int *bs_ptr, *bs_list;
-  // side blocks of k_flux_blk: block k serves sides fb_start[k] .. fb_start[k+1]-1
-  // and stages nodes fb_node[fb_nptr[k] ..]; fb_loc[e] = both ends' positions
-  // in that list (first end in the low 16 bits)
+  // side blocks of k_flux_blk: fb_head[4 k ..] = {first side e0, first
+  // entry n0, first incidence ibase, 0} of block k (nfb + 1 headers, the
+  // last one closing the lists); block k serves sides e0 .. next e0 - 1 and
+  // stages the nodes fb_node[FB_NODES k ..] (n1 - n0 of them, the rest -1);
+  // fb_loc[e] = both ends' positions in that list (first end in the low 16
+  // bits)
   int nfb;
-  const int *fb_start, *fb_nptr, *fb_node;
+  const int *fb_head, *fb_node;
   const unsigned *fb_loc;
-  // per-node partial sums of the side fluxes (k_flux_blk): entry q =
-  // fb_nptr position; fb_inc[fb_iptr[q] ..] = the entry's sides within its
-  // block (local side << 1 | 1 when the node is the side's second end);
-  // node i's entries are pn_ent[pn_ptr[i] ..], in block order
+  // per-node partial sums of the side fluxes (k_flux_blk): entry q = n0 +
+  // the node's position in its block; fb_inc[fb_iptr[q] ..] = the entry's
+  // sides within its block (local side << 1 | 1 when the node is the side's
+  // second end); node i's entries are pn_ent[pn_ptr[i] ..], in block order
   long long nent;
   const int *fb_iptr, *pn_ptr, *pn_ent;
   const unsigned short *fb_inc;
@@ -330,20 +333,20 @@ __device__ __forceinline__ void for_edges_w(const LevelDev<R> &L, int i, int sub
 template <typename R> struct Pair;
 template <> struct Pair<double> { using type = double2; };
 template <> struct Pair<float> { using type = float2; };
-// one staging pass: NW words of each of the nn (<= CAP) listed nodes'
-// records src + STRIDE*g, in pairs when the record allows it, into LDS
-// words OFF.. of the node's S-word slot
-template <int CAP, int NW, int STRIDE, int OFF, int S, typename R>
+// one staging pass: the first NS (default NW) of the NW words at src +
+// STRIDE*g of each of the nn (<= CAP) listed nodes g, loaded in pairs when
+// the record allows it, into LDS words OFF.. of the node's S-word slot
+template <int CAP, int NW, int STRIDE, int OFF, int S, typename R, int NS = NW, int THR = NTHR>
 __device__ __forceinline__ void fb_stage(R *sn, const R *src, const int *nodes, int nn) {
   constexpr bool PAIRS = NW % 2 == 0 && STRIDE % 2 == 0;
   constexpr int U = PAIRS ? NW / 2 : NW;  // load units per node
-  constexpr int PER = (CAP * U + NTHR - 1) / NTHR;
+  constexpr int PER = (CAP * U + THR - 1) / THR;
   using P2 = typename Pair<R>::type;
   P2 pv[PAIRS ? PER : 1];
   R sv[PAIRS ? 1 : PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    int t = threadIdx.x + q * NTHR, k = t / U, j = t - k * U;
+    int t = threadIdx.x + q * THR, k = t / U, j = t - k * U;
     if (k < nn) {
       size_t g = (size_t)nodes[k];
       if constexpr (PAIRS) pv[q] = reinterpret_cast<const P2 *>(src + STRIDE * g)[j];
@@ -352,12 +355,12 @@ __device__ __forceinline__ void fb_stage(R *sn, const R *src, const int *nodes, 
   }
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    int t = threadIdx.x + q * NTHR, k = t / U, j = t - k * U;
+    int t = threadIdx.x + q * THR, k = t / U, j = t - k * U;
     if (k < nn) {
       if constexpr (PAIRS) {
         sn[k * S + OFF + 2 * j] = pv[q].x;
-        sn[k * S + OFF + 2 * j + 1] = pv[q].y;
-      } else {
+        if (NS == NW || 2 * j + 1 < NS) sn[k * S + OFF + 2 * j + 1] = pv[q].y;
+      } else if (NS == NW || j < NS) {
         sn[k * S + OFF + j] = sv[q];
       }
     }
@@ -466,16 +469,28 @@ __global__ void k_pack(LevelDev<R> L, DevParams P) {
 // ---------------------------------------------------------------------------
 // computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459), gathered
 // ---------------------------------------------------------------------------
-// the record of node i into out[26]; false on the lanes that do not own it
-template <typename R, int SUB, typename Walk>
-__device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, int i, int sub, Walk walk, R *out) {
-  const R *qi = L.rec0 + 8 * (size_t)i;
-  R q0[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
-  R g[15];
-  R mn[5], mx[5];
-  for (int c = 0; c < 15; ++c) g[c] = 0;
-  for (int c = 0; c < 5; ++c) { mn[c] = R(1e15); mx[c] = R(1e15); }
-  walk([&](const R *qj, const R *, R wx, R wy, R wz) {
+// computeGradient (DEF:2923-3245) + LocalMinMax (DEF:3248-3459) in the form
+// of a walk part (init / edge / bside / reduce / finish, like the gather
+// parts below): Green-Gauss sums over the interior sides (q_i + q_j) w and
+// the boundary sides (3 q_i + q_o) w, and the minimum and maximum of the
+// neighbours' values.  Carried along the viscous pass 1's walk it made that
+// walk slower than a walk of its own (register pressure), so the gradient
+// kernels walk it alone.
+template <typename R>
+struct GradPart {
+  static constexpr bool AUX = false;
+  R q0[5], g[15], mn[5], mx[5];
+  __device__ void init(const LevelDev<R> &, const DevParams &, int, const R *qi) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      q0[c] = qi[c];
+      mn[c] = R(1e15);
+      mx[c] = R(1e15);
+    }
+#pragma unroll
+    for (int c = 0; c < 15; ++c) g[c] = 0;
+  }
+  __device__ void edge(const LevelDev<R> &, const DevParams &, const R *, const R *, const R *qj, R wx, R wy, R wz) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       R s = q0[c] + qj[c];
@@ -487,12 +502,9 @@ __device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, int i, int sub,
       mn[c] = rmin(mn[c], lo);
       mx[c] = rmin(mx[c], hi);
     }
-  });
-  for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
-    int v = L.bs_list[k], s = v >> 1;
-    int o = (v & 1) ? L.ba[s] : L.bb[s];
-    const R *qo = L.rec0 + 8 * (size_t)o;
-    R wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+  }
+  __device__ void bside(const LevelDev<R> &, const DevParams &, const R *, int, int, int, const R *qo, R wx, R wy,
+                        R wz) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       R t = R(2) * q0[c] + (q0[c] + qo[c]);
@@ -501,7 +513,8 @@ __device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, int i, int sub,
       g[10 + c] += t * wz;
     }
   }
-  if (SUB > 1) {
+  template <int SUB>
+  __device__ void reduce() {
 #pragma unroll
     for (int c = 0; c < 15; ++c) g[c] = lsum<SUB>(g[c]);
 #pragma unroll
@@ -509,26 +522,55 @@ __device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, int i, int sub,
       mn[c] = lmin<SUB>(mn[c]);
       mx[c] = lmin<SUB>(mx[c]);
     }
+  }
+  // the node's record: gradient 15, reciprocal bounds 5 + 5, pad
+  __device__ void values(const LevelDev<R> &L, int i, R *out) const {
+    R vt = R(1) / L.vol[i];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) out[c] = g[c] * vt;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      R lo = mn[c] - q0[c];
+      R hi = -mx[c] - q0[c];
+      out[15 + c] = lo < R(0) ? R(1) / lo : R(0);
+      out[20 + c] = hi > R(0) ? R(1) / hi : R(0);
+    }
+    out[25] = 0;
+  }
+  __device__ void finish(const LevelDev<R> &L, const DevParams &, int i, const R *) {
+    R v[GREC];
+    values(L, i, v);
+    R *out = L.grec + GREC * (size_t)i;
+#pragma unroll
+    for (int c = 0; c < GREC; ++c) out[c] = v[c];
+  }
+};
+
+// the record of node i into out[26]; false on the lanes that do not own it
+template <typename R, int SUB, typename Walk>
+__device__ __forceinline__ bool grad_minmax_vals(LevelDev<R> &L, const DevParams &P, int i, int sub, Walk walk,
+                                                 R *out) {
+  const R *qi = L.rec0 + 8 * (size_t)i;
+  GradPart<R> gp;
+  gp.init(L, P, i, qi);
+  walk([&](const R *qj, const R *tj, R wx, R wy, R wz) { gp.edge(L, P, qi, tj, qj, wx, wy, wz); });
+  for (int k = L.bs_ptr[i] + sub; k < L.bs_ptr[i + 1]; k += SUB) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    gp.bside(L, P, qi, s, o, 0, L.rec0 + 8 * (size_t)o, L.wb[3 * s], L.wb[3 * s + 1], L.wb[3 * s + 2]);
+  }
+  if (SUB > 1) {
+    gp.template reduce<SUB>();
     if (sub) return false;
   }
-  R vt = R(1) / L.vol[i];
-#pragma unroll
-  for (int c = 0; c < 15; ++c) out[c] = g[c] * vt;
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    R lo = mn[c] - q0[c];
-    R hi = -mx[c] - q0[c];
-    out[15 + c] = lo < R(0) ? R(1) / lo : R(0);
-    out[20 + c] = hi > R(0) ? R(1) / hi : R(0);
-  }
-  out[25] = 0;
+  gp.values(L, i, out);
   return true;
 }
 
 template <typename R, int SUB, typename Walk>
-__device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, int i, int sub, Walk walk) {
+__device__ __forceinline__ void grad_minmax_body(LevelDev<R> &L, const DevParams &P, int i, int sub, Walk walk) {
   R v[GREC];
-  if (!grad_minmax_vals<R, SUB>(L, i, sub, walk, v)) return;
+  if (!grad_minmax_vals<R, SUB>(L, P, i, sub, walk, v)) return;
   R *out = L.grec + GREC * (size_t)i;
 #pragma unroll
   for (int c = 0; c < GREC; ++c) out[c] = v[c];
@@ -548,7 +590,7 @@ __global__ void __launch_bounds__(NTHR) k_grad_minmax_st(LevelDev<R> L, DevParam
   P.to_batch(bb_);
   const int i0 = i - (int)threadIdx.x;
   R v[GREC];
-  bool ok = i < L.n && grad_minmax_vals<R, 1>(L, i, 0, [&](auto f) { walk_global<1>(L, i, 0, f); }, v);
+  bool ok = i < L.n && grad_minmax_vals<R, 1>(L, P, i, 0, [&](auto f) { walk_global<1>(L, i, 0, f); }, v);
   const int cnt = L.n - i0 < NTHR ? L.n - i0 : NTHR;
   R *base = L.grec + GREC * (size_t)i0;
   constexpr int HW = GREC / 2;
@@ -577,7 +619,7 @@ __global__ void k_grad_minmax(LevelDev<R> L, DevParams P) {
   L.to_batch(bb_);
   P.to_batch(bb_);
   if (i >= L.n) return;
-  grad_minmax_body<R, SUB>(L, i, sub, [&](auto f) { walk_global<SUB>(L, i, sub, f); });
+  grad_minmax_body<R, SUB>(L, P, i, sub, [&](auto f) { walk_global<SUB>(L, i, sub, f); });
 }
 
 // Reciprocal and reciprocal square root in the FP64 flux: the hardware
@@ -620,7 +662,7 @@ __device__ __forceinline__ R led(R a, R ilo, R ihi) {
   R iv = a1 > R(0) ? ihi : ilo;
   R t = a1 * iv, t2 = t * t;
   R iden = NSIG == 4 ? frsqrt(sqrt(R(1) + t2 * t2)) : frsqrt(R(1) + t2);
-  R r = R(0.5) * a1 * iden;
+  R r = a * iden;  // the reference's 0.5 * a1 * iden: 0.5 * (2 a) is a exactly
   r = iv == R(0) ? R(0) : r;
   return (a1 > R(0) || a1 < R(0)) ? r : a;  // a == 0 (or NaN) passes through
 }
@@ -661,13 +703,6 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   if (SA) fsa = sa_side_convection(A, B, wx, wy, wz);
   T ru1 = roe1 * u1, rv1 = roe1 * v1, rw1 = roe1 * w1, rh1 = re1 + pr1;
   T ru2 = roe2 * u2, rv2 = roe2 * v2, rw2 = roe2 * w2, rh2 = re2 + pr2;
-  // central part: only kept when no HLLC branch is taken (NaN wave speeds)
-  T fc[5];
-  fc[0] = wx * (ru1 + ru2) + wy * (rv1 + rv2) + wz * (rw1 + rw2);
-  fc[1] = wx * (ru1 * u1 + pr1 + ru2 * u2 + pr2) + wy * (ru1 * v1 + ru2 * v2) + wz * (ru1 * w1 + ru2 * w2);
-  fc[2] = wx * (rv1 * u1 + rv2 * u2) + wy * (rv1 * v1 + pr1 + rv2 * v2 + pr2) + wz * (rv1 * w1 + rv2 * w2);
-  fc[3] = wx * (rw1 * u1 + rw2 * u2) + wy * (rw1 * v1 + rw2 * v2) + wz * (rw1 * w1 + pr1 + rw2 * w2 + pr2);
-  fc[4] = wx * (rh1 * u1 + rh2 * u2) + wy * (rh1 * v1 + rh2 * v2) + wz * (rh1 * w1 + rh2 * w2);
   T nx = T(2) * wx, ny = T(2) * wy, nz = T(2) * wz;
   T n2 = nx * nx + ny * ny + nz * nz;
   T inv = n2 > T(0) ? frsqrt(n2) : T(0);
@@ -734,8 +769,15 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   bool lft = sl > T(0) || (sl <= T(0) && sm > T(0));
   bool rgt = (sm <= T(0) && sr >= T(0)) || sr < T(0);
   if (!lft && !rgt) {
-#pragma unroll
-    for (int c = 0; c < 5; ++c) o[c] = fc[c];
+    // the central part of the nodes' states (DEF:1798-1802), kept when no
+    // HLLC branch is taken (NaN wave speeds)
+    T a1 = A[0] * A[1], a2 = A[0] * A[2], a3 = A[0] * A[3], ah = A[4] + A[5];
+    T b1 = B[0] * B[1], b2 = B[0] * B[2], b3 = B[0] * B[3], bh = B[4] + B[5];
+    o[0] = wx * (a1 + b1) + wy * (a2 + b2) + wz * (a3 + b3);
+    o[1] = wx * (a1 * A[1] + A[5] + b1 * B[1] + B[5]) + wy * (a1 * A[2] + b1 * B[2]) + wz * (a1 * A[3] + b1 * B[3]);
+    o[2] = wx * (a2 * A[1] + b2 * B[1]) + wy * (a2 * A[2] + A[5] + b2 * B[2] + B[5]) + wz * (a2 * A[3] + b2 * B[3]);
+    o[3] = wx * (a3 * A[1] + b3 * B[1]) + wy * (a3 * A[2] + b3 * B[2]) + wz * (a3 * A[3] + A[5] + b3 * B[3] + B[5]);
+    o[4] = wx * (ah * A[1] + bh * B[1]) + wy * (ah * A[2] + bh * B[2]) + wz * (ah * A[3] + bh * B[3]);
     if (wg != R(0)) {  // moving mesh: the left state's GCL share (DEF:2081-2085)
       o[0] -= A[0] * wg;
       o[1] -= ru1 * wg;
@@ -825,23 +867,36 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
 // The same flux with the two ends' records staged in LDS per block of
 // sides, and the sides' fluxes summed per node inside the block.  A block
 // serves up to FB_SIDES consecutive sides that touch at most FB_NODES
-// distinct nodes (tables built at upload).  Its threads first copy those
-// nodes' records into LDS with loads in which consecutive lanes read
-// consecutive words of one record, so a wavefront touches a few cache lines
-// per load instead of one line per lane; each thread then evaluates its side
-// from LDS.  The block then sums, for each of its nodes, +f over the sides
-// the node starts and -f over the sides it ends, in side order, and writes
-// that partial sum to pbuf (one 6-word entry per block and node, in the
-// order of the block's node list).  A node's residual is the sum of its
-// entries in block order (k_stage), which replaces the gather of one flux
-// record per incident side by a few partial sums per node.
-constexpr int FB_SIDES = NTHR;
+// distinct nodes (tables built at upload).  Its threads first copy the words
+// of those nodes' records that the flux reads into LDS, with loads in which
+// consecutive lanes read consecutive words of one record, so a wavefront
+// touches a few cache lines per load instead of one line per lane; each
+// thread then evaluates its side from LDS.  The block then sums, for each of
+// its nodes, +f over the sides the node starts and -f over the sides it
+// ends (two lanes per node, each over alternate sides of the node's list in
+// side order, then the two sums added), and writes that partial sum to pbuf
+// (one 6-word entry per block and node, in the order of the block's node
+// list).  A node's
+// residual is the sum of its entries in block order (k_stage), which
+// replaces the gather of one flux record per incident side by a few partial
+// sums per node.
+// Every load of a block depends on its header or on nothing: the node list
+// sits at a fixed place per block, so the records are two loads away from
+// the start, and the sides' own words are requested before the staging.
+constexpr int FB_THR = 256;  // threads of a side block, one side each
+constexpr int FB_SIDES = FB_THR;
 constexpr int FB_NODES = 128;
+static_assert(2 * FB_NODES <= FB_THR, "two threads per staged node in the node sums");
+constexpr int FB_REC = 7;  // gather-record words the flux reads (not mu)
+constexpr int FB_SF = 7;   // LDS stride of a side flux (6 words, odd stride)
+template <int ORDER>
+constexpr int fb_stride() {  // LDS words per node: record 7 | grad + bounds 25 | coordinates 3
+  return ORDER > 1 ? FB_REC + GREC - 1 + 3 : FB_REC;  // odd: lanes reading different nodes spread over banks
+}
 template <typename R, int ORDER, bool SA, int NSIG>
-__global__ __launch_bounds__(NTHR) void k_flux_blk(LevelDev<R> L, DevParams P) {
-  // LDS words per node: gather record 8 | grad + bounds + pad 26 | coordinates 3
-  constexpr int S = ORDER > 1 ? 8 + GREC + 3 : 9;  // odd: lanes reading different nodes spread over banks
-  constexpr int SN = FB_NODES * S > FB_SIDES * 6 ? FB_NODES * S : FB_SIDES * 6;
+__global__ __launch_bounds__(FB_THR) void k_flux_blk(LevelDev<R> L, DevParams P) {
+  constexpr int S = fb_stride<ORDER>();
+  constexpr int SN = FB_NODES * S > FB_SIDES * FB_SF ? FB_NODES * S : FB_SIDES * FB_SF;
   __shared__ R sn[SN];
   __shared__ unsigned short sinc[2 * FB_SIDES];
   int lb = xcd_block();
@@ -849,65 +904,72 @@ __global__ __launch_bounds__(NTHR) void k_flux_blk(LevelDev<R> L, DevParams P) {
   L.to_batch(bb_);
   P.to_batch(bb_);
   if (blk >= L.nfb) return;
-  const int n0 = L.fb_nptr[blk], nn = L.fb_nptr[blk + 1] - n0;
-  const int *nodes = L.fb_node + n0;
-  // this thread's node (if any) and the block's incidence lists
-  const int ibase = L.fb_iptr[n0];
+  const int4 h0 = reinterpret_cast<const int4 *>(L.fb_head)[blk];
+  const int4 h1 = reinterpret_cast<const int4 *>(L.fb_head)[blk + 1];
+  const int n0 = h0.y, nn = h1.y - h0.y, ibase = h0.z;
+  const int *nodes = L.fb_node + (size_t)blk * FB_NODES;
+  // this thread's side: its words are requested before the staging
+  const int e = h0.x + (int)threadIdx.x;
+  const bool active = e < h1.x;
+  unsigned loc = 0;
+  R wx = 0, wy = 0, wz = 0, wg = 0;
+  if (active) {
+    loc = L.fb_loc[e];
+    wx = L.w[3 * (size_t)e];
+    wy = L.w[3 * (size_t)e + 1];
+    wz = L.w[3 * (size_t)e + 2];
+    if (L.gcl) wg = L.gcl[e] / R(P.dtp);
+  }
+  // this thread's node (two threads per node, `half` of its incidences
+  // each) and the block's incidence lists
   int q0 = 0, q1 = 0;
-  if ((int)threadIdx.x < nn) {
-    q0 = L.fb_iptr[n0 + threadIdx.x] - ibase;
-    q1 = L.fb_iptr[n0 + threadIdx.x + 1] - ibase;
+  const int me = (int)threadIdx.x >> 1, half = (int)threadIdx.x & 1;
+  if (me < nn) {
+    q0 = L.fb_iptr[n0 + me] - ibase;
+    q1 = L.fb_iptr[n0 + me + 1] - ibase;
   }
   {
-    const int ni = L.fb_iptr[n0 + nn] - ibase;  // 2 x sides of the block
+    const int ni = h1.z - ibase;  // 2 x sides of the block
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      int t = threadIdx.x + q * NTHR;
+      int t = threadIdx.x + q * FB_THR;
       if (t < ni) sinc[t] = L.fb_inc[ibase + t];
     }
   }
-  fb_stage<FB_NODES, 8, 8, 0, S>(sn, L.rec0, nodes, nn);
+  fb_stage<FB_NODES, 8, 8, 0, S, R, FB_REC, FB_THR>(sn, L.rec0, nodes, nn);
   if constexpr (ORDER > 1) {
-    fb_stage<FB_NODES, GREC, GREC, 8, S>(sn, L.grec, nodes, nn);
-    fb_stage<FB_NODES, 3, 3, 8 + GREC, S>(sn, L.xyz, nodes, nn);
+    fb_stage<FB_NODES, GREC, GREC, FB_REC, S, R, GREC - 1, FB_THR>(sn, L.grec, nodes, nn);
+    fb_stage<FB_NODES, 3, 3, FB_REC + GREC - 1, S, R, 3, FB_THR>(sn, L.xyz, nodes, nn);
   }
   __syncthreads();
-  const int e = L.fb_start[blk] + (int)threadIdx.x;
-  const bool active = e < L.fb_start[blk + 1];
   R fo[6];
   if (active) {
-    unsigned loc = L.fb_loc[e];
     const R *na = sn + (loc & 0xffffu) * S, *nb = sn + (loc >> 16) * S;
-    R A[8], B[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      A[c] = na[c];
-      B[c] = nb[c];
-    }
-    R wx = L.w[3 * e], wy = L.w[3 * e + 1], wz = L.w[3 * e + 2];
     const R *G1 = nullptr, *G2 = nullptr, *X1 = nullptr, *X2 = nullptr;
     if constexpr (ORDER > 1) {
-      G1 = na + 8;
-      G2 = nb + 8;
-      X1 = na + 8 + GREC;
-      X2 = nb + 8 + GREC;
+      G1 = na + FB_REC;
+      G2 = nb + FB_REC;
+      X1 = na + FB_REC + GREC - 1;
+      X2 = nb + FB_REC + GREC - 1;
     }
-    R wg = L.gcl ? L.gcl[e] / R(P.dtp) : R(0);
-    side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, A, B, wx, wy, wz, fo, wg);
+    side_flux<R, R, ORDER, SA, NSIG>(P, G1, G2, X1, X2, na, nb, wx, wy, wz, fo, wg);
   }
   __syncthreads();  // node records no longer needed: the side fluxes take their place
   R *sf = sn;
   if (active) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sf[6 * threadIdx.x + c] = fo[c];
+    for (int c = 0; c < 6; ++c) sf[FB_SF * threadIdx.x + c] = fo[c];
   }
   __syncthreads();
-  if ((int)threadIdx.x < nn) {
-    const int ent = n0 + (int)threadIdx.x;
+  // node sums: the pair of lanes serving a node sums the even and the odd
+  // positions of its incidence list, then adds the two sums (the same bits
+  // on both lanes), and each lane stores three of the six words
+  if (me < nn) {
+    const int ent = n0 + me;
     R r[6] = {0, 0, 0, 0, 0, 0};
-    for (int q = q0; q < q1; ++q) {
+    for (int q = q0 + half; q < q1; q += 2) {
       unsigned v = sinc[q];
-      const R *f = sf + 6 * (v >> 1);
+      const R *f = sf + FB_SF * (v >> 1);
       if (v & 1u) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) r[c] -= f[c];
@@ -916,9 +978,12 @@ __global__ __launch_bounds__(NTHR) void k_flux_blk(LevelDev<R> L, DevParams P) {
         for (int c = 0; c < 6; ++c) r[c] += f[c];
       }
     }
-    R *o = L.pbuf + 6 * (size_t)ent;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) o[c] = r[c];
+    for (int c = 0; c < 6; ++c) r[c] += __shfl_xor(r[c], 1);
+    R *o = L.pbuf + 6 * (size_t)ent + 3 * half;
+    o[0] = half ? r[3] : r[0];
+    o[1] = half ? r[4] : r[1];
+    o[2] = half ? r[5] : r[2];
   }
 }
 

@@ -495,9 +495,17 @@ void metric_vjp(const fp_mesh &m, const double *gw, const double *gwb, double *g
 // directionalityParameter = 0 (gen3d_generator.f90:126): seeds are taken in
 // order of decreasing available connectivity, boundary nodes first; a seed
 // absorbs every neighbour not yet agglomerated; each absorbed node lowers
-// the available connectivity of its free neighbours.  Gen3d keeps this
-// priority as a shifted list; here it is a bucket queue (FIFO within a
-// connectivity), which keeps the rule but not Gen3d's tie order.
+// the available connectivity of its free neighbours.
+// The initial order is Gen3d's (gen3d_grid.f90:2407-2436).  The updates are
+// not: Gen3d keeps the priority as a list in which a node whose available
+// connectivity drops to c moves down only to just before the first later
+// entry with connectivity <= c (the side's first node) or < c (its second
+// node), and entries of agglomerated nodes count as connectivity 0, so they
+// stop the move; with no such entry the node stays where it is
+// (gen3d_grid.f90:2504-2570).  Here a decremented node goes to the back of
+// a FIFO bucket of connectivity c, so a seed always has the largest
+// available connectivity left, and seeds of equal connectivity can come in
+// another order than Gen3d's.
 Level agglomerate(const Level &F) {
   const int n = F.n_nodes, nbf = F.n_bnodes;
   const long long ne = (long long)F.e1.size();

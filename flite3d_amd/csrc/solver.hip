@@ -168,7 +168,7 @@ struct Level {
   long long nslot = 0;
   // side blocks of the LDS-staged flux kernel
   int nfb = 0;
-  int *fb_start = nullptr, *fb_nptr = nullptr, *fb_node = nullptr;
+  int *fb_head = nullptr, *fb_node = nullptr;
   unsigned *fb_loc = nullptr;
   long long nent = 0;  // per-node partial-sum entries of the side blocks
   int *fb_iptr = nullptr, *pn_ptr = nullptr, *pn_ent = nullptr;
@@ -200,7 +200,7 @@ struct Level {
     v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
     v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell_e = ell_e; v.ell_nbr = ell_nbr; v.ellw = (const R *)ellw; v.nslot = nslot;
     v.bs_ptr = bs_ptr; v.bs_list = bs_list;
-    v.nfb = nfb; v.fb_start = fb_start; v.fb_nptr = fb_nptr; v.fb_node = fb_node; v.fb_loc = fb_loc;
+    v.nfb = nfb; v.fb_head = fb_head; v.fb_node = fb_node; v.fb_loc = fb_loc;
     v.nent = nent; v.fb_iptr = fb_iptr; v.fb_inc = fb_inc; v.pn_ptr = pn_ptr; v.pn_ent = pn_ent; v.pbuf = (R *)pbuf;
     v.ngb = ngb; v.gb_first = gb_first; v.gb_nptr = gb_nptr; v.gb_node = gb_node; v.ell_loc = ell_loc;
     v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
@@ -683,7 +683,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   // FB_SIDES of them touching at most FB_NODES distinct nodes; each block's
   // node list is ascending (neighbouring records share cache lines)
   {
-    std::vector<int> fb_start{0}, fb_nptr{0}, fb_node, fb_iptr{0};
+    std::vector<int> fb_head{0, 0, 0, 0}, fb_node, fb_nodep, fb_iptr{0};
     std::vector<unsigned> fb_loc(ne);
     std::vector<unsigned short> fb_inc;
     std::vector<int> stamp(n, -1), local(n, 0), cur;
@@ -704,8 +704,9 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
         fb_iptr.push_back((int)fb_inc.size());
       }
       fb_node.insert(fb_node.end(), cur.begin(), cur.end());
-      fb_nptr.push_back((int)fb_node.size());
-      fb_start.push_back(end);
+      fb_nodep.insert(fb_nodep.end(), cur.begin(), cur.end());
+      fb_nodep.resize((size_t)(blk + 1) * FB_NODES, -1);
+      for (int v : {end, (int)fb_node.size(), fb_iptr.back(), 0}) fb_head.push_back(v);
       cur.clear();
       start = end;
       ++blk;
@@ -727,9 +728,8 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
       std::vector<int> pos(pn_ptr.begin(), pn_ptr.end() - 1);
       for (size_t q = 0; q < fb_node.size(); ++q) pn_ent[pos[fb_node[q]]++] = (int)q;
     }
-    TRY(L.pool.upload(&L.fb_start, fb_start));
-    TRY(L.pool.upload(&L.fb_nptr, fb_nptr));
-    TRY(L.pool.upload(&L.fb_node, fb_node));
+    TRY(L.pool.upload(&L.fb_head, fb_head));
+    TRY(L.pool.upload(&L.fb_node, fb_nodep));
     TRY(L.pool.upload(&L.fb_loc, fb_loc));
     TRY(L.pool.upload(&L.fb_iptr, fb_iptr));
     TRY(L.pool.upload(&L.fb_inc, fb_inc));
@@ -1096,17 +1096,17 @@ int op_flux(flite_ctx *c, Level &L) {
       bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
       L.flux_partials = !c->flux_per_side;
       if (L.flux_partials) {
-        dim3 gb((unsigned)L.nfb * L.nbatch);
+        dim3 gb((unsigned)L.nfb * L.nbatch), bb(FB_THR);
         if (o2 && P.sa) {
-          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, true, 4>), gb, b, 0, c->stream, V, P);
-          else hipLaunchKernelGGL((k_flux_blk<R, 2, true, 2>), gb, b, 0, c->stream, V, P);
+          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, true, 4>), gb, bb, 0, c->stream, V, P);
+          else hipLaunchKernelGGL((k_flux_blk<R, 2, true, 2>), gb, bb, 0, c->stream, V, P);
         } else if (o2) {
-          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, false, 4>), gb, b, 0, c->stream, V, P);
-          else hipLaunchKernelGGL((k_flux_blk<R, 2, false, 2>), gb, b, 0, c->stream, V, P);
+          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, false, 4>), gb, bb, 0, c->stream, V, P);
+          else hipLaunchKernelGGL((k_flux_blk<R, 2, false, 2>), gb, bb, 0, c->stream, V, P);
         } else if (P.sa) {
-          hipLaunchKernelGGL((k_flux_blk<R, 1, true, 2>), gb, b, 0, c->stream, V, P);
+          hipLaunchKernelGGL((k_flux_blk<R, 1, true, 2>), gb, bb, 0, c->stream, V, P);
         } else {
-          hipLaunchKernelGGL((k_flux_blk<R, 1, false, 2>), gb, b, 0, c->stream, V, P);
+          hipLaunchKernelGGL((k_flux_blk<R, 1, false, 2>), gb, bb, 0, c->stream, V, P);
         }
       } else if (o2 && P.sa) {
         if (n4) hipLaunchKernelGGL((k_flux<R, 2, true, 4>), g, b, 0, c->stream, V, P);
@@ -1246,13 +1246,13 @@ int op_step_terms(flite_ctx *c, Level &L, bool timesteps, bool sa_timesteps) {
   return with_level(L, [&](auto V) {
     using R = std::remove_pointer_t<decltype(V.u)>;
     if (timesteps && sa) {
-    launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>(c, L, V, P);
+      launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, SADiff1Part<R>>(c, L, V, P);
     } else if (timesteps) {
-    launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>(c, L, V, P);
+      launch_gather<TimeStepPart<R>, ViscNodePart<R, false>, NoPart<R>>(c, L, V, P);
     } else if (sa) {
-    launch_gather<ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>(c, L, V, P);
+      launch_gather<ViscNodePart<R, true>, SADiff1Part<R>, NoPart<R>>(c, L, V, P);
     } else {
-    launch_gather<ViscNodePart<R, true>, NoPart<R>, NoPart<R>>(c, L, V, P);
+      launch_gather<ViscNodePart<R, true>, NoPart<R>, NoPart<R>>(c, L, V, P);
     }
     if (timesteps)
       hipLaunchKernelGGL(k_mu_record<R>, dim3(nblk(L.n) * L.nbatch), dim3(NTHR), 0, c->stream, V);
