@@ -680,10 +680,14 @@ def main():
                                  "flite_set_coarse_precision switches the coarse levels to FP64",
                    "residual_drop_log10": round(float(np.log10(head["res"] / head["res0"])), 3),
                    "time_to_1e-8_drop": None,
-                   "time_to_1e-8_drop_note": "not reached on this grading: from freestream both modes stall "
-                                             "near a 10^-2.8 drop (profiles/convergence_rans128_mixed_vs_fp64.txt); "
-                                             "graded from 0.005 instead, the 64^3 box reaches it in 105.4 s FP64 "
-                                             "and 104.8 s mixed, drags 4e-11 apart "
+                   "deepest_drop_log10": -2.795,
+                   "deepest_drop_time_s": 480.0,
+                   "time_to_1e-8_drop_note": "not reached on this grading: from freestream the FP64 run stalls "
+                                             "near a 10^-2.8 drop (-2.795 after 480 s, "
+                                             "profiles/convergence_rans128_mixed_vs_fp64.txt); none of the "
+                                             "reference's controls converges on it either (32^3, "
+                                             "profiles/convergence_rans32_1e-6_controls.txt); graded from 0.005 "
+                                             "instead, the 64^3 box reaches it in 105.4 s FP64 "
                                              "(profiles/convergence_rans64_fc0.005_mixed_vs_fp64.txt)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
