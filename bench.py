@@ -71,7 +71,9 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autot
     s = device.Solver(prm(fp32) if prm else params(fp32), levels)
     s.init_freestream()
     lanes = s.autotune(3) if autotune else [s.lanes(k + 1) for k in range(len(levels))]
-    log(f"[bench] lanes per node per level: {lanes}{' (autotuned)' if autotune else ''}")
+    blocks = [s.side_block(k + 1)[0] for k in range(len(levels))]
+    log(f"[bench] lanes per node per level: {lanes}, sides per flux block: {blocks}"
+        f"{' (autotuned)' if autotune else ''}")
     c = 1
     r0 = None
     for _ in range(warmup):
@@ -80,7 +82,7 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autot
         r0 = r0 or m["res"][0]
         c += 1
     evals = device.cycle_evaluations(s)
-    device.timing_enable(s, timed_kernel)
+    # the timed region replays the captured cycle graph, as a run does
     s.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -90,12 +92,21 @@ def run_gpu(levels, fp32, steps, warmup, timed_kernel="flux_l1", prm=None, autot
         c += 1
     s.sync()
     t1 = time.perf_counter()
+    # the timed kernel's launch durations (HIP events on the library's
+    # stream) in cycles right after: kernel timing launches a cycle kernel by
+    # kernel, because a graph replay carries no per-kernel events here
+    device.timing_enable(s, timed_kernel)
+    for _ in range(min(steps, 3)):
+        s.cycle(c)
+        m = s.monitor()
+        c += 1
+    s.sync()
     kms, kcnt = device.timing_get(s)
     device.timing_enable(s, None)
     res = m["res"]
     s.close()
     return dict(time=t1 - t0, evals=evals, kernel_ms=kms, kernel_launches=kcnt,
-                res0=r0, res=res[0], forces=m["forces"][:4].tolist(), lanes=lanes)
+                res0=r0, res=res[0], forces=m["forces"][:4].tolist(), lanes=lanes, side_blocks=blocks)
 
 
 def run_converge(levels, fp32, drop, cap_s, every=250, switch_drop=4.0, prm=None, label=None):
@@ -509,9 +520,15 @@ def bench_config4(nbatch, steps, warmup):
                          restartnumber=0).to_struct(coarse_fp32=True) for m, a in zip(mach, alpha)]
     E = [l.n_sides for l in lv]
 
+    shapes = {}
+
     def timed(plist):
         s = device.Solver(plist, lv)
         s.init_freestream()
+        # launch shapes (lanes per node, side-block shape) picked by timing
+        # on this context's state
+        lanes = s.autotune(3)
+        shapes[len(plist)] = {"lanes": lanes, "sides_per_flux_block": [s.side_block(k + 1)[0] for k in range(len(lv))]}
         for c in range(1, warmup + 1):
             s.cycle(c)
         evals = device.cycle_evaluations(s)
@@ -537,7 +554,8 @@ def bench_config4(nbatch, steps, warmup):
             "batched_gedges": round(ge / tb / 1e9, 4),
             "one_at_a_time_ms_per_cycle": round(1e3 * ts / steps, 3),
             "batching_speedup": round(ts / tb, 3), "single_runs_timed": nsingle,
-            "nodes": lv[0].n_nodes, "sides": E}
+            "nodes": lv[0].n_nodes, "sides": E, "autotuned_shapes": {"batched": shapes.get(nbatch),
+                                                                   "single": shapes.get(1)}}
 
 
 def cpu_baseline(levels, evals, prm=None):
@@ -575,8 +593,9 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick the lanes per node of each level by timing (flite_autotune) before the run")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the size rule for the lanes per node and 256-side flux blocks instead of "
+                         "picking both per level by timing (flite_autotune) before the timed cycles")
     ap.add_argument("--first-cell", type=float, default=1e-6,
                     help="first wall-normal spacing of the graded box (the benchmark case: 1e-6)")
     ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
@@ -673,7 +692,8 @@ def main():
         "config": {"workload": f"RANS-SA bump box {a.n}^3 hexes ({levels[0].n_nodes} nodes, "
                                f"{sides[0]} sides), first cell {a.first_cell:g}, {a.levels} MG levels, growing-V",
                    "evaluations_per_level": head["evals"],
-                   "lanes_per_level": head["lanes"], "autotuned": a.autotune,
+                   "lanes_per_level": head["lanes"], "sides_per_flux_block": head["side_blocks"],
+                   "autotuned": a.autotune,
                    "mixed_gedges": round(results["mixed"]["gedges"], 4) if "mixed" in results else None,
                    "mixed_ms_per_step": round(results["mixed"]["ms_per_cycle"], 3) if "mixed" in results else None,
                    "mixed_note": "FP32 coarse levels, FP64 fine level; usable above a 1e-4 drop, then "

@@ -88,10 +88,11 @@ struct LevelDev {
   const R *ellw;       // signed side weights per ELL slot, SoA: [c * nslot + slot]
   long long nslot;
   const int *bs_ptr, *bs_list;
-  // side blocks of k_flux_blk: fb_head[4 k ..] = {first side e0, first
-  // entry n0, first incidence ibase, 0} of block k (nfb + 1 headers, the
-  // last one closing the lists); block k serves sides e0 .. next e0 - 1 and
-  // stages the nodes fb_node[FB_NODES k ..] (n1 - n0 of them, the rest -1);
+  // side blocks of k_flux_blk (one shape's tables): fb_head[4 k ..] = {first
+  // side e0, first entry n0, first incidence ibase, 0} of block k (nfb + 1
+  // headers, the last one closing the lists); block k serves sides e0 ..
+  // next e0 - 1 and stages the nodes fb_node[FB_SHAPE_NODES[shape] k ..]
+  // (n1 - n0 of them, the rest -1);
   // fb_loc[e] = both ends' positions in that list (first end in the low 16
   // bits)
   int nfb;
@@ -883,18 +884,23 @@ __global__ void k_flux(LevelDev<R> L, DevParams P) {
 // Every load of a block depends on its header or on nothing: the node list
 // sits at a fixed place per block, so the records are two loads away from
 // the start, and the sides' own words are requested before the staging.
-constexpr int FB_THR = 256;  // threads of a side block, one side each
-constexpr int FB_SIDES = FB_THR;
-constexpr int FB_NODES = 128;
-static_assert(2 * FB_NODES <= FB_THR, "two threads per staged node in the node sums");
+// side-block shapes: FB_SHAPE_THR[k] sides per block (one thread each)
+// touching at most FB_SHAPE_NODES[k] distinct nodes; the autotuner picks
+// one per level (shape 0 unless it finds another faster)
+constexpr int FB_NSHAPES = 2;
+constexpr int FB_SHAPE_THR[FB_NSHAPES] = {256, 512};
+constexpr int FB_SHAPE_NODES[FB_NSHAPES] = {128, 224};
+static_assert(2 * FB_SHAPE_NODES[0] <= FB_SHAPE_THR[0] && 2 * FB_SHAPE_NODES[1] <= FB_SHAPE_THR[1],
+              "two threads per staged node in the node sums");
 constexpr int FB_REC = 7;  // gather-record words the flux reads (not mu)
 constexpr int FB_SF = 7;   // LDS stride of a side flux (6 words, odd stride)
 template <int ORDER>
 constexpr int fb_stride() {  // LDS words per node: record 7 | grad + bounds 25 | coordinates 3
   return ORDER > 1 ? FB_REC + GREC - 1 + 3 : FB_REC;  // odd: lanes reading different nodes spread over banks
 }
-template <typename R, int ORDER, bool SA, int NSIG>
-__global__ __launch_bounds__(FB_THR) void k_flux_blk(LevelDev<R> L, DevParams P) {
+template <typename R, int ORDER, bool SA, int NSIG, int SHAPE>
+__global__ __launch_bounds__(FB_SHAPE_THR[SHAPE]) void k_flux_blk(LevelDev<R> L, DevParams P) {
+  constexpr int FB_THR = FB_SHAPE_THR[SHAPE], FB_SIDES = FB_THR, FB_NODES = FB_SHAPE_NODES[SHAPE];
   constexpr int S = fb_stride<ORDER>();
   constexpr int SN = FB_NODES * S > FB_SIDES * FB_SF ? FB_NODES * S : FB_SIDES * FB_SF;
   __shared__ R sn[SN];

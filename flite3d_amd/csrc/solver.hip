@@ -166,14 +166,17 @@ struct Level {
   int *ell_e, *ell_nbr;
   void *ellw;
   long long nslot = 0;
-  // side blocks of the LDS-staged flux kernel
-  int nfb = 0;
-  int *fb_head = nullptr, *fb_node = nullptr;
-  unsigned *fb_loc = nullptr;
-  long long nent = 0;  // per-node partial-sum entries of the side blocks
-  int *fb_iptr = nullptr, *pn_ptr = nullptr, *pn_ent = nullptr;
-  unsigned short *fb_inc = nullptr;
-  void *pbuf = nullptr;
+  // side blocks of the LDS-staged flux kernel, one table set per shape
+  // (FB_SHAPE_THR / FB_SHAPE_NODES); fb_shape is the one in use
+  struct SideBlocks {
+    int nfb = 0;
+    long long nent = 0;  // per-node partial-sum entries
+    int *head = nullptr, *node = nullptr, *iptr = nullptr, *pn_ptr = nullptr, *pn_ent = nullptr;
+    unsigned *loc = nullptr;
+    unsigned short *inc = nullptr;
+  } sb[FB_NSHAPES];
+  int fb_shape = 0;
+  void *pbuf = nullptr;  // sized for the largest nent of the shapes
   bool flux_partials = false;  // the last interior fluxes went to pbuf (k_flux_blk), not fbuf
   // node blocks of the LDS-staged gathers (ngb = 0: the per-node kernels)
   int ngb = 0;
@@ -200,8 +203,10 @@ struct Level {
     v.ba = ba; v.bb = bb; v.bind = bind; v.wb = (const R *)wb;
     v.ell_ptr = ell_ptr; v.ell_k = ell_k; v.ell_e = ell_e; v.ell_nbr = ell_nbr; v.ellw = (const R *)ellw; v.nslot = nslot;
     v.bs_ptr = bs_ptr; v.bs_list = bs_list;
-    v.nfb = nfb; v.fb_head = fb_head; v.fb_node = fb_node; v.fb_loc = fb_loc;
-    v.nent = nent; v.fb_iptr = fb_iptr; v.fb_inc = fb_inc; v.pn_ptr = pn_ptr; v.pn_ent = pn_ent; v.pbuf = (R *)pbuf;
+    const SideBlocks &B = sb[fb_shape];
+    v.nfb = B.nfb; v.fb_head = B.head; v.fb_node = B.node; v.fb_loc = B.loc;
+    v.nent = B.nent; v.fb_iptr = B.iptr; v.fb_inc = B.inc; v.pn_ptr = B.pn_ptr; v.pn_ent = B.pn_ent;
+    v.pbuf = (R *)pbuf;
     v.ngb = ngb; v.gb_first = gb_first; v.gb_nptr = gb_nptr; v.gb_node = gb_node; v.ell_loc = ell_loc;
     v.vol = (const R *)vol; v.xyz = has_coords ? (const R *)xyz : nullptr;
     v.wd = (const R *)wd; v.nrm = (const R *)nrm; v.ncls = ncls;
@@ -225,6 +230,7 @@ struct flite_ctx {
   DevParams P;
   int nl = 0, nbatch = 1;
   int lanes_override = 0;  // FLITE_GATHER_LANES: 1, 4 or 16 on every level (0 = by size)
+  int shape_override = -1;  // FLITE_SIDE_BLOCK: side-block shape by sides per block (-1: shape 0)
   bool flux_per_side = false;  // FLITE_FLUX_KERNEL=side: one lane per side with direct record loads
   bool gather_per_node = false;  // FLITE_GATHER_KERNEL=node: no LDS-staged node blocks
   bool grad_direct = false;      // FLITE_GRAD_STORE=direct: per-thread record stores in the gradient
@@ -609,6 +615,68 @@ std::vector<std::vector<int>> nested_orders(const std::vector<Staged> &st) {
   return h2d;
 }
 
+// Side blocks of one shape for the LDS-staged flux: consecutive sides, at
+// most FB_SHAPE_THR[shape] of them touching at most FB_SHAPE_NODES[shape]
+// distinct nodes; each block's node list is ascending (neighbouring records
+// share cache lines)
+int build_side_blocks(Level &L, int shape, const std::vector<int> &ea, const std::vector<int> &eb) {
+  const int n = L.n, ne = L.ne, FB_SIDES = FB_SHAPE_THR[shape], FB_NODES = FB_SHAPE_NODES[shape];
+  Level::SideBlocks &B = L.sb[shape];
+  std::vector<int> fb_head{0, 0, 0, 0}, fb_node, fb_nodep, fb_iptr{0};
+  std::vector<unsigned> fb_loc(ne);
+  std::vector<unsigned short> fb_inc;
+  std::vector<int> stamp(n, -1), local(n, 0), cur;
+  std::vector<std::vector<unsigned short>> inc;
+  int start = 0, blk = 0;
+  auto close = [&](int end) {
+    std::sort(cur.begin(), cur.end());
+    for (size_t q = 0; q < cur.size(); ++q) local[cur[q]] = (int)q;
+    for (int k = start; k < end; ++k) fb_loc[k] = (unsigned)local[ea[k]] | ((unsigned)local[eb[k]] << 16);
+    // each node's sides within the block, in side order
+    inc.assign(cur.size(), {});
+    for (int k = start; k < end; ++k) {
+      inc[local[ea[k]]].push_back((unsigned short)((k - start) << 1));
+      inc[local[eb[k]]].push_back((unsigned short)(((k - start) << 1) | 1));
+    }
+    for (auto &l : inc) {
+      fb_inc.insert(fb_inc.end(), l.begin(), l.end());
+      fb_iptr.push_back((int)fb_inc.size());
+    }
+    fb_node.insert(fb_node.end(), cur.begin(), cur.end());
+    fb_nodep.insert(fb_nodep.end(), cur.begin(), cur.end());
+    fb_nodep.resize((size_t)(blk + 1) * FB_NODES, -1);
+    for (int v : {end, (int)fb_node.size(), fb_iptr.back(), 0}) fb_head.push_back(v);
+    cur.clear();
+    start = end;
+    ++blk;
+  };
+  for (int k = 0; k < ne; ++k) {
+    int add = (stamp[ea[k]] != blk) + (stamp[eb[k]] != blk && eb[k] != ea[k]);
+    if (k - start == FB_SIDES || (int)cur.size() + add > FB_NODES) close(k);
+    for (int v : {ea[k], eb[k]})
+      if (stamp[v] != blk) { stamp[v] = blk; cur.push_back(v); }
+  }
+  if (start < ne) close(ne);
+  B.nfb = blk;
+  B.nent = (long long)fb_node.size();
+  // node -> its entries, in block order
+  std::vector<int> pn_ptr(n + 1, 0), pn_ent(fb_node.size());
+  for (int g : fb_node) pn_ptr[g + 1]++;
+  for (int i = 0; i < n; ++i) pn_ptr[i + 1] += pn_ptr[i];
+  {
+    std::vector<int> pos(pn_ptr.begin(), pn_ptr.end() - 1);
+    for (size_t q = 0; q < fb_node.size(); ++q) pn_ent[pos[fb_node[q]]++] = (int)q;
+  }
+  TRY(L.pool.upload(&B.head, fb_head));
+  TRY(L.pool.upload(&B.node, fb_nodep));
+  TRY(L.pool.upload(&B.loc, fb_loc));
+  TRY(L.pool.upload(&B.iptr, fb_iptr));
+  TRY(L.pool.upload(&B.inc, fb_inc));
+  TRY(L.pool.upload(&B.pn_ptr, pn_ptr));
+  TRY(L.pool.upload(&B.pn_ent, pn_ent));
+  return 0;
+}
+
 int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int> &h2d, bool coarse_fp32,
                 std::unique_ptr<Level> &out) {
   auto Lp = std::make_unique<Level>();
@@ -620,6 +688,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   // short per-lane gather chains on the small coarse ones
   L.nbatch = c->nbatch;
   L.lanes = c->lanes_override ? c->lanes_override : (S.n >= (1 << 20) ? 1 : S.n >= (1 << 16) ? 4 : 16);
+  L.fb_shape = c->shape_override >= 0 ? c->shape_override : 0;
   L.ne = S.ne;
   L.nbs = S.nbs;
   L.nb = S.nb;
@@ -679,63 +748,7 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
     eb[k] = db[e];
     for (int q = 0; q < 3; ++q) w[3 * (size_t)k + q] = S.w[(size_t)q * ne + e];
   }
-  // side blocks for the LDS-staged flux: consecutive sides, at most
-  // FB_SIDES of them touching at most FB_NODES distinct nodes; each block's
-  // node list is ascending (neighbouring records share cache lines)
-  {
-    std::vector<int> fb_head{0, 0, 0, 0}, fb_node, fb_nodep, fb_iptr{0};
-    std::vector<unsigned> fb_loc(ne);
-    std::vector<unsigned short> fb_inc;
-    std::vector<int> stamp(n, -1), local(n, 0), cur;
-    std::vector<std::vector<unsigned short>> inc;
-    int start = 0, blk = 0;
-    auto close = [&](int end) {
-      std::sort(cur.begin(), cur.end());
-      for (size_t q = 0; q < cur.size(); ++q) local[cur[q]] = (int)q;
-      for (int k = start; k < end; ++k) fb_loc[k] = (unsigned)local[ea[k]] | ((unsigned)local[eb[k]] << 16);
-      // each node's sides within the block, in side order
-      inc.assign(cur.size(), {});
-      for (int k = start; k < end; ++k) {
-        inc[local[ea[k]]].push_back((unsigned short)((k - start) << 1));
-        inc[local[eb[k]]].push_back((unsigned short)(((k - start) << 1) | 1));
-      }
-      for (auto &l : inc) {
-        fb_inc.insert(fb_inc.end(), l.begin(), l.end());
-        fb_iptr.push_back((int)fb_inc.size());
-      }
-      fb_node.insert(fb_node.end(), cur.begin(), cur.end());
-      fb_nodep.insert(fb_nodep.end(), cur.begin(), cur.end());
-      fb_nodep.resize((size_t)(blk + 1) * FB_NODES, -1);
-      for (int v : {end, (int)fb_node.size(), fb_iptr.back(), 0}) fb_head.push_back(v);
-      cur.clear();
-      start = end;
-      ++blk;
-    };
-    for (int k = 0; k < ne; ++k) {
-      int add = (stamp[ea[k]] != blk) + (stamp[eb[k]] != blk && eb[k] != ea[k]);
-      if (k - start == FB_SIDES || (int)cur.size() + add > FB_NODES) close(k);
-      for (int v : {ea[k], eb[k]})
-        if (stamp[v] != blk) { stamp[v] = blk; cur.push_back(v); }
-    }
-    if (start < ne) close(ne);
-    L.nfb = blk;
-    L.nent = (long long)fb_node.size();
-    // node -> its entries, in block order
-    std::vector<int> pn_ptr(n + 1, 0), pn_ent(fb_node.size());
-    for (int g : fb_node) pn_ptr[g + 1]++;
-    for (int i = 0; i < n; ++i) pn_ptr[i + 1] += pn_ptr[i];
-    {
-      std::vector<int> pos(pn_ptr.begin(), pn_ptr.end() - 1);
-      for (size_t q = 0; q < fb_node.size(); ++q) pn_ent[pos[fb_node[q]]++] = (int)q;
-    }
-    TRY(L.pool.upload(&L.fb_head, fb_head));
-    TRY(L.pool.upload(&L.fb_node, fb_nodep));
-    TRY(L.pool.upload(&L.fb_loc, fb_loc));
-    TRY(L.pool.upload(&L.fb_iptr, fb_iptr));
-    TRY(L.pool.upload(&L.fb_inc, fb_inc));
-    TRY(L.pool.upload(&L.pn_ptr, pn_ptr));
-    TRY(L.pool.upload(&L.pn_ent, pn_ent));
-  }
+  for (int k = 0; k < FB_NSHAPES; ++k) TRY(build_side_blocks(L, k, ea, eb));
   // sliced ELL incidence
   std::vector<int> deg(n, 0);
   for (int k = 0; k < ne; ++k) { deg[ea[k]]++; deg[eb[k]]++; }
@@ -917,7 +930,11 @@ int build_level(flite_ctx *c, int level, const Staged &S, const std::vector<int>
   TRY(al_real(&L.tau, (size_t)(VREC * (size_t)n) * NB));  // tau and dnu share the record
   L.dnu = (char *)L.tau + 9 * (L.fp32 ? 4 : 8);
   TRY(al_real(&L.fbuf, (size_t)(6 * (size_t)ne) * NB));
-  TRY(al_real(&L.pbuf, (size_t)(6 * (size_t)L.nent) * NB));
+  {
+    long long nent = 0;
+    for (const auto &B : L.sb) nent = std::max(nent, B.nent);
+    TRY(al_real(&L.pbuf, (size_t)(6 * (size_t)nent) * NB));
+  }
   TRY(al_real(&L.bbuf, (size_t)(10 * (size_t)nbs) * NB));
   TRY(al_real(&L.ws, (size_t)(3 * (size_t)n) * NB));
   TRY(al_real(&L.jst, c->hp.flux_type != 4 ? (size_t)(8 * (size_t)n) * NB : 1));
@@ -1096,18 +1113,23 @@ int op_flux(flite_ctx *c, Level &L) {
       bool n4 = P.nsig == 4;  // LED exponent: 4 inviscid, 2 viscous
       L.flux_partials = !c->flux_per_side;
       if (L.flux_partials) {
-        dim3 gb((unsigned)L.nfb * L.nbatch), bb(FB_THR);
-        if (o2 && P.sa) {
-          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, true, 4>), gb, bb, 0, c->stream, V, P);
-          else hipLaunchKernelGGL((k_flux_blk<R, 2, true, 2>), gb, bb, 0, c->stream, V, P);
-        } else if (o2) {
-          if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, false, 4>), gb, bb, 0, c->stream, V, P);
-          else hipLaunchKernelGGL((k_flux_blk<R, 2, false, 2>), gb, bb, 0, c->stream, V, P);
-        } else if (P.sa) {
-          hipLaunchKernelGGL((k_flux_blk<R, 1, true, 2>), gb, bb, 0, c->stream, V, P);
-        } else {
-          hipLaunchKernelGGL((k_flux_blk<R, 1, false, 2>), gb, bb, 0, c->stream, V, P);
-        }
+        auto blk = [&](auto shape) {
+          constexpr int SH = decltype(shape)::value;
+          dim3 gb((unsigned)L.sb[SH].nfb * L.nbatch), bb(FB_SHAPE_THR[SH]);
+          if (o2 && P.sa) {
+            if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, true, 4, SH>), gb, bb, 0, c->stream, V, P);
+            else hipLaunchKernelGGL((k_flux_blk<R, 2, true, 2, SH>), gb, bb, 0, c->stream, V, P);
+          } else if (o2) {
+            if (n4) hipLaunchKernelGGL((k_flux_blk<R, 2, false, 4, SH>), gb, bb, 0, c->stream, V, P);
+            else hipLaunchKernelGGL((k_flux_blk<R, 2, false, 2, SH>), gb, bb, 0, c->stream, V, P);
+          } else if (P.sa) {
+            hipLaunchKernelGGL((k_flux_blk<R, 1, true, 2, SH>), gb, bb, 0, c->stream, V, P);
+          } else {
+            hipLaunchKernelGGL((k_flux_blk<R, 1, false, 2, SH>), gb, bb, 0, c->stream, V, P);
+          }
+        };
+        if (L.fb_shape == 0) blk(std::integral_constant<int, 0>{});
+        else blk(std::integral_constant<int, 1>{});
       } else if (o2 && P.sa) {
         if (n4) hipLaunchKernelGGL((k_flux<R, 2, true, 4>), g, b, 0, c->stream, V, P);
         else hipLaunchKernelGGL((k_flux<R, 2, true, 2>), g, b, 0, c->stream, V, P);
@@ -1626,6 +1648,12 @@ int flite_ctx_create(const flite_params *p, int n_levels, int n_batch, flite_ctx
       return fail(FLITE_E_ARG, "FLITE_GATHER_LANES must be 1, 4 or 16");
     }
     c->lanes_override = v;
+  }
+  if (const char *e = std::getenv("FLITE_SIDE_BLOCK")) {
+    int v = std::atoi(e);
+    for (int k = 0; k < FB_NSHAPES; ++k)
+      if (v == FB_SHAPE_THR[k]) c->shape_override = k;
+    if (c->shape_override < 0) return fail(FLITE_E_ARG, "FLITE_SIDE_BLOCK must be 256 or 512");
   }
   if (const char *e = std::getenv("FLITE_FLUX_KERNEL")) {
     std::string v(e);
@@ -2500,6 +2528,28 @@ int flite_autotune(flite_ctx *c, int repeats, int *lanes_out) {
     }
     L.lanes = best;
     lanes_out[l] = best;
+    // the flux kernel's side-block shape (levels that use side blocks)
+    if (rc == 0 && L.ne > 0 && !c->flux_per_side && c->P.ftype == 4) {
+      int bs = L.fb_shape;
+      float bms = 1e30f;
+      rc = op_prep(c, L, true);
+      if (rc == 0) rc = op_grad(c, L);
+      for (int k = 0; k < FB_NSHAPES && rc == 0; ++k) {
+        L.fb_shape = k;
+        rc = op_flux(c, L);
+        if (rc == 0 && hipEventRecord(a, c->stream) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventRecord");
+        for (int r = 0; r < repeats && rc == 0; ++r) rc = op_flux(c, L);
+        if (rc == 0 && hipEventRecord(b, c->stream) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventRecord");
+        if (rc == 0) rc = sync(c);
+        float ms = 0;
+        if (rc == 0 && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = fail(FLITE_E_HIP, "hipEventElapsedTime");
+        if (rc == 0 && ms < bms) {
+          bms = ms;
+          bs = k;
+        }
+      }
+      L.fb_shape = bs;
+    }
     off = 0;
     for (auto &k : kept) {
       if (k.second && hipMemcpyAsync(k.first, save + off, k.second, hipMemcpyDeviceToDevice, c->stream) != hipSuccess &&
@@ -2521,6 +2571,15 @@ int flite_level_lanes(flite_ctx *c, int level, int *lanes) {
   TRY(check_level(c, level));
   if (!lanes) return fail(FLITE_E_ARG, "null output");
   *lanes = c->lev[level - 1]->lanes;
+  return 0;
+}
+
+int flite_level_side_block(flite_ctx *c, int level, int *sides, int *nodes) {
+  TRY(check_level(c, level));
+  if (!sides || !nodes) return fail(FLITE_E_ARG, "null output");
+  const int k = c->lev[level - 1]->fb_shape;
+  *sides = FB_SHAPE_THR[k];
+  *nodes = FB_SHAPE_NODES[k];
   return 0;
 }
 

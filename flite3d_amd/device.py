@@ -59,6 +59,7 @@ def lib():
         L.flite_set_coarse_precision.argtypes = [vp, i]
         L.flite_autotune.argtypes = [vp, i, P(i)]
         L.flite_level_lanes.argtypes = [vp, i, P(i)]
+        L.flite_level_side_block.argtypes = [vp, i, P(i), P(i)]
         L.flite_drag_metric_gradient.argtypes = [vp, P(d), P(d), i]
         _bound = L
     return _bound
@@ -174,6 +175,12 @@ class Solver:
         v = ctypes.c_int()
         self._check(self.L.flite_level_lanes(self.ctx, level, ctypes.byref(v)))
         return v.value
+
+    def side_block(self, level):
+        """(sides per block, node cap) of the level's flux side blocks."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.flite_level_side_block(self.ctx, level, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def select(self, batch):
         """Condition that get/set/monitor act on (batched contexts)."""

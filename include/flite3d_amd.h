@@ -307,16 +307,21 @@ int flite_set_coarse_precision(flite_ctx *ctx, int fp32);
 
 /* Autotuning of the launch shape (no counterpart in the reference): per
  * level, the lanes per node (1, 4 or 16) of the gather and stage kernels
- * that run fastest on the context's current state, each candidate timed
- * with HIP events over `repeats` runs of the level's lane-dependent
- * kernels.  The choice holds for the context's life and is written to
- * lanes_out[0 .. n_levels-1]; without autotuning a level gets 1 lane from
- * 2^20 nodes, 4 from 2^16, else 16 (FLITE_GATHER_LANES forces one value).
- * The lanes change the order of the gather sums (results agree to
- * rounding), and timing noise can change the choice from run to run.
- * flite_level_lanes reads a level's current setting. */
+ * and the side-block shape of the flux kernel (256 sides touching at most
+ * 128 nodes, or 512 touching at most 224) that run fastest on the
+ * context's current state, each candidate timed with HIP events over
+ * `repeats` runs of the kernels it changes.  The choice holds for the
+ * context's life; the lanes are written to lanes_out[0 .. n_levels-1].
+ * Without autotuning a level gets 1 lane from 2^20 nodes, 4 from 2^16,
+ * else 16 (FLITE_GATHER_LANES forces one value), and 256-side blocks
+ * (FLITE_SIDE_BLOCK=256|512 forces a shape).  Lanes and shapes change the
+ * order of the gather and node sums (results agree to rounding), and
+ * timing noise can change the choice from run to run.  flite_level_lanes
+ * and flite_level_side_block read a level's current setting (sides per
+ * block and its node cap). */
 int flite_autotune(flite_ctx *ctx, int repeats, int *lanes_out);
 int flite_level_lanes(flite_ctx *ctx, int level, int *lanes);
+int flite_level_side_block(flite_ctx *ctx, int level, int *sides, int *nodes);
 
 /* Condition that flite_field_get/set and flite_monitor act on (default 0);
  * no reference counterpart (the reference has one condition per run). */

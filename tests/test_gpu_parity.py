@@ -198,6 +198,7 @@ def test_gather_lanes(lanes, monkeypatch):
     {"FLITE_GATHER_LANES": "1", "FLITE_GATHER_KERNEL": "node"},    # per-node gathers only
     {"FLITE_FLUX_KERNEL": "side"},                                 # per-side flux records and gather
     {"FLITE_FLUX_KERNEL": "side", "FLITE_GATHER_LANES": "4"},
+    {"FLITE_SIDE_BLOCK": "512"},                                   # 512-side blocks of at most 224 nodes
 ])
 def test_kernel_variants(env, monkeypatch):
     """Every kernel path the library selects (side blocks with per-node
@@ -218,8 +219,9 @@ def test_kernel_variants(env, monkeypatch):
 
 
 def test_autotune():
-    """flite_autotune picks 1, 4 or 16 lanes per level by timing; the cycles
-    that follow, on the chosen launch shapes, match the oracle."""
+    """flite_autotune picks 1, 4 or 16 lanes and a side-block shape per level
+    by timing; the cycles that follow, on the chosen launch shapes, match the
+    oracle."""
     print("[case] autotune lanes, rans 8^3", flush=True)
     lv = levels(8, 3, first_cell=0.005)
     s, o = pair(rans(multigridscheme=3), lv)
@@ -227,6 +229,7 @@ def test_autotune():
     lanes = s.autotune(2)
     assert len(lanes) == 3 and all(x in (1, 4, 16) for x in lanes), lanes
     assert [s.lanes(k + 1) for k in range(3)] == lanes
+    assert all(s.side_block(k + 1) in ((256, 128), (512, 224)) for k in range(3))
     for c in (1, 2, 3):
         s.cycle(c); o.cycle(c)
     ms, mo = s.monitor(), o.monitor()
