@@ -694,9 +694,70 @@ __device__ __forceinline__ T sa_side_convection(const T *A, const T *B, T wx, T 
   return fsa;
 }
 
+// MUSCL slope of component j (DEF:1840-1944): the two ends' projected
+// gradients a, b (before limiting), the LED bounds of the first end (ilo1,
+// ihi1) and of the second (ilo2, ihi2), the clip length hl
+template <int NSIG, typename T>
+__device__ __forceinline__ T muscl_slope(const DevParams &P, T a, T b, T ilo1, T ihi1, T ilo2, T ihi2, T hl) {
+  if (P.autoclip == 1) {
+    a = led<NSIG>(a, ilo1, ihi1);
+    b = -led<NSIG>(-b, ilo2, ihi2);
+  }
+  T c;
+  if (a * b <= T(0)) {
+    c = 0;
+  } else {
+    c = fabs(a) <= fabs(b) ? a : b;
+    if (fabs(c) > hl && hl > T(0)) c = c > T(0) ? hl : -hl;
+  }
+  return c;
+}
+
+// a node's gradient of component j projected on half the side vector
+template <typename T, typename TG>
+__device__ __forceinline__ T muscl_projection(const TG *G, T dx, T dy, T dz, int j) {
+  return T(0.5) * (G[j] * dx + G[j + 5] * dy + G[j + 10] * dz);
+}
+
+// the side's half-length clip hl and projected gradients a[5], b[5]
+template <typename T, typename TG, typename TX>
+__device__ __forceinline__ T muscl_projections(const DevParams &P, const TG *G1, const TG *G2, const TX *X1,
+                                               const TX *X2, T *a, T *b) {
+  T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
+  T hl = T(0.5) * T(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    a[j] = muscl_projection<T>(G1, dx, dy, dz, j);
+    b[j] = muscl_projection<T>(G2, dx, dy, dz, j);
+  }
+  return hl;
+}
+
+template <typename T, typename R, bool SA>
+__device__ __forceinline__ void side_flux_d(const DevParams &P, const T *A, const T *B, const T *d, T wx, T wy, T wz,
+                                            T *o, R wg);
+
 template <typename T, typename R, int ORDER, bool SA, int NSIG, typename TG = R, typename TX = R>
 __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, const TG *G2, const TX *X1, const TX *X2,
                                           const T *A, const T *B, T wx, T wy, T wz, T *o, R wg = R(0)) {
+  if constexpr (ORDER > 1) {
+    T a[5], b[5], d[5];
+    T hl = muscl_projections<T>(P, G1, G2, X1, X2, a, b);
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      d[j] = muscl_slope<NSIG, T>(P, a[j], b[j], T(G1[15 + j]), T(G1[20 + j]), T(G2[15 + j]), T(G2[20 + j]), hl);
+    side_flux_d<T, R, SA>(P, A, B, d, wx, wy, wz, o, wg);
+  } else {
+    side_flux_d<T, R, SA>(P, A, B, (const T *)nullptr, wx, wy, wz, o, wg);
+  }
+}
+
+// the HLLC flux of one side from the two ends' records and the MUSCL slopes
+// d[5] of the primitive states (null: first order): the left state is A + d,
+// the right B - d, with the reference's positivity guards
+template <typename T, typename R, bool SA>
+__device__ __forceinline__ void side_flux_d(const DevParams &P, const T *A, const T *B, const T *d, T wx, T wy, T wz,
+                                            T *o, R wg) {
   T gam = T(P.gamma), gm1 = gam - T(1);
   T roe1 = A[0], u1 = A[1], v1 = A[2], w1 = A[3], re1 = A[4], pr1 = A[5];
   T roe2 = B[0], u2 = B[1], v2 = B[2], w2 = B[3], re2 = B[4], pr2 = B[5];
@@ -709,27 +770,7 @@ __device__ __forceinline__ void side_flux(const DevParams &P, const TG *G1, cons
   T inv = n2 > T(0) ? frsqrt(n2) : T(0);
   T rnorm1 = n2 * inv;  // |2w|
   T rnx = nx * inv, rny = ny * inv, rnz = nz * inv;
-  if constexpr (ORDER > 1) {
-    T dx = X2[0] - X1[0], dy = X2[1] - X1[1], dz = X2[2] - X1[2];
-    T hl = T(0.5) * T(P.clip) * sqrt(dx * dx + dy * dy + dz * dz);
-    T d[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      T a = T(0.5) * (G1[j] * dx + G1[j + 5] * dy + G1[j + 10] * dz);
-      T b = T(0.5) * (G2[j] * dx + G2[j + 5] * dy + G2[j + 10] * dz);
-      if (P.autoclip == 1) {
-        a = led<NSIG>(a, G1[15 + j], G1[20 + j]);
-        b = -led<NSIG>(-b, G2[15 + j], G2[20 + j]);
-      }
-      T c;
-      if (a * b <= T(0)) {
-        c = 0;
-      } else {
-        c = fabs(a) <= fabs(b) ? a : b;
-        if (fabs(c) > hl && hl > T(0)) c = c > T(0) ? hl : -hl;
-      }
-      d[j] = c;
-    }
+  if (d) {
     if (roe1 + d[0] > T(0.1)) roe1 = roe1 + d[0];
     u1 += d[1]; v1 += d[2]; w1 += d[3];
     if (re1 + d[4] > T(0.0001)) re1 = re1 + d[4];
@@ -1763,46 +1804,84 @@ __global__ void k_minmax_arg(LevelDev<double> L, int *amin) {
   }
 }
 
-// adjoint shares of side e: d = lam_a - lam_b against df/d(U_a, U_b,
-// G_a, B_a, G_b, B_b), one dual pass per input
+// adjoint shares of side e: dl = lam_a - lam_b against df/d(U_a, U_b,
+// G_a, B_a, G_b, B_b).  The gradients and bounds reach the flux only through
+// the five MUSCL slopes d_j, so one dual pass per state component and per
+// slope gives dl^T df/d(U_a, U_b, d), and the slopes' own derivatives (a few
+// scalar dual passes of muscl_slope per component) carry the d shares to
+// the gradients (d a_j / d G = half the side vector) and the bounds.
 template <int NSIG>
-__global__ void k_flux_adjoint2(LevelDev<double> L, DevParams P, const double *lam, double *abuf2) {
+__global__ __launch_bounds__(NTHR) void k_flux_adjoint2(LevelDev<double> L, DevParams P, const double *lam,
+                                                        double *abuf2) {
   using ad::Dual;
   int e = gtid_x();
   if (e >= L.ne) return;
   size_t n = L.n;
   int ia = L.ea[e], ib = L.eb[e];
-  double ua[5], ub[5], d[5];
+  double ua[5], ub[5], dl[5];
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
     ua[c] = L.u[c * n + ia];
     ub[c] = L.u[c * n + ib];
-    d[c] = lam[c * n + ia] - lam[c * n + ib];
+    dl[c] = lam[c * n + ia] - lam[c * n + ib];
   }
   const double *ga = L.grec + GREC * (size_t)ia, *gb = L.grec + GREC * (size_t)ib;
+  const double *xa = L.xyz + 3 * (size_t)ia, *xb = L.xyz + 3 * (size_t)ib;
   const double w0 = L.w[3 * e], w1 = L.w[3 * e + 1], w2 = L.w[3 * e + 2];
+  double pa[5], pb[5], d[5];
+  const double hl = muscl_projections<double>(P, ga, gb, xa, xb, pa, pb);
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    d[j] = muscl_slope<NSIG, double>(P, pa[j], pb[j], ga[15 + j], ga[20 + j], gb[15 + j], gb[20 + j], hl);
   double *out = abuf2 + LIN_SIDE * (size_t)e;
+  double sd[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
-  for (int k = 0; k < LIN_SIDE; ++k) {
-    Dual Ua[5], Ub[5], A[8], B[8], GA[LIN_G], GB[LIN_G], o[6];
+  for (int k = 0; k < 15; ++k) {
+    Dual Ua[5], Ub[5], A[8], B[8], D[5], o[6];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       Ua[c] = Dual(ua[c], k == c ? 1.0 : 0.0);
       Ub[c] = Dual(ub[c], k == 5 + c ? 1.0 : 0.0);
-    }
-#pragma unroll
-    for (int c = 0; c < LIN_G; ++c) {
-      GA[c] = Dual(ga[c], k == 10 + c ? 1.0 : 0.0);
-      GB[c] = Dual(gb[c], k == 10 + LIN_G + c ? 1.0 : 0.0);
+      D[c] = Dual(d[c], k == 10 + c ? 1.0 : 0.0);
     }
     state_record(Ua, P.gamma, A);
     state_record(Ub, P.gamma, B);
-    side_flux<Dual, double, 2, false, NSIG, Dual>(P, GA, GB, L.xyz + 3 * (size_t)ia, L.xyz + 3 * (size_t)ib, A, B,
-                                                  Dual(w0), Dual(w1), Dual(w2), o);
+    side_flux_d<Dual, double, false>(P, A, B, D, Dual(w0), Dual(w1), Dual(w2), o, 0.0);
     double acc = 0;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) acc += d[c] * o[c].d;
-    out[k] = acc;
+    for (int c = 0; c < 5; ++c) acc += dl[c] * o[c].d;
+    if (k < 10) out[k] = acc;
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      if (k == 10 + c) sd[c] = acc;  // selects, not an indexed private array
+  }
+  const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+  const double hx = 0.5 * dx, hy = 0.5 * dy, hz = 0.5 * dz;
+  double *oa = out + 10, *ob = out + 10 + LIN_G;
+#pragma unroll 1
+  for (int j = 0; j < 5; ++j) {
+    const double aj = muscl_projection<double>(ga, dx, dy, dz, j), bj = muscl_projection<double>(gb, dx, dy, dz, j);
+    const double s = j == 0 ? sd[0] : j == 1 ? sd[1] : j == 2 ? sd[2] : j == 3 ? sd[3] : sd[4];
+    // d_j's derivatives along (a_j, b_j, ilo_a, ihi_a, ilo_b, ihi_b)
+    double dd[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      Dual c = muscl_slope<NSIG, Dual>(P, Dual(aj, q == 0 ? 1.0 : 0.0), Dual(bj, q == 1 ? 1.0 : 0.0),
+                                       Dual(ga[15 + j], q == 2 ? 1.0 : 0.0), Dual(ga[20 + j], q == 3 ? 1.0 : 0.0),
+                                       Dual(gb[15 + j], q == 4 ? 1.0 : 0.0), Dual(gb[20 + j], q == 5 ? 1.0 : 0.0),
+                                       Dual(hl));
+      dd[q] = s * c.d;
+    }
+    oa[j] = dd[0] * hx;
+    oa[j + 5] = dd[0] * hy;
+    oa[j + 10] = dd[0] * hz;
+    oa[15 + j] = dd[2];
+    oa[20 + j] = dd[3];
+    ob[j] = dd[1] * hx;
+    ob[j + 5] = dd[1] * hy;
+    ob[j + 10] = dd[1] * hz;
+    ob[15 + j] = dd[4];
+    ob[20 + j] = dd[5];
   }
 }
 
