@@ -223,6 +223,14 @@ __device__ __forceinline__ Dual sqrt(Dual a) {
   return Dual(r, r > 0.0 ? a.d / (2.0 * r) : 0.0);
 }
 __device__ __forceinline__ Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+__device__ __forceinline__ Dual exp(Dual a) {
+  double e = ::exp(a.v);
+  return Dual(e, e * a.d);
+}
+__device__ __forceinline__ Dual pow(Dual a, double p) {
+  double r = ::pow(a.v, p);
+  return Dual(r, a.v != 0.0 ? p * r / a.v * a.d : 0.0);
+}
 __device__ __forceinline__ Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
 __device__ __forceinline__ Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
 }  // namespace ad
@@ -1380,6 +1388,44 @@ __global__ void k_jst_flux(LevelDev<R> L, DevParams P) {
   for (int c = 0; c < 5; ++c) L.diss[c * n + i] += acc[c];
 }
 
+// makeSASourceTerm (DEF:9285-9370) of one node on any arithmetic type: the
+// density, nu~ (u(:,6)), the laminar viscosity (after the caller's guard),
+// vorticity and divergence, wall distance and volume
+template <typename T>
+__device__ __forceinline__ T sa_source_t(const DevParams &P, T rho, T nu, T mu, T vort, T divg, double wd, double vt) {
+  double Re = P.Re, ioRe = 1.0 / Re;
+  double cb1 = 0.1355, cb2 = 0.622, k2 = 0.41 * 0.41, sigma = 2.0 / 3.0;
+  double cw1 = cb1 / k2 + (1.0 + cb2) / sigma, cw2 = 0.3, cw36 = 64.0, cv13 = 7.1 * 7.1 * 7.1;
+  double ct3 = 1.1, ct4 = 2.0;
+  T mc = nu;
+  mc = mc > T(0.0) ? mc : T(0.0);
+  T ksi = rho * T(ioRe) * mc / mu;
+  ksi = ksi > T(0.001) ? ksi : T(0.001);
+  T ft2 = T(ct3) * exp(T(-ct4) * ksi * ksi);
+  T k3 = ksi * ksi * ksi;
+  double d = wd;
+  d = d > 1e-10 ? d : 1e-10;
+  T md = mc / T(d);
+  double d2 = d * d;
+  T fv1 = k3 / (k3 + T(cv13));
+  T fv2 = T(1.0) - ksi / (T(1.0) + ksi * fv1);
+  T Sc = vort + T(ioRe) * mc * fv2 / T(k2 * d2);
+  T r;
+  if (fabs(Sc) > T(1e-10)) {
+    r = T(ioRe) * mc / (Sc * T(k2) * T(d2));
+    r = r < T(0.0) ? T(0.0) : (r > T(100.0) ? T(100.0) : r);
+  } else {
+    r = T(100.0);
+  }
+  T r2 = r * r, r6 = r2 * r2 * r2;
+  T g = r + T(cw2) * (r6 - r);
+  T g2 = g * g, g6 = g2 * g2 * g2;
+  T rad = T(1.0 + cw36) / (g6 + T(cw36));
+  T fw = rad > T(1e-10) ? g * pow(rad, 1.0 / 6.0) : T(::pow(1.0 + cw36, 1.0 / 6.0));
+  return T(-vt) * T(cb1) * (T(1.0) - ft2) * Sc * mc + T(ioRe * vt) * (T(cw1) * fw - T(cb1) * ft2 / T(k2)) * md * md -
+         T(vt) * mc * divg;
+}
+
 // ---------------------------------------------------------------------------
 // Tangent and adjoint of the level-1 convective residual R(U) (makeRHS,
 // DEF:1525-2512: interior HLLC sides + boundary sides) for first-order
@@ -1419,6 +1465,7 @@ __device__ __forceinline__ void bside_end_flux(const DevParams &P, int ind, cons
 
 // tangent: fbuf/bbuf <- d(fluxes)/dU along du; the stage gather (MODE_RHS)
 // then sums them into rhs = (dR/dU) du
+template <bool SA>
 __global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du) {
   using ad::Dual;
   int e = gtid_x();
@@ -1433,12 +1480,16 @@ __global__ void k_flux_tangent(LevelDev<double> L, DevParams P, const double *du
   }
   state_record(Ua, P.gamma, A);
   state_record(Ub, P.gamma, B);
-  side_flux<Dual, double, 1, false, 2, double, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
-                                       Dual(L.w[3 * e + 1]), Dual(L.w[3 * e + 2]), o);
+  if (SA) {
+    A[6] = Dual(L.u[5 * n + ia], du[5 * n + ia]);
+    B[6] = Dual(L.u[5 * n + ib], du[5 * n + ib]);
+  }
+  side_flux<Dual, double, 1, SA, 2, double, double>(P, nullptr, nullptr, nullptr, nullptr, A, B, Dual(L.w[3 * e]),
+                                                    Dual(L.w[3 * e + 1]), Dual(L.w[3 * e + 2]), o);
   double *f = L.fbuf + 6 * (size_t)e;
 #pragma unroll
   for (int c = 0; c < 5; ++c) f[c] = o[c].d;
-  f[5] = 0;
+  f[5] = SA ? o[5].d : 0.0;
 }
 
 __global__ void k_bflux_tangent(LevelDev<double> L, DevParams P, const double *du) {
@@ -1676,6 +1727,9 @@ constexpr int LIN_G = 25;                   // gradient 15 | 1/lo 5 | 1/hi 5
 constexpr int LIN_SIDE = 10 + 2 * LIN_G;    // adjoint shares per side: U_a 5, U_b 5, (G,B)_a 25, (G,B)_b 25
 
 // tangent of the gather record (rho, u, v, w, rhoE, p) along du
+// the gather record's tangent dq[DQ i ..] = (rho, u, v, w, E, p, nu~, 0)
+// along du (nu~ = u(:,6) with Spalart-Allmaras, else 0)
+constexpr int DQ = 8;
 __global__ void k_record_tangent(LevelDev<double> L, DevParams P, const double *du, double *dq) {
   using ad::Dual;
   int i = gtid_x();
@@ -1685,8 +1739,11 @@ __global__ void k_record_tangent(LevelDev<double> L, DevParams P, const double *
 #pragma unroll
   for (int c = 0; c < 5; ++c) U[c] = Dual(L.u[c * n + i], du[c * n + i]);
   state_record(U, P.gamma, A);
+  double *o = dq + DQ * (size_t)i;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) dq[6 * (size_t)i + c] = A[c].d;
+  for (int c = 0; c < 6; ++c) o[c] = A[c].d;
+  o[6] = P.sa ? du[5 * n + i] : 0.0;
+  o[7] = 0.0;
 }
 
 // gradient and bounds of grad_minmax_body on dual numbers: gd[26 i + k].
@@ -1701,7 +1758,7 @@ __global__ void k_grad_tangent(LevelDev<double> L, const double *dq, ad::Dual *g
   Dual q0[5], g[15], mn[5], mx[5];
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    q0[c] = Dual(L.rec0[8 * (size_t)i + c], dq[6 * (size_t)i + c]);
+    q0[c] = Dual(L.rec0[8 * (size_t)i + c], dq[DQ * (size_t)i + c]);
     mn[c] = Dual(1e15);
     mx[c] = Dual(1e15);
   }
@@ -1710,7 +1767,7 @@ __global__ void k_grad_tangent(LevelDev<double> L, const double *dq, ad::Dual *g
   for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      Dual qj(L.rec0[8 * (size_t)j + c], dq[6 * (size_t)j + c]);
+      Dual qj(L.rec0[8 * (size_t)j + c], dq[DQ * (size_t)j + c]);
       Dual sq = q0[c] + qj;
       g[c] += sq * Dual(wx);
       g[5 + c] += sq * Dual(wy);
@@ -1725,7 +1782,7 @@ __global__ void k_grad_tangent(LevelDev<double> L, const double *dq, ad::Dual *g
     double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-      Dual qo(L.rec0[8 * (size_t)o + c], dq[6 * (size_t)o + c]);
+      Dual qo(L.rec0[8 * (size_t)o + c], dq[DQ * (size_t)o + c]);
       Dual t = Dual(2.0) * q0[c] + (q0[c] + qo);
       g[c] += t * Dual(wx);
       g[5 + c] += t * Dual(wy);
@@ -1748,7 +1805,8 @@ __global__ void k_grad_tangent(LevelDev<double> L, const double *dq, ad::Dual *g
 }
 
 // second-order side fluxes along (du, dG, dB): fbuf[6 e + c] = d f_c
-template <int NSIG>
+// (component 5: the SA convection along d nu~)
+template <int NSIG, bool SA>
 __global__ void k_flux_tangent2(LevelDev<double> L, DevParams P, const double *dq, const ad::Dual *gd) {
   using ad::Dual;
   int e = gtid_x();
@@ -1757,16 +1815,16 @@ __global__ void k_flux_tangent2(LevelDev<double> L, DevParams P, const double *d
   Dual A[8], B[8], o[6];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    A[c] = Dual(L.rec0[8 * (size_t)ia + c], c < 6 ? dq[6 * (size_t)ia + c] : 0.0);
-    B[c] = Dual(L.rec0[8 * (size_t)ib + c], c < 6 ? dq[6 * (size_t)ib + c] : 0.0);
+    A[c] = Dual(L.rec0[8 * (size_t)ia + c], c < 7 ? dq[DQ * (size_t)ia + c] : 0.0);
+    B[c] = Dual(L.rec0[8 * (size_t)ib + c], c < 7 ? dq[DQ * (size_t)ib + c] : 0.0);
   }
-  side_flux<Dual, double, 2, false, NSIG, Dual>(P, gd + 26 * (size_t)ia, gd + 26 * (size_t)ib, L.xyz + 3 * (size_t)ia,
-                                                L.xyz + 3 * (size_t)ib, A, B, Dual(L.w[3 * e]), Dual(L.w[3 * e + 1]),
-                                                Dual(L.w[3 * e + 2]), o);
+  side_flux<Dual, double, 2, SA, NSIG, Dual>(P, gd + 26 * (size_t)ia, gd + 26 * (size_t)ib, L.xyz + 3 * (size_t)ia,
+                                             L.xyz + 3 * (size_t)ib, A, B, Dual(L.w[3 * e]), Dual(L.w[3 * e + 1]),
+                                             Dual(L.w[3 * e + 2]), o);
   double *f = L.fbuf + 6 * (size_t)e;
 #pragma unroll
   for (int c = 0; c < 5; ++c) f[c] = o[c].d;
-  f[5] = 0;
+  f[5] = SA ? o[5].d : 0.0;
 }
 
 // node i's extremal neighbours: amin[10 i + c] = the node whose q_c sets
@@ -2090,19 +2148,46 @@ __global__ void k_vol_mesh_adjoint(LevelDev<double> L, const double *gbn, double
 // ---------------------------------------------------------------------------
 constexpr int VLIN = 56;  // scratch doubles per node of the viscous linearisation
 
+// Sutherland's laminar viscosity of max(T, 0.001) (ViscNodePart::finish)
 template <typename T>
-__device__ __forceinline__ void visc_tau(const DevParams &P, const T *g, double vol, T Traw, bool wall,
-                                         const double *nrm, T *t) {
-  T vt = T(1.0) / T(vol);
-  T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
-  T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
-  T dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
+__device__ __forceinline__ T sutherland(const DevParams &P, T Traw) {
   T Tn = Traw;
   if (Tn <= T(0.001)) Tn = T(0.001);
   T tc = T((P.gamma - 1.0) * P.mach * P.mach), Tin = T(P.inflow_temperature);
   T tct = tc * Tn;
-  T mu = T(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + T(198.6)) / (tc * Tin * Tn + T(198.6))));
+  return T(1.0 / P.Re) * ((tct * sqrt(tct)) * ((Tin + T(198.6)) / (tc * Tin * Tn + T(198.6))));
+}
+
+// vorticity magnitude and divergence from the Green-Gauss sums (ViscNodePart::finish)
+template <typename T>
+__device__ __forceinline__ void visc_vort_divg(const T *g, double vol, T &vort, T &divg) {
+  T vt = T(1.0) / T(vol);
+  T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt;
+  T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt;
+  T dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt;
+  vort = sqrt((dudy - dvdx) * (dudy - dvdx) + (dvdz - dwdy) * (dvdz - dwdy) + (dwdx - dudz) * (dwdx - dudz));
+  divg = dudx + dvdy + dwdz;
+}
+
+// stress and heat flux of a node; with Spalart-Allmaras the eddy viscosity
+// of (rho, nu~) is added to both (DEF:5556-5566)
+template <typename T>
+__device__ __forceinline__ void visc_tau(const DevParams &P, const T *g, double vol, T Traw, bool wall,
+                                         const double *nrm, T *t, T rho = T(0.0), T nu = T(0.0)) {
+  T vt = T(1.0) / T(vol);
+  T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
+  T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
+  T dudz = g[8] * vt, dvdz = g[9] * vt, dwdz = g[10] * vt, dTdz = g[11] * vt;
+  T mu = sutherland(P, Traw);
   T k = T(1.0 / P.Pr) * mu;
+  if (P.sa) {
+    T tvc = T(1.0 / P.Re) * rho * nu;
+    T ksi = tvc / mu;
+    T k3 = ksi * ksi * ksi;
+    tvc = tvc * k3 / (k3 + T(357.9));
+    mu = mu + tvc;
+    k = k + tvc * T(1.0 / P.Prt);
+  }
   T two3 = T(2.0 / 3.0), four3 = T(4.0 / 3.0);
   t[0] = mu * (four3 * dudx - two3 * (dvdy + dwdz));
   t[1] = mu * (dudy + dvdx);
@@ -2161,15 +2246,17 @@ __device__ __forceinline__ void visc_bside(const T *t1, const T *to, const T *q1
 // node record (u, v, w, T) as dual numbers from the gather record and its
 // tangent dq (null: no tangent)
 __device__ __forceinline__ void visc_rec(const LevelDev<double> &L, const DevParams &P, const double *dq, int i,
-                                         ad::Dual *a) {
+                                         ad::Dual *a, ad::Dual *rho_out = nullptr, ad::Dual *nu_out = nullptr) {
   using ad::Dual;
   const double *q = L.rec0 + 8 * (size_t)i;
-  const double *t = dq ? dq + 6 * (size_t)i : nullptr;
+  const double *t = dq ? dq + DQ * (size_t)i : nullptr;
   Dual rho(q[0], t ? t[0] : 0.0), p(q[5], t ? t[5] : 0.0);
   a[0] = Dual(q[1], t ? t[1] : 0.0);
   a[1] = Dual(q[2], t ? t[2] : 0.0);
   a[2] = Dual(q[3], t ? t[3] : 0.0);
   a[3] = Dual(P.gamma / (P.gamma - 1.0)) * p / rho;
+  if (rho_out) *rho_out = rho;
+  if (nu_out) *nu_out = Dual(q[6], t ? t[6] : 0.0);
 }
 
 __device__ __forceinline__ bool visc_wall(const LevelDev<double> &L, int i) {
@@ -2207,16 +2294,27 @@ __device__ __forceinline__ void visc_grad_sums(const LevelDev<double> &L, const 
 // tangent pass 1: the stress and heat-flux record of each node on dual
 // numbers, taud[18 i + 2 c] = value, [18 i + 2 c + 1] = tangent (dq null:
 // values only, for the adjoint)
-__global__ void k_visc_tangent_node(LevelDev<double> L, DevParams P, const double *dq, ad::Dual *taud) {
+// With Spalart-Allmaras also aux[3 i ..] = (laminar viscosity, vorticity,
+// divergence) on dual numbers, for the SA source and diffusion (aux may be
+// null otherwise)
+__global__ void k_visc_tangent_node(LevelDev<double> L, DevParams P, const double *dq, ad::Dual *taud,
+                                    ad::Dual *aux) {
   using ad::Dual;
   int i = gtid_x();
   if (i >= L.n) return;
-  Dual ai[4], g[12], t[9];
-  visc_rec(L, P, dq, i, ai);
+  Dual ai[4], g[12], t[9], rho, nu;
+  visc_rec(L, P, dq, i, ai, &rho, &nu);
   visc_grad_sums(L, P, dq, i, ai, g);
-  visc_tau(P, g, L.vol[i], ai[3], visc_wall(L, i), L.nrm + 3 * (size_t)i, t);
+  visc_tau(P, g, L.vol[i], ai[3], visc_wall(L, i), L.nrm + 3 * (size_t)i, t, rho, nu);
 #pragma unroll
   for (int c = 0; c < 9; ++c) taud[9 * (size_t)i + c] = t[c];
+  if (P.sa && aux) {
+    Dual vort, divg;
+    visc_vort_divg(g, L.vol[i], vort, divg);
+    aux[3 * (size_t)i] = sutherland(P, ai[3]);
+    aux[3 * (size_t)i + 1] = vort;
+    aux[3 * (size_t)i + 2] = divg;
+  }
 }
 
 // tangent pass 2: dvis(c, i) = -dA (c = 1..4), 0 for c = 0
@@ -2251,6 +2349,89 @@ __global__ void k_visc_tangent_flux(LevelDev<double> L, DevParams P, const doubl
   dvis[i] = 0.0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) dvis[(c + 1) * n + i] = -a[c].d;
+}
+
+// Spalart-Allmaras tangent (makeSADiffusion, DEF:7436-7808, and
+// makeSASourceTerm, DEF:9285-9370, as SADiff1Part / SADiff2Part / sa_source
+// compute them), on dual numbers from the record tangent dq and the node
+// duals aux = (laminar viscosity, vorticity, divergence) of
+// k_visc_tangent_node.
+// pass 1: dnud[4 i ..] = the grad nu~ record (grad 3, nodal cb2 term)
+__global__ void k_sa_diff1_tangent(LevelDev<double> L, DevParams P, const double *dq, ad::Dual *dnud) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  Dual ti(L.rec0[8 * (size_t)i + 6], dq[DQ * (size_t)i + 6]);
+  Dual g[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+    Dual sj = ti + Dual(L.rec0[8 * (size_t)j + 6], dq[DQ * (size_t)j + 6]);
+    g[0] += Dual(wx) * sj;
+    g[1] += Dual(wy) * sj;
+    g[2] += Dual(wz) * sj;
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    Dual to(L.rec0[8 * (size_t)o + 6], dq[DQ * (size_t)o + 6]);
+    Dual c = P.bterm == 1 ? (Dual(3.0) * ti + to) : Dual(4.0) * ti;
+    g[0] += c * Dual(L.wb[3 * s]);
+    g[1] += c * Dual(L.wb[3 * s + 1]);
+    g[2] += c * Dual(L.wb[3 * s + 2]);
+  }
+  Dual vt = Dual(1.0) / Dual(L.vol[i]);
+  Dual dx = g[0] * vt, dy = g[1] * vt, dz = g[2] * vt;
+  Dual fact = Dual(1.0 / P.Re * 0.622 / (2.0 / 3.0));
+  Dual *o = dnud + 4 * (size_t)i;
+  o[0] = dx;
+  o[1] = dy;
+  o[2] = dz;
+  o[3] = -(fact * (g[0] * dx + g[1] * dy + g[2] * dz));
+}
+
+// the diffusion coefficient Re mu / rho + nu~ of node j on dual numbers
+__device__ __forceinline__ ad::Dual sa_diff_coef(const LevelDev<double> &L, const DevParams &P, const double *dq,
+                                                 const ad::Dual *aux, int j) {
+  using ad::Dual;
+  Dual rho(L.rec0[8 * (size_t)j], dq[DQ * (size_t)j]);
+  Dual nu(L.rec0[8 * (size_t)j + 6], dq[DQ * (size_t)j + 6]);
+  return Dual(P.Re) * aux[3 * (size_t)j] / rho + nu;
+}
+
+// pass 2 and the source: dsa[i] = d(tdiff + source) at node i
+__global__ void k_sa_tangent(LevelDev<double> L, DevParams P, const double *dq, const ad::Dual *aux,
+                             const ad::Dual *dnud, double *dsa) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const Dual *di = dnud + 4 * (size_t)i;
+  Dual fact = Dual(1.0 / P.Re / (2.0 / 3.0));
+  Dual v1 = sa_diff_coef(L, P, dq, aux, i);
+  Dual acc(0.0);
+  for_edges_w<1>(L, i, 0, [&](int j, double wx, double wy, double wz) {
+    const Dual *dj = dnud + 4 * (size_t)j;
+    Dual v2 = sa_diff_coef(L, P, dq, aux, j);
+    Dual w0(wx), w1(wy), w2(wz);
+    Dual fd = v1 * (w0 * di[0] + w1 * di[1] + w2 * di[2]) + v2 * (w0 * dj[0] + w1 * dj[1] + w2 * dj[2]);
+    acc -= fact * fd;
+  });
+  for (int k = L.bs_ptr[i]; k < L.bs_ptr[i + 1]; ++k) {
+    int v = L.bs_list[k], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const Dual *dd = dnud + 4 * (size_t)o;
+    Dual vo = sa_diff_coef(L, P, dq, aux, o);
+    Dual w0(L.wb[3 * s]), w1(L.wb[3 * s + 1]), w2(L.wb[3 * s + 2]);
+    Dual fd1 = v1 * (w0 * di[0] + w1 * di[1] + w2 * di[2]);
+    Dual fd2 = vo * (w0 * dd[0] + w1 * dd[1] + w2 * dd[2]);
+    acc -= P.bterm == 1 ? fact * (Dual(3.0) * fd1 + fd2) : fact * Dual(4.0) * fd1;
+  }
+  Dual tdiff = acc + di[3];
+  // the source, with the laminar viscosity guard of sa_source
+  Dual mu = aux[3 * (size_t)i];
+  if (mu < Dual(0.1 / P.Re)) mu = Dual(0.1 * P.Re);
+  Dual rho(L.rec0[8 * (size_t)i], dq[DQ * (size_t)i]);
+  Dual nu(L.rec0[8 * (size_t)i + 6], dq[DQ * (size_t)i + 6]);
+  Dual src = sa_source_t<Dual>(P, rho, nu, mu, aux[3 * (size_t)i + 1], aux[3 * (size_t)i + 2], L.wd[i], L.vol[i]);
+  dsa[i] = tdiff.d + src.d;
 }
 
 // adjoint pass 1: shares of lam^T(-A) on node i's stress record (taub 9)
@@ -2438,11 +2619,10 @@ __global__ void k_visc_adjoint_record(LevelDev<double> L, DevParams P, const dou
 }
 
 // tangent: out += dvis
-__global__ void k_add_vector5(double *out, const double *add, int n) {
+__global__ void k_add_vector(double *out, const double *add, int n, int ncomp) {
   int i = gtid_x();
   if (i >= n) return;
-#pragma unroll
-  for (int c = 0; c < 5; ++c) out[(size_t)c * n + i] += add[(size_t)c * n + i];
+  for (int c = 0; c < ncomp; ++c) out[(size_t)c * n + i] += add[(size_t)c * n + i];
 }
 
 // partial derivative of the pressure drag (calculateLiftAndDrag,
@@ -2915,43 +3095,14 @@ __global__ void k_mu_record(LevelDev<R> L) {
 // makeSASourceTerm (DEF:9285-9370) for node i; returns the rhs(:,6) increment
 template <typename R>
 __device__ __forceinline__ double sa_source(LevelDev<R> &L, const DevParams &P, int i) {
-  double Re = P.Re, ioRe = 1.0 / Re;
-  double cb1 = 0.1355, cb2 = 0.622, k2 = 0.41 * 0.41, sigma = 2.0 / 3.0;
-  double cw1 = cb1 / k2 + (1.0 + cb2) / sigma, cw2 = 0.3, cw36 = 64.0, cv13 = 7.1 * 7.1 * 7.1;
-  double ct3 = 1.1, ct4 = 2.0;
-  int n = L.n;
-  double vt = L.vol[i];
-  double mc = L.u[5 * n + i];
-  mc = mc > 0.0 ? mc : 0.0;
+  const int n = L.n;
   double mu = L.mu[i];
-  if (mu < 0.1 / Re) {
-    mu = 0.1 * Re;
+  if (mu < 0.1 / P.Re) {
+    mu = 0.1 * P.Re;
     L.mu[i] = R(mu);
   }
-  double ksi = double(L.u[i]) * ioRe * mc / mu;
-  ksi = ksi > 0.001 ? ksi : 0.001;
-  double ft2 = ct3 * exp(-ct4 * ksi * ksi);
-  double k3 = ksi * ksi * ksi;
-  double d = L.wd[i];
-  d = d > 1e-10 ? d : 1e-10;
-  double md = mc / d, d2 = d * d;
-  double fv1 = k3 / (k3 + cv13);
-  double fv2 = 1.0 - ksi / (1.0 + ksi * fv1);
-  double Sc = double(L.vort[i]) + ioRe * mc * fv2 / (k2 * d2);
-  double r;
-  if (fabs(Sc) > 1e-10) {
-    r = ioRe * mc / (Sc * k2 * d2);
-    r = r < 0.0 ? 0.0 : (r > 100.0 ? 100.0 : r);
-  } else {
-    r = 100.0;
-  }
-  double r2 = r * r, r6 = r2 * r2 * r2;
-  double g = r + cw2 * (r6 - r);
-  double g2 = g * g, g6 = g2 * g2 * g2;
-  double rad = (1.0 + cw36) / (g6 + cw36);
-  double fw = rad > 1e-10 ? g * pow(rad, 1.0 / 6.0) : pow(1.0 + cw36, 1.0 / 6.0);
-  return -vt * cb1 * (1.0 - ft2) * Sc * mc + ioRe * vt * (cw1 * fw - cb1 * ft2 / k2) * md * md -
-         vt * mc * double(L.divg[i]);
+  return sa_source_t<double>(P, double(L.u[i]), double(L.u[5 * n + i]), mu, double(L.vort[i]), double(L.divg[i]),
+                             double(L.wd[i]), double(L.vol[i]));
 }
 
 // setBCsOnIncrementField (DEF:4978-5140) for node i on r[0..5]

@@ -2066,7 +2066,7 @@ int flite_monitor_noalloc(flite_ctx *c, double res[7], double maxres[7], int max
 
 // ---- tangent / adjoint of the level-1 residual ---------------------------
 namespace {
-int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_ok = false) {
+int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_ok = false, bool sa_ok = false) {
   TRY(check_ctx(c));
   const Level &L = *c->lev[0];
   const flite_params &hp = c->hp;
@@ -2075,8 +2075,10 @@ int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_o
   if (hp.flux_type != 4) return fail(FLITE_E_UNSUPPORTED, "tangent/adjoint: HLLC (fluxType=4) only");
   if (!second_order_ok && L.has_coords && hp.hllc_flux_order > 1)
     return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
-  if (hp.turbulence_model != 0)
-    return fail(FLITE_E_UNSUPPORTED, "linearisation: no turbulence model (the Spalart-Allmaras terms are not linearised)");
+  if (hp.turbulence_model != 0 && !(sa_ok && hp.turbulence_model == 1))
+    return fail(FLITE_E_UNSUPPORTED, sa_ok ? "tangent: no turbulence model or Spalart-Allmaras only"
+                                           : "this linearisation: no turbulence model (Spalart-Allmaras is "
+                                             "linearised in the tangent only)");
   if (hp.flow_type == 2) return fail(FLITE_E_UNSUPPORTED, "linearisation: fixed meshes only (no GCL terms)");
   if (!laminar_ok && hp.reynolds_number > 0.0)
     return fail(FLITE_E_UNSUPPORTED, "metric and drag derivatives: inviscid flow (ReynoldsNumber=0) only");
@@ -2092,18 +2094,29 @@ int alloc_vlin(Level &L) {
 }
 
 // tangent of the viscous term along the direction in V.src, added to V.rhs
+// With Spalart-Allmaras it also adds the tangent of the SA diffusion and
+// source to component 6 (the eddy viscosity is in the stress already).
+// Scratch per node: dq 8 | tau duals 18 | aux duals 6 | grad nu~ duals 8 |
+// out 6
 int visc_tangent(flite_ctx *c, Level &L) {
   TRY(alloc_vlin(L));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   const size_t n = L.n;
-  double *dq = L.vlin, *dvis = L.vlin + 24 * n;
-  ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin + 6 * n);
-  hipLaunchKernelGGL(k_record_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, dq);
-  hipLaunchKernelGGL(k_visc_tangent_node, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq, taud);
-  hipLaunchKernelGGL(k_visc_tangent_flux, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
-                     (const ad::Dual *)taud, dvis);
-  hipLaunchKernelGGL(k_add_vector5, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V.rhs, (const double *)dvis, L.n);
+  double *dq = L.vlin, *dvis = L.vlin + 40 * n;
+  ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin + 8 * n);
+  ad::Dual *aux = reinterpret_cast<ad::Dual *>(L.vlin + 26 * n);
+  ad::Dual *dnud = reinterpret_cast<ad::Dual *>(L.vlin + 32 * n);
+  dim3 g(nblk(L.n)), b(NTHR);
+  hipLaunchKernelGGL(k_record_tangent, g, b, 0, c->stream, V, P, V.src, dq);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)dq, taud, aux);
+  hipLaunchKernelGGL(k_visc_tangent_flux, g, b, 0, c->stream, V, P, (const double *)dq, (const ad::Dual *)taud, dvis);
+  if (P.sa) {
+    hipLaunchKernelGGL(k_sa_diff1_tangent, g, b, 0, c->stream, V, P, (const double *)dq, dnud);
+    hipLaunchKernelGGL(k_sa_tangent, g, b, 0, c->stream, V, P, (const double *)dq, (const ad::Dual *)aux,
+                       (const ad::Dual *)dnud, dvis + 5 * n);
+  }
+  hipLaunchKernelGGL(k_add_vector, g, b, 0, c->stream, V.rhs, (const double *)dvis, L.n, P.sa ? 6 : 5);
   return check_launch();
 }
 
@@ -2117,7 +2130,7 @@ int visc_adjoint(flite_ctx *c, Level &L) {
   double *taub = L.vlin + 18 * n, *ubv = L.vlin + 27 * n, *gbar = L.vlin + 30 * n, *tbar = L.vlin + 42 * n,
          *abar = L.vlin + 43 * n;
   dim3 g(nblk(L.n)), b(NTHR);
-  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, taud);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, taud, (ad::Dual *)nullptr);
   hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)taud, taub, ubv);
   hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, (const double *)taub, gbar, tbar);
   hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)gbar, (const double *)tbar,
@@ -2126,44 +2139,49 @@ int visc_adjoint(flite_ctx *c, Level &L) {
   return check_launch();
 }
 
-// host (N, 5) column-major -> level-1 device array src (6 components)
-int put_vector5(flite_ctx *c, const double *x, int ld) {
+// host (N, ncomp) column-major -> level-1 device array src (6 components;
+// ncomp = 5, or 6 with nu~ as the sixth)
+int put_vector(flite_ctx *c, const double *x, int ld, int ncomp = 5) {
   Level &L = *c->lev[0];
   if (!x || ld < L.n) return fail(FLITE_E_ARG, "bad input vector");
   std::vector<double> h(6 * (size_t)L.n, 0.0);
-  for (int k = 0; k < 5; ++k)
+  for (int k = 0; k < ncomp; ++k)
     for (int i = 0; i < L.n; ++i) h[(size_t)k * L.n + i] = x[(size_t)k * ld + i];
   return flite_field_set(c, 1, FLITE_FIELD_SOURCE, h.data(), L.n);
 }
+int put_vector5(flite_ctx *c, const double *x, int ld) { return put_vector(c, x, ld, 5); }
 
-int get_vector5(flite_ctx *c, double *y, int ld) {
+int get_vector(flite_ctx *c, double *y, int ld, int ncomp = 5) {
   Level &L = *c->lev[0];
   if (!y || ld < L.n) return fail(FLITE_E_ARG, "bad output vector");
   std::vector<double> h(6 * (size_t)L.n);
   TRY(flite_field_get(c, 1, FLITE_FIELD_RHS, h.data(), L.n));
-  for (int k = 0; k < 5; ++k)
+  for (int k = 0; k < ncomp; ++k)
     for (int i = 0; i < L.n; ++i) y[(size_t)k * ld + i] = h[(size_t)k * L.n + i];
   return 0;
 }
+int get_vector5(flite_ctx *c, double *y, int ld) { return get_vector(c, y, ld, 5); }
 
 bool second_order(const flite_ctx *c, const Level &L) {
   return L.has_coords && c->hp.hllc_flux_order > 1 && c->hp.flux_type == 4;
 }
 
-// second-order linearisation scratch: per node 58 doubles (tangent: dq 6 +
-// dual (G, B) 52; adjoint: state shares 5, (G, B) shares 25, record shares 5)
+// second-order linearisation scratch: per node DQ + 52 doubles (tangent: dq
+// DQ + dual (G, B) 52; adjoint: state shares 5, (G, B) shares 25, record
+// shares 5)
 // and 10 extremal-neighbour indices
 int alloc_lin(Level &L) {
-  if (!L.lin) TRY(L.pool.alloc(&L.lin, 58 * (size_t)std::max(1, L.n)));
+  if (!L.lin) TRY(L.pool.alloc(&L.lin, (DQ + 52) * (size_t)std::max(1, L.n)));
   if (!L.amin) TRY(L.pool.alloc(&L.amin, 10 * (size_t)std::max(1, L.n)));
   return 0;
 }
 }  // namespace
 
 int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
-  TRY(check_linearisable(c, true, true));
+  TRY(check_linearisable(c, true, true, true));
   Level &L = *c->lev[0];
-  TRY(put_vector5(c, du, ld_du));
+  const int nc = c->P.sa ? 6 : 5;
+  TRY(put_vector(c, du, ld_du, nc));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   if (second_order(c, L)) {
@@ -2173,19 +2191,21 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
     TRY(op_grad(c, L));
     TRY(alloc_lin(L));
     double *dq = L.lin;
-    ad::Dual *gd = reinterpret_cast<ad::Dual *>(L.lin + 6 * (size_t)L.n);
+    ad::Dual *gd = reinterpret_cast<ad::Dual *>(L.lin + DQ * (size_t)L.n);
     hipLaunchKernelGGL(k_record_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, dq);
     hipLaunchKernelGGL(k_grad_tangent, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, (const double *)dq, gd);
     if (L.ne > 0) {
-      if (P.nsig == 4)
-        hipLaunchKernelGGL(k_flux_tangent2<4>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
-                           (const ad::Dual *)gd);
-      else
-        hipLaunchKernelGGL(k_flux_tangent2<2>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, (const double *)dq,
-                           (const ad::Dual *)gd);
+      dim3 ge(nblk(L.ne)), be(NTHR);
+      const double *dqc = dq;
+      const ad::Dual *gdc = gd;
+      if (P.nsig == 4 && P.sa) hipLaunchKernelGGL((k_flux_tangent2<4, true>), ge, be, 0, c->stream, V, P, dqc, gdc);
+      else if (P.nsig == 4) hipLaunchKernelGGL((k_flux_tangent2<4, false>), ge, be, 0, c->stream, V, P, dqc, gdc);
+      else if (P.sa) hipLaunchKernelGGL((k_flux_tangent2<2, true>), ge, be, 0, c->stream, V, P, dqc, gdc);
+      else hipLaunchKernelGGL((k_flux_tangent2<2, false>), ge, be, 0, c->stream, V, P, dqc, gdc);
     }
   } else if (L.ne > 0) {
-    hipLaunchKernelGGL(k_flux_tangent, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+    if (P.sa) hipLaunchKernelGGL(k_flux_tangent<true>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
+    else hipLaunchKernelGGL(k_flux_tangent<false>, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src);
   }
   L.flux_partials = false;  // the tangent fluxes are per-side records
   if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_tangent, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
@@ -2196,7 +2216,7 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
     if (!second_order(c, L)) TRY(op_prep(c, L, true));
     TRY(visc_tangent(c, L));
   }
-  return get_vector5(c, out, ld_out);
+  return get_vector(c, out, ld_out, nc);
 }
 
 int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {

@@ -243,11 +243,15 @@ int flite_apply(flite_ctx *ctx, int level, int op);
  * second-order HLLC (MUSCL reconstruction with the Green-Gauss gradients and
  * LED clipping), inviscid or laminar: with ReynoldsNumber > 0 the residual
  * includes the viscous term of makeViscosityTerm (mgnsg3d_def_par.F90:
- * 5198-5893, the dissipation the smoother adds).  Spalart-Allmaras settings
- * return FLITE_E_UNSUPPORTED.  The second-order residual is piecewise smooth: at
- * a limiter switch these give the derivative of the branch the state
- * selects.  Vectors are (N, 5) column-major in host numbering with leading
- * dimension ld.  The reference has no differentiated code; these entry
+ * 5198-5893, the dissipation the smoother adds).  With Spalart-Allmaras
+ * (turbulence_model 1) flite_tangent_apply also differentiates the eddy
+ * viscosity in the stress, the nu~ convection, the SA diffusion
+ * (SADiff1Part/SADiff2Part, DEF:7436-7808) and the source (DEF:9285-9370):
+ * du and out are then (N, 6), nu~ last; flite_adjoint_apply and
+ * flite_drag_state_gradient return FLITE_E_UNSUPPORTED for SA settings.  The
+ * second-order residual is piecewise smooth: at a limiter switch these give
+ * the derivative of the branch the state selects.  Vectors are otherwise
+ * (N, 5) column-major in host numbering with leading dimension ld.  The reference has no differentiated code; these entry
  * points serve the sensitivity calculations a maintainer adds on top of it.
  * Both recompute the level's pressure, gather records and (second order)
  * gradients from the current state.

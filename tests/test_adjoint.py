@@ -352,11 +352,112 @@ def test_laminar_adjoint_dot_product(order):
 
 
 def test_linearisation_rejects_unsupported_settings():
+    """Spalart-Allmaras is linearised in the tangent only: the adjoint and
+    the drag gradient refuse it."""
     from flite3d_amd.device import DeviceError
     from cases import rans
     lv = levels(8, 2, first_cell=0.005)
     s = Solver(rans().to_struct(), list(lv))
     s.init_freestream()
+    assert s.tangent(np.zeros((6, lv[0].n_nodes))).shape == (6, lv[0].n_nodes)
     with pytest.raises(DeviceError):
-        s.tangent(np.zeros((5, lv[0].n_nodes)))
+        s.adjoint(np.zeros((5, lv[0].n_nodes)))
+    with pytest.raises(DeviceError):
+        s.drag_gradient()
+    s.close()
+
+
+# ---- RANS-SA: the Spalart-Allmaras terms in the tangent ----
+def make_rans(n, order, re=1.0e4):
+    """A smooth state with nu~ of a few times the laminar viscosity (chi of
+    a few units, where the eddy viscosity, the source's fv1/fv2/fw and the
+    diffusion all vary), on the freestream before the wall conditions."""
+    from cases import rans
+    lv = levels(n, 2, jitter=0.15)
+    iv = rans(hllcfluxorder=order, reynoldsnumber=re)
+    s = Solver(iv.to_struct(), list(lv))
+    s.init_freestream()
+    inf = np.array(iv.inflowfield[0])
+    base = np.zeros((6, lv[0].n_nodes))
+    base[:5] = inf[:5, None]
+    base[1:4] *= inf[0]
+    u = smooth_state(lv, base)
+    x, y, z = lv[0].coords
+    u[5] = 3.0 * (1.0 + 0.3 * np.sin(1.7 * x + 0.9 * y - 1.3 * z + 0.2))
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    return s, lv, u
+
+
+def residual_rans(s, u):
+    """The RANS-SA residual the tangent linearises: makeRHS and the SA
+    convection and source (RHS), plus the viscous dissipation (VISCOUS, with
+    the eddy viscosity) on the mean flow and the SA diffusion (SA_DIFFUSION)
+    on nu~."""
+    s.set(1, "U", u)
+    s.apply(1, "PRESSURE")
+    s.apply(1, "VISCOUS")
+    s.apply(1, "SA_DIFFUSION")
+    s.apply(1, "RHS")
+    r = s.get(1, "RHS")[:6].copy()
+    r[:5] += s.get(1, "DISSIPATION")[:5]
+    r[5] += s.get(1, "TURB_DIFFUSION")[0]
+    return r
+
+
+def directions6(rng, u, k):
+    out = []
+    for _ in range(k):
+        d = rng.standard_normal((6, u.shape[1])) * np.abs(u[:6]).max(axis=1, keepdims=True)
+        out.append(d / np.linalg.norm(d))
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_rans_tangent_matches_finite_differences(order):
+    """Every entry of the 6-component tangent equals the central difference
+    of the RANS-SA residual, or a one-sided one where a switch (limiter,
+    the source's clips) lies within H."""
+    print(f"[case] RANS-SA tangent FD order={order}", flush=True)
+    s, lv, u = make_rans(8, order)
+    rng = np.random.default_rng(41)
+    one_sided = 0
+    for du in directions6(rng, u, 3):
+        jt = s.tangent(du)
+        assert jt.shape == (6, lv[0].n_nodes)
+        r0 = residual_rans(s, u)
+        up, um = u.copy(), u.copy()
+        up[:6] += H * du
+        um[:6] -= H * du
+        rp, rm = residual_rans(s, up), residual_rans(s, um)
+        s.set(1, "U", u)
+        for c in range(6):  # each component on its own scale (nu~'s residual is small)
+            tol = 1e-6 * np.abs(jt[c]).max() + 1e-300
+            central = np.abs((rp[c] - rm[c]) / (2 * H) - jt[c]) <= tol
+            fwd = np.abs((rp[c] - r0[c]) / H - jt[c]) <= tol
+            bwd = np.abs((r0[c] - rm[c]) / H - jt[c]) <= tol
+            ok = central | fwd | bwd
+            assert np.all(ok), (c, np.argwhere(~ok)[:5],
+                                np.abs((rp[c] - rm[c]) / (2 * H) - jt[c]).max() / np.abs(jt[c]).max())
+            one_sided += int((~central).sum())
+    assert one_sided <= 0.002 * jt.size * 3, one_sided
+    s.close()
+
+
+def test_rans_tangent_couples_nu_and_mean_flow():
+    """A nu~-only direction moves the mean flow's residual (eddy viscosity
+    in the stress) and nu~'s (convection, diffusion, source); a mean-flow
+    direction moves nu~'s residual (convection velocity, vorticity,
+    laminar viscosity)."""
+    print("[case] RANS-SA tangent coupling", flush=True)
+    s, lv, u = make_rans(8, 1)
+    rng = np.random.default_rng(43)
+    dn = np.zeros((6, lv[0].n_nodes))
+    dn[5] = rng.standard_normal(lv[0].n_nodes)
+    jn = s.tangent(dn)
+    assert np.abs(jn[1:5]).max() > 0 and np.abs(jn[5]).max() > 0
+    dm = np.zeros((6, lv[0].n_nodes))
+    dm[1:4] = rng.standard_normal((3, lv[0].n_nodes)) * np.abs(u[1:4]).max()
+    jm = s.tangent(dm)
+    assert np.abs(jm[5]).max() > 0
     s.close()
