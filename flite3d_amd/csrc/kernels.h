@@ -1627,6 +1627,45 @@ __global__ void k_adjoint_gather(LevelDev<double> L, const double *abuf, double 
   out[5 * n + i] = 0;
 }
 
+// adjoint of the SA convection (the side flux's component 6, makeSARHS
+// DEF:6873-6880, first order at both orders: the node values of nu~ and
+// the velocities): node i's share of sum over sides of (lam6_ea - lam6_eb) f6
+// from six dual passes per side seeded on its own (U, nu~), added to out
+__global__ void k_sa_conv_adjoint(LevelDev<double> L, DevParams P, const double *lam, double *out) {
+  using ad::Dual;
+  int i = gtid_x();
+  if (i >= L.n) return;
+  const size_t n = L.n;
+  double ui[6], acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) ui[c] = L.u[c * n + i];
+  for_edges<1>(L, i, 0, [&](int e, int role) {
+    const int o = role ? L.ea[e] : L.eb[e];
+    const double dl = lam[5 * n + L.ea[e]] - lam[5 * n + L.eb[e]];
+    double uo[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) uo[c] = L.u[c * n + o];
+    const Dual wx(L.w[3 * e]), wy(L.w[3 * e + 1]), wz(L.w[3 * e + 2]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      Dual Ui[5], Uo[5], S[8], O[8];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        Ui[c] = Dual(ui[c], k == c ? 1.0 : 0.0);
+        Uo[c] = Dual(uo[c]);
+      }
+      state_record(Ui, P.gamma, S);
+      state_record(Uo, P.gamma, O);
+      S[6] = Dual(ui[5], k == 5 ? 1.0 : 0.0);
+      O[6] = Dual(uo[5]);
+      const Dual f = role ? sa_side_convection(O, S, wx, wy, wz) : sa_side_convection(S, O, wx, wy, wz);
+      acc[k] += dl * f.d;
+    }
+  });
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[c * n + i] += acc[c];
+}
+
 // Metric adjoint: the derivative of lam^T R with respect to the side
 // weights at the current state, for the residual of the tangent and adjoint
 // (first-order HLLC, inviscid).  The interior flux depends on w through
@@ -2526,29 +2565,44 @@ __global__ void k_visc_adjoint_tau(LevelDev<double> L, DevParams P, const double
 }
 
 // adjoint pass 2: the stress law transposed at each node, gbar[12] on the
-// Green-Gauss sums and tbar on T (through the viscosity)
-__global__ void k_visc_adjoint_node(LevelDev<double> L, DevParams P, const double *taub, double *gbar,
-                                    double *tbar) {
+// Green-Gauss sums and tbar on T (through the viscosity).  With
+// Spalart-Allmaras the stress reads the eddy viscosity of (rho, nu~) and
+// the node's SA source (multiplier lam(6)) reads rho, nu~, the laminar
+// viscosity of T and the vorticity and divergence of the same sums: two more
+// passes, rnbar[2 i ..] = the shares on (rho, nu~)
+__global__ void k_visc_adjoint_node(LevelDev<double> L, DevParams P, const double *lam, const double *taub,
+                                    double *gbar, double *tbar, double *rnbar) {
   using ad::Dual;
   int i = gtid_x();
   if (i >= L.n) return;
-  Dual ai[4], g[12];
-  visc_rec(L, P, nullptr, i, ai);
+  Dual ai[4], g[12], rho, nu;
+  visc_rec(L, P, nullptr, i, ai, &rho, &nu);
   visc_grad_sums(L, P, nullptr, i, ai, g);
   const double *tb = taub + 9 * (size_t)i;
   const bool wall = visc_wall(L, i);
+  const double l6 = P.sa ? lam[5 * (size_t)L.n + i] : 0.0;
+  const int passes = P.sa ? 15 : 13;
 #pragma unroll 1
-  for (int k = 0; k < 13; ++k) {
+  for (int k = 0; k < passes; ++k) {
     Dual gs[12], t[9];
 #pragma unroll
     for (int c = 0; c < 12; ++c) gs[c] = Dual(g[c].v, k == c ? 1.0 : 0.0);
-    Dual T(ai[3].v, k == 12 ? 1.0 : 0.0);
-    visc_tau(P, gs, L.vol[i], T, wall, L.nrm + 3 * (size_t)i, t);
+    Dual T(ai[3].v, k == 12 ? 1.0 : 0.0), R(rho.v, k == 13 ? 1.0 : 0.0), N(nu.v, k == 14 ? 1.0 : 0.0);
+    visc_tau(P, gs, L.vol[i], T, wall, L.nrm + 3 * (size_t)i, t, R, N);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc += tb[c] * t[c].d;
+    if (P.sa) {
+      // the source with sa_source's laminar viscosity guard
+      Dual mu = sutherland(P, T);
+      if (mu < Dual(0.1 / P.Re)) mu = Dual(0.1 * P.Re);
+      Dual vort, divg;
+      visc_vort_divg(gs, L.vol[i], vort, divg);
+      acc += l6 * sa_source_t<Dual>(P, R, N, mu, vort, divg, L.wd[i], L.vol[i]).d;
+    }
     if (k < 12) gbar[12 * (size_t)i + k] = acc;
-    else tbar[i] = acc;
+    else if (k == 12) tbar[i] = acc;
+    else rnbar[2 * (size_t)i + (k - 13)] = acc;
   }
 }
 
@@ -2616,6 +2670,83 @@ __global__ void k_visc_adjoint_record(LevelDev<double> L, DevParams P, const dou
     Dual T = Dual(P.gamma / (P.gamma - 1.0)) * A[5] / A[0];
     out[k * n + i] += ab[0] * A[1].d + ab[1] * A[2].d + ab[2] * A[3].d + ab[3] * T.d;
   }
+}
+
+// adjoint of the SA diffusion (k_sa_diff1_tangent / k_sa_tangent
+// transposed), multipliers m = lam(6).  Forward, node k: g_k = sum_j
+// (nu_k + nu_j) w_kj + boundary terms, d_k = g_k / V_k, the nodal term
+// -f1 g_k.d_k, and the side terms -f2 (v_k w.d_k + v_j w.d_j) with
+// v = Re mu_lam / rho + nu~.  Pass 1, node k: W_k = sum_j (m_k - m_j) w_kj
+// (+ (3 m_k + m_o) w_b, or 4 m_k w_b), the share on d_k is -f2 v_k W_k and
+// on v_k -f2 W_k.d_k; gnbar[3 k] = the share on g_k; v_k's share goes to
+// rho and nu~ (rnbar) and to T through Sutherland's law (tbar).
+__global__ void k_sa_diff_adjoint1(LevelDev<double> L, DevParams P, const double *lam, double *gnbar, double *tbar,
+                                   double *rnbar) {
+  using ad::Dual;
+  int k = gtid_x();
+  if (k >= L.n) return;
+  const size_t n = L.n;
+  const double *qk = L.rec0 + 8 * (size_t)k;
+  const double mk = lam[5 * n + k], nk = qk[6];
+  double g[3] = {0, 0, 0}, W[3] = {0, 0, 0};
+  for_edges_w<1>(L, k, 0, [&](int j, double wx, double wy, double wz) {
+    const double s = nk + L.rec0[8 * (size_t)j + 6], dm = mk - lam[5 * n + j];
+    g[0] += wx * s;
+    g[1] += wy * s;
+    g[2] += wz * s;
+    W[0] += wx * dm;
+    W[1] += wy * dm;
+    W[2] += wz * dm;
+  });
+  for (int q = L.bs_ptr[k]; q < L.bs_ptr[k + 1]; ++q) {
+    int v = L.bs_list[q], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const double c = P.bterm == 1 ? 3.0 * nk + L.rec0[8 * (size_t)o + 6] : 4.0 * nk;
+    const double cm = P.bterm == 1 ? 3.0 * mk + lam[5 * n + o] : 4.0 * mk;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      g[d] += c * L.wb[3 * s + d];
+      W[d] += cm * L.wb[3 * s + d];
+    }
+  }
+  const double vt = 1.0 / L.vol[k];
+  const double f1 = 1.0 / P.Re * 0.622 / (2.0 / 3.0), f2 = 1.0 / P.Re / (2.0 / 3.0);
+  const double rho = qk[0], T = P.gamma / (P.gamma - 1.0) * qk[5] / rho;
+  const Dual mu = sutherland(P, Dual(T, 1.0));
+  const double vk = P.Re * mu.v / rho + nk;
+  const double vb = -f2 * (W[0] * g[0] + W[1] * g[1] + W[2] * g[2]) * vt;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gnbar[3 * (size_t)k + d] = vt * (-f2 * vk * W[d] - 2.0 * f1 * mk * g[d]);
+  tbar[k] += vb * P.Re / rho * mu.d;
+  rnbar[2 * (size_t)k] -= vb * P.Re * mu.v / (rho * rho);
+  rnbar[2 * (size_t)k + 1] += vb;
+}
+
+// pass 2, node m: the grad nu~ sums transposed (node m's nu~ enters its own
+// g with sum_j w_mj + 3 (or 4) sum_b w_b, its neighbours' g with w_jm =
+// -w_mj and, boundaryTerm 1, its boundary partners' g with w_b), plus the
+// (rho, nu~) shares; added to out
+__global__ void k_sa_diff_adjoint2(LevelDev<double> L, DevParams P, const double *gnbar, const double *rnbar,
+                                   double *out) {
+  int m = gtid_x();
+  if (m >= L.n) return;
+  const size_t n = L.n;
+  const double *gm = gnbar + 3 * (size_t)m;
+  double r = rnbar[2 * (size_t)m + 1];
+  for_edges_w<1>(L, m, 0, [&](int j, double wx, double wy, double wz) {
+    const double *gj = gnbar + 3 * (size_t)j;
+    r += wx * (gm[0] - gj[0]) + wy * (gm[1] - gj[1]) + wz * (gm[2] - gj[2]);
+  });
+  for (int q = L.bs_ptr[m]; q < L.bs_ptr[m + 1]; ++q) {
+    int v = L.bs_list[q], s = v >> 1;
+    int o = (v & 1) ? L.ba[s] : L.bb[s];
+    const double wx = L.wb[3 * s], wy = L.wb[3 * s + 1], wz = L.wb[3 * s + 2];
+    const double *go = gnbar + 3 * (size_t)o;
+    const double self = wx * gm[0] + wy * gm[1] + wz * gm[2];
+    r += P.bterm == 1 ? 3.0 * self + (wx * go[0] + wy * go[1] + wz * go[2]) : 4.0 * self;
+  }
+  out[m] += rnbar[2 * (size_t)m];
+  out[5 * n + m] += r;
 }
 
 // tangent: out += dvis

@@ -2076,9 +2076,9 @@ int check_linearisable(flite_ctx *c, bool second_order_ok = true, bool laminar_o
   if (!second_order_ok && L.has_coords && hp.hllc_flux_order > 1)
     return fail(FLITE_E_UNSUPPORTED, "metric derivatives: first-order HLLC (HLLCFluxOrder=1) only");
   if (hp.turbulence_model != 0 && !(sa_ok && hp.turbulence_model == 1))
-    return fail(FLITE_E_UNSUPPORTED, sa_ok ? "tangent: no turbulence model or Spalart-Allmaras only"
+    return fail(FLITE_E_UNSUPPORTED, sa_ok ? "tangent/adjoint: no turbulence model or Spalart-Allmaras only"
                                            : "this linearisation: no turbulence model (Spalart-Allmaras is "
-                                             "linearised in the tangent only)");
+                                             "linearised in the tangent and adjoint only)");
   if (hp.flow_type == 2) return fail(FLITE_E_UNSUPPORTED, "linearisation: fixed meshes only (no GCL terms)");
   if (!laminar_ok && hp.reynolds_number > 0.0)
     return fail(FLITE_E_UNSUPPORTED, "metric and drag derivatives: inviscid flow (ReynoldsNumber=0) only");
@@ -2121,6 +2121,10 @@ int visc_tangent(flite_ctx *c, Level &L) {
 }
 
 // adjoint of the viscous term against the multipliers in V.src, added to V.rhs
+// With Spalart-Allmaras it adds the transposes of the eddy viscosity in the
+// stress, the SA source and the SA diffusion (multipliers lam(6)).
+// Scratch per node: tau duals 18 | taub 9 | ubv 3 | gbar 12 | tbar 1 |
+// abar 4 | (rho, nu~) shares 2 | grad nu~ shares 3
 int visc_adjoint(flite_ctx *c, Level &L) {
   TRY(alloc_vlin(L));
   DevParams P = c->P;
@@ -2128,14 +2132,18 @@ int visc_adjoint(flite_ctx *c, Level &L) {
   const size_t n = L.n;
   ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin);
   double *taub = L.vlin + 18 * n, *ubv = L.vlin + 27 * n, *gbar = L.vlin + 30 * n, *tbar = L.vlin + 42 * n,
-         *abar = L.vlin + 43 * n;
+         *abar = L.vlin + 43 * n, *rnbar = L.vlin + 47 * n, *gnbar = L.vlin + 49 * n;
   dim3 g(nblk(L.n)), b(NTHR);
   hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, taud, (ad::Dual *)nullptr);
   hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)taud, taub, ubv);
-  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, (const double *)taub, gbar, tbar);
+  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, V.src, (const double *)taub, gbar, tbar, rnbar);
+  if (P.sa) hipLaunchKernelGGL(k_sa_diff_adjoint1, g, b, 0, c->stream, V, P, V.src, gnbar, tbar, rnbar);
   hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)gbar, (const double *)tbar,
                      (const double *)ubv, abar);
   hipLaunchKernelGGL(k_visc_adjoint_record, g, b, 0, c->stream, V, P, (const double *)abar, V.rhs);
+  if (P.sa)
+    hipLaunchKernelGGL(k_sa_diff_adjoint2, g, b, 0, c->stream, V, P, (const double *)gnbar, (const double *)rnbar,
+                       V.rhs);
   return check_launch();
 }
 
@@ -2220,8 +2228,9 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
 }
 
 int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
-  TRY(check_linearisable(c, true, true));
+  TRY(check_linearisable(c, true, true, true));
   Level &L = *c->lev[0];
+  const int nc = c->P.sa ? 6 : 5;
   if (second_order(c, L)) {
     // records, gradients and bounds of the current state, the bounds'
     // extremal neighbours, then the per-side shares and their transposes
@@ -2229,7 +2238,7 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
     TRY(op_grad(c, L));
     TRY(alloc_lin(L));
     if (!L.abuf2) TRY(L.pool.alloc(&L.abuf2, (size_t)LIN_SIDE * std::max(1, L.ne)));
-    TRY(put_vector5(c, lam, ld_lam));
+    TRY(put_vector(c, lam, ld_lam, nc));
     DevParams P = c->P;
     LevelDev<double> V = L.view<double>();
     double *ub = L.lin, *gbn = L.lin + 5 * (size_t)L.n, *qb = L.lin + 30 * (size_t)L.n;
@@ -2246,23 +2255,25 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
                        (const int *)L.amin, qb);
     hipLaunchKernelGGL(k_record_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, (const double *)ub,
                        (const double *)qb, V.rhs);
+    if (P.sa) hipLaunchKernelGGL(k_sa_conv_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, V.rhs);
     TRY(check_launch());
     if (laminar(c)) TRY(visc_adjoint(c, L));
-    return get_vector5(c, out, ld_out);
+    return get_vector(c, out, ld_out, nc);
   }
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
-  TRY(put_vector5(c, lam, ld_lam));
+  TRY(put_vector(c, lam, ld_lam, nc));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   if (L.ne > 0) hipLaunchKernelGGL(k_flux_adjoint, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf);
   if (L.nbs > 0) hipLaunchKernelGGL(k_bflux_adjoint, dim3(nblk(L.nbs)), dim3(NTHR), 0, c->stream, V, P, V.src);
   hipLaunchKernelGGL(k_adjoint_gather, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, L.abuf, V.rhs);
+  if (P.sa) hipLaunchKernelGGL(k_sa_conv_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, V.rhs);
   TRY(check_launch());
   if (laminar(c)) {
     TRY(op_prep(c, L, true));
     TRY(visc_adjoint(c, L));
   }
-  return get_vector5(c, out, ld_out);
+  return get_vector(c, out, ld_out, nc);
 }
 
 int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) {

@@ -219,10 +219,12 @@ class Solver:
         return out
 
     def adjoint(self, lam):
-        """(dR/dU)^T lam of the level-1 residual at the current state."""
+        """(dR/dU)^T lam of the level-1 residual at the current state: 5
+        components, or 6 (nu~ last) with Spalart-Allmaras."""
         n = self.n(1)
-        x = self._vec5(lam)
-        out = np.zeros((5, n))
+        nc = 6 if self.params[0].turbulence_model == 1 else 5
+        x = np.ascontiguousarray(np.asarray(lam, dtype=np.float64)[:nc].reshape(nc, n))
+        out = np.zeros((nc, n))
         D = ctypes.POINTER(ctypes.c_double)
         self._check(self.L.flite_adjoint_apply(self.ctx, x.ctypes.data_as(D), n, out.ctypes.data_as(D), n))
         return out

@@ -244,11 +244,11 @@ int flite_apply(flite_ctx *ctx, int level, int op);
  * LED clipping), inviscid or laminar: with ReynoldsNumber > 0 the residual
  * includes the viscous term of makeViscosityTerm (mgnsg3d_def_par.F90:
  * 5198-5893, the dissipation the smoother adds).  With Spalart-Allmaras
- * (turbulence_model 1) flite_tangent_apply also differentiates the eddy
- * viscosity in the stress, the nu~ convection, the SA diffusion
- * (SADiff1Part/SADiff2Part, DEF:7436-7808) and the source (DEF:9285-9370):
- * du and out are then (N, 6), nu~ last; flite_adjoint_apply and
- * flite_drag_state_gradient return FLITE_E_UNSUPPORTED for SA settings.  The
+ * (turbulence_model 1) the tangent and the adjoint also differentiate the
+ * eddy viscosity in the stress (DEF:5556-5566), the nu~ convection
+ * (makeSARHS, DEF:6809-6950), the SA diffusion (DEF:7436-7808) and the
+ * source (DEF:9285-9370): du, lam and out are then (N, 6), nu~ last;
+ * flite_drag_state_gradient returns FLITE_E_UNSUPPORTED for SA settings.  The
  * second-order residual is piecewise smooth: at a limiter switch these give
  * the derivative of the branch the state selects.  Vectors are otherwise
  * (N, 5) column-major in host numbering with leading dimension ld.  The reference has no differentiated code; these entry

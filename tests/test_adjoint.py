@@ -352,16 +352,15 @@ def test_laminar_adjoint_dot_product(order):
 
 
 def test_linearisation_rejects_unsupported_settings():
-    """Spalart-Allmaras is linearised in the tangent only: the adjoint and
-    the drag gradient refuse it."""
+    """Spalart-Allmaras is linearised in the tangent and the adjoint (6
+    components); the drag gradient refuses it."""
     from flite3d_amd.device import DeviceError
     from cases import rans
     lv = levels(8, 2, first_cell=0.005)
     s = Solver(rans().to_struct(), list(lv))
     s.init_freestream()
     assert s.tangent(np.zeros((6, lv[0].n_nodes))).shape == (6, lv[0].n_nodes)
-    with pytest.raises(DeviceError):
-        s.adjoint(np.zeros((5, lv[0].n_nodes)))
+    assert s.adjoint(np.zeros((6, lv[0].n_nodes))).shape == (6, lv[0].n_nodes)
     with pytest.raises(DeviceError):
         s.drag_gradient()
     s.close()
@@ -460,4 +459,58 @@ def test_rans_tangent_couples_nu_and_mean_flow():
     dm[1:4] = rng.standard_normal((3, lv[0].n_nodes)) * np.abs(u[1:4]).max()
     jm = s.tangent(dm)
     assert np.abs(jm[5]).max() > 0
+    s.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_rans_adjoint_dot_product(order):
+    """<lam, J du> = <J^T lam, du> for the RANS-SA residual, over the whole
+    vectors and over blocks that isolate the couplings: nu~'s row against
+    each state component (convection, diffusion and source) and the mean
+    flow's rows against nu~ (the eddy viscosity)."""
+    print(f"[case] RANS-SA adjoint dot product order={order}", flush=True)
+    s, lv, u = make_rans(8, order)
+    rng = np.random.default_rng(47)
+    blocks = [(range(6), range(6)), ([5], range(6)), ([5], [0]), ([5], [1, 2, 3]), ([5], [4]), ([5], [5]),
+              ([1, 2, 3, 4], [5]), (range(5), range(5))]
+    for rows, cols in blocks:
+        du, lam = directions6(rng, u, 2)
+        du[[c for c in range(6) if c not in cols]] = 0.0
+        lam[[c for c in range(6) if c not in rows]] = 0.0
+        a = np.sum(lam * s.tangent(du))
+        b = np.sum(s.adjoint(lam) * du)
+        assert abs(a) > 0, (rows, cols)
+        assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)), (list(rows), list(cols), a, b)
+    s.close()
+
+
+def test_rans_adjoint_columns_are_jacobian_rows():
+    """J^T e_k equals row k of J for nu~'s row and a momentum row, entry by
+    entry against tangent columns over the nodes of k's two-ring."""
+    print("[case] RANS-SA adjoint vs explicit Jacobian rows", flush=True)
+    s, lv, u = make_rans(6, 2)
+    n = u.shape[1]
+    a, b = lv[0].side_index
+    nbr = [set() for _ in range(n)]
+    for p, q in zip(a, b):
+        nbr[p].add(q)
+        nbr[q].add(p)
+    rng = np.random.default_rng(53)
+    for node in rng.choice(n, 2, replace=False):
+        ring = set(nbr[node]) | {node}
+        ring2 = set().union(*(nbr[k] for k in ring)) | ring
+        for comp in (5, 2):
+            e = np.zeros((6, n))
+            e[comp, node] = 1.0
+            row = s.adjoint(e)
+            for j in rng.choice(sorted(ring2), 4, replace=False):
+                for c in range(6):
+                    d = np.zeros((6, n))
+                    d[c, j] = 1.0
+                    t = s.tangent(d)[comp, node]
+                    assert abs(t - row[c, j]) <= 1e-12 * (1 + abs(row[c, j])), (node, comp, j, c, t, row[c, j])
+            # no entries outside the two-ring
+            outside = np.ones(n, bool)
+            outside[sorted(ring2)] = False
+            assert np.all(row[:, outside] == 0.0)
     s.close()
