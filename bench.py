@@ -350,7 +350,7 @@ def second_order_checks(lv, cycles, ndir, h, smooth=False, raw=None, nxdir=0, hx
         lam = rng.standard_normal((5, n))
         s.set(1, "U", u)
         t0 = time.perf_counter()
-        gw, gwb, gv, gxd = s.mesh_adjoint(lam)
+        gw, gwb, gv, gxd, _ = s.mesh_adjoint(lam)
         _, gjwb = s.drag_metric_gradient()
         xyz, tets, btri, bind = raw
         gx = mesh.mesh_vjp(xyz, tets, btri, bind, -gw, gjwb - gwb, gvol=-gv, gxn=-gxd)

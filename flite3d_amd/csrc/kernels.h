@@ -1390,9 +1390,10 @@ __global__ void k_jst_flux(LevelDev<R> L, DevParams P) {
 
 // makeSASourceTerm (DEF:9285-9370) of one node on any arithmetic type: the
 // density, nu~ (u(:,6)), the laminar viscosity (after the caller's guard),
-// vorticity and divergence, wall distance and volume
-template <typename T>
-__device__ __forceinline__ T sa_source_t(const DevParams &P, T rho, T nu, T mu, T vort, T divg, double wd, double vt) {
+// vorticity and divergence, wall distance and volume (TW: double, or dual
+// numbers for the derivatives with respect to the mesh)
+template <typename T, typename TW = double>
+__device__ __forceinline__ T sa_source_t(const DevParams &P, T rho, T nu, T mu, T vort, T divg, TW wd, TW vt) {
   double Re = P.Re, ioRe = 1.0 / Re;
   double cb1 = 0.1355, cb2 = 0.622, k2 = 0.41 * 0.41, sigma = 2.0 / 3.0;
   double cw1 = cb1 / k2 + (1.0 + cb2) / sigma, cw2 = 0.3, cw36 = 64.0, cv13 = 7.1 * 7.1 * 7.1;
@@ -1403,16 +1404,16 @@ __device__ __forceinline__ T sa_source_t(const DevParams &P, T rho, T nu, T mu, 
   ksi = ksi > T(0.001) ? ksi : T(0.001);
   T ft2 = T(ct3) * exp(T(-ct4) * ksi * ksi);
   T k3 = ksi * ksi * ksi;
-  double d = wd;
-  d = d > 1e-10 ? d : 1e-10;
-  T md = mc / T(d);
-  double d2 = d * d;
+  T d = T(wd);
+  d = d > T(1e-10) ? d : T(1e-10);
+  T md = mc / d;
+  T d2 = d * d;
   T fv1 = k3 / (k3 + T(cv13));
   T fv2 = T(1.0) - ksi / (T(1.0) + ksi * fv1);
-  T Sc = vort + T(ioRe) * mc * fv2 / T(k2 * d2);
+  T Sc = vort + T(ioRe) * mc * fv2 / (T(k2) * d2);
   T r;
   if (fabs(Sc) > T(1e-10)) {
-    r = T(ioRe) * mc / (Sc * T(k2) * T(d2));
+    r = T(ioRe) * mc / (Sc * T(k2) * d2);
     r = r < T(0.0) ? T(0.0) : (r > T(100.0) ? T(100.0) : r);
   } else {
     r = T(100.0);
@@ -1422,7 +1423,7 @@ __device__ __forceinline__ T sa_source_t(const DevParams &P, T rho, T nu, T mu, 
   T g2 = g * g, g6 = g2 * g2 * g2;
   T rad = T(1.0 + cw36) / (g6 + T(cw36));
   T fw = rad > T(1e-10) ? g * pow(rad, 1.0 / 6.0) : T(::pow(1.0 + cw36, 1.0 / 6.0));
-  return T(-vt) * T(cb1) * (T(1.0) - ft2) * Sc * mc + T(ioRe * vt) * (T(cw1) * fw - T(cb1) * ft2 / T(k2)) * md * md -
+  return -T(vt) * T(cb1) * (T(1.0) - ft2) * Sc * mc + T(ioRe) * T(vt) * (T(cw1) * fw - T(cb1) * ft2 / T(k2)) * md * md -
          T(vt) * mc * divg;
 }
 
@@ -2185,7 +2186,7 @@ __global__ void k_vol_mesh_adjoint(LevelDev<double> L, const double *gbn, double
 // follow ViscNodePart / ViscFluxPart operation for operation, on any
 // arithmetic type.
 // ---------------------------------------------------------------------------
-constexpr int VLIN = 56;  // scratch doubles per node of the viscous linearisation
+constexpr int VLIN = 66;  // scratch doubles per node of the viscous linearisation
 
 // Sutherland's laminar viscosity of max(T, 0.001) (ViscNodePart::finish)
 template <typename T>
@@ -2198,8 +2199,8 @@ __device__ __forceinline__ T sutherland(const DevParams &P, T Traw) {
 }
 
 // vorticity magnitude and divergence from the Green-Gauss sums (ViscNodePart::finish)
-template <typename T>
-__device__ __forceinline__ void visc_vort_divg(const T *g, double vol, T &vort, T &divg) {
+template <typename T, typename TV = double>
+__device__ __forceinline__ void visc_vort_divg(const T *g, TV vol, T &vort, T &divg) {
   T vt = T(1.0) / T(vol);
   T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt;
   T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt;
@@ -2209,10 +2210,11 @@ __device__ __forceinline__ void visc_vort_divg(const T *g, double vol, T &vort, 
 }
 
 // stress and heat flux of a node; with Spalart-Allmaras the eddy viscosity
-// of (rho, nu~) is added to both (DEF:5556-5566)
-template <typename T>
-__device__ __forceinline__ void visc_tau(const DevParams &P, const T *g, double vol, T Traw, bool wall,
-                                         const double *nrm, T *t, T rho = T(0.0), T nu = T(0.0)) {
+// of (rho, nu~) is added to both (DEF:5556-5566).  The volume and the wall
+// normal are of type TV (double, or dual numbers for the mesh derivatives).
+template <typename T, typename TV = double>
+__device__ __forceinline__ void visc_tau(const DevParams &P, const T *g, TV vol, T Traw, bool wall, const TV *nrm,
+                                         T *t, T rho = T(0.0), T nu = T(0.0)) {
   T vt = T(1.0) / T(vol);
   T dudx = g[0] * vt, dvdx = g[1] * vt, dwdx = g[2] * vt, dTdx = g[3] * vt;
   T dudy = g[4] * vt, dvdy = g[5] * vt, dwdy = g[6] * vt, dTdy = g[7] * vt;
@@ -2569,9 +2571,13 @@ __global__ void k_visc_adjoint_tau(LevelDev<double> L, DevParams P, const double
 // Spalart-Allmaras the stress reads the eddy viscosity of (rho, nu~) and
 // the node's SA source (multiplier lam(6)) reads rho, nu~, the laminar
 // viscosity of T and the vorticity and divergence of the same sums: two more
-// passes, rnbar[2 i ..] = the shares on (rho, nu~)
+// passes, rnbar[2 i ..] = the shares on (rho, nu~).  lam may be null (no
+// source term: the drag's friction alone drives taub).  With mbar (5 per
+// node) also the shares on the mesh quantities the node reads: its volume,
+// its wall normal (wall nodes, the adiabatic projection) and, with
+// Spalart-Allmaras, its wall distance (the source).
 __global__ void k_visc_adjoint_node(LevelDev<double> L, DevParams P, const double *lam, const double *taub,
-                                    double *gbar, double *tbar, double *rnbar) {
+                                    double *gbar, double *tbar, double *rnbar, double *mbar) {
   using ad::Dual;
   int i = gtid_x();
   if (i >= L.n) return;
@@ -2580,30 +2586,47 @@ __global__ void k_visc_adjoint_node(LevelDev<double> L, DevParams P, const doubl
   visc_grad_sums(L, P, nullptr, i, ai, g);
   const double *tb = taub + 9 * (size_t)i;
   const bool wall = visc_wall(L, i);
-  const double l6 = P.sa ? lam[5 * (size_t)L.n + i] : 0.0;
-  const int passes = P.sa ? 15 : 13;
+  const double l6 = (P.sa && lam) ? lam[5 * (size_t)L.n + i] : 0.0;
+  const double *nr = L.nrm + 3 * (size_t)i;
+  double m[5] = {0, 0, 0, 0, 0};
+  // seeds: 0-11 the sums, 12 T, 13 rho, 14 nu~, 15 V, 16-18 the normal,
+  // 19 the wall distance
+  const int passes = mbar ? 20 : (P.sa ? 15 : 13);
 #pragma unroll 1
   for (int k = 0; k < passes; ++k) {
+    if ((k == 13 || k == 14 || k == 19) && !P.sa) continue;
+    if (k >= 16 && k <= 18 && !wall) continue;
     Dual gs[12], t[9];
 #pragma unroll
     for (int c = 0; c < 12; ++c) gs[c] = Dual(g[c].v, k == c ? 1.0 : 0.0);
-    Dual T(ai[3].v, k == 12 ? 1.0 : 0.0), R(rho.v, k == 13 ? 1.0 : 0.0), N(nu.v, k == 14 ? 1.0 : 0.0);
-    visc_tau(P, gs, L.vol[i], T, wall, L.nrm + 3 * (size_t)i, t, R, N);
+    const Dual T(ai[3].v, k == 12 ? 1.0 : 0.0), R(rho.v, k == 13 ? 1.0 : 0.0), N(nu.v, k == 14 ? 1.0 : 0.0);
+    const Dual Vd(L.vol[i], k == 15 ? 1.0 : 0.0), Wd(L.wd[i], k == 19 ? 1.0 : 0.0);
+    const Dual nd[3] = {Dual(nr[0], k == 16 ? 1.0 : 0.0), Dual(nr[1], k == 17 ? 1.0 : 0.0),
+                        Dual(nr[2], k == 18 ? 1.0 : 0.0)};
+    visc_tau<Dual, Dual>(P, gs, Vd, T, wall, nd, t, R, N);
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc += tb[c] * t[c].d;
-    if (P.sa) {
+    if (l6 != 0.0) {
       // the source with sa_source's laminar viscosity guard
       Dual mu = sutherland(P, T);
       if (mu < Dual(0.1 / P.Re)) mu = Dual(0.1 * P.Re);
       Dual vort, divg;
-      visc_vort_divg(gs, L.vol[i], vort, divg);
-      acc += l6 * sa_source_t<Dual>(P, R, N, mu, vort, divg, L.wd[i], L.vol[i]).d;
+      visc_vort_divg(gs, Vd, vort, divg);
+      acc += l6 * sa_source_t<Dual, Dual>(P, R, N, mu, vort, divg, Wd, Vd).d;
     }
     if (k < 12) gbar[12 * (size_t)i + k] = acc;
     else if (k == 12) tbar[i] = acc;
-    else rnbar[2 * (size_t)i + (k - 13)] = acc;
+    else if (k < 15) rnbar[2 * (size_t)i + (k - 13)] = acc;
+    else if (k == 15) m[0] = acc;
+    else if (k == 16) m[1] = acc;
+    else if (k == 17) m[2] = acc;
+    else if (k == 18) m[3] = acc;
+    else m[4] = acc;
   }
+  if (mbar)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) mbar[5 * (size_t)i + c] = m[c];
 }
 
 // adjoint pass 3: the Green-Gauss sums transposed, abar[4 j + q] on node j's
@@ -2680,8 +2703,11 @@ __global__ void k_visc_adjoint_record(LevelDev<double> L, DevParams P, const dou
 // (+ (3 m_k + m_o) w_b, or 4 m_k w_b), the share on d_k is -f2 v_k W_k and
 // on v_k -f2 W_k.d_k; gnbar[3 k] = the share on g_k; v_k's share goes to
 // rho and nu~ (rnbar) and to T through Sutherland's law (tbar).
+// With mbar it also adds the share on node k's volume (through d_k and the
+// nodal term) to mbar[5 k] and keeps (d_k, v_k) in sad[4 k ..] for the side
+// weights' shares (k_visc_mesh_side / k_visc_mesh_bside).
 __global__ void k_sa_diff_adjoint1(LevelDev<double> L, DevParams P, const double *lam, double *gnbar, double *tbar,
-                                   double *rnbar) {
+                                   double *rnbar, double *mbar, double *sad) {
   using ad::Dual;
   int k = gtid_x();
   if (k >= L.n) return;
@@ -2720,6 +2746,19 @@ __global__ void k_sa_diff_adjoint1(LevelDev<double> L, DevParams P, const double
   tbar[k] += vb * P.Re / rho * mu.d;
   rnbar[2 * (size_t)k] -= vb * P.Re * mu.v / (rho * rho);
   rnbar[2 * (size_t)k + 1] += vb;
+  if (mbar) {
+    // vt = 1 / V scales d_k = g_k vt and the nodal term -f1 vt g_k.g_k
+    double dg = 0, gg = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      dg += -f2 * vk * W[d] * g[d];
+      gg += g[d] * g[d];
+    }
+    mbar[5 * (size_t)k] += -vt * vt * (dg - f1 * mk * gg);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sad[4 * (size_t)k + d] = g[d] * vt;
+    sad[4 * (size_t)k + 3] = vk;
+  }
 }
 
 // pass 2, node m: the grad nu~ sums transposed (node m's nu~ enters its own
@@ -2747,6 +2786,206 @@ __global__ void k_sa_diff_adjoint2(LevelDev<double> L, DevParams P, const double
   }
   out[m] += rnbar[2 * (size_t)m];
   out[5 * n + m] += r;
+}
+
+// (u, v, w, T) of node i's record, as visc_rec forms it
+__device__ __forceinline__ void visc_rec_v(const LevelDev<double> &L, const DevParams &P, int i, double *a) {
+  const double *q = L.rec0 + 8 * (size_t)i;
+  a[0] = q[1];
+  a[1] = q[2];
+  a[2] = q[3];
+  a[3] = P.gamma / (P.gamma - 1.0) * q[5] / q[0];
+}
+
+// Mesh adjoint of the viscous and SA terms, interior side e: the share on
+// its weight w (3), added to dgw[3 e ..].  w enters the Green-Gauss sums of
+// both ends (g_a += (a_a + a_b) w, g_b -= (a_a + a_b) w; the same for nu~),
+// the viscous edge flux of both ends (linear in w; node b's weight is -w)
+// and the SA diffusion's side terms -f2 (v_a d_a + v_b d_b).w of node a
+// (node b: the negative).  gbar / gnbar are the shares on the sums; lam
+// null: the sums of the viscous records only (the drag's friction).
+__global__ void k_visc_mesh_side(LevelDev<double> L, DevParams P, const double *lam, const ad::Dual *taud,
+                                 const double *gbar, const double *gnbar, const double *sad, double *dgw) {
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  const size_t n = L.n;
+  const int ia = L.ea[e], ib = L.eb[e];
+  double aa[4], ab[4], r[3];
+  visc_rec_v(L, P, ia, aa);
+  visc_rec_v(L, P, ib, ab);
+  const double *ga = gbar + 12 * (size_t)ia, *gb = gbar + 12 * (size_t)ib;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r[d] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[d] += (ga[4 * d + q] - gb[4 * d + q]) * (aa[q] + ab[q]);
+  }
+  if (lam) {
+    double ta[9], tb[9], ma[4], mb[4];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      ta[c] = taud[9 * (size_t)ia + c].v;
+      tb[c] = taud[9 * (size_t)ib + c].v;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ma[c] = -lam[(c + 1) * n + ia];
+      mb[c] = -lam[(c + 1) * n + ib];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+      visc_edge(ta, tb, aa, ab, d == 0 ? 1.0 : 0.0, d == 1 ? 1.0 : 0.0, d == 2 ? 1.0 : 0.0, e1);
+      visc_edge(tb, ta, ab, aa, d == 0 ? 1.0 : 0.0, d == 1 ? 1.0 : 0.0, d == 2 ? 1.0 : 0.0, e2);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) r[d] += ma[c] * e1[c] - mb[c] * e2[c];
+    }
+    if (P.sa) {
+      const double *na = gnbar + 3 * (size_t)ia, *nb = gnbar + 3 * (size_t)ib;
+      const double *sa_ = sad + 4 * (size_t)ia, *sb = sad + 4 * (size_t)ib;
+      const double sn = L.rec0[8 * (size_t)ia + 6] + L.rec0[8 * (size_t)ib + 6];
+      const double f2 = 1.0 / P.Re / (2.0 / 3.0), dl6 = lam[5 * n + ia] - lam[5 * n + ib];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) r[d] += (na[d] - nb[d]) * sn - f2 * dl6 * (sa_[3] * sa_[d] + sb[3] * sb[d]);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dgw[3 * (size_t)e + d] += r[d];
+}
+
+// the same for boundary side s (ends a, b, weight w_b, the same for both
+// ends): the sums (3 a_i + a_o) w_b at each end, the viscous boundary flux
+// of each end (indicators other than 2 and 9) and the SA diffusion's
+// boundary terms (boundaryTerm 1: -f2 (3 v_i d_i + v_o d_o).w_b, 2:
+// -4 f2 v_i d_i.w_b); added to dgwb[3 s ..]
+__global__ void k_visc_mesh_bside(LevelDev<double> L, DevParams P, const double *lam, const ad::Dual *taud,
+                                  const double *gbar, const double *gnbar, const double *sad, double *dgwb) {
+  int s = gtid_x();
+  if (s >= L.nbs) return;
+  const size_t n = L.n;
+  const int ia = L.ba[s], ib = L.bb[s], ind = L.bind[s];
+  double aa[4], ab[4], r[3];
+  visc_rec_v(L, P, ia, aa);
+  visc_rec_v(L, P, ib, ab);
+  const double *ga = gbar + 12 * (size_t)ia, *gb = gbar + 12 * (size_t)ib;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r[d] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      r[d] += ga[4 * d + q] * (3.0 * aa[q] + ab[q]) + gb[4 * d + q] * (3.0 * ab[q] + aa[q]);
+  }
+  if (lam) {
+    if (!(ind == 2 || ind == 9)) {
+      double ta[9], tb[9], ma[4], mb[4];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        ta[c] = taud[9 * (size_t)ia + c].v;
+        tb[c] = taud[9 * (size_t)ib + c].v;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ma[c] = -lam[(c + 1) * n + ia];
+        mb[c] = -lam[(c + 1) * n + ib];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+        visc_bside(ta, tb, aa, ab, d == 0 ? 1.0 : 0.0, d == 1 ? 1.0 : 0.0, d == 2 ? 1.0 : 0.0, e1);
+        visc_bside(tb, ta, ab, aa, d == 0 ? 1.0 : 0.0, d == 1 ? 1.0 : 0.0, d == 2 ? 1.0 : 0.0, e2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r[d] += ma[c] * e1[c] + mb[c] * e2[c];
+      }
+    }
+    if (P.sa) {
+      const double *na = gnbar + 3 * (size_t)ia, *nb = gnbar + 3 * (size_t)ib;
+      const double *sa_ = sad + 4 * (size_t)ia, *sb = sad + 4 * (size_t)ib;
+      const double va = L.rec0[8 * (size_t)ia + 6], vb = L.rec0[8 * (size_t)ib + 6];
+      const double f2 = 1.0 / P.Re / (2.0 / 3.0), la = lam[5 * n + ia], lb = lam[5 * n + ib];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (P.bterm == 1)
+          r[d] += na[d] * (3.0 * va + vb) + nb[d] * (3.0 * vb + va) -
+                  f2 * (la * (3.0 * sa_[3] * sa_[d] + sb[3] * sb[d]) + lb * (3.0 * sb[3] * sb[d] + sa_[3] * sa_[d]));
+        else
+          r[d] += na[d] * 4.0 * va + nb[d] * 4.0 * vb - f2 * 4.0 * (la * sa_[3] * sa_[d] + lb * sb[3] * sb[d]);
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dgwb[3 * (size_t)s + d] += r[d];
+}
+
+// the friction term of the drag (calculateLiftAndDrag, DEF:14757-14803,
+// with ReynoldsNumber > 0): fct A_i ws_i.t at each counted wall node, with
+// ws_i = tau_i n_i (the symmetric stress times the unit normal).  Writes its
+// shares on the stress record (taub[9 i ..], the heat-flux slots 0), on the
+// wall area (gaf[i]) and on the normal (gnf[3 i ..]: fct A_i tau_i t); the
+// caller clears the arrays elsewhere.
+__global__ void k_friction_shares(LevelDev<double> L, const int *wall_nodes, int nw, const double *warea,
+                                  double fct, double t0, double t1, double t2, double groundplane,
+                                  const ad::Dual *taud, double *taub, double *gaf, double *gnf) {
+  int k = gtid();
+  if (k >= nw) return;
+  const int ip = wall_nodes[k];
+  const double z = L.xyz ? L.xyz[3 * (size_t)ip + 2] : 0.0;
+  if (!(z > groundplane + 0.01)) return;
+  const double nx = L.nrm[3 * (size_t)ip], ny = L.nrm[3 * (size_t)ip + 1], nz = L.nrm[3 * (size_t)ip + 2];
+  const double fa = fct * warea[ip];
+  double tau[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) tau[c] = taud[9 * (size_t)ip + c].v;
+  double *tb = taub + 9 * (size_t)ip;
+  tb[0] = fa * nx * t0;
+  tb[1] = fa * (ny * t0 + nx * t1);
+  tb[2] = fa * (nz * t0 + nx * t2);
+  tb[3] = fa * ny * t1;
+  tb[4] = fa * (nz * t1 + ny * t2);
+  tb[5] = fa * nz * t2;
+  tb[6] = tb[7] = tb[8] = 0.0;
+  const double tt0 = tau[0] * t0 + tau[1] * t1 + tau[2] * t2;
+  const double tt1 = tau[1] * t0 + tau[3] * t1 + tau[4] * t2;
+  const double tt2 = tau[2] * t0 + tau[4] * t1 + tau[5] * t2;
+  gaf[ip] = fct * (tt0 * nx + tt1 * ny + tt2 * nz);
+  gnf[3 * (size_t)ip] = fa * tt0;
+  gnf[3 * (size_t)ip + 1] = fa * tt1;
+  gnf[3 * (size_t)ip + 2] = fa * tt2;
+}
+
+// mesh adjoint of the SA convection (first order at both HLLC orders):
+// side e's share on its weight w, (lam6_a - lam6_b) df6/dw from three dual
+// passes; added to dgw[3 e ..]
+__global__ void k_sa_conv_mesh_adjoint(LevelDev<double> L, DevParams P, const double *lam, double *dgw) {
+  using ad::Dual;
+  int e = gtid_x();
+  if (e >= L.ne) return;
+  const size_t n = L.n;
+  const int ia = L.ea[e], ib = L.eb[e];
+  Dual Ua[5], Ub[5], A[8], B[8];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Ua[c] = Dual(L.u[c * n + ia]);
+    Ub[c] = Dual(L.u[c * n + ib]);
+  }
+  state_record(Ua, P.gamma, A);
+  state_record(Ub, P.gamma, B);
+  A[6] = Dual(L.u[5 * n + ia]);
+  B[6] = Dual(L.u[5 * n + ib]);
+  const double dl = lam[5 * n + ia] - lam[5 * n + ib];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const Dual wx(L.w[3 * e], d == 0 ? 1.0 : 0.0), wy(L.w[3 * e + 1], d == 1 ? 1.0 : 0.0),
+        wz(L.w[3 * e + 2], d == 2 ? 1.0 : 0.0);
+    dgw[3 * (size_t)e + d] += dl * sa_side_convection(A, B, wx, wy, wz).d;
+  }
+}
+
+// out(1) += rnbar(rho), out(6) += rnbar(nu~)
+__global__ void k_add_rho_nu(const double *rnbar, int n, double *out) {
+  int i = gtid_x();
+  if (i >= n) return;
+  out[i] += rnbar[2 * (size_t)i];
+  out[5 * (size_t)n + i] += rnbar[2 * (size_t)i + 1];
 }
 
 // tangent: out += dvis

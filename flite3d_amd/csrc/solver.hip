@@ -188,6 +188,7 @@ struct Level {
   // monitor records: MON_BLOCKS block records and the total (MON_REC doubles each)
   double *m_part = nullptr;
   double *abuf = nullptr;  // adjoint side shares, 10 per side (allocated on first use)
+  double *vmesh = nullptr;  // viscous mesh-adjoint shares, 3 per side and boundary side (first use)
   // second-order linearisation (allocated on first use): per-side adjoint
   // shares, per-node scratch, extremal neighbours of the limiter bounds
   double *abuf2 = nullptr, *lin = nullptr;
@@ -2120,30 +2121,125 @@ int visc_tangent(flite_ctx *c, Level &L) {
   return check_launch();
 }
 
+// the viscous linearisation's scratch (VLIN doubles per node): stress duals
+// 18 | taub 9 | ubv 3 | gbar 12 | tbar 1 | abar 4 | (rho, nu~) shares 2 |
+// grad nu~ shares 3 | SA (d, v) 4 | mesh shares 5 | friction area 1 and
+// normal 3 shares
+struct Vlin {
+  ad::Dual *taud;
+  double *taub, *ubv, *gbar, *tbar, *abar, *rnbar, *gnbar, *sad, *mbar, *gaf, *gnf;
+  explicit Vlin(Level &L) {
+    const size_t n = L.n;
+    double *b = L.vlin;
+    taud = reinterpret_cast<ad::Dual *>(b);
+    taub = b + 18 * n;
+    ubv = b + 27 * n;
+    gbar = b + 30 * n;
+    tbar = b + 42 * n;
+    abar = b + 43 * n;
+    rnbar = b + 47 * n;
+    gnbar = b + 49 * n;
+    sad = b + 52 * n;
+    mbar = b + 56 * n;
+    gaf = b + 61 * n;
+    gnf = b + 62 * n;
+  }
+};
+
 // adjoint of the viscous term against the multipliers in V.src, added to V.rhs
 // With Spalart-Allmaras it adds the transposes of the eddy viscosity in the
 // stress, the SA source and the SA diffusion (multipliers lam(6)).
-// Scratch per node: tau duals 18 | taub 9 | ubv 3 | gbar 12 | tbar 1 |
-// abar 4 | (rho, nu~) shares 2 | grad nu~ shares 3
 int visc_adjoint(flite_ctx *c, Level &L) {
   TRY(alloc_vlin(L));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
-  const size_t n = L.n;
-  ad::Dual *taud = reinterpret_cast<ad::Dual *>(L.vlin);
-  double *taub = L.vlin + 18 * n, *ubv = L.vlin + 27 * n, *gbar = L.vlin + 30 * n, *tbar = L.vlin + 42 * n,
-         *abar = L.vlin + 43 * n, *rnbar = L.vlin + 47 * n, *gnbar = L.vlin + 49 * n;
+  Vlin S(L);
   dim3 g(nblk(L.n)), b(NTHR);
-  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, taud, (ad::Dual *)nullptr);
-  hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)taud, taub, ubv);
-  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, V.src, (const double *)taub, gbar, tbar, rnbar);
-  if (P.sa) hipLaunchKernelGGL(k_sa_diff_adjoint1, g, b, 0, c->stream, V, P, V.src, gnbar, tbar, rnbar);
-  hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)gbar, (const double *)tbar,
-                     (const double *)ubv, abar);
-  hipLaunchKernelGGL(k_visc_adjoint_record, g, b, 0, c->stream, V, P, (const double *)abar, V.rhs);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, S.taud, (ad::Dual *)nullptr);
+  hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)S.taud, S.taub, S.ubv);
+  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, V.src, (const double *)S.taub, S.gbar, S.tbar,
+                     S.rnbar, (double *)nullptr);
   if (P.sa)
-    hipLaunchKernelGGL(k_sa_diff_adjoint2, g, b, 0, c->stream, V, P, (const double *)gnbar, (const double *)rnbar,
+    hipLaunchKernelGGL(k_sa_diff_adjoint1, g, b, 0, c->stream, V, P, V.src, S.gnbar, S.tbar, S.rnbar, (double *)nullptr,
+                       (double *)nullptr);
+  hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)S.gbar, (const double *)S.tbar,
+                     (const double *)S.ubv, S.abar);
+  hipLaunchKernelGGL(k_visc_adjoint_record, g, b, 0, c->stream, V, P, (const double *)S.abar, V.rhs);
+  if (P.sa)
+    hipLaunchKernelGGL(k_sa_diff_adjoint2, g, b, 0, c->stream, V, P, (const double *)S.gnbar, (const double *)S.rnbar,
                        V.rhs);
+  return check_launch();
+}
+
+// per-side and per-boundary-side mesh shares of the viscous terms (device
+// side order), allocated on first use
+int alloc_vmesh(Level &L) {
+  if (!L.vmesh) TRY(L.pool.alloc(&L.vmesh, 3 * ((size_t)std::max(1, L.ne) + (size_t)std::max(1, L.nbs))));
+  return 0;
+}
+
+// d(lam^T R_visc)/d(mesh) for lam in V.src: side weights into L.vmesh
+// (3 per side), boundary-side weights after them (3 per boundary side), and
+// per node in S.mbar: volume, wall normal, wall distance.  Records current.
+int visc_mesh_adjoint(flite_ctx *c, Level &L) {
+  TRY(alloc_vlin(L));
+  TRY(alloc_vmesh(L));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  Vlin S(L);
+  double *dgw = L.vmesh, *dgwb = L.vmesh + 3 * (size_t)std::max(1, L.ne);
+  dim3 g(nblk(L.n)), b(NTHR);
+  const size_t ns = 3 * ((size_t)std::max(1, L.ne) + (size_t)std::max(1, L.nbs));
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)ns)), dim3(NTHR), 0, c->stream, L.vmesh, 0.0, ns);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, S.taud, (ad::Dual *)nullptr);
+  hipLaunchKernelGGL(k_visc_adjoint_tau, g, b, 0, c->stream, V, P, V.src, (const ad::Dual *)S.taud, S.taub, S.ubv);
+  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, V.src, (const double *)S.taub, S.gbar, S.tbar,
+                     S.rnbar, S.mbar);
+  if (P.sa)
+    hipLaunchKernelGGL(k_sa_diff_adjoint1, g, b, 0, c->stream, V, P, V.src, S.gnbar, S.tbar, S.rnbar, S.mbar, S.sad);
+  if (L.ne > 0)
+    hipLaunchKernelGGL(k_visc_mesh_side, dim3(nblk(L.ne)), b, 0, c->stream, V, P, V.src, (const ad::Dual *)S.taud,
+                       (const double *)S.gbar, (const double *)S.gnbar, (const double *)S.sad, dgw);
+  if (L.nbs > 0)
+    hipLaunchKernelGGL(k_visc_mesh_bside, dim3(nblk(L.nbs)), b, 0, c->stream, V, P, V.src, (const ad::Dual *)S.taud,
+                       (const double *)S.gbar, (const double *)S.gnbar, (const double *)S.sad, dgwb);
+  return check_launch();
+}
+
+// the friction term of the drag (ReynoldsNumber > 0) reversed: its shares
+// on the stress records through the node pass (lam null) into S.gbar,
+// S.tbar, S.rnbar and, with mesh, S.mbar; the direct area and normal shares
+// in S.gaf, S.gnf.  Records current.
+int visc_friction_reverse(flite_ctx *c, Level &L, bool mesh) {
+  TRY(alloc_vlin(L));
+  DevParams P = c->P;
+  const flite_params &hp = c->hp;
+  LevelDev<double> V = L.view<double>();
+  Vlin S(L);
+  const size_t n = L.n;
+  dim3 g(nblk(L.n)), b(NTHR);
+  double PI = 4.0 * std::atan(1.0);
+  double ar = hp.alpha * PI / 180., br = hp.beta * PI / 180.;
+  double t0 = std::cos(ar) * std::cos(br), t1 = std::sin(br), t2 = std::sin(ar);
+  hipLaunchKernelGGL(k_visc_tangent_node, g, b, 0, c->stream, V, P, (const double *)nullptr, S.taud, (ad::Dual *)nullptr);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)(9 * n))), b, 0, c->stream, S.taub, 0.0, 9 * n);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)(4 * n))), b, 0, c->stream, S.gaf, 0.0, 4 * n);
+  const int nw = (int)L.wall_dev.size();
+  if (nw > 0)
+    hipLaunchKernelGGL(k_friction_shares, dim3(nblk(nw)), b, 0, c->stream, V, (const int *)L.wall_nodes, nw,
+                       (const double *)L.warea, 2.0 / hp.reference_area, t0, t1, t2, hp.groundplane,
+                       (const ad::Dual *)S.taud, S.taub, S.gaf, S.gnf);
+  hipLaunchKernelGGL(k_visc_adjoint_node, g, b, 0, c->stream, V, P, (const double *)nullptr, (const double *)S.taub,
+                     S.gbar, S.tbar, S.rnbar, mesh ? S.mbar : (double *)nullptr);
+  return check_launch();
+}
+
+// the wall stress of the current state (the viscous node pass), for the
+// drag's friction term
+int wall_stress(flite_ctx *c, Level &L) {
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  launch_gather<ViscNodePart<double, true>, NoPart<double>, NoPart<double>>(c, L, V, P);
   return check_launch();
 }
 
@@ -2277,11 +2373,12 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
 }
 
 int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) {
-  TRY(check_linearisable(c));
+  TRY(check_linearisable(c, true, true, true));
   Level &L = *c->lev[0];
   if (!dJdU || ld < L.n) return fail(FLITE_E_ARG, "bad output vector");
   const flite_params &hp = c->hp;
   TRY(op_prep(c, L, true));
+  if (laminar(c)) TRY(wall_stress(c, L));
   double res[7], mx[7], forces[9];
   int idx[7];
   TRY(flite_monitor(c, res, mx, idx, forces));
@@ -2297,15 +2394,65 @@ int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) 
     hipLaunchKernelGGL(k_drag_state_gradient, dim3(nblk(nw)), dim3(NTHR), 0, c->stream, V, c->P, L.wall_nodes, nw,
                        (const double *)L.warea, 2.0 / hp.reference_area, t0, t1, t2, hp.groundplane, V.rhs);
   TRY(check_launch());
-  return get_vector5(c, dJdU, ld);
+  if (laminar(c)) {
+    // the friction term through the wall stress: its shares on the stress
+    // records, then the Green-Gauss transpose and the records' dependence
+    // on the state (and the eddy viscosity's on rho and nu~)
+    TRY(visc_friction_reverse(c, L, false));
+    Vlin S(L);
+    dim3 g(nblk(L.n)), b(NTHR);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)(3 * (size_t)L.n))), b, 0, c->stream, S.ubv, 0.0,
+                       3 * (size_t)L.n);
+    hipLaunchKernelGGL(k_visc_adjoint_grad, g, b, 0, c->stream, V, (const double *)S.gbar, (const double *)S.tbar,
+                       (const double *)S.ubv, S.abar);
+    hipLaunchKernelGGL(k_visc_adjoint_record, g, b, 0, c->stream, V, c->P, (const double *)S.abar, V.rhs);
+    if (c->P.sa) hipLaunchKernelGGL(k_add_rho_nu, g, b, 0, c->stream, (const double *)S.rnbar, L.n, V.rhs);
+    TRY(check_launch());
+  }
+  return get_vector(c, dJdU, ld, c->P.sa ? 6 : 5);
 }
 
-int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb,
-                         int ld_wb) {
-  TRY(check_linearisable(c, false));
+namespace {
+// the chain from per-node shares on the wall areas A_i = sum 4 |w_b| and the
+// unit normals n_i = m_i / |m_i|, m_i = -sum w_b (sums over the boundary
+// sides at the node, as build_level forms warea and nrm; host numbering) to
+// the boundary-side weights, added to gwb
+void wb_chain(const Staged &S, int n, int nbs, const std::vector<double> &gA, const std::vector<double> &gN,
+              double *gwb, int ld_wb) {
+  std::vector<double> m(3 * (size_t)n, 0.0), gM(3 * (size_t)n, 0.0);
+  for (int s = 0; s < nbs; ++s)
+    for (int q = 0; q < 3; ++q) {
+      const double w = S.wb[(size_t)q * nbs + s];
+      m[3 * (size_t)S.b1[s] + q] -= w;
+      m[3 * (size_t)S.b2[s] + q] -= w;
+    }
+  for (int i = 0; i < n; ++i) {
+    const double *gn = &gN[3 * (size_t)i], *mi = &m[3 * (size_t)i];
+    if (gn[0] == 0.0 && gn[1] == 0.0 && gn[2] == 0.0) continue;
+    const double mn = std::sqrt(mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]);
+    if (!(mn > 1e-10)) continue;  // zero normal: no dependence
+    const double nh[3] = {mi[0] / mn, mi[1] / mn, mi[2] / mn};
+    const double gNn = gn[0] * nh[0] + gn[1] * nh[1] + gn[2] * nh[2];
+    for (int q = 0; q < 3; ++q) gM[3 * (size_t)i + q] = (gn[q] - gNn * nh[q]) / mn;
+  }
+  for (int s = 0; s < nbs; ++s) {
+    double w[3], a = 0;
+    for (int q = 0; q < 3; ++q) { w[q] = S.wb[(size_t)q * nbs + s]; a += w[q] * w[q]; }
+    a = std::sqrt(a);
+    double g[3] = {0, 0, 0};
+    for (int i : {S.b1[s], S.b2[s]}) {
+      for (int q = 0; q < 3; ++q) g[q] -= gM[3 * (size_t)i + q];
+      if (a > 0)
+        for (int q = 0; q < 3; ++q) g[q] += gA[i] * 4.0 * w[q] / a;
+    }
+    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] += g[q];
+  }
+}
+
+// the first-order convective residual's side and boundary-side weight
+// shares for the multipliers in src, into gw and gwb (host layouts)
+int metric_adjoint_conv(flite_ctx *c, double *gw, int ld_w, double *gwb, int ld_wb) {
   Level &L = *c->lev[0];
-  if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output arrays");
-  TRY(put_vector5(c, lam, ld_lam));
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
@@ -2325,19 +2472,70 @@ int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw
   return 0;
 }
 
+// the viscous and SA terms' mesh shares for the multipliers in src, added
+// to gw, gwb and gvol; gwd (may be null without SA) receives the wall
+// distance shares.  The wall normals' shares go to gwb through wb_chain.
+int mesh_adjoint_visc(flite_ctx *c, double *gw, int ld_w, double *gwb, int ld_wb, double *gvol, double *gwd) {
+  Level &L = *c->lev[0];
+  const int n = L.n, ne = L.ne, nbs = L.nbs;
+  TRY(op_prep(c, L, true));
+  TRY(visc_mesh_adjoint(c, L));
+  DevParams P = c->P;
+  LevelDev<double> V = L.view<double>();
+  double *dgw = L.vmesh, *dgwb = L.vmesh + 3 * (size_t)std::max(1, ne);
+  if (P.sa && ne > 0)
+    hipLaunchKernelGGL(k_sa_conv_mesh_adjoint, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, V.src, dgw);
+  TRY(check_launch());
+  Vlin S(L);
+  std::vector<double> hw(3 * (size_t)ne), hb(3 * (size_t)nbs), hm(5 * (size_t)n);
+  if (ne > 0) HIPTRY(hipMemcpyAsync(hw.data(), dgw, hw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (nbs > 0) HIPTRY(hipMemcpyAsync(hb.data(), dgwb, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipMemcpyAsync(hm.data(), S.mbar, hm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ne; ++k)
+    for (int q = 0; q < 3; ++q) gw[(size_t)q * ld_w + L.side_host[k]] += hw[3 * (size_t)k + q];
+  for (int s = 0; s < nbs; ++s)
+    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] += hb[3 * (size_t)s + q];
+  std::vector<double> gA(n, 0.0), gN(3 * (size_t)n, 0.0);
+  for (int hh = 0; hh < n; ++hh) {
+    const int d = L.h2d[hh];
+    gvol[hh] += hm[5 * (size_t)d];
+    if (gwd) gwd[hh] = hm[5 * (size_t)d + 4];
+    for (int q = 0; q < 3; ++q) gN[3 * (size_t)hh + q] = hm[5 * (size_t)d + 1 + q];
+  }
+  wb_chain(c->staged[0], n, nbs, gA, gN, gwb, ld_wb);
+  return 0;
+}
+}  // namespace
+
+int flite_metric_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb,
+                         int ld_wb) {
+  TRY(check_linearisable(c, false));
+  Level &L = *c->lev[0];
+  if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output arrays");
+  TRY(put_vector5(c, lam, ld_lam));
+  return metric_adjoint_conv(c, gw, ld_w, gwb, ld_wb);
+}
+
 int flite_mesh_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb,
-                       double *gvol, double *gxyz, int ld_x) {
-  TRY(check_linearisable(c));
+                       double *gvol, double *gxyz, int ld_x, double *gwd) {
+  TRY(check_linearisable(c, true, true, true));
   Level &L = *c->lev[0];
   if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs || !gvol || !gxyz || ld_x < L.n)
     return fail(FLITE_E_ARG, "bad output arrays");
+  if (c->P.sa && !gwd) return fail(FLITE_E_ARG, "Spalart-Allmaras: gwd (wall distance shares) is required");
   const int n = L.n, ne = L.ne, nbs = L.nbs;
+  const int nc = c->P.sa ? 6 : 5;
+  if (gwd)
+    for (int i = 0; i < n; ++i) gwd[i] = 0.0;
   if (!second_order(c, L)) {
-    // the first-order residual depends on the side weights only
-    TRY(flite_metric_adjoint(c, lam, ld_lam, gw, ld_w, gwb, ld_wb));
+    // the first-order convective residual depends on the side weights only
+    TRY(put_vector(c, lam, ld_lam, nc));
+    TRY(metric_adjoint_conv(c, gw, ld_w, gwb, ld_wb));
     for (int i = 0; i < n; ++i) gvol[i] = 0.0;
     for (int k = 0; k < 3; ++k)
       for (int i = 0; i < n; ++i) gxyz[(size_t)k * ld_x + i] = 0.0;
+    if (laminar(c)) TRY(mesh_adjoint_visc(c, gw, ld_w, gwb, ld_wb, gvol, gwd));
     return 0;
   }
   // records, gradients and bounds of the state; the state adjoint's side
@@ -2348,7 +2546,7 @@ int flite_mesh_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, 
   TRY(alloc_lin(L));
   if (!L.abuf2) TRY(L.pool.alloc(&L.abuf2, (size_t)LIN_SIDE * std::max(1, ne)));
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, ne)));
-  TRY(put_vector5(c, lam, ld_lam));
+  TRY(put_vector(c, lam, ld_lam, nc));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   double *ub = L.lin, *gbn = L.lin + 5 * (size_t)n, *gv = L.lin + 30 * (size_t)n;
@@ -2403,20 +2601,20 @@ int flite_mesh_adjoint(flite_ctx *c, const double *lam, int ld_lam, double *gw, 
     gvol[hh] = hv[d];
     for (int q = 0; q < 3; ++q) gxyz[(size_t)q * ld_x + hh] = gx[3 * (size_t)d + q];
   }
+  if (laminar(c)) TRY(mesh_adjoint_visc(c, gw, ld_w, gwb, ld_wb, gvol, gwd));
   return 0;
 }
 
-// pressure drag of calculateLiftAndDrag (DEF:14757-14803) as a function of
-// the boundary-side weights at a fixed state: the wall node areas
-// A_i = sum 4|w_b| and normals n_i = m_i/|m_i|, m_i = -sum w_b, over the
-// boundary sides at the node (as build_level forms warea and nrm)
-int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_wb) {
-  TRY(check_linearisable(c));  // the drag itself does not depend on the residual's order
+namespace {
+// the drag at the current state (records and, with ReynoldsNumber > 0, the
+// wall stress recomputed first), and its pressure term's shares on the
+// counted wall nodes' areas and unit normals (host numbering)
+int drag_pressure_shares(flite_ctx *c, double *drag, std::vector<double> &gA, std::vector<double> &gN) {
   Level &L = *c->lev[0];
   const Staged &S = c->staged[0];
-  if (!gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output array");
   const flite_params &hp = c->hp;
   TRY(op_prep(c, L, true));
+  if (laminar(c)) TRY(wall_stress(c, L));
   double res[7], mx[7], forces[9];
   int idx[7];
   TRY(flite_monitor(c, res, mx, idx, forces));
@@ -2429,7 +2627,6 @@ int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_w
   const double t[3] = {std::cos(ar) * std::cos(br), std::sin(br), std::sin(ar)};
   const double f = 2.0 / hp.reference_area, g1 = hp.gamma - 1.0;
   const double pinf = 1.0 / (hp.gamma * hp.mach_number * hp.mach_number);
-  // node sums m_i and A_i over all boundary sides, as build_level forms them
   std::vector<double> m(3 * (size_t)n, 0.0), area(n, 0.0);
   for (int s = 0; s < nbs; ++s) {
     double w[3], a = 0;
@@ -2439,8 +2636,8 @@ int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_w
     area[S.b1[s]] += a;
     area[S.b2[s]] += a;
   }
-  // per counted wall node: dJ/dA_i and dJ/dm_i
-  std::vector<double> gA(n, 0.0), gM(3 * (size_t)n, 0.0);
+  gA.assign(n, 0.0);
+  gN.assign(3 * (size_t)n, 0.0);
   for (int d : L.wall_dev) {
     int i = L.d2h[d];
     double z = S.xyz.empty() ? 0.0 : S.xyz[2 * (size_t)n + i];
@@ -2452,25 +2649,84 @@ int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_w
     const double *mi = &m[3 * (size_t)i];
     double mn = std::sqrt(mi[0] * mi[0] + mi[1] * mi[1] + mi[2] * mi[2]);
     if (!(mn > 1e-10)) continue;  // zero normal: no dependence
-    double nh[3] = {mi[0] / mn, mi[1] / mn, mi[2] / mn};
-    double nt = nh[0] * t[0] + nh[1] * t[1] + nh[2] * t[2];
+    double nt = (mi[0] * t[0] + mi[1] * t[1] + mi[2] * t[2]) / mn;
     gA[i] = -f * (p - pinf) * nt;
-    double gN[3], gNn = 0;
-    for (int q = 0; q < 3; ++q) { gN[q] = -f * area[i] * (p - pinf) * t[q]; gNn += gN[q] * nh[q]; }
-    for (int q = 0; q < 3; ++q) gM[3 * (size_t)i + q] = (gN[q] - gNn * nh[q]) / mn;
+    for (int q = 0; q < 3; ++q) gN[3 * (size_t)i + q] = -f * area[i] * (p - pinf) * t[q];
   }
-  for (int s = 0; s < nbs; ++s) {
-    double w[3], a = 0;
-    for (int q = 0; q < 3; ++q) { w[q] = S.wb[(size_t)q * nbs + s]; a += w[q] * w[q]; }
-    a = std::sqrt(a);
-    double g[3] = {0, 0, 0};
-    for (int i : {S.b1[s], S.b2[s]}) {
-      for (int q = 0; q < 3; ++q) g[q] -= gM[3 * (size_t)i + q];
-      if (a > 0)
-        for (int q = 0; q < 3; ++q) g[q] += gA[i] * 4.0 * w[q] / a;
+  return 0;
+}
+}  // namespace
+
+// pressure drag of calculateLiftAndDrag (DEF:14757-14803) as a function of
+// the boundary-side weights at a fixed state: the wall node areas
+// A_i = sum 4|w_b| and normals n_i = m_i/|m_i|, m_i = -sum w_b, over the
+// boundary sides at the node (as build_level forms warea and nrm)
+int flite_drag_metric_gradient(flite_ctx *c, double *drag, double *gwb, int ld_wb) {
+  TRY(check_linearisable(c));  // the drag itself does not depend on the residual's order
+  Level &L = *c->lev[0];
+  if (!gwb || ld_wb < L.nbs) return fail(FLITE_E_ARG, "bad output array");
+  std::vector<double> gA, gN;
+  TRY(drag_pressure_shares(c, drag, gA, gN));
+  for (int q = 0; q < 3; ++q)
+    for (int s = 0; s < L.nbs; ++s) gwb[(size_t)q * ld_wb + s] = 0.0;
+  wb_chain(c->staged[0], L.n, L.nbs, gA, gN, gwb, ld_wb);
+  return 0;
+}
+
+// the drag (pressure and, with ReynoldsNumber > 0, friction) as a function
+// of the side weights, boundary-side weights and volumes at a fixed state.
+// The friction fct A_i ws_i.t reads the wall areas and normals and the wall
+// stress, whose Green-Gauss sums read the weights and whose gradients read
+// the volumes.
+int flite_drag_mesh_gradient(flite_ctx *c, double *drag, double *gw, int ld_w, double *gwb, int ld_wb, double *gvol) {
+  TRY(check_linearisable(c, true, true, true));
+  Level &L = *c->lev[0];
+  if (!gw || ld_w < L.ne || !gwb || ld_wb < L.nbs || !gvol) return fail(FLITE_E_ARG, "bad output arrays");
+  const int n = L.n, ne = L.ne, nbs = L.nbs;
+  std::vector<double> gA, gN;
+  TRY(drag_pressure_shares(c, drag, gA, gN));
+  for (int q = 0; q < 3; ++q) {
+    for (int k = 0; k < ne; ++k) gw[(size_t)q * ld_w + k] = 0.0;
+    for (int s = 0; s < nbs; ++s) gwb[(size_t)q * ld_wb + s] = 0.0;
+  }
+  for (int i = 0; i < n; ++i) gvol[i] = 0.0;
+  if (laminar(c)) {
+    TRY(visc_friction_reverse(c, L, true));
+    TRY(alloc_vmesh(L));
+    DevParams P = c->P;
+    LevelDev<double> V = L.view<double>();
+    Vlin S(L);
+    double *dgw = L.vmesh, *dgwb = L.vmesh + 3 * (size_t)std::max(1, ne);
+    const size_t ns = 3 * ((size_t)std::max(1, ne) + (size_t)std::max(1, nbs));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)ns)), dim3(NTHR), 0, c->stream, L.vmesh, 0.0, ns);
+    const double *nul = nullptr;
+    if (ne > 0)
+      hipLaunchKernelGGL(k_visc_mesh_side, dim3(nblk(ne)), dim3(NTHR), 0, c->stream, V, P, nul,
+                         (const ad::Dual *)S.taud, (const double *)S.gbar, nul, nul, dgw);
+    if (nbs > 0)
+      hipLaunchKernelGGL(k_visc_mesh_bside, dim3(nblk(nbs)), dim3(NTHR), 0, c->stream, V, P, nul,
+                         (const ad::Dual *)S.taud, (const double *)S.gbar, nul, nul, dgwb);
+    TRY(check_launch());
+    std::vector<double> hw(3 * (size_t)ne), hb(3 * (size_t)nbs), hm(5 * (size_t)n), ha(4 * (size_t)n);
+    if (ne > 0) HIPTRY(hipMemcpyAsync(hw.data(), dgw, hw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (nbs > 0) HIPTRY(hipMemcpyAsync(hb.data(), dgwb, hb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(hm.data(), S.mbar, hm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(ha.data(), S.gaf, ha.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < ne; ++k)
+      for (int q = 0; q < 3; ++q) gw[(size_t)q * ld_w + L.side_host[k]] += hw[3 * (size_t)k + q];
+    for (int s = 0; s < nbs; ++s)
+      for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] += hb[3 * (size_t)s + q];
+    // the friction's area and normal shares (gaf, gnf: device numbering) and
+    // the stress law's normal shares join the pressure term's
+    for (int hh = 0; hh < n; ++hh) {
+      const int d = L.h2d[hh];
+      gvol[hh] += hm[5 * (size_t)d];
+      gA[hh] += ha[d];
+      for (int q = 0; q < 3; ++q) gN[3 * (size_t)hh + q] += ha[n + 3 * (size_t)d + q] + hm[5 * (size_t)d + 1 + q];
     }
-    for (int q = 0; q < 3; ++q) gwb[(size_t)q * ld_wb + s] = g[q];
   }
+  wb_chain(c->staged[0], n, nbs, gA, gN, gwb, ld_wb);
   return 0;
 }
 

@@ -55,7 +55,8 @@ def lib():
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
-        L.flite_mesh_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i, P(d), P(d), i]
+        L.flite_mesh_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i, P(d), P(d), i, P(d)]
+        L.flite_drag_mesh_gradient.argtypes = [vp, P(d), P(d), i, P(d), i, P(d)]
         L.flite_set_coarse_precision.argtypes = [vp, i]
         L.flite_autotune.argtypes = [vp, i, P(i)]
         L.flite_level_lanes.argtypes = [vp, i, P(i)]
@@ -230,9 +231,10 @@ class Solver:
         return out
 
     def drag_gradient(self):
-        """(pressure drag, dDrag/dU) at the current state."""
+        """(drag, dDrag/dU) at the current state: the pressure drag, plus the
+        friction with ReynoldsNumber > 0; 6 components with Spalart-Allmaras."""
         n = self.n(1)
-        out = np.zeros((5, n))
+        out = np.zeros((6 if self.params[0].turbulence_model == 1 else 5, n))
         drag = ctypes.c_double()
         D = ctypes.POINTER(ctypes.c_double)
         self._check(self.L.flite_drag_state_gradient(self.ctx, ctypes.byref(drag), out.ctypes.data_as(D), n))
@@ -259,20 +261,39 @@ class Solver:
 
     def mesh_adjoint(self, lam):
         """d(lam^T R) with respect to side_w (3, E), bside_w (3, Eb), the
-        dual volumes (N,) and the coordinates directly (3, N), at the
-        current state (first- or second-order HLLC)."""
+        dual volumes (N,), the coordinates directly (3, N) and, with
+        Spalart-Allmaras, the wall distances (N,; zeros otherwise), at the
+        current state (first- or second-order HLLC, inviscid, laminar or
+        RANS-SA)."""
         n = self.n(1)
         ne, nbs = self._sizes1()
-        x = self._vec5(lam)
+        nc = 6 if self.params[0].turbulence_model == 1 else 5
+        x = np.ascontiguousarray(np.asarray(lam, dtype=np.float64)[:nc].reshape(nc, n))
         gw = np.zeros((3, ne))
         gwb = np.zeros((3, nbs))
         gv = np.zeros(n)
         gx = np.zeros((3, n))
+        gd = np.zeros(n)
         D = ctypes.POINTER(ctypes.c_double)
         self._check(self.L.flite_mesh_adjoint(self.ctx, x.ctypes.data_as(D), n, gw.ctypes.data_as(D), ne,
                                               gwb.ctypes.data_as(D), nbs, gv.ctypes.data_as(D),
-                                              gx.ctypes.data_as(D), n))
-        return gw, gwb, gv, gx
+                                              gx.ctypes.data_as(D), n, gd.ctypes.data_as(D)))
+        return gw, gwb, gv, gx, gd
+
+    def drag_mesh_gradient(self):
+        """(drag, dDrag/d side_w (3, E), dDrag/d bside_w (3, Eb), dDrag/d
+        volume (N,)) at the current state (friction included with
+        ReynoldsNumber > 0)."""
+        n = self.n(1)
+        ne, nbs = self._sizes1()
+        gw = np.zeros((3, ne))
+        gwb = np.zeros((3, nbs))
+        gv = np.zeros(n)
+        drag = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.flite_drag_mesh_gradient(self.ctx, ctypes.byref(drag), gw.ctypes.data_as(D), ne,
+                                                    gwb.ctypes.data_as(D), nbs, gv.ctypes.data_as(D)))
+        return drag.value, gw, gwb, gv
 
     def drag_metric_gradient(self):
         """(pressure drag, dDrag/d bside_w (3, Eb)) at the current state."""

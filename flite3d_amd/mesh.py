@@ -316,6 +316,23 @@ def mesh_vjp(xyz, tets, btri, bind, gw, gwb, gvol=None, gxn=None):
     return gx
 
 
+def wall_distance_vjp(level, gwd):
+    """Transpose of the wall distance d_i = |x_i - x_w(i)| (w(i) the nearest
+    wall node the build found): (3, N) coordinate shares in the level's
+    numbering, to pass as (part of) mesh_vjp's gxn.  Nodes at the wall
+    (d = 0) carry no share: the SA source clamps d at 1e-10."""
+    x = level.coords
+    w = level.wall_node
+    d = level.wall_dist
+    gx = np.zeros_like(x, dtype=np.float64)
+    on = d > 1e-10
+    i = np.nonzero(on)[0]
+    v = (x[:, i] - x[:, w[i]]) * (np.asarray(gwd)[i] / d[i])
+    np.add.at(gx, (slice(None), i), v)
+    np.subtract.at(gx, (slice(None), w[i]), v)
+    return gx
+
+
 def bump_levels(n, n_levels, jitter=0.0, seed=0, first_cell=None,
                 ratio=1.15, h=0.05, wall_indicator=1, progress=None):
     xyz, tets, btri, bind = bump_box(n, jitter=jitter, seed=seed,

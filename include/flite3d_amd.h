@@ -247,8 +247,7 @@ int flite_apply(flite_ctx *ctx, int level, int op);
  * (turbulence_model 1) the tangent and the adjoint also differentiate the
  * eddy viscosity in the stress (DEF:5556-5566), the nu~ convection
  * (makeSARHS, DEF:6809-6950), the SA diffusion (DEF:7436-7808) and the
- * source (DEF:9285-9370): du, lam and out are then (N, 6), nu~ last;
- * flite_drag_state_gradient returns FLITE_E_UNSUPPORTED for SA settings.  The
+ * source (DEF:9285-9370): du, lam, out and dJdU are then (N, 6), nu~ last.  The
  * second-order residual is piecewise smooth: at a limiter switch these give
  * the derivative of the branch the state selects.  Vectors are otherwise
  * (N, 5) column-major in host numbering with leading dimension ld.  The reference has no differentiated code; these entry
@@ -259,9 +258,12 @@ int flite_apply(flite_ctx *ctx, int level, int op);
  *   flite_adjoint_apply:  out = (dR/dU)^T lam  (a reversed side loop that
  *                         re-evaluates each side's inputs on dual numbers;
  *                         no Jacobian is stored)
- *   flite_drag_state_gradient: the pressure drag of calculateLiftAndDrag
- *                         (mgnsg3d_def_par.F90:14757-14803) at the current
- *                         state and its partial derivative dJ/dU. */
+ *   flite_drag_state_gradient: the drag of calculateLiftAndDrag
+ *                         (mgnsg3d_def_par.F90:14757-14803: the pressure
+ *                         term and, with ReynoldsNumber > 0, the friction
+ *                         term A_i ws_i.t of the wall stress, recomputed at
+ *                         the current state) and its partial derivative
+ *                         dJ/dU. */
 int flite_tangent_apply(flite_ctx *ctx, const double *du, int ld_du, double *out, int ld_out);
 int flite_adjoint_apply(flite_ctx *ctx, const double *lam, int ld_lam, double *out, int ld_out);
 int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld);
@@ -284,19 +286,36 @@ int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld
  * coordinates (fp_metric_vjp in flite3d_prepro.h). */
 int flite_metric_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb);
 int flite_drag_metric_gradient(flite_ctx *ctx, double *drag, double *gwb, int ld_wb);
-/* The same for first- and second-order HLLC (no counterpart in the
- * reference; for shape-sensitivity callers): d(lam^T R) with respect to
- * every mesh quantity the level-1 residual reads,
+/* The same for first- and second-order HLLC, inviscid, laminar or
+ * Spalart-Allmaras (no counterpart in the reference; for shape-sensitivity
+ * callers): d(lam^T R) with respect to every mesh quantity the level-1
+ * residual of flite_adjoint_apply reads,
  *   gw, gwb   side and boundary-side weights (layout as flite_metric_adjoint),
  *   gvol[N]   dual volumes (host numbering),
- *   gxyz      coordinates directly (N,3) column-major, ld_x >= N.
+ *   gxyz      coordinates directly (N,3) column-major, ld_x >= N,
+ *   gwd[N]    wall distances (host numbering; Spalart-Allmaras only, where
+ *             the source reads them; may be NULL otherwise).
  * Second order reads the volumes and the weights through the Green-Gauss
  * gradients and the coordinates through the reconstruction's side vector
- * (x_b - x_a) and clip length; first order gives gvol = gxyz = 0.  The
- * coordinate gradient of J is then fp_mesh_vjp's chain of
- * (-gw, gJwb - gwb, -gvol, -gxyz) (flite3d_prepro.h). */
+ * (x_b - x_a) and clip length; first order gives gxyz = 0.  The viscous
+ * term reads the weights and volumes through the stress's Green-Gauss
+ * gradients, the weights in its side and boundary fluxes and the wall
+ * normals in the adiabatic-wall projection (their shares reach gwb through
+ * the normals' sums of boundary-side weights); the SA terms read the same
+ * through the nu~ gradients, the diffusion's side terms and the source.
+ * lam is (N, 6) with Spalart-Allmaras.
+ *   flite_drag_mesh_gradient: the drag (as flite_drag_state_gradient, with
+ *             friction when ReynoldsNumber > 0) and its partial derivatives
+ *             with respect to the side weights, boundary-side weights and
+ *             volumes (the friction's wall stress reads all three; the wall
+ *             areas and normals read the boundary-side weights).
+ * The coordinate gradient of J is then fp_mesh_vjp's chain of
+ * (gJw - gw, gJwb - gwb, gJvol - gvol, -gxyz) (flite3d_prepro.h), plus
+ * -gwd through the wall distance |x_i - x_w(i)| to the nearest wall node. */
 int flite_mesh_adjoint(flite_ctx *ctx, const double *lam, int ld_lam, double *gw, int ld_w, double *gwb, int ld_wb,
-                       double *gvol, double *gxyz, int ld_x);
+                       double *gvol, double *gxyz, int ld_x, double *gwd);
+int flite_drag_mesh_gradient(flite_ctx *ctx, double *drag, double *gw, int ld_w, double *gwb, int ld_wb,
+                             double *gvol);
 
 /* Precision of the coarse levels from the next call on: 1 = FP32, 0 = FP64
  * (flite_params.coarse_fp32 sets the initial one).  No counterpart in the

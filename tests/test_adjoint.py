@@ -352,17 +352,23 @@ def test_laminar_adjoint_dot_product(order):
 
 
 def test_linearisation_rejects_unsupported_settings():
-    """Spalart-Allmaras is linearised in the tangent and the adjoint (6
-    components); the drag gradient refuses it."""
+    """Spalart-Allmaras runs in the tangent, the adjoint, the drag's state
+    gradient and the mesh adjoint (6 components); the first-order inviscid
+    entry points (metric adjoint, drag metric gradient) refuse it."""
     from flite3d_amd.device import DeviceError
     from cases import rans
     lv = levels(8, 2, first_cell=0.005)
     s = Solver(rans().to_struct(), list(lv))
     s.init_freestream()
-    assert s.tangent(np.zeros((6, lv[0].n_nodes))).shape == (6, lv[0].n_nodes)
-    assert s.adjoint(np.zeros((6, lv[0].n_nodes))).shape == (6, lv[0].n_nodes)
+    n = lv[0].n_nodes
+    assert s.tangent(np.zeros((6, n))).shape == (6, n)
+    assert s.adjoint(np.zeros((6, n))).shape == (6, n)
+    assert s.drag_gradient()[1].shape == (6, n)
+    assert s.mesh_adjoint(np.zeros((6, n)))[4].shape == (n,)
     with pytest.raises(DeviceError):
-        s.drag_gradient()
+        s.metric_adjoint(np.zeros((5, n)))
+    with pytest.raises(DeviceError):
+        s.drag_metric_gradient()
     s.close()
 
 

@@ -22,7 +22,12 @@ C-ABI (libflite3d_amd.so) and through the oracle on the same inputs:
    coarse levels: four conditions equal their single runs bit for bit; one
    condition with FP64 coarse levels equals the oracle to 1e-9.
 5. RANS-SA, 200^3 graded from 1e-6 (8.1M nodes), 5 levels: one fine
-   residual per node as in 2, and freestream preservation.
+   residual per node as in 2, and freestream preservation.  Its
+   sensitivities (HLLC-2 + SA): the tangent/adjoint dot product of the
+   6-component residual (to 1e-12 of |lam| |J du|), and the coordinate gradient of drag - lam^T R at a
+   fixed state against central differences with the fine level rebuilt
+   from the deformed coordinates (2 degrees of sideslip, so that no
+   farfield face sits on the Roe flux's |u.n| kink).
 """
 import dataclasses
 
@@ -324,9 +329,16 @@ def test_config4_batch_64_conditions():
 
 
 # ---- configuration 5 ------------------------------------------------------------
-def test_config5_rans_200_fine_residual():
+@pytest.fixture(scope="module")
+def box200():
     say("config 5 mesh: 200^3 graded from 1e-6, 5 levels")
-    lv = mesh.bump_levels(200, 5, first_cell=1e-6, ratio=1.15, progress=say)
+    raw = mesh.bump_box(200, first_cell=1e-6, ratio=1.15)
+    lv, _ = mesh.build_levels(*raw, 5, progress=say)
+    return raw, lv
+
+
+def test_config5_rans_200_fine_residual(box200):
+    _, lv = box200
     assert lv[0].n_nodes == 8120601 and lv[0].n_sides == 56360600
     iv = rans(reynoldsnumber=1.0e7, machnumber=0.75, alpha=2.0, multigridscheme=3)
     P = iv.to_struct()
@@ -356,3 +368,71 @@ def test_config5_rans_200_fine_residual():
     r.close()
     check_fine_operations(out, sens)
     assert freestream_residual(lv[0], iv, "config 5") <= 1e-12
+
+
+def test_config5_rans_200_sensitivities(box200):
+    """d(drag)/dX's two halves at 200^3: the state part through the adjoint
+    (its dot product with the tangent), and the mesh part, the coordinate
+    gradient of L(X) = drag - lam^T R(U, X) at a fixed state U and vector
+    lam, through every mesh quantity the preprocessor rebuilds (weights,
+    volumes, coordinates, wall normals and areas, wall distances), against a
+    central difference along a smooth deformation of the bump: dz =
+    sin(pi x) sin(pi y) (1 - z), which moves the wall and its graded cells
+    together."""
+    raw, lv = box200
+    xyz, tets, btri, bind = raw
+    L = lv[0]
+    n = L.n_nodes
+    iv = rans(reynoldsnumber=1.0e7, machnumber=0.75, alpha=2.0, beta=2.0, multigridscheme=3)
+    P = iv.to_struct()
+    u = rans_state(L, 0)
+    u[5] = np.where(L.wall_dist > 0, u[5], 0.0)  # the wall condition's nu~ = 0
+    rng = np.random.default_rng(5)
+    lam = rng.standard_normal((6, n))
+    lam /= np.linalg.norm(lam)
+    say("config 5 sensitivities: tangent/adjoint dot product (6 components, HLLC-2 + SA)")
+    s = Solver(P, [L])
+    s.set(1, "U", u)
+    du = rng.standard_normal((6, n)) * np.abs(u).max(axis=1, keepdims=True)
+    du /= np.linalg.norm(du)
+    jt, ja = s.tangent(du), s.adjoint(lam)
+    a, b = float(np.sum(lam * jt)), float(np.sum(ja * du))
+    # random vectors of 48M entries: their dot product cancels to ~1e-4 of
+    # the Cauchy-Schwarz bound |lam| |J du|, which is the scale of the sums'
+    # rounding
+    scale = max(np.linalg.norm(lam) * np.linalg.norm(jt), np.linalg.norm(ja) * np.linalg.norm(du))
+    print(f"config 5 dot product: {a:.15e} {b:.15e}; |a - b| / |a| {abs(a - b) / abs(a):.2e}, "
+          f"/ (|lam| |J du|) {abs(a - b) / scale:.2e}", flush=True)
+    assert abs(a - b) <= 1e-12 * scale, (a, b, scale)
+    say("config 5 sensitivities: mesh shares and their chain to the coordinates")
+    gw, gwb, gv, gxd, gd = s.mesh_adjoint(lam)
+    drag, jw, jwb, jv = s.drag_mesh_gradient()
+    s.close()
+    gx = mesh.mesh_vjp(xyz, tets, btri, bind, jw - gw, jwb - gwb, gvol=jv - gv,
+                       gxn=-gxd + mesh.wall_distance_vjp(L, -gd))
+    x, y, z = xyz.T
+    d = np.zeros_like(xyz)
+    d[:, 2] = np.sin(np.pi * x) * np.sin(np.pi * y) * (1.0 - z)
+    an = float(np.sum(gx * d))
+    h = 1e-5
+    vals = []
+    for sgn in (1.0, -1.0):
+        say(f"config 5 sensitivities: fine level rebuilt at X {'+' if sgn > 0 else '-'} h dX")
+        lvp, _ = mesh.build_levels(xyz + sgn * h * d, tets, btri, bind, 1)
+        t = Solver(P, [lvp[0]])
+        t.set(1, "U", u)
+        t.apply(1, "PRESSURE")
+        t.apply(1, "VISCOUS")
+        t.apply(1, "SA_DIFFUSION")
+        t.apply(1, "RHS")
+        r = t.get(1, "RHS")[:6].copy()
+        r[:5] += t.get(1, "DISSIPATION")[:5]
+        r[5] += t.get(1, "TURB_DIFFUSION")[0]
+        dr, _ = t.drag_gradient()
+        t.close()
+        vals.append(dr - float(np.sum(lam * r)))
+        del lvp
+    fd = (vals[0] - vals[1]) / (2 * h)
+    print(f"config 5 coordinate gradient: fd {fd:.12e} analytic {an:.12e} rel {abs(fd - an) / abs(an):.2e} "
+          f"(drag {drag:.9e})", flush=True)
+    assert abs(fd - an) <= 1e-5 * abs(an), (fd, an)

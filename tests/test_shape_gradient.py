@@ -153,7 +153,7 @@ def base2():
 def mesh_gradients(lv, u, lam):
     s = Solver(params(2), lv)
     s.set(1, "U", u)
-    out = s.mesh_adjoint(lam)
+    out = s.mesh_adjoint(lam)[:4]
     drag, gjwb = s.drag_metric_gradient()
     s.close()
     return out, drag, gjwb
