@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from flite3d_amd.device import Solver
+from oracle import Oracle
 
 from cases import euler, levels
 
@@ -293,33 +294,50 @@ def make_laminar(n, order):
     return s, lv, u
 
 
+def oracle_for(s, lv):
+    """The oracle on the solver's fine level and parameters: the finite
+    differences below are of its residual, the CPU restatement of the
+    reference's, not of the GPU's."""
+    o = Oracle(s.params[0], [lv[0]])
+    o.init_freestream()
+    return o
+
+
+def viscous_alone(s):
+    """The viscous op on a cleared dissipation array: makeViscosityTerm
+    subtracts into dissipation(:,2:5) (the oracle keeps what is there)."""
+    s.set(1, "DISSIPATION", np.zeros((5, s.n(1))))
+    s.apply(1, "VISCOUS")
+
+
 def residual_laminar(s, u):
     """makeRHS plus the viscous dissipation (dissipation(:,2:5) = -viscous)."""
     s.set(1, "U", u)
     s.apply(1, "PRESSURE")
     s.apply(1, "RHS")
-    s.apply(1, "VISCOUS")
+    viscous_alone(s)
     return s.get(1, "RHS")[:5] + s.get(1, "DISSIPATION")[:5]
 
 
 @pytest.mark.parametrize("order", [1, 2])
 def test_laminar_tangent_matches_finite_differences(order):
-    """Every entry equals the central difference, or, at second order, a
-    one-sided one where a limiter switch lies within H; the viscous part
+    """Every entry equals the central difference of the oracle's residual,
+    or, at second order, a one-sided one where a limiter switch lies within
+    H; the viscous part
     (Green-Gauss gradients of u, v, w, T, Sutherland viscosity, adiabatic
     wall projection, edge and boundary fluxes) is smooth."""
     print(f"[case] laminar tangent FD order={order}", flush=True)
     s, lv, u = make_laminar(8, order)
+    o = oracle_for(s, lv)
     rng = np.random.default_rng(31)
     one_sided = 0
     for du in directions(rng, u, 3):
         jt = s.tangent(du)
-        r0 = residual_laminar(s, u)
+        r0 = residual_laminar(o, u)
         up, um = u.copy(), u.copy()
         up[:5] += H * du
         um[:5] -= H * du
-        rp, rm = residual_laminar(s, up), residual_laminar(s, um)
-        s.set(1, "U", u)
+        rp, rm = residual_laminar(o, up), residual_laminar(o, um)
         tol = 1e-6 * np.abs(jt).max()
         central = np.abs((rp - rm) / (2 * H) - jt) <= tol
         fwd = np.abs((rp - r0) / H - jt) <= tol
@@ -337,6 +355,7 @@ def test_laminar_tangent_matches_finite_differences(order):
     assert np.abs(s.tangent(du) - si.tangent(du)).max() > 1e-2 * np.abs(s.tangent(du)).max()
     si.close()
     s.close()
+    o.close()
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -401,7 +420,7 @@ def residual_rans(s, u):
     on nu~."""
     s.set(1, "U", u)
     s.apply(1, "PRESSURE")
-    s.apply(1, "VISCOUS")
+    viscous_alone(s)
     s.apply(1, "SA_DIFFUSION")
     s.apply(1, "RHS")
     r = s.get(1, "RHS")[:6].copy()
@@ -421,21 +440,21 @@ def directions6(rng, u, k):
 @pytest.mark.parametrize("order", [1, 2])
 def test_rans_tangent_matches_finite_differences(order):
     """Every entry of the 6-component tangent equals the central difference
-    of the RANS-SA residual, or a one-sided one where a switch (limiter,
-    the source's clips) lies within H."""
+    of the oracle's RANS-SA residual, or a one-sided one where a switch
+    (limiter, the source's clips) lies within H."""
     print(f"[case] RANS-SA tangent FD order={order}", flush=True)
     s, lv, u = make_rans(8, order)
+    o = oracle_for(s, lv)
     rng = np.random.default_rng(41)
     one_sided = 0
     for du in directions6(rng, u, 3):
         jt = s.tangent(du)
         assert jt.shape == (6, lv[0].n_nodes)
-        r0 = residual_rans(s, u)
+        r0 = residual_rans(o, u)
         up, um = u.copy(), u.copy()
         up[:6] += H * du
         um[:6] -= H * du
-        rp, rm = residual_rans(s, up), residual_rans(s, um)
-        s.set(1, "U", u)
+        rp, rm = residual_rans(o, up), residual_rans(o, um)
         for c in range(6):  # each component on its own scale (nu~'s residual is small)
             tol = 1e-6 * np.abs(jt[c]).max() + 1e-300
             central = np.abs((rp[c] - rm[c]) / (2 * H) - jt[c]) <= tol
@@ -447,6 +466,7 @@ def test_rans_tangent_matches_finite_differences(order):
             one_sided += int((~central).sum())
     assert one_sided <= 0.002 * jt.size * 3, one_sided
     s.close()
+    o.close()
 
 
 def test_rans_tangent_couples_nu_and_mean_flow():

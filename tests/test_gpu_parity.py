@@ -27,6 +27,15 @@ def rel_err(a, b, floor=1e-30):
     return np.array(out)
 
 
+def node_check(a, b, mag, what, tol=1e-11):
+    """Per node and component: |a - b| <= tol max(|b|, the node's sum of
+    |terms| from the oracle), so that no small-residual node hides behind a
+    component's global maximum."""
+    den = np.maximum(np.maximum(np.abs(b), mag), 1e-300)
+    e = (np.abs(a - b) / den).max(axis=1)
+    assert e.max() < tol, (what, e)
+
+
 def pair(iv, lv, fp32=False):
     P = iv.to_struct(coarse_fp32=fp32)
     return Solver(P, list(lv)), Oracle(iv.to_struct(), list(lv))
@@ -55,6 +64,7 @@ def test_rhs_euler(order, level):
     assert np.abs(s.get(level, "PRESSURE") - o.get(level, "PRESSURE")).max() < 1e-13
     e = rel_err(s.get(level, "RHS")[:5], o.get(level, "RHS")[:5])
     assert e.max() < 1e-11, e
+    node_check(s.get(level, "RHS")[:5], o.get(level, "RHS")[:5], o.magnitudes(level, "RHS")[:5], "rhs")
     if order == 2 and level == 1:
         eg = rel_err(s.get(1, "GRADIENT"), o.get(1, "GRADIENT"))
         assert eg.max() < 1e-11, eg
@@ -100,6 +110,12 @@ def test_sa_terms():
     for f in ("DT", "DISSIPATION", "DT_TURB", "TURB_DIFFUSION", "RHS"):
         e = rel_err(s.get(1, f), o.get(1, f))
         assert e.max() < 1e-11, (f, e)
+    # per node: the residual, the viscous term (components 2-5 of the
+    # dissipation) and the SA diffusion (its magnitudes sit in row 6)
+    node_check(s.get(1, "RHS")[:6], o.get(1, "RHS")[:6], o.magnitudes(1, "RHS"), "rhs")
+    node_check(s.get(1, "DISSIPATION")[1:5], o.get(1, "DISSIPATION")[1:5], o.magnitudes(1, "VISCOUS")[1:5], "viscous")
+    node_check(s.get(1, "TURB_DIFFUSION"), o.get(1, "TURB_DIFFUSION"), o.magnitudes(1, "SA_DIFFUSION")[5:6],
+               "sa diffusion")
 
 
 @pytest.mark.parametrize("case", ["euler1", "euler2", "rans"])
