@@ -505,7 +505,15 @@ void metric_vjp(const fp_mesh &m, const double *gw, const double *gwb, double *g
 // (gen3d_grid.f90:2504-2570).  Here a decremented node goes to the back of
 // a FIFO bucket of connectivity c, so a seed always has the largest
 // available connectivity left, and seeds of equal connectivity can come in
-// another order than Gen3d's.
+// another order than Gen3d's.  The fine sides here are sorted by (lower,
+// upper) node, where Gen3d numbers them in order of first appearance over
+// the elements (gen3d_grid.f90:2110-2330), so a seed's neighbours come in
+// another order too.  The boundary-first renumbering below moves whole
+// agglomerates; Gen3d's swap (gen3d_grid.f90:2748-2790) exchanges the
+// members but not the seeds, whose mapping values are negated at that point.
+// Lonesome agglomerates join their one neighbour as in gen3d_grid.f90:
+// 2611-2700, and the boundary count is that of the agglomerates holding a
+// fine boundary node.
 Level agglomerate(const Level &F) {
   const int n = F.n_nodes, nbf = F.n_bnodes;
   const long long ne = (long long)F.e1.size();
