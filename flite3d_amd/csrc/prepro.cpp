@@ -509,7 +509,7 @@ void metric_vjp(const fp_mesh &m, const double *gw, const double *gwb, double *g
 // upper) node, where Gen3d numbers them in order of first appearance over
 // the elements (gen3d_grid.f90:2110-2330), so a seed's neighbours come in
 // another order too.  The boundary-first renumbering below moves whole
-// agglomerates; Gen3d's swap (gen3d_grid.f90:2748-2790) exchanges the
+// agglomerates; Gen3d's swap (gen3d_grid.f90:2739-2796) exchanges the
 // members but not the seeds, whose mapping values are negated at that point.
 // Lonesome agglomerates join their one neighbour as in gen3d_grid.f90:
 // 2611-2700, and the boundary count is that of the agglomerates holding a
