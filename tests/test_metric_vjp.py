@@ -98,3 +98,24 @@ def test_volume_vjp():
     ref = np.zeros_like(xyz)
     ref[np.asarray(old)] = gn.T
     assert np.array_equal(gx2, ref)
+
+
+def test_wall_distance_vjp():
+    """mesh.wall_distance_vjp against central differences of the
+    preprocessor's wall distance: the distance to the nearest wall node,
+    smooth while that node stays the same."""
+    xyz, tets, btri, bind = mesh.bump_box(5, jitter=0.15, seed=0)
+    lv, old = mesh.build_levels(xyz, tets, btri, bind, 1)
+    L0 = lv[0]
+    rng = np.random.default_rng(5)
+    gwd = rng.standard_normal(L0.n_nodes)
+    gx = mesh.wall_distance_vjp(L0, gwd)
+    d = rng.standard_normal(xyz.shape)
+    h = 1e-7
+    lp, _ = mesh.build_levels(xyz + h * d, tets, btri, bind, 1)
+    lm, _ = mesh.build_levels(xyz - h * d, tets, btri, bind, 1)
+    assert np.array_equal(lp[0].wall_node, L0.wall_node) and np.array_equal(lm[0].wall_node, L0.wall_node)
+    fd = float(np.sum(gwd * (lp[0].wall_dist - lm[0].wall_dist))) / (2 * h)
+    an = float(np.sum(gx * d[np.asarray(old)].T))
+    assert abs(an) > 0
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
