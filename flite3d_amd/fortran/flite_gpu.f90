@@ -279,6 +279,34 @@ module flite_gpu
        integer(c_int), value :: ld_wb
      end function flite_drag_metric_gradient
 
+     ! gwd may be c_null_ptr without Spalart-Allmaras
+     integer(c_int) function flite_mesh_adjoint(ctx, lam, ld_lam, gw, ld_w, gwb, ld_wb, gvol, gxyz, ld_x, gwd) &
+          bind(C, name="flite_mesh_adjoint")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: lam(*)
+       integer(c_int), value :: ld_lam
+       real(c_double), intent(out) :: gw(*)
+       integer(c_int), value :: ld_w
+       real(c_double), intent(out) :: gwb(*)
+       integer(c_int), value :: ld_wb
+       real(c_double), intent(out) :: gvol(*), gxyz(*)
+       integer(c_int), value :: ld_x
+       type(c_ptr), value :: gwd
+     end function flite_mesh_adjoint
+
+     integer(c_int) function flite_drag_mesh_gradient(ctx, drag, gw, ld_w, gwb, ld_wb, gvol) &
+          bind(C, name="flite_drag_mesh_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: drag
+       real(c_double), intent(out) :: gw(*)
+       integer(c_int), value :: ld_w
+       real(c_double), intent(out) :: gwb(*)
+       integer(c_int), value :: ld_wb
+       real(c_double), intent(out) :: gvol(*)
+     end function flite_drag_mesh_gradient
+
      integer(c_int) function flite_set_coarse_precision(ctx, fp32) bind(C, name="flite_set_coarse_precision")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
