@@ -2980,6 +2980,21 @@ __global__ void k_sa_conv_mesh_adjoint(LevelDev<double> L, DevParams P, const do
   }
 }
 
+// vectors (nc, N) in device memory and host numbering <-> the level's src /
+// rhs (device numbering; d2h = the level's device-to-host index)
+__global__ void k_load_host_order(const double *in, const int *d2h, int n, int nc, double *src) {
+  int d = gtid_x();
+  if (d >= n) return;
+  const int h = d2h[d];
+  for (int k = 0; k < 6; ++k) src[(size_t)k * n + d] = k < nc ? in[(size_t)k * n + h] : 0.0;
+}
+__global__ void k_store_host_order(const double *rhs, const int *d2h, int n, int nc, double *out) {
+  int d = gtid_x();
+  if (d >= n) return;
+  const int h = d2h[d];
+  for (int k = 0; k < nc; ++k) out[(size_t)k * n + h] = rhs[(size_t)k * n + d];
+}
+
 // out(1) += rnbar(rho), out(6) += rnbar(nu~)
 __global__ void k_add_rho_nu(const double *rnbar, int n, double *out) {
   int i = gtid_x();

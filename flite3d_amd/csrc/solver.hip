@@ -2281,11 +2281,10 @@ int alloc_lin(Level &L) {
 }
 }  // namespace
 
-int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
-  TRY(check_linearisable(c, true, true, true));
+namespace {
+// (dR/dU) V.src into V.rhs on level 1 (src filled by the caller)
+int tangent_core(flite_ctx *c) {
   Level &L = *c->lev[0];
-  const int nc = c->P.sa ? 6 : 5;
-  TRY(put_vector(c, du, ld_du, nc));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   if (second_order(c, L)) {
@@ -2320,13 +2319,12 @@ int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, 
     if (!second_order(c, L)) TRY(op_prep(c, L, true));
     TRY(visc_tangent(c, L));
   }
-  return get_vector(c, out, ld_out, nc);
+  return 0;
 }
 
-int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
-  TRY(check_linearisable(c, true, true, true));
+// (dR/dU)^T V.src into V.rhs on level 1 (src filled by the caller)
+int adjoint_core(flite_ctx *c) {
   Level &L = *c->lev[0];
-  const int nc = c->P.sa ? 6 : 5;
   if (second_order(c, L)) {
     // records, gradients and bounds of the current state, the bounds'
     // extremal neighbours, then the per-side shares and their transposes
@@ -2334,7 +2332,6 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
     TRY(op_grad(c, L));
     TRY(alloc_lin(L));
     if (!L.abuf2) TRY(L.pool.alloc(&L.abuf2, (size_t)LIN_SIDE * std::max(1, L.ne)));
-    TRY(put_vector(c, lam, ld_lam, nc));
     DevParams P = c->P;
     LevelDev<double> V = L.view<double>();
     double *ub = L.lin, *gbn = L.lin + 5 * (size_t)L.n, *qb = L.lin + 30 * (size_t)L.n;
@@ -2354,10 +2351,9 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
     if (P.sa) hipLaunchKernelGGL(k_sa_conv_adjoint, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, V, P, V.src, V.rhs);
     TRY(check_launch());
     if (laminar(c)) TRY(visc_adjoint(c, L));
-    return get_vector(c, out, ld_out, nc);
+    return 0;
   }
   if (!L.abuf) TRY(L.pool.alloc(&L.abuf, 10 * (size_t)std::max(1, L.ne)));
-  TRY(put_vector(c, lam, ld_lam, nc));
   DevParams P = c->P;
   LevelDev<double> V = L.view<double>();
   if (L.ne > 0) hipLaunchKernelGGL(k_flux_adjoint, dim3(nblk(L.ne)), dim3(NTHR), 0, c->stream, V, P, V.src, L.abuf);
@@ -2369,7 +2365,59 @@ int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out
     TRY(op_prep(c, L, true));
     TRY(visc_adjoint(c, L));
   }
+  return 0;
+}
+
+// device-vector form of an apply: in and out on the context's GPU
+int apply_device(flite_ctx *c, const double *in, double *out, int (*core)(flite_ctx *)) {
+  TRY(check_linearisable(c, true, true, true));
+  if (!in || !out) return fail(FLITE_E_ARG, "null vector");
+  for (const void *p : {(const void *)in, (const void *)out}) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice || a.device != c->device) {
+      (void)hipGetLastError();
+      return fail(FLITE_E_ARG, "vectors must be device memory on the context's GPU");
+    }
+  }
+  Level &L = *c->lev[0];
+  const int nc = c->P.sa ? 6 : 5;
+  LevelDev<double> V = L.view<double>();
+  // the producer of `in` may run on another stream: wait for the device
+  HIPTRY(hipDeviceSynchronize());
+  hipLaunchKernelGGL(k_load_host_order, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, in, (const int *)L.host_index,
+                     L.n, nc, V.src);
+  TRY(check_launch());
+  TRY(core(c));
+  hipLaunchKernelGGL(k_store_host_order, dim3(nblk(L.n)), dim3(NTHR), 0, c->stream, (const double *)V.rhs,
+                     (const int *)L.host_index, L.n, nc, out);
+  TRY(check_launch());
+  HIPTRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+}  // namespace
+
+int flite_tangent_apply(flite_ctx *c, const double *du, int ld_du, double *out, int ld_out) {
+  TRY(check_linearisable(c, true, true, true));
+  const int nc = c->P.sa ? 6 : 5;
+  TRY(put_vector(c, du, ld_du, nc));
+  TRY(tangent_core(c));
   return get_vector(c, out, ld_out, nc);
+}
+
+int flite_adjoint_apply(flite_ctx *c, const double *lam, int ld_lam, double *out, int ld_out) {
+  TRY(check_linearisable(c, true, true, true));
+  const int nc = c->P.sa ? 6 : 5;
+  TRY(put_vector(c, lam, ld_lam, nc));
+  TRY(adjoint_core(c));
+  return get_vector(c, out, ld_out, nc);
+}
+
+int flite_tangent_apply_device(flite_ctx *c, const double *du, double *out) {
+  return apply_device(c, du, out, tangent_core);
+}
+
+int flite_adjoint_apply_device(flite_ctx *c, const double *lam, double *out) {
+  return apply_device(c, lam, out, adjoint_core);
 }
 
 int flite_drag_state_gradient(flite_ctx *c, double *drag, double *dJdU, int ld) {

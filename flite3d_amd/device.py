@@ -53,6 +53,8 @@ def lib():
         L.flite_sync.argtypes = [vp]
         L.flite_tangent_apply.argtypes = [vp, P(d), i, P(d), i]
         L.flite_adjoint_apply.argtypes = [vp, P(d), i, P(d), i]
+        L.flite_tangent_apply_device.argtypes = [vp, vp, vp]
+        L.flite_adjoint_apply_device.argtypes = [vp, vp, vp]
         L.flite_drag_state_gradient.argtypes = [vp, P(d), P(d), i]
         L.flite_metric_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i]
         L.flite_mesh_adjoint.argtypes = [vp, P(d), i, P(d), i, P(d), i, P(d), P(d), i, P(d)]
@@ -229,6 +231,15 @@ class Solver:
         D = ctypes.POINTER(ctypes.c_double)
         self._check(self.L.flite_adjoint_apply(self.ctx, x.ctypes.data_as(D), n, out.ctypes.data_as(D), n))
         return out
+
+    def tangent_device(self, du_ptr, out_ptr):
+        """flite_tangent_apply_device: du and out are device addresses of
+        (nc, N) float64 arrays in host numbering (e.g. tensor.data_ptr())."""
+        self._check(self.L.flite_tangent_apply_device(self.ctx, ctypes.c_void_p(du_ptr), ctypes.c_void_p(out_ptr)))
+
+    def adjoint_device(self, lam_ptr, out_ptr):
+        """flite_adjoint_apply_device, as tangent_device."""
+        self._check(self.L.flite_adjoint_apply_device(self.ctx, ctypes.c_void_p(lam_ptr), ctypes.c_void_p(out_ptr)))
 
     def drag_gradient(self):
         """(drag, dDrag/dU) at the current state: the pressure drag, plus the

@@ -267,6 +267,15 @@ int flite_apply(flite_ctx *ctx, int level, int op);
 int flite_tangent_apply(flite_ctx *ctx, const double *du, int ld_du, double *out, int ld_out);
 int flite_adjoint_apply(flite_ctx *ctx, const double *lam, int ld_lam, double *out, int ld_out);
 int flite_drag_state_gradient(flite_ctx *ctx, double *drag, double *dJdU, int ld);
+/* The tangent and adjoint with the vectors in device memory on the
+ * context's GPU (no counterpart in the reference; for solvers that keep
+ * their vectors in HBM and iterate the applies without host copies): du,
+ * lam and out are (nc, N) contiguous, host numbering, nc = 5 (6 with
+ * Spalart-Allmaras).  The call waits for all earlier work on the device
+ * (a producer on another stream has then finished), and returns when out
+ * is written.  Host pointers return FLITE_E_ARG. */
+int flite_tangent_apply_device(flite_ctx *ctx, const double *du, double *out);
+int flite_adjoint_apply_device(flite_ctx *ctx, const double *lam, double *out);
 
 /* Derivatives with respect to the level-1 mesh metrics, for gradients with
  * respect to the node coordinates (no counterpart in the reference; for

@@ -540,3 +540,30 @@ def test_rans_adjoint_columns_are_jacobian_rows():
             outside[sorted(ring2)] = False
             assert np.all(row[:, outside] == 0.0)
     s.close()
+
+
+@pytest.mark.parametrize("kind", ["euler2", "rans2"])
+def test_device_vector_applies_equal_host_ones(kind):
+    """flite_tangent_apply_device / flite_adjoint_apply_device on torch
+    device tensors give the host entry points' results bit for bit, and
+    host pointers are refused (before any device access)."""
+    import torch
+    from flite3d_amd.device import DeviceError
+    print(f"[case] device-vector applies {kind}", flush=True)
+    if kind == "euler2":
+        s, lv, u = make_smooth(8, 2)
+        nc = 5
+    else:
+        s, lv, u = make_rans(8, 2)
+        nc = 6
+    n = lv[0].n_nodes
+    x = np.random.default_rng(71).standard_normal((nc, n))
+    for host, dev in ((s.tangent, s.tangent_device), (s.adjoint, s.adjoint_device)):
+        ref = host(x)
+        xd = torch.from_numpy(x.copy()).to("cuda:0")
+        od = torch.zeros((nc, n), dtype=torch.float64, device="cuda:0")
+        dev(xd.data_ptr(), od.data_ptr())
+        assert np.array_equal(od.cpu().numpy(), ref)
+    with pytest.raises(DeviceError):
+        s.tangent_device(x.ctypes.data, x.ctypes.data)
+    s.close()
