@@ -12,6 +12,7 @@ kinds (wall and farfield) are exercised.
 """
 import numpy as np
 import pytest
+import torch  # torch before the first context: its device init after one has failed on this image
 
 from flite3d_amd.device import Solver
 from oracle import Oracle
@@ -547,7 +548,6 @@ def test_device_vector_applies_equal_host_ones(kind):
     """flite_tangent_apply_device / flite_adjoint_apply_device on torch
     device tensors give the host entry points' results bit for bit, and
     host pointers are refused (before any device access)."""
-    import torch
     from flite3d_amd.device import DeviceError
     print(f"[case] device-vector applies {kind}", flush=True)
     if kind == "euler2":
