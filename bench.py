@@ -388,6 +388,49 @@ def second_order_checks(lv, cycles, ndir, h, smooth=False, raw=None, nxdir=0, hx
             "ms_per_adjoint_apply": round(1e3 * t_adj / ndir, 3)}
 
 
+def bench_sensitivities(levels, cycles, prm=None, reps=3):
+    """Time the sensitivity entry points on the case after `cycles` cycles
+    from freestream (FP64): the tangent and adjoint with vectors in HBM
+    (flite_*_apply_device, on torch tensors) and through host vectors, the
+    drag's state gradient, the mesh adjoint and the drag's mesh gradient.
+    Each is called once untimed, then `reps` times."""
+    import torch
+    torch.cuda.init()  # torch's device before the library's first context
+    from flite3d_amd import device
+    s = device.Solver((prm or params)(False), levels)
+    s.init_freestream()
+    for c in range(1, cycles + 1):
+        s.cycle(c)
+    s.sync()
+    n = s.n(1)
+    nc = 6 if s.params[0].turbulence_model == 1 else 5
+    x = np.random.default_rng(0).standard_normal((nc, n))
+    xd = torch.from_numpy(x.copy()).to("cuda:0")
+    od = torch.zeros_like(xd)
+    out = {}
+
+    def t(name, f):
+        f()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        s.sync()
+        out[f"ms_{name}"] = round(1e3 * (time.perf_counter() - t0) / reps, 2)
+        print(f"[bench] sensitivities: {name} {out[f'ms_{name}']} ms", flush=True)
+
+    t("tangent_device", lambda: s.tangent_device(xd.data_ptr(), od.data_ptr()))
+    t("adjoint_device", lambda: s.adjoint_device(xd.data_ptr(), od.data_ptr()))
+    t("tangent_host_vectors", lambda: s.tangent(x))
+    t("adjoint_host_vectors", lambda: s.adjoint(x))
+    t("drag_state_gradient", lambda: s.drag_gradient())
+    t("mesh_adjoint", lambda: s.mesh_adjoint(x))
+    t("drag_mesh_gradient", lambda: s.drag_mesh_gradient())
+    t("cycle", lambda: s.cycle(cycles + 1))
+    s.close()
+    return out
+
+
 def bench_config3(cycles, ndir=16, h=1e-7, hd=1e-5, nxdir=4, hx=1e-6, ndir2=4):
     """Discrete adjoint case: Euler M=0.8, alpha=1.25 on the 64^3 bump box,
     first-order HLLC (second order in second_order_checks), after `cycles`
@@ -607,6 +650,9 @@ def main():
     ap.add_argument("--fp32-alone", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the timed steps: FP32 coarse levels with no switch to FP64, then "
                          "FP64, each to a 1e-8 drop (1e-10 with --euler) or this many seconds")
+    ap.add_argument("--sens", type=int, default=0, metavar="CYCLES",
+                    help="instead of the timed steps: after this many cycles from freestream, time the "
+                         "sensitivity entry points (tangent, adjoint, dJ/dU, mesh adjoint) on the case")
     ap.add_argument("--euler", action="store_true",
                     help="with --fp32-alone: second-order Euler (M 0.5, alpha 2, CFL 2) instead of RANS-SA")
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
@@ -636,6 +682,10 @@ def main():
     sides = [l.n_sides for l in levels]
     workload = (f"{'Euler' if a.euler else 'RANS-SA'} bump box {a.n}^3 hexes, "
                 f"{f'first cell {a.first_cell:g}' if a.first_cell > 0 else 'uniform spacing'}, {a.levels} MG levels")
+    if a.sens > 0:
+        print(json.dumps(dict(workload=workload, cycles=a.sens, **bench_sensitivities(levels, a.sens, prm))),
+              flush=True)
+        return
     if a.controls > 0:
         print(json.dumps(dict(workload=workload, cap_s=a.controls, runs=sweep_controls(levels, a.controls))),
               flush=True)
