@@ -286,7 +286,7 @@ int flite_adjoint_apply_device(flite_ctx *ctx, const double *lam, double *out);
  *   flite_metric_adjoint:  gw  = d(lam^T R)/d side_weights,
  *                          gwb = d(lam^T R)/d bside_weights, at the current
  *                          state (dual numbers seeded on the weights);
- *   flite_drag_metric_gradient: the pressure drag (as flite_drag_state_gradient)
+ *   flite_drag_metric_gradient: the drag of an inviscid run (its pressure term)
  *                          and its partial derivative with respect to the
  *                          boundary-side weights, through the wall-node areas
  *                          and normals the library forms from them.
