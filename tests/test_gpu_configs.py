@@ -376,9 +376,9 @@ def test_config5_rans_200_sensitivities(box200):
     gradient of L(X) = drag - lam^T R(U, X) at a fixed state U and vector
     lam, through every mesh quantity the preprocessor rebuilds (weights,
     volumes, coordinates, wall normals and areas, wall distances), against a
-    central difference along a smooth deformation of the bump: dz =
-    sin(pi x) sin(pi y) (1 - z), which moves the wall and its graded cells
-    together."""
+    central difference of the oracle's residual and drag along a smooth
+    deformation of the bump: dz = sin(pi x) sin(pi y) (1 - z), which moves
+    the wall and its graded cells together."""
     raw, lv = box200
     xyz, tets, btri, bind = raw
     L = lv[0]
@@ -419,17 +419,19 @@ def test_config5_rans_200_sensitivities(box200):
     for sgn in (1.0, -1.0):
         say(f"config 5 sensitivities: fine level rebuilt at X {'+' if sgn > 0 else '-'} h dX")
         lvp, _ = mesh.build_levels(xyz + sgn * h * d, tets, btri, bind, 1)
-        t = Solver(P, [lvp[0]])
-        t.set(1, "U", u)
-        t.apply(1, "PRESSURE")
-        t.apply(1, "VISCOUS")
-        t.apply(1, "SA_DIFFUSION")
-        t.apply(1, "RHS")
-        r = t.get(1, "RHS")[:6].copy()
-        r[:5] += t.get(1, "DISSIPATION")[:5]
-        r[5] += t.get(1, "TURB_DIFFUSION")[0]
-        dr, _ = t.drag_gradient()
-        t.close()
+        o = Oracle(P, [lvp[0]])  # the difference is of the oracle's residual and drag
+        o.init_freestream()
+        o.set(1, "U", u)
+        o.apply(1, "PRESSURE")
+        o.set(1, "DISSIPATION", np.zeros((5, n)))  # makeViscosityTerm subtracts into it
+        o.apply(1, "VISCOUS")
+        o.apply(1, "SA_DIFFUSION")
+        o.apply(1, "RHS")
+        r = o.get(1, "RHS")[:6].copy()
+        r[:5] += o.get(1, "DISSIPATION")[:5]
+        r[5] += o.get(1, "TURB_DIFFUSION")[0]
+        dr = float(o.monitor()["forces"][1])
+        o.close()
         vals.append(dr - float(np.sum(lam * r)))
         del lvp
     fd = (vals[0] - vals[1]) / (2 * h)

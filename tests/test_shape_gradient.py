@@ -3,7 +3,8 @@ the GPU (flite_metric_adjoint, flite_drag_metric_gradient, and the
 preprocessor's fp_metric_vjp).
 
 The reference has no differentiated code, so these are pinned by central
-finite differences of what the library and the preprocessor compute:
+finite differences of the oracle's residual and drag, with the metrics the
+preprocessor computes:
   * d(lam^T R)/d(side weights): re-upload the level with perturbed weights;
   * d(pressure drag)/d(boundary-side weights) at a fixed state;
   * the chain to the coordinates, L(X) = drag(U, X) - lam^T R(U, X) at a
@@ -29,6 +30,7 @@ import pytest
 
 from flite3d_amd import mesh
 from flite3d_amd.device import Solver
+from oracle import Oracle
 
 from cases import euler
 
@@ -58,14 +60,16 @@ def base():
 
 
 def evaluate(lv, u, lam, order=1):
-    """(lam^T R, drag) at state u on levels lv."""
-    s = Solver(params(order), list(lv))
-    s.set(1, "U", u)
-    s.apply(1, "PRESSURE")
-    s.apply(1, "RHS")
-    r = s.get(1, "RHS")[:5]
-    drag, _ = s.drag_gradient()
-    s.close()
+    """(lam^T R, drag) at state u on the fine level of lv, by the oracle
+    (its residual and its monitor's pressure drag)."""
+    o = Oracle(params(order), [lv[0]])
+    o.init_freestream()
+    o.set(1, "U", u)
+    o.apply(1, "PRESSURE")
+    o.apply(1, "RHS")
+    r = o.get(1, "RHS")[:5]
+    drag = float(o.monitor()["forces"][1])
+    o.close()
     return float(np.sum(lam * r)), drag
 
 

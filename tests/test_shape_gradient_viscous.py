@@ -3,9 +3,9 @@ residual and of the drag with its friction term (flite_mesh_adjoint and
 flite_drag_mesh_gradient with ReynoldsNumber > 0, flite_drag_state_gradient
 with friction), on the GPU.
 
-The reference has no differentiated code, so these are pinned, as in
-test_shape_gradient.py, by central differences of what the library itself
-computes:
+The reference has no differentiated code, so these are pinned by central
+differences of the oracle's residual and drag (the CPU restatement of the
+reference's):
   * each mesh quantity on its own: volumes, wall distances and coordinates
     along a random direction; side and boundary-side weights one entry at a
     time, where the entry must equal the central difference or a one-sided
@@ -28,6 +28,7 @@ import pytest
 
 from flite3d_amd import mesh
 from flite3d_amd.device import Solver
+from oracle import Oracle
 
 from cases import laminar, rans
 from test_adjoint import smooth_state
@@ -69,23 +70,26 @@ def make(kind, order, n=5):
 
 
 def evaluate(kind, order, lv, u, lam):
-    """(lam^T R, drag) at state u on levels lv: R as the tangent and adjoint
-    linearise it (makeRHS, the viscous dissipation and, with SA, the nu~
-    convection, diffusion and source)."""
-    s = Solver(inputs(kind, order).to_struct(), list(lv))
-    s.set(1, "U", u)
-    s.apply(1, "PRESSURE")
-    s.apply(1, "VISCOUS")
+    """(lam^T R, drag) at state u on the fine level of lv, by the oracle: R
+    as the tangent and adjoint linearise it (makeRHS, the viscous
+    dissipation and, with SA, the nu~ convection, diffusion and source), the
+    drag of its monitor (pressure and friction)."""
+    o = Oracle(inputs(kind, order).to_struct(), [lv[0]])
+    o.init_freestream()
+    o.set(1, "U", u)
+    o.apply(1, "PRESSURE")
+    o.set(1, "DISSIPATION", np.zeros((5, lv[0].n_nodes)))  # makeViscosityTerm subtracts into it
+    o.apply(1, "VISCOUS")
     if kind == "rans":
-        s.apply(1, "SA_DIFFUSION")
-    s.apply(1, "RHS")
+        o.apply(1, "SA_DIFFUSION")
+    o.apply(1, "RHS")
     nc = lam.shape[0]
-    r = s.get(1, "RHS")[:nc].copy()
-    r[:5] += s.get(1, "DISSIPATION")[:5]
+    r = o.get(1, "RHS")[:nc].copy()
+    r[:5] += o.get(1, "DISSIPATION")[:5]
     if kind == "rans":
-        r[5] += s.get(1, "TURB_DIFFUSION")[0]
-    drag, _ = s.drag_gradient()
-    s.close()
+        r[5] += o.get(1, "TURB_DIFFUSION")[0]
+    drag = float(o.monitor()["forces"][1])
+    o.close()
     return float(np.sum(lam * r)), drag
 
 
