@@ -27,8 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per side of one evaluation, FP64 (counting
-# rule: every distinct input read once, every output written once)
+# algorithmic bytes per side of one evaluation, FP64 (counting rule: every
+# distinct input read once, every output written once)
 ALGO_BYTES_PER_EDGE = {"flux": 76.0, "grad": 67.9, "stage": 19.7}
 
 
@@ -662,7 +662,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the sharding across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if a.config == 1:
         print(json.dumps(bench_config1(200)), flush=True)

@@ -50,8 +50,7 @@
  * indicator 9, dissipation schemes 1 and 3, k-omega/SST) are not restated:
  * the calls that would need them return FLITE_E_UNSUPPORTED.
  *
- * Known reference defects and how they are mirrored (SURVEY Appendix B
- * decisions, restated here):
+ * Known reference defects and how they are mirrored:
  *   - coarse-level coordinates are never filled (DEF:14002,14011,14071):
  *     taken as zero, so HLLC-2 reconstruction vanishes on coarse levels;
  *   - nullifyOuterBoundary reads an uninitialised local ivd (DEF:4259):
