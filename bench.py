@@ -661,7 +661,7 @@ def main():
                          "5: RANS-SA 200^3, 5 levels")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: this library runs on one GPU; --gpus {a.gpus} is not supported "
               "(the sharding across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if a.config == 1:

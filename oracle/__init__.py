@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
-this package, and only as the checker / CPU comparator.  The product package
+this package, and only to check results against / as the CPU comparator.  The product package
 flite3d_amd never imports it.
 """
 import ctypes

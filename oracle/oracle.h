@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of FLITE3D's flow-solver hot path, in FP64.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg may load this library, and only as the checker
- * (or, for cpu_baseline, as the timed CPU comparator).  The product path
+ * bench.py's cpu_baseline leg may load this library, and only to check
+ * results against (or, for cpu_baseline, as the timed CPU comparator).  The product path
  * (flite3d_amd, libflite3d_amd.so) never links, loads or calls it.
  *
  * It restates, routine by routine and in the reference's loop order, the
