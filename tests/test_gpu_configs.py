@@ -30,6 +30,7 @@ C-ABI (libflite3d_amd.so) and through the oracle on the same inputs:
    farfield face sits on the Roe flux's |u.n| kink).
 """
 import dataclasses
+import os
 
 import numpy as np
 import pytest
@@ -37,11 +38,21 @@ import pytest
 from flite3d_amd import device, mesh
 from flite3d_amd.device import Solver
 from flite3d_amd.params import InputVariables
-from oracle import Oracle
+from oracle import Oracle as _Oracle
 
 from cases import euler, rans
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(1200)]
+
+
+class Oracle(_Oracle):
+    """The oracle with its side-flux and node loops on up to 16 host threads:
+    its results do not depend on the thread count (test_oracle_fields.py),
+    and the 128^3 and 200^3 cases then take seconds instead of minutes."""
+
+    def __init__(self, params, levels):
+        super().__init__(params, levels)
+        self.set_threads(min(16, os.cpu_count() or 1))
 
 RK5 = [0.25, 1.0 / 6.0, 0.375, 0.5, 1.0]
 
